@@ -1,0 +1,1008 @@
+/*
+ * moldy_oracle.c -- CPU restatement of MOLDY's Fe-Fe force hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle): the product library never links it.
+ * Every routine cites the reference file:line it restates.  Arithmetic is kept
+ * in the reference's operation order and literal precision (several Fortran
+ * literals are default REAL(4); they are reproduced with float literals), and
+ * the file must be compiled with -ffp-contract=off so that no FMA is formed.
+ */
+#include "moldy_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define T(a, i, j) (a)[((j)-1) * 3 + ((i)-1)] /* Fortran-style 1-based (i,j) */
+
+/* src/constants_m.F90:84-113 (unit system: eV, Angstrom, amu) */
+static const double k_electron = 1.602176487e-19;
+static const double k_angstrom = 1.e-10;
+static const double k_boltzmann = 1.3806504e-23;
+static double k_fs_to_time(void) {
+  const double time_to_sec = 1.018050530893617e-014;
+  const double time_to_fs = time_to_sec / 1.e-15;
+  return 1.0 / time_to_fs;
+}
+static double k_gpa_to_press(void) {
+  const double a3 = k_angstrom * k_angstrom * k_angstrom;
+  const double press_to_gpa = (k_electron / a3) / 1.0e9;
+  return 1.0 / press_to_gpa;
+}
+
+static int g_threads = 1;
+void orc_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
+
+/* Fortran NINT: round half away from zero (C round()). */
+static inline double f_nint(double x) { return round(x); }
+
+/* ======================================================================
+ * Potential: src/fetransmetal.F90 (Fe-Fe only: find_coeff factors are 1,
+ * SolSol_(:,5) = 1 at :831,937-940, so r*p == r and *dp is exact)
+ * ====================================================================== */
+
+/* Heaviside xH0, src/fetransmetal.F90:569-579 */
+static inline double xh0(double x) { return (x < 0.0) ? 0.0 : 1.0; }
+
+/* Knot sums, src/fetransmetal.F90:133-139 (vee), :186-190 (dvee),
+ * :259-265 (phi), :331-336 (dphi).  (x)**3 is evaluated as (x*x)*x. */
+static double knot_v(const orc_pot *p, double r) {
+  double s = 0.0;
+  for (int i = 0; i < p->nv; i++) {
+    double d = p->rv[i] - r;
+    s = s + p->av[i] * (d * d * d) * xh0(d);
+  }
+  return s * 1.0;
+}
+static double knot_dv(const orc_pot *p, double r) {
+  double s = 0.0;
+  for (int i = 0; i < p->nv; i++) {
+    double d = p->rv[i] - r;
+    s = s - 1.0 * 3.0 * p->av[i] * (d * d) * xh0(d);
+  }
+  return s * 1.0;
+}
+static double knot_p(const orc_pot *p, double r) {
+  double s = 0.0;
+  for (int i = 0; i < p->np; i++) {
+    double d = p->rp[i] - r;
+    s = s + p->ap[i] * (d * d * d) * xh0(d);
+  }
+  return s * 1.0;
+}
+static double knot_dp(const orc_pot *p, double r) {
+  double s = 0.0;
+  for (int i = 0; i < p->np; i++) {
+    double d = p->rp[i] - r;
+    s = s - 3.0 * 1.0 * p->ap[i] * (d * d) * xh0(d);
+  }
+  return s * 1.0;
+}
+
+/* Biersack-Ziegler screening length for Z=26: src/fetransmetal.F90:423,445.
+ * All literals there are default REAL(4): the length is a float. */
+static double bier_a(void) {
+  volatile float c = 0.8854f * 0.529f;
+  float t = powf(26.0f, 0.666666667f);
+  volatile float s = t + t;
+  float den = powf(s, 0.5f);
+  float a = c / den;
+  return (double)a;
+}
+static double bier_zed(void) {
+  /* na1*na2 * 8.98755178d9 * electron / angstrom, :420 */
+  return (double)(26 * 26) * 8.98755178e9 * k_electron / k_angstrom;
+}
+
+/* biersack, src/fetransmetal.F90:414-431 (REAL(4) literals promoted) */
+static double biersack(double r) {
+  double zed = bier_zed();
+  double a = bier_a();
+  double x = r / a;
+  double ph = (double)0.1818f * exp(-((double)3.2f * x)) +
+              (double)0.5099f * exp(-((double)0.9423f * x)) +
+              (double)0.2802f * exp(-((double)0.4029f * x)) +
+              (double)0.02817f * exp(-((double)0.2016f * x));
+  return zed * ph / r;
+}
+/* dvee_biersack, src/fetransmetal.F90:434-454 (REAL(8) literals) */
+static double dvee_biersack(double r) {
+  double zed = bier_zed();
+  double a = bier_a();
+  double x = r / a;
+  double s = (1.0 / r + 3.2 / a) * 0.1818 * exp(-(3.2 * x)) +
+             (1.0 / r + 0.9423 / a) * 0.5099 * exp(-(0.9423 * x)) +
+             (1.0 / r + 0.4029 / a) * 0.2802 * exp(-(0.4029 * x)) +
+             (1.0 / r + 0.2016 / a) * 0.02817 * exp(-(0.2016 * x));
+  return -(zed / r) * s;
+}
+
+/* expjoin / dexpjoin, src/fetransmetal.F90:511-551 */
+static double expjoin(double a1, double x1, double v1, double a2, double x2, double v2,
+                      double r) {
+  double vv1 = v1 / x1, xx1 = log(x1), vv2 = v2 / x2, xx2 = log(x2);
+  double t = (r - a1) / (a2 - a1);
+  double sp = (2.0 * t * t * t - 3.0 * t * t + 1.0) * xx1 +
+              (t * t * t - 2.0 * t * t + t) * vv1 * (a2 - a1) +
+              (-(2.0 * t * t * t) + 3.0 * t * t) * xx2 +
+              (t * t * t - t * t) * vv2 * (a2 - a1);
+  return exp(sp);
+}
+static double dexpjoin(double a1, double x1, double v1, double a2, double x2, double v2,
+                       double r) {
+  double vv1 = v1 / x1, xx1 = log(x1), vv2 = v2 / x2, xx2 = log(x2);
+  double t = (r - a1) / (a2 - a1);
+  double ds = (6.0 * t * t - 6.0 * t) * xx1 + (3.0 * t * t - 4.0 * t + 1.0) * vv1 * (a2 - a1) +
+              (-(6.0 * t * t) + 6.0 * t) * xx2 + (3.0 * t * t - 2.0 * t) * vv2 * (a2 - a1);
+  ds = ds / (a2 - a1);
+  double sp = (2.0 * t * t * t - 3.0 * t * t + 1.0) * xx1 +
+              (t * t * t - 2.0 * t * t + t) * vv1 * (a2 - a1) +
+              (-(2.0 * t * t * t) + 3.0 * t * t) * xx2 +
+              (t * t * t - t * t) * vv2 * (a2 - a1);
+  return ds * exp(sp);
+}
+
+/* vee_src, src/fetransmetal.F90:100-159.  r == bier_cut1 exactly gives 0. */
+double orc_vee_src(const orc_pot *p, double r) {
+  if (r > p->bier_cut2) return knot_v(p, r);
+  if (r > p->bier_cut1)
+    return expjoin(p->bier_cut1, p->sc[0], p->sc[1], p->bier_cut2, p->sc[2], p->sc[3], r);
+  if (r < p->bier_cut1) return biersack(r);
+  return 0.0;
+}
+/* dvee_src, src/fetransmetal.F90:167-219 */
+double orc_dvee_src(const orc_pot *p, double r) {
+  if (r > p->bier_cut2) return knot_dv(p, r);
+  if (r > p->bier_cut1)
+    return dexpjoin(p->bier_cut1, p->sc[0], p->sc[1], p->bier_cut2, p->sc[2], p->sc[3], r);
+  if (r < p->bier_cut1) return dvee_biersack(r);
+  return 0.0;
+}
+/* phi_src, src/fetransmetal.F90:227-285 (flat below phi_cut(1)) */
+double orc_phi_src(const orc_pot *p, double r) {
+  if (r < p->phi_cut_r) return p->phi_cut_v;
+  return knot_p(p, r);
+}
+/* dphi_src, src/fetransmetal.F90:293-366 */
+double orc_dphi_src(const orc_pot *p, double r) {
+  if (r < p->phi_cut_r) return 0.0;
+  return knot_dp(p, r);
+}
+/* emb_src / demb_src, src/fetransmetal.F90:375-404 */
+static double emb_src(double rho) { return -1.0 * sqrt(rho > 0.0 ? rho : 0.0); }
+static double demb_src(double rho) {
+  double d = 0.0;
+  if (rho > 0.0) d = -0.5 / sqrt(rho);
+  return d;
+}
+
+/* phi_turn, src/fetransmetal.F90:896-926: 500-step bisection on dphi */
+static double phi_turn(const orc_pot *p) {
+  double lo = 0.0, hi = 3.0;
+  for (int i = 0; i < 500; i++) {
+    double mid = (hi + lo) / 2.0;
+    double g = orc_dphi_src(p, mid);
+    if (g > 0.0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return (hi + lo) / 2.0;
+}
+
+/* check_supported_atomic_numbers (:671-672 cut values, REAL(4) literals)
+ * and set_up_m (:768-894) for the Fe-Fe pair. */
+void orc_pot_init(orc_pot *p, int nv, const double *av, const double *rv, int np,
+                  const double *ap, const double *rp) {
+  memset(p, 0, sizeof(*p));
+  p->nv = nv;
+  p->np = np;
+  for (int i = 0; i < nv; i++) { p->av[i] = av[i]; p->rv[i] = rv[i]; }
+  for (int i = 0; i < np; i++) { p->ap[i] = ap[i]; p->rp[i] = rp[i]; }
+  p->bier_cut1 = (double)0.9f;
+  p->bier_cut2 = (double)1.9f;
+  const double eps = (double)0.000000000001f;
+  p->sc[0] = biersack(p->bier_cut1 - eps);
+  p->sc[1] = dvee_biersack(p->bier_cut1 - eps);
+  p->sc[2] = orc_vee_src(p, p->bier_cut2 + eps);
+  p->sc[3] = orc_dvee_src(p, p->bier_cut2 + eps);
+  p->phi_cut_r = 0.0; /* not yet set while bisecting */
+  p->phi_cut_v = 0.0;
+  double rs = phi_turn(p);
+  p->phi_cut_r = rs;
+  p->phi_cut_v = orc_phi_src(p, rs);
+}
+
+/* calculate_lookup_table_veephi/_emb, src/lookup_m.F90:161-251;
+ * ranges from src/fetransmetal.F90:589-593 and src/potential_m.F90:185 */
+int orc_pot_build_lookup(orc_pot *p, int nlookup) {
+  p->nlookup = nlookup;
+  p->r_min = 0.0;
+  p->r_max = 6.0;
+  p->vp_range = p->r_max - p->r_min;
+  p->vp_reciprange = 1.0 / p->vp_range;
+  p->vp_recipnlkp = 1.0 / (double)nlookup;
+  p->vp_recipstep = (double)nlookup / p->vp_range;
+  p->rho_min = 0.0;
+  p->rho_max = 10000.0;
+  p->e_range = p->rho_max - p->rho_min;
+  p->e_reciprange = 1.0 / p->e_range;
+  p->e_recipnlkp = 1.0 / (double)nlookup;
+  p->e_recipstep = (double)nlookup / p->e_range;
+  for (int k = 0; k < 6; k++) {
+    p->tab[k] = (double *)malloc(sizeof(double) * (size_t)(nlookup + 1));
+    if (!p->tab[k]) return -1;
+  }
+  for (int i = 0; i <= nlookup; i++) {
+    double r = p->r_min + (double)i * p->vp_range * p->vp_recipnlkp;
+    p->tab[0][i] = orc_vee_src(p, r);
+    p->tab[1][i] = orc_dvee_src(p, r);
+    p->tab[2][i] = orc_phi_src(p, r);
+    p->tab[3][i] = orc_dphi_src(p, r);
+    double rh = p->rho_min + (double)i * p->e_range * p->e_recipnlkp;
+    p->tab[4][i] = emb_src(rh);
+    p->tab[5][i] = demb_src(rh);
+  }
+  p->use_lookup = 1;
+  return 0;
+}
+void orc_pot_free(orc_pot *p) {
+  for (int k = 0; k < 6; k++) { free(p->tab[k]); p->tab[k] = NULL; }
+  p->use_lookup = 0;
+}
+
+/* get_value_from_lookup_veephi, src/lookup_m.F90:307-346 */
+static double lk_vp(const orc_pot *p, const double *t, double r) {
+  if (r >= p->r_max) return t[p->nlookup];
+  int lo = (int)((double)p->nlookup * (r - p->r_min) * p->vp_reciprange);
+  double rl = p->r_min + p->vp_range * (double)lo * p->vp_recipnlkp;
+  double dr = (r - rl) * p->vp_recipstep;
+  return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
+}
+/* get_value_from_lookup_emb, src/lookup_m.F90:356-394 */
+static double lk_e(const orc_pot *p, const double *t, double rho) {
+  if (rho >= p->rho_max) return t[p->nlookup];
+  int lo = (int)((double)p->nlookup * (rho - p->rho_min) * p->e_reciprange);
+  double rl = p->rho_min + p->e_range * (double)lo * p->e_recipnlkp;
+  double dr = (rho - rl) * p->e_recipstep;
+  return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
+}
+
+/* potential_m dispatch, src/potential_m.F90:288-417 */
+double orc_vee(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[0], r) : orc_vee_src(p, r); }
+double orc_dvee(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[1], r) : orc_dvee_src(p, r); }
+double orc_phi(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[2], r) : orc_phi_src(p, r); }
+double orc_dphi(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[3], r) : orc_dphi_src(p, r); }
+double orc_emb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[4], rho) : emb_src(rho); }
+double orc_demb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[5], rho) : demb_src(rho); }
+
+/* ======================================================================
+ * System state (src/particles_m.F90:52-66, src/parrinellorahman_m.F90:102-115)
+ * ====================================================================== */
+struct orc_sys {
+  int n, ivol, nnbrs, ntcm, ntc;
+  int nlcx, nlcy, nlcz;
+  double deltat, rpad, rcut, rnear, press, bmass, bdel2;
+  double *x0, *y0, *z0, *x1, *y1, *z1, *x2, *y2, *z2;
+  double *x1dt2, *y1dt2, *z1dt2;
+  double *mass, *del2, *rho, *afrho, *dafrho, *fx, *fy, *fz;
+  int *zat;
+  int *l0, *link, *ic, *numn, *nlist;
+  double b0[9], b1[9], b2[9], tk[9], tc[9], tg[9], tp[9], tginv[9], tgdot[9], tgid[9], fb[9];
+  double vol, pe, ke, s1, s2, s3, te, h, th, boxke;
+  orc_pot pot;
+};
+
+static double *dalloc(int n) { return (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+/* pr_get_metric_tensor, src/parrinellorahman_m.F90:223-227 */
+static void pr_volume(orc_sys *s) {
+  const double *b = s->b0;
+  s->vol = T(b, 1, 1) * (T(b, 2, 2) * T(b, 3, 3) - T(b, 2, 3) * T(b, 3, 2)) +
+           T(b, 2, 1) * (T(b, 3, 2) * T(b, 1, 3) - T(b, 1, 2) * T(b, 3, 3)) +
+           T(b, 3, 1) * (T(b, 1, 2) * T(b, 2, 3) - T(b, 2, 2) * T(b, 1, 3));
+}
+
+/* pr_get_realsep_from_dx, src/parrinellorahman_m.F90:149-173 (dx updated in place) */
+static inline double realsep(const double *b, int ivol, double *dx, double *dy, double *dz) {
+  if (ivol == 0 || ivol == 1 || ivol == 4) {
+    *dx = *dx - f_nint(*dx);
+    *dy = *dy - f_nint(*dy);
+    *dz = *dz - f_nint(*dz);
+  } else if (ivol == 2) {
+    *dx = *dx - f_nint(*dx);
+    *dy = *dy - f_nint(*dy);
+  } else if (ivol == 5) {
+    *dz = *dz - f_nint(*dz);
+  }
+  double rx = T(b, 1, 1) * *dx + T(b, 1, 2) * *dy + T(b, 1, 3) * *dz;
+  double ry = T(b, 2, 1) * *dx + T(b, 2, 2) * *dy + T(b, 2, 3) * *dz;
+  double rz = T(b, 3, 1) * *dx + T(b, 3, 2) * *dy + T(b, 3, 3) * *dz;
+  return sqrt(rx * rx + ry * ry + rz * rz);
+}
+
+orc_sys *orc_create(int n, const double *b0, const double *x0, const double *y0,
+                    const double *z0, const double *x1, const double *y1,
+                    const double *z1, const double *mass, const int *z, int ivol,
+                    double deltat_fs, double rpad, double rcut, int nnbrs, int ntcm,
+                    double press_gpa, double boxmass, const orc_pot *pot) {
+  orc_sys *s = (orc_sys *)calloc(1, sizeof(orc_sys));
+  if (!s) return NULL;
+  s->n = n;
+  s->ivol = ivol;
+  s->nnbrs = nnbrs;
+  s->ntcm = ntcm;
+  s->ntc = 0;
+  /* src/params_m.F90:412 (fs -> model time), :454 (GPa -> model), :207-208 */
+  s->deltat = deltat_fs * k_fs_to_time();
+  s->press = press_gpa * k_gpa_to_press();
+  s->bmass = (double)n * boxmass;
+  s->bdel2 = s->deltat * s->deltat / (2.0 * s->bmass);
+  s->rpad = rpad;
+  s->rcut = rcut > 6.0 ? rcut : 6.0; /* moldin.F90:226 with range 6 (fetransmetal.F90:589-593) */
+  s->rnear = s->rcut + s->rpad;      /* moldin.F90:248 */
+  memcpy(s->b0, b0, sizeof(s->b0));
+  /* moldin.F90:252-257: link cells from the diagonal, at least 3, never recomputed */
+  s->nlcx = (int)(T(b0, 1, 1) / s->rnear);
+  s->nlcy = (int)(T(b0, 2, 2) / s->rnear);
+  s->nlcz = (int)(T(b0, 3, 3) / s->rnear);
+  if (s->nlcx < 3) s->nlcx = 3;
+  if (s->nlcy < 3) s->nlcy = 3;
+  if (s->nlcz < 3) s->nlcz = 3;
+  s->x0 = dalloc(n); s->y0 = dalloc(n); s->z0 = dalloc(n);
+  s->x1 = dalloc(n); s->y1 = dalloc(n); s->z1 = dalloc(n);
+  s->x2 = dalloc(n); s->y2 = dalloc(n); s->z2 = dalloc(n);
+  s->x1dt2 = dalloc(n); s->y1dt2 = dalloc(n); s->z1dt2 = dalloc(n);
+  s->mass = dalloc(n); s->del2 = dalloc(n);
+  s->rho = dalloc(n); s->afrho = dalloc(n); s->dafrho = dalloc(n);
+  s->fx = dalloc(n); s->fy = dalloc(n); s->fz = dalloc(n);
+  s->zat = (int *)calloc((size_t)n, sizeof(int));
+  int ncell = s->nlcx * s->nlcy * s->nlcz;
+  s->l0 = (int *)calloc((size_t)ncell, sizeof(int));
+  s->link = (int *)calloc((size_t)n + 1, sizeof(int));
+  s->ic = (int *)calloc((size_t)n + 1, sizeof(int));
+  s->numn = (int *)calloc((size_t)n, sizeof(int));
+  s->nlist = (int *)calloc((size_t)n * (size_t)nnbrs, sizeof(int));
+  if (!s->nlist || !s->zat) { orc_destroy(s); return NULL; }
+  for (int i = 0; i < n; i++) {
+    s->x0[i] = x0[i]; s->y0[i] = y0[i]; s->z0[i] = z0[i];
+    s->x1[i] = x1 ? x1[i] : 0.0; s->y1[i] = y1 ? y1[i] : 0.0; s->z1[i] = z1 ? z1[i] : 0.0;
+    s->mass[i] = mass[i];
+    s->zat[i] = z[i];
+    /* moldin.F90:294-300 (2.0 is an exact REAL(4) literal) */
+    s->del2[i] = (mass[i] > (double)0.01f) ? s->deltat * s->deltat / (2.0 * mass[i]) : 0.0;
+  }
+  s->pot = *pot;
+  pr_volume(s);
+  return s;
+}
+
+void orc_destroy(orc_sys *s) {
+  if (!s) return;
+  double *d[] = {s->x0, s->y0, s->z0, s->x1, s->y1, s->z1, s->x2, s->y2, s->z2,
+                 s->x1dt2, s->y1dt2, s->z1dt2, s->mass, s->del2, s->rho, s->afrho,
+                 s->dafrho, s->fx, s->fy, s->fz};
+  for (size_t k = 0; k < sizeof(d) / sizeof(d[0]); k++) free(d[k]);
+  free(s->zat); free(s->l0); free(s->link); free(s->ic); free(s->numn); free(s->nlist);
+  free(s);
+}
+
+/* ======================================================================
+ * Link cells, src/linkcell_m.F90 (1-based atom and cell indices inside)
+ * ====================================================================== */
+static const double hmeps = 0.4999999999;
+
+static inline int cell_of(const orc_sys *s, int i /*0-based*/) {
+  int ix = (int)((s->x0[i] - f_nint(s->x0[i]) + hmeps) * (double)s->nlcx);
+  int iy = (int)((s->y0[i] - f_nint(s->y0[i]) + hmeps) * (double)s->nlcy);
+  int iz = (int)((s->z0[i] - f_nint(s->z0[i]) + hmeps) * (double)s->nlcz);
+  return ix + s->nlcx * iy + s->nlcx * s->nlcy * iz + 1;
+}
+
+/* linkup, src/linkcell_m.F90:122-198 */
+int orc_linkup(orc_sys *s) {
+  int ncell = s->nlcx * s->nlcy * s->nlcz;
+  int *l1 = (int *)calloc((size_t)ncell + 1, sizeof(int));
+  if (!l1) return -1;
+  for (int c = 0; c < ncell; c++) s->l0[c] = 0;
+  for (int i = 1; i <= s->n; i++) {
+    int ip = cell_of(s, i - 1);
+    s->ic[i] = ip;
+    int me = l1[ip - 1];
+    if (me != 0) {
+      s->link[me] = i;
+      l1[ip - 1] = i;
+    } else {
+      s->l0[ip - 1] = i;
+      l1[ip - 1] = i;
+    }
+  }
+  for (int ip = 1; ip <= ncell; ip++) {
+    int me = l1[ip - 1];
+    if (me == 0) continue;
+    s->link[me] = s->l0[ip - 1];
+  }
+  free(l1);
+  return 0;
+}
+
+/* relink, src/linkcell_m.F90:215-274 */
+int orc_relink(orc_sys *s) {
+  int *l0 = s->l0, *link = s->link, *ic = s->ic;
+  for (int i = 1; i <= s->n; i++) {
+    int ip = cell_of(s, i - 1);
+    if (ip == ic[i]) continue;
+    int me = ic[i];
+    int j = l0[me - 1];
+    for (;;) {
+      if (link[j] == i) break;
+      j = link[j];
+    }
+    link[j] = link[i];
+    l0[me - 1] = link[j];
+    if (i == j) l0[me - 1] = 0;
+    j = l0[ip - 1];
+    if (l0[ip - 1] == 0) j = i;
+    link[i] = link[j];
+    link[j] = i;
+    ic[i] = ip;
+    l0[ip - 1] = i;
+  }
+  return 0;
+}
+
+/* update_neighbourlist, src/neighbourlist_m.F90:103-276: half list over the
+ * own-cell forward chain and 13 forward cells; NLIST(k,i) stored at
+ * nlist[(i-1)*nnbrs + k-1] (column-major NLIST(nnbrs,nm)). */
+int orc_update_neighbourlist(orc_sys *s) {
+  const int nlcx = s->nlcx, nlcy = s->nlcy, nlcz = s->nlcz;
+  const int *l0 = s->l0, *link = s->link;
+  for (int i = 0; i < s->n; i++) s->numn[i] = 0;
+  int overflow = 0;
+  for (int ip = 1; ip <= nlcx * nlcy * nlcz && !overflow; ip++) {
+    int ix = (ip - 1) % nlcx + 1;
+    int iy = ((ip - 1) / nlcx) % nlcy + 1;
+    int iz = ((ip - 1) / (nlcx * nlcy)) % nlcz + 1;
+    int i = l0[ip - 1];
+    if (i == 0) continue;
+    for (int kz = 2; kz <= 3; kz++) {
+      int jz = iz + kz - 2;
+      if (jz - nlcz - 1 == 0) jz = 1;
+      int kk = (kz == 3) ? 1 : 2;
+      int jzn = (jz - 1) * nlcx * nlcy;
+      for (int ky = kk; ky <= 3; ky++) {
+        int jy = iy + ky - 2;
+        if (jy == 0) jy = nlcy;
+        if (jy - nlcy - 1 == 0) jy = 1;
+        int kl = (kz == 2 && ky == 2) ? 2 : 1;
+        int jyzn = jzn + (jy - 1) * nlcx;
+        for (int kx = kl; kx <= 3; kx++) {
+          int jx = ix + kx - 2;
+          if (jx == 0) jx = nlcx;
+          if (jx - nlcx - 1 == 0) jx = 1;
+          int jp = jx + jyzn;
+          for (;;) { /* linkloop */
+            int j;
+            if (ip == jp) {
+              j = link[i];
+              if (j == l0[ip - 1]) { i = l0[ip - 1]; goto next_kx; }
+            } else {
+              j = l0[jp - 1];
+              if (j == 0) goto next_kx;
+            }
+            for (;;) { /* testcandidates */
+              double dx = s->x0[i - 1] - s->x0[j - 1];
+              double dy = s->y0[i - 1] - s->y0[j - 1];
+              double dz = s->z0[i - 1] - s->z0[j - 1];
+              double r = realsep(s->b0, s->ivol, &dx, &dy, &dz);
+              if (r < s->rnear) {
+                s->numn[i - 1] += 1;
+                if (s->numn[i - 1] > s->nnbrs) { overflow = 1; goto done; }
+                s->nlist[(size_t)(i - 1) * s->nnbrs + s->numn[i - 1] - 1] = j;
+              }
+              j = link[j];
+              if (j == l0[jp - 1]) break;
+            }
+            i = link[i];
+            if (i == l0[ip - 1]) goto next_kx;
+          }
+        next_kx:;
+        }
+      }
+    }
+  }
+done:
+  return overflow ? 1 : 0;
+}
+
+/* rhoset, src/dynamics_m.F90:497-579 (serial order: scatter to j, gather to i) */
+void orc_rhoset(orc_sys *s) {
+  const int n = s->n, nn = s->nnbrs;
+  for (int i = 0; i < n; i++) s->rho[i] = 0.0;
+#ifdef _OPENMP
+  if (g_threads > 1) {
+    int nt = g_threads;
+    double *acc = (double *)calloc((size_t)n * nt, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+      double *my = acc + (size_t)omp_get_thread_num() * n;
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i++) {
+        double rt = 0.0;
+        for (int k = 0; k < s->numn[i]; k++) {
+          int j = s->nlist[(size_t)i * nn + k] - 1;
+          double dx = s->x0[i] - s->x0[j], dy = s->y0[i] - s->y0[j], dz = s->z0[i] - s->z0[j];
+          double r = realsep(s->b0, s->ivol, &dx, &dy, &dz);
+          my[j] = my[j] + orc_phi(&s->pot, r);
+          rt = rt + orc_phi(&s->pot, r);
+        }
+        my[i] = my[i] + rt;
+      }
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i++) {
+        double v = 0.0;
+        for (int t = 0; t < nt; t++) v += acc[(size_t)t * n + i];
+        s->rho[i] = v;
+      }
+    }
+    free(acc);
+    return;
+  }
+#endif
+  for (int i = 0; i < n; i++) {
+    double rt = 0.0;
+    for (int k = 0; k < s->numn[i]; k++) {
+      int j = s->nlist[(size_t)i * nn + k] - 1;
+      double dx = s->x0[i] - s->x0[j], dy = s->y0[i] - s->y0[j], dz = s->z0[i] - s->z0[j];
+      double r = realsep(s->b0, s->ivol, &dx, &dy, &dz);
+      s->rho[j] = s->rho[j] + orc_phi(&s->pot, r);
+      rt = rt + orc_phi(&s->pot, r);
+    }
+    s->rho[i] = s->rho[i] + rt;
+  }
+}
+
+/* embed, src/potential_m.F90:148-169 (rho <= 1e-40 leaves stale values) */
+void orc_embed(orc_sys *s) {
+  double pe = 0.0;
+  for (int i = 0; i < s->n; i++) {
+    if (s->rho[i] <= 1e-40) continue;
+    s->afrho[i] = orc_emb(&s->pot, s->rho[i]);
+    pe = pe + s->afrho[i];
+    s->dafrho[i] = orc_demb(&s->pot, s->rho[i]);
+  }
+  s->pe = pe;
+}
+
+/* pair pass of force, src/dynamics_m.F90:235-346 */
+static void force_pair_loop(orc_sys *s, int i, double *fxs, double *fys, double *fzs,
+                            double *pe, double *tp) {
+  const double *b = s->b0;
+  const int nn = s->nnbrs;
+  double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+  for (int k = 0; k < s->numn[i]; k++) {
+    int j = s->nlist[(size_t)i * nn + k] - 1;
+    double dx = s->x0[i] - s->x0[j], dy = s->y0[i] - s->y0[j], dz = s->z0[i] - s->z0[j];
+    double r = realsep(b, s->ivol, &dx, &dy, &dz);
+    double rr = 1.0 / r;
+    double pp = orc_vee(&s->pot, r);
+    double fpp = -orc_dvee(&s->pot, r) * rr;
+    *pe = *pe + pp;
+    double fcp = -1.0 * (orc_dphi(&s->pot, r) * s->dafrho[i] + orc_dphi(&s->pot, r) * s->dafrho[j]) * rr;
+    double fp = fpp + fcp;
+    double rx = T(b, 1, 1) * dx + T(b, 1, 2) * dy + T(b, 1, 3) * dz;
+    double ry = T(b, 2, 1) * dx + T(b, 2, 2) * dy + T(b, 2, 3) * dz;
+    double rz = T(b, 3, 1) * dx + T(b, 3, 2) * dy + T(b, 3, 3) * dz;
+    if ((s->mass[i] + s->mass[j]) > (double)0.1f) {
+      T(tp, 1, 1) = T(tp, 1, 1) + dx * rx * fp;
+      T(tp, 2, 1) = T(tp, 2, 1) + dx * ry * fp;
+      T(tp, 3, 1) = T(tp, 3, 1) + dx * rz * fp;
+      T(tp, 1, 2) = T(tp, 1, 2) + dy * rx * fp;
+      T(tp, 2, 2) = T(tp, 2, 2) + dy * ry * fp;
+      T(tp, 3, 2) = T(tp, 3, 2) + dy * rz * fp;
+      T(tp, 1, 3) = T(tp, 1, 3) + dz * rx * fp;
+      T(tp, 2, 3) = T(tp, 2, 3) + dz * ry * fp;
+      T(tp, 3, 3) = T(tp, 3, 3) + dz * rz * fp;
+    }
+    double ddfx = -dx * fp, ddfy = -dy * fp, ddfz = -dz * fp;
+    fxs[j] = fxs[j] + ddfx;
+    fys[j] = fys[j] + ddfy;
+    fzs[j] = fzs[j] + ddfz;
+    fxi = fxi + ddfx;
+    fyi = fyi + ddfy;
+    fzi = fzi + ddfz;
+  }
+  fxs[i] = fxs[i] - fxi;
+  fys[i] = fys[i] - fyi;
+  fzs[i] = fzs[i] - fzi;
+}
+
+/* kinten, src/thermostat_m.F90:127-177 */
+void orc_kinten(orc_sys *s) {
+  const double *b = s->b0;
+  for (int k = 0; k < 9; k++) s->tk[k] = 0.0;
+  for (int i = 0; i < s->n; i++) {
+    if (s->zat[i] == 0) continue;
+    double x1 = s->x1[i], y1 = s->y1[i], z1 = s->z1[i], m = s->mass[i];
+    double mvx = (T(b, 1, 1) * x1 + T(b, 1, 2) * y1 + T(b, 1, 3) * z1) * m;
+    double mvy = (T(b, 2, 1) * x1 + T(b, 2, 2) * y1 + T(b, 2, 3) * z1) * m;
+    double mvz = (T(b, 3, 1) * x1 + T(b, 3, 2) * y1 + T(b, 3, 3) * z1) * m;
+    T(s->tk, 1, 1) += x1 * mvx; T(s->tk, 2, 1) += x1 * mvy; T(s->tk, 3, 1) += x1 * mvz;
+    T(s->tk, 1, 2) += y1 * mvx; T(s->tk, 2, 2) += y1 * mvy; T(s->tk, 3, 2) += y1 * mvz;
+    T(s->tk, 1, 3) += z1 * mvx; T(s->tk, 2, 3) += z1 * mvy; T(s->tk, 3, 3) += z1 * mvz;
+  }
+  for (int k = 0; k < 9; k++) s->tk[k] = s->tk[k] / (s->deltat * s->deltat);
+  for (int k = 0; k < 9; k++) s->tgdot[k] = 0.0;
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++)
+        T(s->tgdot, i, j) = T(s->tgdot, i, j) + T(b, k, i) * T(s->b1, k, j) + T(s->b1, k, i) * T(b, k, j);
+}
+
+/* remainder of force after the pair loop: tc, kinten, fb (dynamics_m.F90:365-394) */
+static void force_tail(orc_sys *s) {
+  const double *b = s->b0;
+  double *tc = s->tc;
+  T(tc, 1, 1) = T(b, 2, 2) * T(b, 3, 3) - T(b, 2, 3) * T(b, 3, 2);
+  T(tc, 2, 1) = T(b, 3, 2) * T(b, 1, 3) - T(b, 1, 2) * T(b, 3, 3);
+  T(tc, 3, 1) = T(b, 1, 2) * T(b, 2, 3) - T(b, 2, 2) * T(b, 1, 3);
+  T(tc, 1, 2) = T(b, 2, 3) * T(b, 3, 1) - T(b, 3, 3) * T(b, 2, 1);
+  T(tc, 2, 2) = T(b, 3, 3) * T(b, 1, 1) - T(b, 1, 3) * T(b, 3, 1);
+  T(tc, 3, 2) = T(b, 1, 3) * T(b, 2, 1) - T(b, 1, 1) * T(b, 2, 3);
+  T(tc, 1, 3) = T(b, 2, 1) * T(b, 3, 2) - T(b, 3, 1) * T(b, 2, 2);
+  T(tc, 2, 3) = T(b, 3, 1) * T(b, 1, 2) - T(b, 1, 1) * T(b, 3, 2);
+  T(tc, 3, 3) = T(b, 1, 1) * T(b, 2, 2) - T(b, 2, 1) * T(b, 1, 2);
+  orc_kinten(s);
+  for (int i = 1; i <= 3; i++)
+    for (int j = 1; j <= 3; j++)
+      T(s->fb, i, j) = T(s->tk, i, j) + T(s->tp, i, j) - s->press * T(s->tc, i, j);
+}
+
+void orc_force_pairs(orc_sys *s) {
+  const int n = s->n;
+  for (int k = 0; k < 9; k++) { s->tp[k] = 0.0; s->tg[k] = 0.0; }
+  for (int i = 0; i < n; i++) { s->fx[i] = 0.0; s->fy[i] = 0.0; s->fz[i] = 0.0; }
+  pr_volume(s);
+  orc_rhoset(s);
+  orc_embed(s);
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) T(s->tg, i, j) = T(s->tg, i, j) + T(s->b0, k, i) * T(s->b0, k, j);
+#ifdef _OPENMP
+  if (g_threads > 1) {
+    int nt = g_threads;
+    double *acc = (double *)calloc((size_t)3 * n * nt, sizeof(double));
+    double *pes = (double *)calloc((size_t)nt, sizeof(double));
+    double *tps = (double *)calloc((size_t)9 * nt, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+      int t = omp_get_thread_num();
+      double *mx = acc + (size_t)3 * n * t, *my = mx + n, *mz = my + n;
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i++) force_pair_loop(s, i, mx, my, mz, &pes[t], &tps[9 * t]);
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i++) {
+        double ax = 0, ay = 0, az = 0;
+        for (int q = 0; q < nt; q++) {
+          ax += acc[(size_t)3 * n * q + i];
+          ay += acc[(size_t)3 * n * q + n + i];
+          az += acc[(size_t)3 * n * q + 2 * n + i];
+        }
+        s->fx[i] = ax; s->fy[i] = ay; s->fz[i] = az;
+      }
+    }
+    for (int q = 0; q < nt; q++) {
+      s->pe += pes[q];
+      for (int k = 0; k < 9; k++) s->tp[k] += tps[9 * q + k];
+    }
+    free(acc); free(pes); free(tps);
+    force_tail(s);
+    return;
+  }
+#endif
+  for (int i = 0; i < n; i++) force_pair_loop(s, i, s->fx, s->fy, s->fz, &s->pe, s->tp);
+  force_tail(s);
+}
+
+/* force, src/dynamics_m.F90:135-401 (ntc bookkeeping :186-198) */
+int orc_force(orc_sys *s) {
+  int st = 0;
+  s->ntc = s->ntc + 1;
+  if (s->ntc == 1) st = orc_update_neighbourlist(s);
+  if (s->ntc == s->ntcm) s->ntc = 0;
+  if (st) return st;
+  orc_force_pairs(s);
+  return 0;
+}
+
+/* matrix_invert / matrix_multiply, src/matrix_m.F90:60-119 */
+static void mat_invert(const double *a, double *b) {
+  double det = T(a, 1, 1) * (T(a, 2, 2) * T(a, 3, 3) - T(a, 2, 3) * T(a, 3, 2));
+  det = det + T(a, 2, 1) * (T(a, 3, 2) * T(a, 1, 3) - T(a, 1, 2) * T(a, 3, 3));
+  det = det + T(a, 3, 1) * (T(a, 1, 2) * T(a, 2, 3) - T(a, 1, 3) * T(a, 2, 2));
+  T(b, 1, 1) = T(a, 2, 2) * T(a, 3, 3) - T(a, 2, 3) * T(a, 3, 2);
+  T(b, 1, 2) = T(a, 3, 1) * T(a, 2, 3) - T(a, 2, 1) * T(a, 3, 3);
+  T(b, 1, 3) = T(a, 2, 1) * T(a, 3, 2) - T(a, 3, 1) * T(a, 2, 2);
+  T(b, 2, 1) = T(a, 3, 2) * T(a, 1, 3) - T(a, 1, 2) * T(a, 3, 3);
+  T(b, 2, 2) = T(a, 1, 1) * T(a, 3, 3) - T(a, 1, 3) * T(a, 3, 1);
+  T(b, 2, 3) = T(a, 3, 1) * T(a, 1, 2) - T(a, 1, 1) * T(a, 3, 2);
+  T(b, 3, 1) = T(a, 1, 2) * T(a, 2, 3) - T(a, 1, 3) * T(a, 2, 2);
+  T(b, 3, 2) = T(a, 2, 1) * T(a, 1, 3) - T(a, 2, 3) * T(a, 1, 1);
+  T(b, 3, 3) = T(a, 1, 1) * T(a, 2, 2) - T(a, 1, 2) * T(a, 2, 1);
+  for (int k = 0; k < 9; k++) b[k] = b[k] / det;
+}
+static void mat_mul(const double *a, const double *b, double *c) {
+  for (int i = 1; i <= 3; i++) {
+    for (int j = 1; j <= 3; j++) T(c, i, j) = 0.0;
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) T(c, i, j) = T(c, i, j) + T(a, i, k) * T(b, k, j);
+  }
+}
+
+/* moldin.F90:446-477: first force, tgid, initial accelerations and box acceleration */
+int orc_md_init(orc_sys *s) {
+  int st = orc_force(s);
+  if (st) return st;
+  mat_invert(s->tg, s->tginv);
+  mat_mul(s->tginv, s->tgdot, s->tgid);
+  const double *g = s->tgid;
+  for (int i = 0; i < s->n; i++) {
+    if (s->zat[i] == 0) continue; /* atomic_index==0 marks a vacancy */
+    double a1 = T(g, 1, 1) * s->x1[i] + T(g, 1, 2) * s->y1[i] + T(g, 1, 3) * s->z1[i];
+    double a2 = T(g, 2, 1) * s->x1[i] + T(g, 2, 2) * s->y1[i] + T(g, 2, 3) * s->z1[i];
+    double a3 = T(g, 3, 1) * s->x1[i] + T(g, 3, 2) * s->y1[i] + T(g, 3, 3) * s->z1[i];
+    s->x2[i] = s->del2[i] * s->fx[i] - 0.5 * a1;
+    s->y2[i] = s->del2[i] * s->fy[i] - 0.5 * a2;
+    s->z2[i] = s->del2[i] * s->fz[i] - 0.5 * a3;
+  }
+  for (int k = 0; k < 9; k++) s->b2[k] = s->bdel2 * s->fb[k];
+  return 0;
+}
+
+/* velocityverlet, src/quench_m.F90:562-624 */
+static int velocity_verlet(orc_sys *s) {
+  const int n = s->n;
+  for (int i = 0; i < n; i++) {
+    s->x1dt2[i] = s->x1[i] + s->x2[i];
+    s->y1dt2[i] = s->y1[i] + s->y2[i];
+    s->z1dt2[i] = s->z1[i] + s->z2[i];
+  }
+  for (int i = 0; i < n; i++) {
+    s->x0[i] = s->x0[i] + s->x1dt2[i];
+    s->y0[i] = s->y0[i] + s->y1dt2[i];
+    s->z0[i] = s->z0[i] + s->z1dt2[i];
+  }
+  if (s->ivol == 4) {
+    T(s->b1, 3, 3) = T(s->b1, 3, 3) + T(s->b2, 3, 3);
+    T(s->b0, 3, 3) = T(s->b0, 3, 3) + T(s->b1, 3, 3);
+  }
+  if (s->ivol < 1) {
+    for (int k = 0; k < 9; k++) s->b1[k] = s->b1[k] + s->b2[k];
+    for (int k = 0; k < 9; k++) s->b0[k] = s->b0[k] + s->b1[k];
+  }
+  int st = orc_force(s);
+  if (st) return st;
+  double g[9];
+  memcpy(g, s->tgid, sizeof(g)); /* lagged tgid, quench_m.F90:595 */
+  if (s->ivol < 1) {
+    for (int k = 0; k < 9; k++) s->b2[k] = s->bdel2 * s->fb[k];
+    for (int k = 0; k < 9; k++) s->b1[k] = s->b1[k] + s->b2[k];
+  }
+  if (s->ivol == 4) {
+    T(s->b2, 3, 3) = s->bdel2 * T(s->fb, 3, 3);
+    for (int k = 0; k < 9; k++) s->b1[k] = s->b1[k] + s->b2[k];
+  }
+  double sxi = 0.0, syi = 0.0, szi = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (s->zat[i] == 0 || s->mass[i] < (double)0.1f) continue;
+    if (s->ivol == 0) {
+      sxi = T(g, 1, 1) * s->x1[i] + T(g, 1, 2) * s->y1[i] + T(g, 1, 3) * s->z1[i];
+      syi = T(g, 2, 1) * s->x1[i] + T(g, 2, 2) * s->y1[i] + T(g, 2, 3) * s->z1[i];
+      szi = T(g, 3, 1) * s->x1[i] + T(g, 3, 2) * s->y1[i] + T(g, 3, 3) * s->z1[i];
+    }
+    if (s->ivol == 4) szi = T(g, 3, 3) * s->z1[i];
+    s->x2[i] = s->del2[i] * s->fx[i] - 0.5 * sxi;
+    s->y2[i] = s->del2[i] * s->fy[i] - 0.5 * syi;
+    s->z2[i] = s->del2[i] * s->fz[i] - 0.5 * szi;
+  }
+  for (int i = 0; i < n; i++) {
+    s->x1[i] = s->x1dt2[i] + s->x2[i];
+    s->y1[i] = s->y1dt2[i] + s->y2[i];
+    s->z1[i] = s->z1dt2[i] + s->z2[i];
+  }
+  return 0;
+}
+
+/* update_therm_avs, src/dynamics_m.F90:649-734 */
+static void update_therm_avs(orc_sys *s) {
+  const double *b0 = s->b0, *b1 = s->b1;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int k = 0; k < 9; k++) { s->tg[k] = 0.0; s->tgdot[k] = 0.0; }
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) {
+        T(s->tg, i, j) = T(s->tg, i, j) + T(b0, k, i) * T(b0, k, j);
+        if (s->ivol != 1)
+          T(s->tgdot, i, j) = T(s->tgdot, i, j) + T(b1, k, i) * T(b0, k, j) + T(b0, k, i) * T(b1, k, j);
+      }
+  mat_invert(s->tg, s->tginv);          /* pr_get_tgid */
+  mat_mul(s->tginv, s->tgdot, s->tgid);
+  pr_volume(s);
+  const double v = s->vol;
+  double xx = (T(b0, 2, 2) * T(b0, 3, 3) - T(b0, 2, 3) * T(b0, 3, 2)) / v;
+  double xy = (T(b0, 2, 3) * T(b0, 3, 1) - T(b0, 3, 3) * T(b0, 2, 1)) / v;
+  double xz = (T(b0, 2, 1) * T(b0, 3, 2) - T(b0, 3, 1) * T(b0, 2, 2)) / v;
+  double yx = (T(b0, 3, 2) * T(b0, 1, 3) - T(b0, 1, 2) * T(b0, 3, 3)) / v;
+  double yy = (T(b0, 1, 1) * T(b0, 3, 3) - T(b0, 1, 3) * T(b0, 3, 1)) / v;
+  double yz = (T(b0, 3, 1) * T(b0, 1, 2) - T(b0, 1, 1) * T(b0, 3, 2)) / v;
+  double zx = (T(b0, 1, 2) * T(b0, 2, 3) - T(b0, 2, 2) * T(b0, 1, 3)) / v;
+  double zy = (T(b0, 1, 3) * T(b0, 2, 1) - T(b0, 1, 1) * T(b0, 2, 3)) / v;
+  double zz = (T(b0, 1, 1) * T(b0, 2, 2) - T(b0, 1, 2) * T(b0, 2, 1)) / v;
+  const double *g = s->tgid;
+  for (int i = 0; i < s->n; i++) {
+    if (s->zat[i] == 0) continue;
+    double sxi = T(g, 1, 1) * s->x1[i] + T(g, 1, 2) * s->y1[i] + T(g, 1, 3) * s->z1[i];
+    double syi = T(g, 2, 1) * s->x1[i] + T(g, 2, 2) * s->y1[i] + T(g, 2, 3) * s->z1[i];
+    double szi = T(g, 3, 1) * s->x1[i] + T(g, 3, 2) * s->y1[i] + T(g, 3, 3) * s->z1[i];
+    double fxp = 2.0 * s->x2[i] + sxi, fyp = 2.0 * s->y2[i] + syi, fzp = 2.0 * s->z2[i] + szi;
+    double fxi = xx * fxp + xy * fyp + xz * fzp;
+    double fyi = yx * fxp + yy * fyp + yz * fzp;
+    double fzi = zx * fxp + zy * fyp + zz * fzp;
+    double f2 = fxi * fxi + fyi * fyi + fzi * fzi;
+    s1 = s1 + f2;
+    s2 = s2 + f2 * f2;
+    double rx = T(b0, 1, 1) * s->x1[i] + T(b0, 1, 2) * s->y1[i] + T(b0, 1, 3) * s->z1[i];
+    double ry = T(b0, 2, 1) * s->x1[i] + T(b0, 2, 2) * s->y1[i] + T(b0, 2, 3) * s->z1[i];
+    double rz = T(b0, 3, 1) * s->x1[i] + T(b0, 3, 2) * s->y1[i] + T(b0, 3, 3) * s->z1[i];
+    s3 = s3 + s->mass[i] * (rx * rx + ry * ry + rz * rz);
+  }
+  s->s1 = s1;
+  s->s2 = s2;
+  s->s3 = s3;
+}
+
+/* one MD step, src/moldin.F90:752-791 (iverlet=1) */
+int orc_md_step(orc_sys *s) {
+  int st = velocity_verlet(s);
+  if (st) return st;
+  update_therm_avs(s);
+  s->ke = s->s3 / (s->deltat * s->deltat) / 2.0; /* deltat**2 */
+  double bk = 0.0;
+  for (int i = 1; i <= 3; i++)
+    for (int j = 1; j <= 3; j++) bk = bk + (T(s->b1, i, j) * T(s->b1, i, j)) / s->bdel2;
+  s->boxke = bk;
+  s->te = s->ke + s->pe;
+  s->h = s->te + s->press * s->vol;
+  s->th = s->h + s->boxke;
+  if (s->ntc == 0) orc_relink(s);
+  return 0;
+}
+
+/* ---------------- accessors ---------------- */
+static double *arr_by_name(const orc_sys *s, const char *nm) {
+  static const char *names[] = {"x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2",
+                                "rho", "afrho", "dafrho", "fx", "fy", "fz", "del2", "mass"};
+  double *ptrs[] = {s->x0, s->y0, s->z0, s->x1, s->y1, s->z1, s->x2, s->y2, s->z2,
+                    s->rho, s->afrho, s->dafrho, s->fx, s->fy, s->fz, s->del2, s->mass};
+  for (size_t k = 0; k < sizeof(names) / sizeof(names[0]); k++)
+    if (!strcmp(nm, names[k])) return ptrs[k];
+  return NULL;
+}
+int orc_get_array(const orc_sys *s, const char *name, double *out) {
+  double *a = arr_by_name(s, name);
+  if (!a) return -1;
+  memcpy(out, a, sizeof(double) * (size_t)s->n);
+  return 0;
+}
+int orc_set_array(orc_sys *s, const char *name, const double *in) {
+  double *a = arr_by_name(s, name);
+  if (!a) return -1;
+  memcpy(a, in, sizeof(double) * (size_t)s->n);
+  return 0;
+}
+static double *ten_by_name(const orc_sys *s, const char *nm) {
+  orc_sys *m = (orc_sys *)s;
+  if (!strcmp(nm, "b0")) return m->b0;
+  if (!strcmp(nm, "b1")) return m->b1;
+  if (!strcmp(nm, "b2")) return m->b2;
+  if (!strcmp(nm, "tk")) return m->tk;
+  if (!strcmp(nm, "tc")) return m->tc;
+  if (!strcmp(nm, "tg")) return m->tg;
+  if (!strcmp(nm, "tp")) return m->tp;
+  if (!strcmp(nm, "tginv")) return m->tginv;
+  if (!strcmp(nm, "tgdot")) return m->tgdot;
+  if (!strcmp(nm, "tgid")) return m->tgid;
+  if (!strcmp(nm, "fb")) return m->fb;
+  return NULL;
+}
+int orc_get_tensor(const orc_sys *s, const char *name, double *out) {
+  double *t = ten_by_name(s, name);
+  if (!t) return -1;
+  memcpy(out, t, 9 * sizeof(double));
+  return 0;
+}
+int orc_set_tensor(orc_sys *s, const char *name, const double *in) {
+  double *t = ten_by_name(s, name);
+  if (!t) return -1;
+  memcpy(t, in, 9 * sizeof(double));
+  if (!strcmp(name, "b0")) pr_volume(s);
+  return 0;
+}
+double orc_get_scalar(const orc_sys *s, int which) {
+  switch (which) {
+    case 0: return s->pe;
+    case 1: return s->ke;
+    case 2: return s->vol;
+    case 3: return s->s1;
+    case 4: return s->s2;
+    case 5: return s->s3;
+    case 6: return s->te;
+    case 7: return s->h;
+    case 8: return s->th;
+    case 9: return s->boxke;
+    case 10: return s->deltat;
+    case 11: return s->bdel2;
+    case 12: return s->press;
+    case 13: return s->rnear;
+    default: return NAN;
+  }
+}
+int orc_get_int(const orc_sys *s, int which) {
+  switch (which) {
+    case 0: return s->n;
+    case 1: return s->nlcx;
+    case 2: return s->nlcy;
+    case 3: return s->nlcz;
+    case 4: return s->ntc;
+    case 5: return s->ntcm;
+    case 6: return s->nnbrs;
+    case 7: {
+      long t = 0;
+      for (int i = 0; i < s->n; i++) t += s->numn[i];
+      return (int)t;
+    }
+    default: return -1;
+  }
+}
+void orc_set_ntc(orc_sys *s, int ntc) { s->ntc = ntc; }
+
+long orc_get_pairs(const orc_sys *s, int *pi, int *pj, long cap) {
+  long c = 0;
+  for (int i = 0; i < s->n; i++)
+    for (int k = 0; k < s->numn[i]; k++) {
+      if (c < cap) {
+        pi[c] = i;
+        pj[c] = s->nlist[(size_t)i * s->nnbrs + k] - 1;
+      }
+      c++;
+    }
+  return c;
+}
+int orc_get_numn(const orc_sys *s, int *numn) {
+  memcpy(numn, s->numn, sizeof(int) * (size_t)s->n);
+  return 0;
+}
+int orc_get_ic(const orc_sys *s, int *ic) {
+  for (int i = 0; i < s->n; i++) ic[i] = s->ic[i + 1];
+  return 0;
+}
+
+long orc_bruteforce_pairs(int n, const double *b0, const double *x, const double *y,
+                          const double *z, int ivol, double rcut, int *pi, int *pj,
+                          long cap) {
+  long c = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+      double r = realsep(b0, ivol, &dx, &dy, &dz);
+      if (r < rcut) {
+        if (c < cap) { pi[c] = i; pj[c] = j; }
+        c++;
+      }
+    }
+  return c;
+}

@@ -1,0 +1,239 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product path (moldy_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+# Fe-Fe coefficients of the published potential (pot_026_026.in format):
+from moldy_amd.potential import FE_POTENTIAL  # noqa: E402
+
+_lib = None
+
+
+class OrcPot(C.Structure):
+    _fields_ = [
+        ("nv", C.c_int), ("np", C.c_int),
+        ("av", C.c_double * 8), ("rv", C.c_double * 8),
+        ("ap", C.c_double * 4), ("rp", C.c_double * 4),
+        ("bier_cut1", C.c_double), ("bier_cut2", C.c_double),
+        ("sc", C.c_double * 4),
+        ("phi_cut_r", C.c_double), ("phi_cut_v", C.c_double),
+        ("use_lookup", C.c_int), ("nlookup", C.c_int),
+        ("tab", C.c_void_p * 6),
+        ("r_min", C.c_double), ("r_max", C.c_double), ("vp_range", C.c_double),
+        ("vp_reciprange", C.c_double), ("vp_recipnlkp", C.c_double), ("vp_recipstep", C.c_double),
+        ("rho_min", C.c_double), ("rho_max", C.c_double), ("e_range", C.c_double),
+        ("e_reciprange", C.c_double), ("e_recipnlkp", C.c_double), ("e_recipstep", C.c_double),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_pot_init.argtypes = [C.POINTER(OrcPot), C.c_int, dp, dp, C.c_int, dp, dp]
+        L.orc_pot_build_lookup.argtypes = [C.POINTER(OrcPot), C.c_int]
+        L.orc_pot_free.argtypes = [C.POINTER(OrcPot)]
+        for nm in ("orc_vee", "orc_dvee", "orc_phi", "orc_dphi", "orc_emb", "orc_demb",
+                   "orc_vee_src", "orc_dvee_src", "orc_phi_src", "orc_dphi_src"):
+            f = getattr(L, nm)
+            f.argtypes = [C.POINTER(OrcPot), C.c_double]
+            f.restype = C.c_double
+        L.orc_create.argtypes = [C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, ip, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.POINTER(OrcPot)]
+        L.orc_create.restype = C.c_void_p
+        L.orc_destroy.argtypes = [C.c_void_p]
+        for nm in ("orc_linkup", "orc_relink", "orc_update_neighbourlist", "orc_force",
+                   "orc_md_init", "orc_md_step"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = C.c_int
+        for nm in ("orc_rhoset", "orc_embed", "orc_force_pairs", "orc_kinten"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = None
+        L.orc_get_array.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_set_array.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_get_tensor.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_set_tensor.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.orc_get_scalar.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_scalar.restype = C.c_double
+        L.orc_get_int.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_ntc.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_pairs.argtypes = [C.c_void_p, ip, ip, C.c_long]
+        L.orc_get_pairs.restype = C.c_long
+        L.orc_get_numn.argtypes = [C.c_void_p, ip]
+        L.orc_get_ic.argtypes = [C.c_void_p, ip]
+        L.orc_bruteforce_pairs.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_double,
+                                           ip, ip, C.c_long]
+        L.orc_bruteforce_pairs.restype = C.c_long
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Potential:
+    """Fe-Fe potential of the oracle (set_up_m constants computed at init)."""
+
+    def __init__(self, coeffs=None, lookup: bool = False, nlookup: int = 5000):
+        c = coeffs if coeffs is not None else FE_POTENTIAL
+        self._keep = [np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp")]
+        av, rv, ap, rp = self._keep
+        self.s = OrcPot()
+        lib().orc_pot_init(C.byref(self.s), len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp))
+        if lookup:
+            lib().orc_pot_build_lookup(C.byref(self.s), nlookup)
+
+    def __del__(self):
+        try:
+            if self.s.use_lookup:
+                lib().orc_pot_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def f(self, name: str, x):
+        fn = getattr(lib(), "orc_" + name)
+        x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+        return np.array([fn(C.byref(self.s), float(v)) for v in x])
+
+    @property
+    def spline_coeff(self):
+        return np.array(self.s.sc[:])
+
+    @property
+    def phi_cut(self):
+        return (self.s.phi_cut_r, self.s.phi_cut_v)
+
+
+SCALARS = {"pe": 0, "ke": 1, "vol": 2, "s1": 3, "s2": 4, "s3": 5, "te": 6, "h": 7,
+           "th": 8, "boxke": 9, "deltat": 10, "bdel2": 11, "press": 12, "rnear": 13}
+INTS = {"n": 0, "nlcx": 1, "nlcy": 2, "nlcz": 3, "ntc": 4, "ntcm": 5, "nnbrs": 6, "npairs": 7}
+
+
+class OracleSystem:
+    """Reference-semantics MD state on the CPU (serial unless threads>1)."""
+
+    def __init__(self, sysdef, pot: Potential, ivol: int = 1, rpad: float = 0.3,
+                 rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10,
+                 press_gpa: float = 0.0, boxmass: float = 50.0):
+        self.pot = pot
+        n = sysdef.n
+        b0 = np.asfortranarray(sysdef.b0).ravel(order="F").copy()
+        s = np.ascontiguousarray(sysdef.s.T)
+        x1 = np.ascontiguousarray(sysdef.x1.T)
+        self._z = np.ascontiguousarray(sysdef.z, dtype=np.int32)
+        mass = np.ascontiguousarray(sysdef.mass, dtype=np.float64)
+        self.n = n
+        self.h = lib().orc_create(n, _dp(b0), _dp(s[0]), _dp(s[1]), _dp(s[2]), _dp(x1[0]),
+                                  _dp(x1[1]), _dp(x1[2]), _dp(mass), _ip(self._z), ivol,
+                                  sysdef.deltat_fs, rpad, rcut, nnbrs, ntcm, press_gpa,
+                                  boxmass, C.byref(pot.s))
+        if not self.h:
+            raise MemoryError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def call(self, name: str) -> int:
+        r = getattr(lib(), "orc_" + name)(self.h)
+        return 0 if r is None else int(r)
+
+    def array(self, name: str) -> np.ndarray:
+        out = np.empty(self.n)
+        if lib().orc_get_array(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def set_array(self, name: str, v) -> None:
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if lib().orc_set_array(self.h, name.encode(), _dp(v)) != 0:
+            raise KeyError(name)
+
+    def tensor(self, name: str) -> np.ndarray:
+        out = np.empty(9)
+        if lib().orc_get_tensor(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out.reshape(3, 3, order="F")
+
+    def set_tensor(self, name: str, t) -> None:
+        t = np.asfortranarray(t, dtype=np.float64).ravel(order="F").copy()
+        lib().orc_set_tensor(self.h, name.encode(), _dp(t))
+
+    def scalar(self, name: str) -> float:
+        return lib().orc_get_scalar(self.h, SCALARS[name])
+
+    def int(self, name: str) -> int:
+        return lib().orc_get_int(self.h, INTS[name])
+
+    def pairs(self) -> np.ndarray:
+        cnt = lib().orc_get_pairs(self.h, None, None, 0)
+        pi = np.empty(max(cnt, 1), dtype=np.int32)
+        pj = np.empty(max(cnt, 1), dtype=np.int32)
+        lib().orc_get_pairs(self.h, _ip(pi), _ip(pj), cnt)
+        return np.stack([pi[:cnt], pj[:cnt]], axis=1)
+
+    def numn(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        lib().orc_get_numn(self.h, _ip(out))
+        return out
+
+    def ic(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        lib().orc_get_ic(self.h, _ip(out))
+        return out
+
+    def forces_cart(self) -> np.ndarray:
+        f = np.stack([self.array("fx"), self.array("fy"), self.array("fz")], axis=1)
+        return f @ self.tensor("b0").T
+
+
+def bruteforce_pairs(b0, s, rcut: float, ivol: int = 1) -> np.ndarray:
+    b = np.asfortranarray(b0).ravel(order="F").copy()
+    st = np.ascontiguousarray(np.asarray(s).T)
+    n = st.shape[1]
+    cnt = lib().orc_bruteforce_pairs(n, _dp(b), _dp(st[0]), _dp(st[1]), _dp(st[2]), ivol,
+                                     rcut, None, None, 0)
+    pi = np.empty(max(cnt, 1), dtype=np.int32)
+    pj = np.empty(max(cnt, 1), dtype=np.int32)
+    lib().orc_bruteforce_pairs(n, _dp(b), _dp(st[0]), _dp(st[1]), _dp(st[2]), ivol, rcut,
+                               _ip(pi), _ip(pj), cnt)
+    return np.stack([pi[:cnt], pj[:cnt]], axis=1)
+
+
+def canonical_pairs(p: np.ndarray) -> np.ndarray:
+    """Unordered pair set as sorted (min,max) rows."""
+    if len(p) == 0:
+        return np.zeros((0, 2), dtype=np.int64)
+    q = np.sort(np.asarray(p, dtype=np.int64), axis=1)
+    key = q[:, 0] * (1 << 32) + q[:, 1]
+    order = np.argsort(key, kind="stable")
+    return q[order]
+
+
+def pairset_digest(p: np.ndarray) -> str:
+    import hashlib
+    q = canonical_pairs(p).astype("<i8")
+    return hashlib.sha256(q.tobytes()).hexdigest()
