@@ -1,0 +1,162 @@
+/*
+ * moldy_hip.h -- C ABI of libmoldy_hip.so, the MI355X (gfx950) implementation
+ * of MOLDY's force hot path (link cells, neighbour list, Finnis-Sinclair
+ * density/embedding/force+virial, velocity Verlet + Parrinello-Rahman).
+ *
+ * Every entry point returns an int status (MDK_OK == 0).  No C++ types, no
+ * torch types and no exceptions cross this boundary: plain pointers, sizes
+ * and an opaque handle.  Host arrays are always in the reference's atom order
+ * (index i of X0(i) in src/particles_m.F90:58) and are copied; the handle
+ * owns device-resident mirrors (atoms are kept sorted by cell on the device).
+ * One handle is used by one host thread and owns one HIP stream.
+ *
+ * Tensors (3x3) are Fortran column-major like the reference's B0(3,3):
+ * t[(j-1)*3 + (i-1)] == T(i,j); column j of B0 is cell vector j.
+ *
+ * Reference interfaces replaced (gjackland/moldy):
+ *   src/linkcell_m.F90:52-54        linkup, relink, get_l0, get_link
+ *   src/neighbourlist_m.F90:52-58   update_neighbourlist, NLIST, NUMN
+ *   src/potential_m.F90:100-118     set_up, potential_set_lookup/exact, embed
+ *   src/dynamics_m.F90:69-83        force, rhoset, update_therm_avs, fx/fy/fz
+ *   src/thermostat_m.F90:127-177    kinten (inside force)
+ *   src/parrinellorahman_m.F90:91-115  B0..B2, TK, TC, TG, TP, TGINV, TGDOT, TGID, FB, vol
+ *   src/quench_m.F90:59-63          velocityverlet
+ *   src/moldin.F90:446-477,752-791  initial accelerations; MD-step glue
+ */
+#ifndef MOLDY_HIP_H
+#define MOLDY_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MDK_OK 0
+#define MDK_ERR_HIP 1           /* a HIP runtime call failed (message via mdk_last_error) */
+#define MDK_ERR_NBR_OVERFLOW 2  /* 'NEIGHBOURLIST: EXCESS NEIGHBOURS!' (src/neighbourlist_m.F90:222-230) */
+#define MDK_ERR_BAD_ARG 3       /* invalid argument / size */
+#define MDK_ERR_SINGULAR_BOX 4  /* matrix_invert on a zero determinant (src/matrix_m.F90:110-113) */
+#define MDK_ERR_STATE 5         /* call out of order (e.g. force before set_potential) */
+#define MDK_ERR_SKIN 6          /* an atom moved farther than half the list skin between rebuilds */
+#define MDK_ERR_NO_DEVICE 7     /* no GPU visible */
+
+/* ---- precision forms ---- */
+#define MDK_PREC_FP64 0         /* plain FP64 (parity form) */
+#define MDK_PREC_FP32 1         /* FP32 pair/spline arithmetic, FP64 accumulation/integration */
+
+/* ---- per-atom fields for mdk_upload / mdk_download (n doubles each) ---- */
+enum {
+  MDK_X0 = 0, MDK_Y0, MDK_Z0,   /* fractional positions */
+  MDK_X1, MDK_Y1, MDK_Z1,       /* fractional velocity * deltat */
+  MDK_X2, MDK_Y2, MDK_Z2,       /* fractional acceleration * deltat^2/2 */
+  MDK_FX, MDK_FY, MDK_FZ,       /* forces f (Cartesian force = B0 . f) */
+  MDK_RHO, MDK_AFRHO, MDK_DAFRHO,
+  MDK_MASS, MDK_DEL2,
+  MDK_NFIELDS
+};
+/* ---- 3x3 tensors for mdk_get_tensor / mdk_set_tensor ---- */
+enum {
+  MDK_B0 = 0, MDK_B1, MDK_B2, MDK_TK, MDK_TC, MDK_TG, MDK_TP, MDK_TGINV, MDK_TGDOT,
+  MDK_TGID, MDK_FB, MDK_NTENSORS
+};
+/* ---- scalars for mdk_get_scalars (array of MDK_NSCALARS doubles) ---- */
+enum {
+  MDK_S_PE = 0, MDK_S_KE, MDK_S_VOL, MDK_S_S1, MDK_S_S2, MDK_S_S3, MDK_S_TE, MDK_S_H,
+  MDK_S_TH, MDK_S_BOXKE, MDK_S_MAXDISP, MDK_NSCALARS
+};
+
+/* Simulation parameters: the subset of type(simparameters)
+ * (src/params_m.F90:44-162) that the hot path reads, already in model units
+ * (deltat in 10.1805 fs units, press in eV/A^3, bdel2 = deltat^2/(2*bmass*nm)). */
+typedef struct mdk_params {
+  int nm;          /* number of atoms */
+  int ivol;        /* 0 NPT (PR), 1 NVE, 2 free surface z, 4 z-only box, 5 pillar */
+  int nnbrs;       /* capacity of the reference half list per atom (NLIST) */
+  int ntcm;        /* neighbour-list update interval (nbrupdate) */
+  int nlcx, nlcy, nlcz; /* reference link-cell grid (moldin.F90:252-257) */
+  int precision;   /* MDK_PREC_FP64 or MDK_PREC_FP32 */
+  int rebuild;     /* 0: reference interval ntcm only; 1: also on displacement > skin/2 */
+  int max_full;    /* capacity of the device full list per atom (0 = default 64) */
+  double deltat;   /* model time units */
+  double rnear;    /* rcut + rpad: reference list radius */
+  double rskin;    /* skin of the device full list (default rpad) */
+  double press;    /* external pressure, model units */
+  double bdel2;    /* box time scale */
+} mdk_params;
+
+/* Handle lifecycle.  Replaces init_linkcell_m/init_neighbourlist_m/init_dynamics_m. */
+int mdk_create(void **handle, const mdk_params *p);
+int mdk_destroy(void *handle);
+const char *mdk_last_error(void);
+
+/* Potential (src/fetransmetal.F90:603-954 set_up; src/potential_m.F90:181-232).
+ * Knot coefficients of the Fe-Fe pair and density functions; set_up constants
+ * (Biersack join, phi flat cap) are derived on the host exactly as set_up_m. */
+int mdk_set_potential(void *h, int nv, const double *av, const double *rv, int np,
+                      const double *ap, const double *rp);
+/* potential_set_lookup: nlookup-point linear tables on [0,6] A and rho in [0,1e4];
+ * nlookup <= 0 selects exact evaluation (potential_set_exact). */
+int mdk_set_lookup(void *h, int nlookup);
+/* out[0..7] = bier_cut1, bier_cut2, spline_coeff(1:4), phi_cut(1:2) */
+int mdk_potential_constants(void *h, double *out);
+/* Evaluate potential_m's vee,dvee,phi,dphi (which=0..3) at r[k], or emb,demb
+ * (which=4,5) at rho[k], on the device, k < count. */
+int mdk_eval_potential(void *h, int which, const double *x, double *y, int count);
+
+/* State transfer (ownership stays with the caller; arrays in reference order). */
+int mdk_upload(void *h, int field, const double *data);
+int mdk_download(void *h, int field, double *data);
+int mdk_upload_species(void *h, const int *atomic_number);
+int mdk_set_tensor(void *h, int which, const double *t9);
+int mdk_get_tensor(void *h, int which, double *t9);
+int mdk_get_scalars(void *h, double *out);
+int mdk_set_ntc(void *h, int ntc);
+int mdk_get_ntc(void *h, int *ntc);
+
+/* Link cells: linkup (src/linkcell_m.F90:122-198), relink (:215-274).
+ * get_l0/get_link return chains that visit the same cell membership;
+ * l0 has nlcx*nlcy*nlcz entries, link and ic have nm entries (1-based). */
+int mdk_linkup(void *h);
+int mdk_relink(void *h);
+int mdk_get_linkcells(void *h, int *l0, int *link, int *ic);
+
+/* update_neighbourlist (src/neighbourlist_m.F90:103-276): the reference half
+ * list at rnear over the 13 forward link cells, on the device.  Returns
+ * MDK_ERR_NBR_OVERFLOW when some atom has more than nnbrs entries. */
+int mdk_update_neighbourlist(void *h);
+/* NUMN(nm) and NLIST(nnbrs,nm) (column-major, 1-based indices like the reference). */
+int mdk_get_neighbourlist(void *h, int *numn, int *nlist);
+/* Unordered half-list pair count and pairs (0-based (i,j)), cap entries max. */
+int mdk_get_pairs(void *h, long *count, int *pi, int *pj, long cap);
+
+/* Force pieces (src/dynamics_m.F90). mdk_force is the whole reference force
+ * routine including its ntc/ntcm list bookkeeping; rhoset computes RHO only,
+ * embed computes AFRHO/DAFRHO and the embedding energy. */
+int mdk_force(void *h);
+int mdk_rhoset(void *h);
+int mdk_embed(void *h, double *pe_embed);
+int mdk_kinten(void *h);
+/* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
+int mdk_md_init(void *h);
+/* velocityverlet(dt) (src/quench_m.F90:562-624), including force */
+int mdk_velocityverlet(void *h);
+/* update_therm_avs(s1,s2,s3) (src/dynamics_m.F90:649-734) */
+int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3);
+/* nsteps MD steps of the moldin loop (velocityverlet, update_therm_avs,
+ * KE/TE/H/TH, relink when ntc==0) without host round trips; if trace is not
+ * NULL it receives per step {pe, ke, te, h, th, vol} (6*nsteps doubles). */
+int mdk_md_steps(void *h, int nsteps, double *trace);
+/* Number of full-list rebuilds performed so far and the current full-list
+ * pair count (sum over atoms, each pair counted twice). */
+int mdk_stats(void *h, long *rebuilds, long *full_pairs);
+
+/* Timing: per-kernel-class HIP-event accumulators on the handle's stream.
+ * enable=1 starts recording; mdk_timing_read returns total milliseconds and
+ * launch counts for classes {0 drift, 1 density, 2 force, 3 kick, 4 rebuild}. */
+int mdk_timing(void *h, int enable);
+int mdk_timing_read(void *h, double *ms, long *launches, int nclasses);
+int mdk_synchronize(void *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1801 @@
+// mdk_core.hip -- libmoldy_hip: MOLDY's Fe force hot path on MI355X (gfx950).
+//
+// Device design (one system per handle, one HIP stream):
+//  * per-atom state is FP64 SoA, kept SORTED BY CELL of a fine grid (cell
+//    width >= interaction range + skin) so that neighbour gathers are local;
+//    the sort is redone at every full-list rebuild.  slot_of/orig map between
+//    the reference's atom order and the device order.
+//  * the force path uses a FULL neighbour list (both directions) pruned at
+//    r_int + skin, r_int = 3.72645 A being the outermost knot beyond which
+//    V, V', phi and phi' are identically zero; per-atom sums are gathers (no
+//    atomics, deterministic).  It is rebuilt on the reference's ntcm cadence
+//    and whenever an atom may have moved more than skin/2.
+//  * the reference half list at rnear (NLIST/NUMN) is built on request by a
+//    separate kernel that reproduces the reference's link-cell grid, stencil
+//    and bit-exact distance test.
+//  * reductions (energies, virial, kinetic tensor, thermodynamic sums) are
+//    per-block partials reduced in a fixed order by one finishing block.
+//  * box/thermo tensors live on the device (BoxState, double-buffered per step)
+//    so an MD step needs no host round trip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/moldy_hip.h"
+#include "mdk_internal.h"
+
+// ---------------------------------------------------------------------------
+// error handling
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+extern "C" const char *mdk_last_error(void) { return g_err; }
+
+#define HIPCK(call)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_err(MDK_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                              \
+  } while (0)
+#define RCK(expr)          \
+  do {                     \
+    int r_ = (expr);       \
+    if (r_ != MDK_OK) return r_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// host-side potential set-up (src/fetransmetal.F90:603-954), evaluated with
+// the host libm exactly in the reference's operation order so that the
+// derived constants and lookup tables are the reference's bit for bit.
+// ---------------------------------------------------------------------------
+namespace hostpot {
+struct Knots {
+  int nv, np;
+  double av[MDK_MAXV], rv[MDK_MAXV], ap[MDK_MAXP], rp[MDK_MAXP];
+  double bc1, bc2, sc[4], pcr, pcv;
+};
+static double h_step(double x) { return x < 0.0 ? 0.0 : 1.0; }
+static double knot_v(const Knots &k, double r) {
+  double s = 0.0;
+  for (int i = 0; i < k.nv; i++) {
+    double d = k.rv[i] - r;
+    s = s + k.av[i] * (d * d * d) * h_step(d);
+  }
+  return s;
+}
+static double knot_dv(const Knots &k, double r) {
+  double s = 0.0;
+  for (int i = 0; i < k.nv; i++) {
+    double d = k.rv[i] - r;
+    s = s - 3.0 * k.av[i] * (d * d) * h_step(d);
+  }
+  return s;
+}
+static double knot_p(const Knots &k, double r) {
+  double s = 0.0;
+  for (int i = 0; i < k.np; i++) {
+    double d = k.rp[i] - r;
+    s = s + k.ap[i] * (d * d * d) * h_step(d);
+  }
+  return s;
+}
+static double knot_dp(const Knots &k, double r) {
+  double s = 0.0;
+  for (int i = 0; i < k.np; i++) {
+    double d = k.rp[i] - r;
+    s = s - 3.0 * k.ap[i] * (d * d) * h_step(d);
+  }
+  return s;
+}
+// screening length: every literal of src/fetransmetal.F90:423 is REAL(4)
+static double screening_length() {
+  volatile float c = 0.8854f * 0.529f;
+  float t = powf(26.0f, 0.666666667f);
+  volatile float s = t + t;
+  float den = powf(s, 0.5f);
+  return (double)(c / den);
+}
+static double zed() { return (double)(26 * 26) * 8.98755178e9 * 1.602176487e-19 / 1.e-10; }
+static double bier(double r) {
+  double a = screening_length(), x = r / a;
+  double ph = (double)0.1818f * exp(-((double)3.2f * x)) + (double)0.5099f * exp(-((double)0.9423f * x)) +
+              (double)0.2802f * exp(-((double)0.4029f * x)) + (double)0.02817f * exp(-((double)0.2016f * x));
+  return zed() * ph / r;
+}
+static double dbier(double r) {
+  double a = screening_length(), x = r / a;
+  double s = (1.0 / r + 3.2 / a) * 0.1818 * exp(-(3.2 * x)) + (1.0 / r + 0.9423 / a) * 0.5099 * exp(-(0.9423 * x)) +
+             (1.0 / r + 0.4029 / a) * 0.2802 * exp(-(0.4029 * x)) + (1.0 / r + 0.2016 / a) * 0.02817 * exp(-(0.2016 * x));
+  return -(zed() / r) * s;
+}
+static double join(const Knots &k, double r, int deriv) {
+  double a1 = k.bc1, a2 = k.bc2, x1 = k.sc[0], v1 = k.sc[1], x2 = k.sc[2], v2 = k.sc[3];
+  double vv1 = v1 / x1, xx1 = log(x1), vv2 = v2 / x2, xx2 = log(x2);
+  double t = (r - a1) / (a2 - a1);
+  double sp = (2.0 * t * t * t - 3.0 * t * t + 1.0) * xx1 + (t * t * t - 2.0 * t * t + t) * vv1 * (a2 - a1) +
+              (-(2.0 * t * t * t) + 3.0 * t * t) * xx2 + (t * t * t - t * t) * vv2 * (a2 - a1);
+  if (!deriv) return exp(sp);
+  double ds = (6.0 * t * t - 6.0 * t) * xx1 + (3.0 * t * t - 4.0 * t + 1.0) * vv1 * (a2 - a1) +
+              (-(6.0 * t * t) + 6.0 * t) * xx2 + (3.0 * t * t - 2.0 * t) * vv2 * (a2 - a1);
+  ds = ds / (a2 - a1);
+  return ds * exp(sp);
+}
+static double vee(const Knots &k, double r) {
+  if (r > k.bc2) return knot_v(k, r);
+  if (r > k.bc1) return join(k, r, 0);
+  if (r < k.bc1) return bier(r);
+  return 0.0;
+}
+static double dvee(const Knots &k, double r) {
+  if (r > k.bc2) return knot_dv(k, r);
+  if (r > k.bc1) return join(k, r, 1);
+  if (r < k.bc1) return dbier(r);
+  return 0.0;
+}
+static double phi(const Knots &k, double r) { return r < k.pcr ? k.pcv : knot_p(k, r); }
+static double dphi(const Knots &k, double r) { return r < k.pcr ? 0.0 : knot_dp(k, r); }
+static double emb(double rho) { return -1.0 * sqrt(rho > 0.0 ? rho : 0.0); }
+static double demb(double rho) { return rho > 0.0 ? -0.5 / sqrt(rho) : 0.0; }
+// set_up_m: join constants and the maximum of phi by 500-step bisection
+static void set_up(Knots &k) {
+  k.bc1 = (double)0.9f;
+  k.bc2 = (double)1.9f;
+  const double eps = (double)0.000000000001f;
+  k.sc[0] = bier(k.bc1 - eps);
+  k.sc[1] = dbier(k.bc1 - eps);
+  k.sc[2] = vee(k, k.bc2 + eps);
+  k.sc[3] = dvee(k, k.bc2 + eps);
+  k.pcr = 0.0;
+  k.pcv = 0.0;
+  double lo = 0.0, hi = 3.0;
+  for (int i = 0; i < 500; i++) {
+    double mid = (hi + lo) / 2.0;
+    if (dphi(k, mid) > 0.0) lo = mid; else hi = mid;
+  }
+  double rs = (hi + lo) / 2.0;
+  k.pcr = rs;
+  k.pcv = phi(k, rs);
+}
+}  // namespace hostpot
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+enum {
+  F_X0 = 0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2, F_FX, F_FY, F_FZ,
+  F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2, F_N
+};
+
+struct TimedEv {
+  int cls;
+  hipEvent_t a, b;
+};
+
+struct mdk_ctx {
+  mdk_params p;
+  int n;
+  int pbc;                 // periodic dims mask from ivol (bit0 x, bit1 y, bit2 z)
+  int device;
+  hipStream_t stream;
+  // per-atom state (sorted order): F_N doubles per atom, field-major
+  double *atoms, *atoms_alt;
+  int *zat, *zat_alt, *orig, *orig_alt, *slot_of;
+  float *sref;             // fractional positions at the last full-list build
+  // reference link cells (1-based cell per atom, sorted order)
+  int *ic;
+  // parity half list in reference layout: numn_ref[nm] and nlist_ref[nm*nnbrs] (orig order)
+  int *numn_ref, *nlist_ref;
+  // full list
+  int maxn;
+  int *nbr, *numn;
+  int *cell_of, *cell_count, *cell_start, *cell_fill, *members;
+  int ncell_cap;
+  int grid[3];
+  int *dflags;             // [0] full-list overflow, [1] half-list overflow, [2] grid too coarse
+  long rebuilds;
+  bool list_valid;
+  int last_rebuild_step;
+  // box state: 2 copies
+  BoxState *box;
+  int cur;
+  // partials
+  double *part_d, *part_f, *part_k;
+  int nblk;
+  // host mapped monitor ring: [slot][0]=maxdisp2,[1]=maxstep2
+  double *mon_host, *mon_dev;
+  hipEvent_t mon_ev[4];
+  long step_counter;
+  // trace buffer on device
+  double *trace_dev;
+  int trace_cap;
+  // potential
+  PotDev pot;
+  bool pot_set;
+  double *tabs;
+  hostpot::Knots knots;
+  // reference list threshold: r < rnear  <=>  r^2 < rnear2_exact (see near_threshold)
+  double rnear2_exact;
+  double r_list, skin;
+  int ntc;
+  // timing
+  bool timing;
+  std::vector<TimedEv> evs;
+  double t_ms[TC_NCLASS];
+  long t_cnt[TC_NCLASS];
+  bool linked;
+};
+
+static inline double *fld(double *base, int f, int n) { return base + (size_t)f * n; }
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block-reduce K sums (and the last KM entries as maxima) into part[blockIdx.x*K+k].
+template <int K, int KM>
+__device__ __forceinline__ void block_store(double (&v)[K], double *part) {
+  __shared__ double sh[MDK_BLOCK / 64][K];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sh[w][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    double s = sh[0][k];
+    for (int q = 1; q < MDK_BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
+    part[(size_t)blockIdx.x * K + k] = s;
+  }
+}
+
+// Fixed-order reduction of nb block partials inside one block.
+template <int K, int KM>
+__device__ __forceinline__ void block_final(const double *part, int nb, double (&out)[K]) {
+  __shared__ double sh[MDK_BLOCK / 64][K];
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double x = part[(size_t)b * K + k];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sh[w][k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double s = sh[0][k];
+    for (int q = 1; q < (int)(blockDim.x / 64); q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
+    out[k] = s;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void min_image(int pbc, double &dx, double &dy, double &dz) {
+  if (pbc & 1) dx = dx - round(dx);
+  if (pbc & 2) dy = dy - round(dy);
+  if (pbc & 4) dz = dz - round(dz);
+}
+
+// Reference-order Cartesian separation squared (src/parrinellorahman_m.F90:167-172),
+// no FMA: bit-identical to the reference's r*r before the square root.
+__device__ __forceinline__ double sep2_exact(const double *b, double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  const double rx = b[0] * dx + b[3] * dy + b[6] * dz;
+  const double ry = b[1] * dx + b[4] * dy + b[7] * dz;
+  const double rz = b[2] * dx + b[5] * dy + b[8] * dz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// 3x3 helpers on column-major arrays (device, one thread)
+__device__ void dev_invert(const double *a, double *b, int *status) {
+#pragma clang fp contract(off)
+  double det = TT(a, 1, 1) * (TT(a, 2, 2) * TT(a, 3, 3) - TT(a, 2, 3) * TT(a, 3, 2));
+  det = det + TT(a, 2, 1) * (TT(a, 3, 2) * TT(a, 1, 3) - TT(a, 1, 2) * TT(a, 3, 3));
+  det = det + TT(a, 3, 1) * (TT(a, 1, 2) * TT(a, 2, 3) - TT(a, 1, 3) * TT(a, 2, 2));
+  TT(b, 1, 1) = TT(a, 2, 2) * TT(a, 3, 3) - TT(a, 2, 3) * TT(a, 3, 2);
+  TT(b, 1, 2) = TT(a, 3, 1) * TT(a, 2, 3) - TT(a, 2, 1) * TT(a, 3, 3);
+  TT(b, 1, 3) = TT(a, 2, 1) * TT(a, 3, 2) - TT(a, 3, 1) * TT(a, 2, 2);
+  TT(b, 2, 1) = TT(a, 3, 2) * TT(a, 1, 3) - TT(a, 1, 2) * TT(a, 3, 3);
+  TT(b, 2, 2) = TT(a, 1, 1) * TT(a, 3, 3) - TT(a, 1, 3) * TT(a, 3, 1);
+  TT(b, 2, 3) = TT(a, 3, 1) * TT(a, 1, 2) - TT(a, 1, 1) * TT(a, 3, 2);
+  TT(b, 3, 1) = TT(a, 1, 2) * TT(a, 2, 3) - TT(a, 1, 3) * TT(a, 2, 2);
+  TT(b, 3, 2) = TT(a, 2, 1) * TT(a, 1, 3) - TT(a, 2, 3) * TT(a, 1, 1);
+  TT(b, 3, 3) = TT(a, 1, 1) * TT(a, 2, 2) - TT(a, 1, 2) * TT(a, 2, 1);
+  if (fabs(det) == 0.0) {
+    *status = MDK_ERR_SINGULAR_BOX;
+    return;
+  }
+  for (int k = 0; k < 9; k++) b[k] = b[k] / det;
+}
+__device__ void dev_matmul(const double *a, const double *b, double *c) {
+#pragma clang fp contract(off)
+  for (int i = 1; i <= 3; i++) {
+    for (int j = 1; j <= 3; j++) TT(c, i, j) = 0.0;
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) TT(c, i, j) = TT(c, i, j) + TT(a, i, k) * TT(b, k, j);
+  }
+}
+__device__ double dev_volume(const double *b) {
+#pragma clang fp contract(off)
+  return TT(b, 1, 1) * (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) +
+         TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
+         TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
+}
+
+// ---------------------------------------------------------------------------
+// permutation / transfer kernels
+// ---------------------------------------------------------------------------
+__global__ void k_scatter_d(double *dst, const double *src, const int *slot_of, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[slot_of[i]] = src[i];
+}
+__global__ void k_gather_d(double *dst, const double *src, const int *slot_of, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[slot_of[i]];
+}
+__global__ void k_scatter_i(int *dst, const int *src, const int *slot_of, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[slot_of[i]] = src[i];
+}
+__global__ void k_gather_i(int *dst, const int *src, const int *slot_of, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[slot_of[i]];
+}
+__global__ void k_iota(int *a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+// new slot k takes old slot members[k]
+__global__ void k_permute(double *dst, const double *src, const int *members, int n, int nf) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * nf) return;
+  int f = (int)(t / n), k = (int)(t % n);
+  dst[(size_t)f * n + k] = src[(size_t)f * n + members[k]];
+}
+__global__ void k_permute_int(int *zdst, const int *zsrc, int *odst, const int *osrc, int *slot_of,
+                              const int *members, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int o = members[k];
+  zdst[k] = zsrc[o];
+  int orig = osrc[o];
+  odst[k] = orig;
+  slot_of[orig] = k;
+}
+
+// ---------------------------------------------------------------------------
+// cells: counting sort helpers
+// ---------------------------------------------------------------------------
+__global__ void k_count(const int *key, int n, int *count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&count[key[i]], 1);
+}
+// exclusive scan of count[0..m) into start[0..m], start[m] = total; one block
+__global__ void k_scan1(const int *count, int *start, int m) {
+  __shared__ int sh[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int chunk = (m + nt - 1) / nt;
+  const int b = t * chunk, e = min(m, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; i++) s += count[i];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    int v = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = sh[t] - s;
+  for (int i = b; i < e; i++) {
+    start[i] = run;
+    run += count[i];
+  }
+  if (t == nt - 1) start[m] = sh[t];
+}
+__global__ void k_fill(const int *key, int n, const int *start, int *fill, int *members) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = key[i];
+  int pos = start[c] + atomicAdd(&fill[c], 1);
+  members[pos] = i;
+}
+// deterministic order inside each cell: ascending sort key (key[members] or members)
+__global__ void k_sort_cells(const int *start, int m, int *members, const int *key) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; i++) {
+    int v = members[i];
+    int kv = key ? key[v] : v;
+    int j = i - 1;
+    while (j >= b) {
+      int u = members[j];
+      int ku = key ? key[u] : u;
+      if (ku <= kv) break;
+      members[j + 1] = u;
+      j--;
+    }
+    members[j + 1] = v;
+  }
+}
+
+// reference link cell of each atom (src/linkcell_m.F90:149-153), 1-based
+__global__ void k_ref_cells(const double *x0, const double *y0, const double *z0, int n, int nlcx,
+                            int nlcy, int nlcz, int *ic) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hm = 0.4999999999;
+  double x = x0[i], y = y0[i], z = z0[i];
+  int ix = (int)((x - round(x) + hm) * (double)nlcx);
+  int iy = (int)((y - round(y) + hm) * (double)nlcy);
+  int iz = (int)((z - round(z) + hm) * (double)nlcz);
+  // the reference's range is [0, nlc-1] for finite positions; clamp guards NaN
+  ix = min(max(ix, 0), nlcx - 1);
+  iy = min(max(iy, 0), nlcy - 1);
+  iz = min(max(iz, 0), nlcz - 1);
+  ic[i] = ix + nlcx * iy + nlcx * nlcy * iz + 1;
+}
+__global__ void k_minus1(const int *in, int *out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] - 1;
+}
+
+// Reference half list (src/neighbourlist_m.F90:143-255) as an unordered set:
+// atom i lists j when j lies in i's own link cell after it (ascending original
+// index) or in one of the 13 forward cells, and r < rnear with the reference's
+// arithmetic.  Output in reference layout NLIST(nnbrs, nm), 1-based.
+__global__ void k_half_list(const double *x0, const double *y0, const double *z0, const int *ic,
+                            const int *orig, const int *cstart, const int *members, int n,
+                            int nlcx, int nlcy, int nlcz, const BoxState *bs, double rn2,
+                            int pbc, int nnbrs, int *numn_ref, int *nlist_ref, int *flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *b = bs->b0;
+  const int c = ic[i] - 1;
+  const int cx = c % nlcx, cy = (c / nlcx) % nlcy, cz = c / (nlcx * nlcy);
+  const double xi = x0[i], yi = y0[i], zi = z0[i];
+  const int oi = orig[i];
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; dz++) {
+    int jz = (cz + dz + nlcz) % nlcz;
+    for (int dy = -1; dy <= 1; dy++) {
+      int jy = (cy + dy + nlcy) % nlcy;
+      for (int dx = -1; dx <= 1; dx++) {
+        int jx = (cx + dx + nlcx) % nlcx;
+        // forward stencil of src/neighbourlist_m.F90:163-191
+        bool fwd = (dz == 1) || (dz == 0 && dy == 1) || (dz == 0 && dy == 0 && dx == 1);
+        bool own = (dz == 0 && dy == 0 && dx == 0);
+        if (!fwd && !own) continue;
+        int cj = jx + nlcx * (jy + nlcy * jz);
+        for (int q = cstart[cj]; q < cstart[cj + 1]; q++) {
+          int j = members[q];
+          if (own && orig[j] <= oi) continue;
+          double ddx = xi - x0[j], ddy = yi - y0[j], ddz = zi - z0[j];
+          min_image(pbc, ddx, ddy, ddz);
+          double r2 = sep2_exact(b, ddx, ddy, ddz);
+          if (r2 < rn2) {
+            cnt++;
+            if (cnt <= nnbrs) nlist_ref[(size_t)oi * nnbrs + cnt - 1] = orig[j] + 1;
+          }
+        }
+      }
+    }
+  }
+  numn_ref[oi] = cnt;
+  if (cnt > nnbrs) atomicOr(&flags[1], 1);
+}
+
+// fine-grid cell of each atom (fractional, folded into [0,1))
+__global__ void k_fast_cells(const double *x0, const double *y0, const double *z0, int n, int gx,
+                             int gy, int gz, int *cell) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fx = x0[i] - floor(x0[i]), fy = y0[i] - floor(y0[i]), fz = z0[i] - floor(z0[i]);
+  int cx = min(max((int)(fx * gx), 0), gx - 1);
+  int cy = min(max((int)(fy * gy), 0), gy - 1);
+  int cz = min(max((int)(fz * gz), 0), gz - 1);
+  cell[i] = cx + gx * (cy + gy * cz);
+}
+
+// Full neighbour list at r_list over the fine grid; atoms are already sorted by
+// cell so cell c holds slots [cstart[c], cstart[c+1]).  nbr is column-major
+// nbr[k*n + i] for coalesced loads.
+__global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
+    const double *x0, const double *y0, const double *z0, const int *cell, const int *cstart,
+    int n, int gx, int gy, int gz, const BoxState *bs, double rl2, int pbc, int maxn, int *nbr,
+    int *numn, float *sref, int *flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *b = bs->b0;
+  const int c = cell[i];
+  const int cx = c % gx, cy = (c / gx) % gy, cz = c / (gx * gy);
+  const int sx = gx >= 3 ? 1 : 0, sy = gy >= 3 ? 1 : 0, sz = gz >= 3 ? 1 : 0;
+  const double xi = x0[i], yi = y0[i], zi = z0[i];
+  int cnt = 0;
+  for (int dz = -sz; dz <= sz; dz++) {
+    int jz = (cz + dz + gz) % gz;
+    for (int dy = -sy; dy <= sy; dy++) {
+      int jy = (cy + dy + gy) % gy;
+      for (int dx = -sx; dx <= sx; dx++) {
+        int jx = (cx + dx + gx) % gx;
+        int cj = jx + gx * (jy + gy * jz);
+        for (int j = cstart[cj]; j < cstart[cj + 1]; j++) {
+          if (j == i) continue;
+          double ddx = xi - x0[j], ddy = yi - y0[j], ddz = zi - z0[j];
+          min_image(pbc, ddx, ddy, ddz);
+          double rx = b[0] * ddx + b[3] * ddy + b[6] * ddz;
+          double ry = b[1] * ddx + b[4] * ddy + b[7] * ddz;
+          double rz = b[2] * ddx + b[5] * ddy + b[8] * ddz;
+          double r2 = rx * rx + ry * ry + rz * rz;
+          if (r2 < rl2) {
+            if (cnt < maxn) nbr[(size_t)cnt * n + i] = j;
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  numn[i] = min(cnt, maxn);
+  if (cnt > maxn) atomicMax(&flags[0], cnt);
+  sref[3 * i + 0] = (float)xi;
+  sref[3 * i + 1] = (float)yi;
+  sref[3 * i + 2] = (float)zi;
+}
+
+// ---------------------------------------------------------------------------
+// MD-step kernels
+// ---------------------------------------------------------------------------
+// velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2; one thread
+// copies the box state into the other buffer and advances b1, b0 there.
+__global__ void k_drift(double *atoms, int n, const BoxState *bin, BoxState *bout, int ivol,
+                        int copy_box) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
+    const double *x1 = atoms + 3 * (size_t)n, *y1 = atoms + 4 * (size_t)n, *z1 = atoms + 5 * (size_t)n;
+    const double *x2 = atoms + 6 * (size_t)n, *y2 = atoms + 7 * (size_t)n, *z2 = atoms + 8 * (size_t)n;
+    x0[i] = x0[i] + (x1[i] + x2[i]);
+    y0[i] = y0[i] + (y1[i] + y2[i]);
+    z0[i] = z0[i] + (z1[i] + z2[i]);
+  }
+  if (copy_box && blockIdx.x == 0 && threadIdx.x == 0) {
+    BoxState s = *bin;
+    if (ivol == 4) {
+      TT(s.b1, 3, 3) = TT(s.b1, 3, 3) + TT(s.b2, 3, 3);
+      TT(s.b0, 3, 3) = TT(s.b0, 3, 3) + TT(s.b1, 3, 3);
+    }
+    if (ivol < 1) {
+      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
+      for (int k = 0; k < 9; k++) s.b0[k] = s.b0[k] + s.b1[k];
+    }
+    *bout = s;
+  }
+}
+
+// geometry of pair (i,j): fractional separation (minimum image) and Cartesian
+template <bool EXACT>
+__device__ __forceinline__ double pair_geom(const double *b, int pbc, double xi, double yi, double zi,
+                                            double xj, double yj, double zj, double &dx, double &dy,
+                                            double &dz, double &rx, double &ry, double &rz) {
+  dx = xi - xj;
+  dy = yi - yj;
+  dz = zi - zj;
+  min_image(pbc, dx, dy, dz);
+  rx = b[0] * dx + b[3] * dy + b[6] * dz;
+  ry = b[1] * dx + b[4] * dy + b[7] * dz;
+  rz = b[2] * dx + b[5] * dy + b[8] * dz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
+// (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list;
+// F, F' per atom; per-block embedding energy.
+template <typename T>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, int n, const int *nbr,
+                                                       const int *numn, const BoxState *bs,
+                                                       PotDev P, int pbc, int do_embed,
+                                                       double *part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
+  double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
+         *daf = atoms + (size_t)F_DAFRHO * n;
+  double b[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  double pe[1] = {0.0};
+  if (i < n) {
+    const double xi = x0[i], yi = y0[i], zi = z0[i];
+    const int nn = numn[i];
+    double acc = 0.0;
+    T accf = T(0);
+    for (int k = 0; k < nn; k++) {
+      const int j = nbr[(size_t)k * n + i];
+      double dx, dy, dz, rx, ry, rz;
+      double r2 = pair_geom<false>(b, pbc, xi, yi, zi, x0[j], y0[j], z0[j], dx, dy, dz, rx, ry, rz);
+      T r = sizeof(T) == 8 ? (T)sqrt(r2) : (T)sqrtf((float)r2);
+      T ph = mdk_pair_phi<T>(P, r);
+      if (sizeof(T) == 8)
+        acc += (double)ph;
+      else
+        accf += ph;
+    }
+    if (sizeof(T) != 8) acc = (double)accf;
+    rho[i] = acc;
+    if (do_embed && acc > 1e-40) {
+      double f, df;
+      mdk_embed_f(P, acc, f, df);
+      af[i] = f;
+      daf[i] = df;
+      pe[0] = f;
+    }
+  }
+  if (do_embed) block_store<1, 0>(pe, part);
+}
+
+// Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
+// fp = -(V' + phi'(F'_i + F'_j))/r;  f_i = sum_j s_ij fp;  the virial is
+// accumulated as W = 1/2 sum s s^T fp (tp = B0 W in the finishing block);
+// the pair energy as 1/2 sum V.
+template <typename T>
+__global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const int *zat, int n,
+                                                     const int *nbr, const int *numn,
+                                                     const BoxState *bs, PotDev P, int pbc,
+                                                     double *part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
+  const double *x1 = atoms + (size_t)F_X1 * n, *y1 = atoms + (size_t)F_Y1 * n, *z1 = atoms + (size_t)F_Z1 * n;
+  double *fx = atoms + (size_t)F_FX * n, *fy = atoms + (size_t)F_FY * n, *fz = atoms + (size_t)F_FZ * n;
+  const double *daf = atoms + (size_t)F_DAFRHO * n, *mass = atoms + (size_t)F_MASS * n;
+  double b[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  double v[NPART_FORCE];
+#pragma unroll
+  for (int k = 0; k < NPART_FORCE; k++) v[k] = 0.0;
+  if (i < n) {
+    const double xi = x0[i], yi = y0[i], zi = z0[i];
+    const double dfi = daf[i], mi = mass[i];
+    const int nn = numn[i];
+    double fxi = 0.0, fyi = 0.0, fzi = 0.0, pe2 = 0.0;
+    double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
+    for (int k = 0; k < nn; k++) {
+      const int j = nbr[(size_t)k * n + i];
+      double dx, dy, dz, rx, ry, rz;
+      double r2 = pair_geom<false>(b, pbc, xi, yi, zi, x0[j], y0[j], z0[j], dx, dy, dz, rx, ry, rz);
+      T r = sizeof(T) == 8 ? (T)sqrt(r2) : (T)sqrtf((float)r2);
+      T rr = T(1) / r;
+      T vv, dv;
+      mdk_pair_vdv<T>(P, r, vv, dv);
+      T dph = mdk_pair_dphi<T>(P, r);
+      T fp = -dv * rr + (-T(1) * (dph * (T)dfi + dph * (T)daf[j]) * rr);
+      const double fpd = (double)fp;
+      pe2 += 0.5 * (double)vv;
+      fxi += dx * fpd;
+      fyi += dy * fpd;
+      fzi += dz * fpd;
+      if (mi + mass[j] > (double)0.1f) {
+        const double h = 0.5 * fpd;
+        wxx += dx * dx * h; wyy += dy * dy * h; wzz += dz * dz * h;
+        wxy += dx * dy * h; wxz += dx * dz * h; wyz += dy * dz * h;
+      }
+    }
+    fx[i] = fxi;
+    fy[i] = fyi;
+    fz[i] = fzi;
+    v[0] = pe2;
+    v[1] = wxx; v[2] = wyy; v[3] = wzz; v[4] = wxy; v[5] = wxz; v[6] = wyz;
+    if (zat[i] != 0) {  // kinten
+      const double a1 = x1[i], a2 = y1[i], a3 = z1[i];
+      const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
+      const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
+      const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
+      // tk(a,b) += x1_b * mv_a, stored column-major from v[10]
+      v[10] = a1 * mvx; v[11] = a1 * mvy; v[12] = a1 * mvz;
+      v[13] = a2 * mvx; v[14] = a2 * mvy; v[15] = a2 * mvz;
+      v[16] = a3 * mvx; v[17] = a3 * mvy; v[18] = a3 * mvz;
+    }
+  }
+  block_store<NPART_FORCE, 0>(v, part);
+}
+
+enum { FIN_FORCE = 0, FIN_VV = 1, FIN_VV_THERM = 2, FIN_INIT = 3 };
+
+// update_therm_avs box part (src/dynamics_m.F90:672-700)
+__device__ void therm_box(BoxState &s, int ivol) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 9; k++) { s.tg[k] = 0.0; s.tgdot[k] = 0.0; }
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) {
+        TT(s.tg, i, j) = TT(s.tg, i, j) + TT(s.b0, k, i) * TT(s.b0, k, j);
+        if (ivol != 1)
+          TT(s.tgdot, i, j) = TT(s.tgdot, i, j) + TT(s.b1, k, i) * TT(s.b0, k, j) + TT(s.b0, k, i) * TT(s.b1, k, j);
+      }
+  dev_invert(s.tg, s.tginv, &s.status);
+  dev_matmul(s.tginv, s.tgdot, s.tgid);
+  s.vol = dev_volume(s.b0);
+  const double *b = s.b0, v = s.vol;
+  TT(s.binv, 1, 1) = (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) / v;  // xx
+  TT(s.binv, 1, 2) = (TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 3, 3) * TT(b, 2, 1)) / v;  // xy
+  TT(s.binv, 1, 3) = (TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 3, 1) * TT(b, 2, 2)) / v;  // xz
+  TT(s.binv, 2, 1) = (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) / v;  // yx
+  TT(s.binv, 2, 2) = (TT(b, 1, 1) * TT(b, 3, 3) - TT(b, 1, 3) * TT(b, 3, 1)) / v;  // yy
+  TT(s.binv, 2, 3) = (TT(b, 3, 1) * TT(b, 1, 2) - TT(b, 1, 1) * TT(b, 3, 2)) / v;  // yz
+  TT(s.binv, 3, 1) = (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3)) / v;  // zx
+  TT(s.binv, 3, 2) = (TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3)) / v;  // zy
+  TT(s.binv, 3, 3) = (TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 1, 2) * TT(b, 2, 1)) / v;  // zz
+}
+
+// Finishing block of force: pe, tp, tk, tc, fb (dynamics_m.F90:365-394),
+// then per mode the velocity-Verlet box kick (quench_m.F90:595-604), the
+// initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part.
+__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pd, int nbd,
+                                                              const double *pf, int nbf,
+                                                              BoxState *bs, int mode, int ivol,
+                                                              double press, double bdel2,
+                                                              double dt, int have_emb) {
+#pragma clang fp contract(off)
+  double e[1] = {0.0};
+  if (have_emb) block_final<1, 0>(pd, nbd, e);
+  double v[NPART_FORCE];
+  block_final<NPART_FORCE, 0>(pf, nbf, v);
+  if (threadIdx.x != 0) return;
+  BoxState s = *bs;
+  if (have_emb) s.pe_emb = e[0];
+  s.pe = s.pe_emb + v[0];
+  const double *b = s.b0;
+  // W (symmetric) -> tp = B0 W
+  double W[9];
+  TT(W, 1, 1) = v[1]; TT(W, 2, 2) = v[2]; TT(W, 3, 3) = v[3];
+  TT(W, 1, 2) = TT(W, 2, 1) = v[4];
+  TT(W, 1, 3) = TT(W, 3, 1) = v[5];
+  TT(W, 2, 3) = TT(W, 3, 2) = v[6];
+  for (int a = 1; a <= 3; a++)
+    for (int c = 1; c <= 3; c++)
+      TT(s.tp, a, c) = TT(b, a, 1) * TT(W, 1, c) + TT(b, a, 2) * TT(W, 2, c) + TT(b, a, 3) * TT(W, 3, c);
+  for (int k = 0; k < 9; k++) s.tk[k] = v[10 + k] / (dt * dt);
+  TT(s.tc, 1, 1) = TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2);
+  TT(s.tc, 2, 1) = TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3);
+  TT(s.tc, 3, 1) = TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3);
+  TT(s.tc, 1, 2) = TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 3, 3) * TT(b, 2, 1);
+  TT(s.tc, 2, 2) = TT(b, 3, 3) * TT(b, 1, 1) - TT(b, 1, 3) * TT(b, 3, 1);
+  TT(s.tc, 3, 2) = TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3);
+  TT(s.tc, 1, 3) = TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 3, 1) * TT(b, 2, 2);
+  TT(s.tc, 2, 3) = TT(b, 3, 1) * TT(b, 1, 2) - TT(b, 1, 1) * TT(b, 3, 2);
+  TT(s.tc, 3, 3) = TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 2, 1) * TT(b, 1, 2);
+  // kinten's tgdot and force's tg
+  for (int k = 0; k < 9; k++) { s.tgdot[k] = 0.0; s.tg[k] = 0.0; }
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) {
+        TT(s.tgdot, i, j) = TT(s.tgdot, i, j) + TT(b, k, i) * TT(s.b1, k, j) + TT(s.b1, k, i) * TT(b, k, j);
+        TT(s.tg, i, j) = TT(s.tg, i, j) + TT(b, k, i) * TT(b, k, j);
+      }
+  for (int k = 0; k < 9; k++) s.fb[k] = s.tk[k] + s.tp[k] - press * s.tc[k];
+  s.vol = dev_volume(b);
+  if (mode == FIN_VV || mode == FIN_VV_THERM) {
+    for (int k = 0; k < 9; k++) s.tgid_lag[k] = s.tgid[k];
+    if (ivol < 1) {
+      for (int k = 0; k < 9; k++) s.b2[k] = bdel2 * s.fb[k];
+      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
+    }
+    if (ivol == 4) {
+      TT(s.b2, 3, 3) = bdel2 * TT(s.fb, 3, 3);
+      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
+    }
+    if (mode == FIN_VV_THERM) therm_box(s, ivol);
+  } else if (mode == FIN_INIT) {
+    dev_invert(s.tg, s.tginv, &s.status);
+    dev_matmul(s.tginv, s.tgdot, s.tgid);
+    for (int k = 0; k < 9; k++) s.b2[k] = bdel2 * s.fb[k];
+  }
+  *bs = s;
+}
+
+// update_therm_avs box part only (stand-alone entry point)
+__global__ void k_therm_box(BoxState *bs, int ivol) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    BoxState s = *bs;
+    therm_box(s, ivol);
+    *bs = s;
+  }
+}
+
+// Velocity half-kick (src/quench_m.F90:604-622) fused with the per-atom sums of
+// update_therm_avs (src/dynamics_m.F90:702-732) and the list-skin monitor.
+// mode 0: VV kick + sums; 1: sums only; 2: initial accelerations (moldin.F90:461-472)
+__global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *zat, int n,
+                                                    const BoxState *bs, int ivol, int mode,
+                                                    const float *sref, double *part) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    double *x0p = atoms, *y0p = atoms + n, *z0p = atoms + 2 * (size_t)n;
+    double *x1p = atoms + (size_t)F_X1 * n, *y1p = atoms + (size_t)F_Y1 * n, *z1p = atoms + (size_t)F_Z1 * n;
+    double *x2p = atoms + (size_t)F_X2 * n, *y2p = atoms + (size_t)F_Y2 * n, *z2p = atoms + (size_t)F_Z2 * n;
+    const double *fxp = atoms + (size_t)F_FX * n, *fyp = atoms + (size_t)F_FY * n, *fzp = atoms + (size_t)F_FZ * n;
+    const double m = atoms[(size_t)F_MASS * n + i], d2 = atoms[(size_t)F_DEL2 * n + i];
+    const int z = zat[i];
+    double x1 = x1p[i], y1 = y1p[i], z1 = z1p[i];
+    double x2 = x2p[i], y2 = y2p[i], z2 = z2p[i];
+    if (mode == 0) {
+      const double *g = bs->tgid_lag;
+      const double xd = x1 + x2, yd = y1 + y2, zd = z1 + z2;  // x1_dt2
+      if (!(z == 0 || m < (double)0.1f)) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (ivol == 0) {
+          sx = TT(g, 1, 1) * x1 + TT(g, 1, 2) * y1 + TT(g, 1, 3) * z1;
+          sy = TT(g, 2, 1) * x1 + TT(g, 2, 2) * y1 + TT(g, 2, 3) * z1;
+          sz = TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1;
+        }
+        if (ivol == 4) sz = TT(g, 3, 3) * z1;
+        x2 = d2 * fxp[i] - 0.5 * sx;
+        y2 = d2 * fyp[i] - 0.5 * sy;
+        z2 = d2 * fzp[i] - 0.5 * sz;
+        x2p[i] = x2; y2p[i] = y2; z2p[i] = z2;
+      }
+      x1 = xd + x2; y1 = yd + y2; z1 = zd + z2;
+      x1p[i] = x1; y1p[i] = y1; z1p[i] = z1;
+    } else if (mode == 2) {
+      if (z != 0) {
+        const double *g = bs->tgid;
+        double a1 = TT(g, 1, 1) * x1 + TT(g, 1, 2) * y1 + TT(g, 1, 3) * z1;
+        double a2 = TT(g, 2, 1) * x1 + TT(g, 2, 2) * y1 + TT(g, 2, 3) * z1;
+        double a3 = TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1;
+        x2 = d2 * fxp[i] - 0.5 * a1;
+        y2 = d2 * fyp[i] - 0.5 * a2;
+        z2 = d2 * fzp[i] - 0.5 * a3;
+        x2p[i] = x2; y2p[i] = y2; z2p[i] = z2;
+      }
+    }
+    const double *b = bs->b0;
+    if (mode != 2 && z != 0) {
+      const double *g = bs->tgid, *bi = bs->binv;
+      double sx = TT(g, 1, 1) * x1 + TT(g, 1, 2) * y1 + TT(g, 1, 3) * z1;
+      double sy = TT(g, 2, 1) * x1 + TT(g, 2, 2) * y1 + TT(g, 2, 3) * z1;
+      double sz = TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1;
+      double fxq = 2.0 * x2 + sx, fyq = 2.0 * y2 + sy, fzq = 2.0 * z2 + sz;
+      double fxi = TT(bi, 1, 1) * fxq + TT(bi, 1, 2) * fyq + TT(bi, 1, 3) * fzq;
+      double fyi = TT(bi, 2, 1) * fxq + TT(bi, 2, 2) * fyq + TT(bi, 2, 3) * fzq;
+      double fzi = TT(bi, 3, 1) * fxq + TT(bi, 3, 2) * fyq + TT(bi, 3, 3) * fzq;
+      double f2 = fxi * fxi + fyi * fyi + fzi * fzi;
+      v[0] = f2;
+      v[1] = f2 * f2;
+      double rx = TT(b, 1, 1) * x1 + TT(b, 1, 2) * y1 + TT(b, 1, 3) * z1;
+      double ry = TT(b, 2, 1) * x1 + TT(b, 2, 2) * y1 + TT(b, 2, 3) * z1;
+      double rz = TT(b, 3, 1) * x1 + TT(b, 3, 2) * y1 + TT(b, 3, 3) * z1;
+      v[2] = m * (rx * rx + ry * ry + rz * rz);
+    }
+    if (sref) {
+      double ex = x0p[i] - (double)sref[3 * i], ey = y0p[i] - (double)sref[3 * i + 1],
+             ez = z0p[i] - (double)sref[3 * i + 2];
+      double qx = b[0] * ex + b[3] * ey + b[6] * ez, qy = b[1] * ex + b[4] * ey + b[7] * ez,
+             qz = b[2] * ex + b[5] * ey + b[8] * ez;
+      v[3] = qx * qx + qy * qy + qz * qz;
+      double sx = x1 + x2, sy = y1 + y2, sz = z1 + z2;
+      qx = b[0] * sx + b[3] * sy + b[6] * sz;
+      qy = b[1] * sx + b[4] * sy + b[7] * sz;
+      qz = b[2] * sx + b[5] * sy + b[8] * sz;
+      v[4] = qx * qx + qy * qy + qz * qz;
+    }
+  }
+  block_store<NPART_KICK, 2>(v, part);
+}
+
+// Finishing block of the kick: s1,s2,s3, KE/TE/H/TH (moldin.F90:763-790),
+// trace row and the host-visible skin monitor.
+__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_kick(const double *pk, int nb, BoxState *bs,
+                                                             double dt, double bdel2, double press,
+                                                             double *trace_row, double *mon) {
+#pragma clang fp contract(off)
+  double v[NPART_KICK];
+  block_final<NPART_KICK, 2>(pk, nb, v);
+  if (threadIdx.x != 0) return;
+  BoxState s = *bs;
+  s.s1 = v[0];
+  s.s2 = v[1];
+  s.s3 = v[2];
+  s.maxdisp2 = v[3];
+  s.maxstep2 = v[4];
+  s.ke = s.s3 / (dt * dt) / 2.0;
+  double bk = 0.0;
+  for (int i = 1; i <= 3; i++)
+    for (int j = 1; j <= 3; j++) bk = bk + (TT(s.b1, i, j) * TT(s.b1, i, j)) / bdel2;
+  s.boxke = bk;
+  s.te = s.ke + s.pe;
+  s.h = s.te + press * s.vol;
+  s.th = s.h + s.boxke;
+  *bs = s;
+  if (trace_row) {
+    trace_row[0] = s.pe; trace_row[1] = s.ke; trace_row[2] = s.te;
+    trace_row[3] = s.h; trace_row[4] = s.th; trace_row[5] = s.vol;
+  }
+  if (mon) {
+    mon[0] = v[3];
+    mon[1] = v[4];
+  }
+}
+
+// potential evaluation for mdk_eval_potential
+__global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  double r = x[k], v = 0.0, dv = 0.0;
+  switch (which) {
+    case 0: mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
+    case 1: mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
+    case 2: y[k] = mdk_pair_phi<double>(P, r); break;
+    case 3: y[k] = mdk_pair_dphi<double>(P, r); break;
+    case 4: mdk_embed_f(P, r, v, dv); y[k] = v; break;
+    case 5: mdk_embed_f(P, r, v, dv); y[k] = dv; break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+static inline int nblocks(long n) { return (int)((n + MDK_BLOCK - 1) / MDK_BLOCK); }
+
+static void tbegin(mdk_ctx *c, int cls, TimedEv &te) {
+  te.cls = -1;
+  if (!c->timing) return;
+  te.cls = cls;
+  hipEventCreate(&te.a);
+  hipEventCreate(&te.b);
+  hipEventRecord(te.a, c->stream);
+}
+static void tend(mdk_ctx *c, TimedEv &te) {
+  if (te.cls < 0) return;
+  hipEventRecord(te.b, c->stream);
+  c->evs.push_back(te);
+  if (c->evs.size() > 4096) {  // resolve old ones to bound memory
+    hipEventSynchronize(te.b);
+    for (auto &e : c->evs) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e.a, e.b);
+      c->t_ms[e.cls] += ms;
+      c->t_cnt[e.cls] += 1;
+      hipEventDestroy(e.a);
+      hipEventDestroy(e.b);
+    }
+    c->evs.clear();
+  }
+}
+
+static int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(MDK_ERR_HIP, "launch %s: %s", what, hipGetErrorString(e));
+  return MDK_OK;
+}
+
+static double near_threshold(double rn) {
+  // smallest double q with sqrt(q) >= rn (host sqrt is correctly rounded), so
+  // that sqrt(q) < rn <=> q < threshold without a device square root.
+  double q = rn * rn;
+  while (sqrt(q) >= rn) q = nextafter(q, 0.0);
+  while (sqrt(q) < rn) q = nextafter(q, INFINITY);
+  return q;
+}
+
+static int read_box(mdk_ctx *c, BoxState *out) {
+  HIPCK(hipMemcpyAsync(out, c->box + c->cur, sizeof(BoxState), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+// Fine grid from the current box: cells at least r_list wide (perpendicular width)
+static void plan_grid(mdk_ctx *c, const double *b, int g[3]) {
+  double vol = fabs(TT(b, 1, 1) * (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) +
+                    TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
+                    TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3)));
+  for (int d = 0; d < 3; d++) {
+    int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+    const double *u = b + 3 * e1, *w = b + 3 * e2;
+    double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+    double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
+    int nc = (int)floor(width / c->r_list);
+    if (nc < 3) nc = 1;  // too few cells for a distinct 27-cell stencil: one cell in this dim
+    if (nc > 1024) nc = 1024;
+    g[d] = nc;
+  }
+}
+
+static int ensure_cells(mdk_ctx *c, int m) {
+  if (m <= c->ncell_cap) return MDK_OK;
+  if (c->cell_count) hipFree(c->cell_count);
+  if (c->cell_start) hipFree(c->cell_start);
+  if (c->cell_fill) hipFree(c->cell_fill);
+  int cap = std::max(m, 64);
+  HIPCK(hipMalloc(&c->cell_count, sizeof(int) * cap));
+  HIPCK(hipMalloc(&c->cell_start, sizeof(int) * (cap + 1)));
+  HIPCK(hipMalloc(&c->cell_fill, sizeof(int) * cap));
+  c->ncell_cap = cap;
+  return MDK_OK;
+}
+
+// counting sort of atoms by key[] into cell_start/members with a deterministic
+// in-cell order (ascending sortkey, or slot when sortkey is null)
+static int bin_atoms(mdk_ctx *c, const int *key, int m, const int *sortkey) {
+  const int n = c->n;
+  RCK(ensure_cells(c, m));
+  HIPCK(hipMemsetAsync(c->cell_count, 0, sizeof(int) * m, c->stream));
+  HIPCK(hipMemsetAsync(c->cell_fill, 0, sizeof(int) * m, c->stream));
+  k_count<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(key, n, c->cell_count);
+  k_scan1<<<1, 1024, 0, c->stream>>>(c->cell_count, c->cell_start, m);
+  k_fill<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(key, n, c->cell_start, c->cell_fill, c->members);
+  k_sort_cells<<<nblocks(m), MDK_BLOCK, 0, c->stream>>>(c->cell_start, m, c->members, sortkey);
+  return check_launch("bin_atoms");
+}
+
+// Full-list rebuild: fine grid from the current box, re-sort atoms by cell,
+// build the list; grows the per-atom capacity and retries on overflow.
+static int rebuild_full(mdk_ctx *c) {
+  const int n = c->n;
+  c->list_valid = false;
+  TimedEv te;
+  tbegin(c, TC_REBUILD, te);
+  BoxState bs;
+  RCK(read_box(c, &bs));
+  int g[3];
+  plan_grid(c, bs.b0, g);
+  const int m = g[0] * g[1] * g[2];
+  c->grid[0] = g[0]; c->grid[1] = g[1]; c->grid[2] = g[2];
+  k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
+                                                       n, g[0], g[1], g[2], c->cell_of);
+  RCK(bin_atoms(c, c->cell_of, m, nullptr));
+  // physical re-sort of every per-atom array into cell order
+  k_permute<<<nblocks((long)n * F_N), MDK_BLOCK, 0, c->stream>>>(c->atoms_alt, c->atoms, c->members, n, F_N);
+  k_permute_int<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->zat, c->orig_alt, c->orig,
+                                                        c->slot_of, c->members, n);
+  std::swap(c->atoms, c->atoms_alt);
+  std::swap(c->zat, c->zat_alt);
+  std::swap(c->orig, c->orig_alt);
+  // reference cells move with the atoms
+  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
+  HIPCK(hipMemcpyAsync(c->ic, c->zat_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+  k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
+                                                       n, g[0], g[1], g[2], c->cell_of);
+  const double rl2 = c->r_list * c->r_list;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
+    k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, c->cell_of, c->cell_start, n, g[0], g[1], g[2],
+        c->box + c->cur, rl2, c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+    RCK(check_launch("k_full_list"));
+    int fl[4];
+    HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (fl[0] == 0) {
+      c->list_valid = true;
+      c->rebuilds++;
+      tend(c, te);
+      return MDK_OK;
+    }
+    // grow capacity to the observed maximum (+25%) and retry
+    int newmax = (int)(fl[0] * 1.25) + 8;
+    HIPCK(hipFree(c->nbr));
+    HIPCK(hipMalloc(&c->nbr, sizeof(int) * (size_t)newmax * n));
+    c->maxn = newmax;
+  }
+  tend(c, te);
+  HIPCK(hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream));
+  return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
+}
+
+static int launch_density(mdk_ctx *c, int do_embed) {
+  TimedEv te;
+  tbegin(c, TC_DENSITY, te);
+  if (c->p.precision == MDK_PREC_FP32)
+    k_density<float><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->nbr, c->numn, c->box + c->cur,
+                                                          c->pot, c->pbc, do_embed, c->part_d);
+  else
+    k_density<double><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->nbr, c->numn, c->box + c->cur,
+                                                           c->pot, c->pbc, do_embed, c->part_d);
+  tend(c, te);
+  return check_launch("k_density");
+}
+
+static int launch_force(mdk_ctx *c, int fin_mode) {
+  TimedEv te;
+  tbegin(c, TC_FORCE, te);
+  if (c->p.precision == MDK_PREC_FP32)
+    k_force<float><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->nbr, c->numn,
+                                                        c->box + c->cur, c->pot, c->pbc, c->part_f);
+  else
+    k_force<double><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->nbr, c->numn,
+                                                         c->box + c->cur, c->pot, c->pbc, c->part_f);
+  tend(c, te);
+  RCK(check_launch("k_force"));
+  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_d, c->nblk, c->part_f, c->nblk, c->box + c->cur,
+                                                   fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
+                                                   c->p.deltat, 1);
+  return check_launch("k_finalize_force");
+}
+
+// force() bookkeeping of src/dynamics_m.F90:186-198 plus the skin criterion
+static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
+  if (!c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
+  c->ntc = c->ntc + 1;
+  bool ref_due = (c->ntc == 1);
+  if (c->ntc == c->p.ntcm) c->ntc = 0;
+  bool due = !c->list_valid || skin_due || (ref_due && c->p.rebuild == 0);
+  if (due) RCK(rebuild_full(c));
+  RCK(launch_density(c, 1));
+  return launch_force(c, fin_mode);
+}
+
+static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon) {
+  TimedEv te;
+  tbegin(c, TC_KICK, te);
+  k_kick<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode,
+                                               c->sref, c->part_k);
+  tend(c, te);
+  RCK(check_launch("k_kick"));
+  if (mode == 2) return MDK_OK;
+  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, c->box + c->cur, c->p.deltat,
+                                                  c->p.bdel2, c->p.press, trace_row_dev, mon);
+  return check_launch("k_finalize_kick");
+}
+
+static int launch_drift(mdk_ctx *c) {
+  TimedEv te;
+  tbegin(c, TC_DRIFT, te);
+  const int nxt = 1 - c->cur;
+  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->box + c->cur, c->box + nxt, c->p.ivol, 1);
+  tend(c, te);
+  c->cur = nxt;
+  return check_launch("k_drift");
+}
+
+// Decide from a completed monitor sample whether the next force evaluation
+// could see an atom displaced by more than half the skin since the last build.
+static bool skin_check(mdk_ctx *c) {
+  if (!c->list_valid) return true;
+  long s = c->step_counter;
+  if (s - c->last_rebuild_step < 3) return false;
+  long probe = s - 2;
+  int slot = (int)(probe & 3);
+  hipEventSynchronize(c->mon_ev[slot]);
+  double disp = sqrt(fmax(c->mon_host[2 * slot], 0.0));
+  double stp = sqrt(fmax(c->mon_host[2 * slot + 1], 0.0));
+  return disp + 3.0 * stp >= 0.5 * c->skin;
+}
+
+static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm) {
+  bool skin_due = skin_check(c);
+  long before = c->rebuilds;
+  RCK(launch_drift(c));
+  RCK(force_body(c, fused_therm ? FIN_VV_THERM : FIN_VV, skin_due));
+  if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
+  int slot = (int)(c->step_counter & 3);
+  RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + 2 * slot));
+  HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
+  c->step_counter++;
+  return MDK_OK;
+}
+
+static int check_device_status(mdk_ctx *c) {
+  BoxState bs;
+  RCK(read_box(c, &bs));
+  if (bs.status != 0) return set_err(bs.status, "singular box matrix in matrix_invert");
+  return MDK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static mdk_ctx *H(void *h) { return (mdk_ctx *)h; }
+
+extern "C" int mdk_create(void **handle, const mdk_params *p) {
+  if (!handle || !p) return set_err(MDK_ERR_BAD_ARG, "null argument");
+  if (p->nm < 1 || p->nnbrs < 1 || p->ntcm < 1 || p->nlcx < 3 || p->nlcy < 3 || p->nlcz < 3 ||
+      p->deltat <= 0.0 || p->rnear <= 0.0)
+    return set_err(MDK_ERR_BAD_ARG, "bad parameters (nm=%d nnbrs=%d ntcm=%d)", p->nm, p->nnbrs, p->ntcm);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(MDK_ERR_NO_DEVICE, "no HIP device");
+  void *mem = ::operator new(sizeof(mdk_ctx));
+  memset(mem, 0, sizeof(mdk_ctx));
+  mdk_ctx *c = (mdk_ctx *)mem;
+  new (&c->evs) std::vector<TimedEv>();
+  c->p = *p;
+  c->n = p->nm;
+  const int n = c->n;
+  switch (p->ivol) {
+    case 0: case 1: case 4: c->pbc = 7; break;
+    case 2: c->pbc = 3; break;
+    case 5: c->pbc = 4; break;
+    default: c->pbc = 0; break;
+  }
+  hipGetDevice(&c->device);
+  c->maxn = p->max_full > 0 ? p->max_full : 64;
+  c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
+  c->rnear2_exact = near_threshold(p->rnear);
+  c->nblk = nblocks(n);
+  int rc = MDK_OK;
+#define ALLOC(ptr, bytes)                                                          \
+  do {                                                                             \
+    if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                       \
+      rc = set_err(MDK_ERR_HIP, "hipMalloc %zu bytes failed", (size_t)(bytes));    \
+      goto fail;                                                                   \
+    }                                                                              \
+  } while (0)
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    rc = set_err(MDK_ERR_HIP, "stream create failed");
+    goto fail;
+  }
+  ALLOC(c->atoms, sizeof(double) * (size_t)F_N * n);
+  ALLOC(c->atoms_alt, sizeof(double) * (size_t)F_N * n);
+  ALLOC(c->zat, sizeof(int) * n);
+  ALLOC(c->zat_alt, sizeof(int) * n);
+  ALLOC(c->orig, sizeof(int) * n);
+  ALLOC(c->orig_alt, sizeof(int) * n);
+  ALLOC(c->slot_of, sizeof(int) * n);
+  ALLOC(c->sref, sizeof(float) * 3 * (size_t)n);
+  ALLOC(c->ic, sizeof(int) * n);
+  ALLOC(c->numn_ref, sizeof(int) * n);
+  ALLOC(c->nlist_ref, sizeof(int) * (size_t)n * p->nnbrs);
+  ALLOC(c->nbr, sizeof(int) * (size_t)c->maxn * n);
+  ALLOC(c->numn, sizeof(int) * n);
+  ALLOC(c->cell_of, sizeof(int) * n);
+  ALLOC(c->members, sizeof(int) * n);
+  ALLOC(c->dflags, sizeof(int) * 4);
+  ALLOC(c->box, sizeof(BoxState) * 2);
+  ALLOC(c->part_d, sizeof(double) * (size_t)c->nblk);
+  ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * NPART_FORCE);
+  ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
+  if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 8, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->mon_dev, c->mon_host, 0) != hipSuccess) {
+    rc = set_err(MDK_ERR_HIP, "mapped host buffer failed");
+    goto fail;
+  }
+  for (int k = 0; k < 4; k++) hipEventCreateWithFlags(&c->mon_ev[k], hipEventDisableTiming);
+  hipMemsetAsync(c->atoms, 0, sizeof(double) * (size_t)F_N * n, c->stream);
+  hipMemsetAsync(c->zat, 0, sizeof(int) * n, c->stream);
+  hipMemsetAsync(c->box, 0, sizeof(BoxState) * 2, c->stream);
+  hipMemsetAsync(c->ic, 0, sizeof(int) * n, c->stream);
+  hipMemsetAsync(c->numn_ref, 0, sizeof(int) * n, c->stream);
+  hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream);
+  k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->orig, n);
+  k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->slot_of, n);
+  memset(c->mon_host, 0, sizeof(double) * 8);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    rc = set_err(MDK_ERR_HIP, "init sync failed");
+    goto fail;
+  }
+  *handle = c;
+  return MDK_OK;
+fail:
+  mdk_destroy(c);
+  return rc;
+#undef ALLOC
+}
+
+extern "C" int mdk_destroy(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return MDK_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
+                  c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
+                  c->members, c->dflags, c->box, c->part_d, c->part_f, c->part_k, c->cell_count,
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  if (c->mon_host) hipHostFree(c->mon_host);
+  for (int k = 0; k < 4; k++)
+    if (c->mon_ev[k]) hipEventDestroy(c->mon_ev[k]);
+  for (auto &e : c->evs) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  if (c->stream) hipStreamDestroy(c->stream);
+  c->evs.~vector();
+  ::operator delete(c);
+  return MDK_OK;
+}
+
+extern "C" int mdk_set_potential(void *h, int nv, const double *av, const double *rv, int np,
+                                 const double *ap, const double *rp) {
+  mdk_ctx *c = H(h);
+  if (!c || nv < 1 || nv > MDK_MAXV || np < 1 || np > MDK_MAXP || !av || !rv || !ap || !rp)
+    return set_err(MDK_ERR_BAD_ARG, "bad potential coefficients");
+  hostpot::Knots &k = c->knots;
+  memset(&k, 0, sizeof(k));
+  k.nv = nv;
+  k.np = np;
+  for (int i = 0; i < nv; i++) { k.av[i] = av[i]; k.rv[i] = rv[i]; }
+  for (int i = 0; i < np; i++) { k.ap[i] = ap[i]; k.rp[i] = rp[i]; }
+  hostpot::set_up(k);
+  PotDev &P = c->pot;
+  const int keep_lookup = P.use_lookup, keep_n = P.nlookup;
+  memset(&P, 0, sizeof(P));
+  P.nv = nv;
+  P.np = np;
+  double rint = 0.0;
+  for (int i = 0; i < nv; i++) {
+    P.av[i] = av[i]; P.rv[i] = rv[i]; P.avf[i] = (float)av[i]; P.rvf[i] = (float)rv[i];
+    rint = std::max(rint, rv[i]);
+  }
+  for (int i = 0; i < np; i++) {
+    P.ap[i] = ap[i]; P.rp[i] = rp[i]; P.apf[i] = (float)ap[i]; P.rpf[i] = (float)rp[i];
+    rint = std::max(rint, rp[i]);
+  }
+  P.bc1 = k.bc1; P.bc2 = k.bc2;
+  for (int i = 0; i < 4; i++) P.sc[i] = k.sc[i];
+  P.phi_cut_r = k.pcr; P.phi_cut_v = k.pcv;
+  P.bier_a = hostpot::screening_length();
+  P.bier_zed = hostpot::zed();
+  P.r_int = rint;
+  P.jx1 = log(k.sc[0]); P.jv1 = k.sc[1] / k.sc[0];
+  P.jx2 = log(k.sc[2]); P.jv2 = k.sc[3] / k.sc[2];
+  P.bc1f = (float)P.bc1; P.bc2f = (float)P.bc2; P.phi_cut_rf = (float)P.phi_cut_r;
+  P.phi_cut_vf = (float)P.phi_cut_v; P.r_intf = (float)rint;
+  P.jx1f = (float)P.jx1; P.jv1f = (float)P.jv1; P.jx2f = (float)P.jx2; P.jv2f = (float)P.jv2;
+  P.bier_af = (float)P.bier_a; P.bier_zedf = (float)P.bier_zed;
+  c->r_list = rint + c->skin;
+  c->pot_set = true;
+  c->list_valid = false;
+  if (keep_lookup) return mdk_set_lookup(h, keep_n > 0 ? keep_n : 5000);
+  return MDK_OK;
+}
+
+extern "C" int mdk_set_lookup(void *h, int nlookup) {
+  mdk_ctx *c = H(h);
+  if (!c || !c->pot_set) return set_err(MDK_ERR_STATE, "set the potential before the lookup tables");
+  PotDev &P = c->pot;
+  if (nlookup <= 0) {
+    P.use_lookup = 0;
+    return MDK_OK;
+  }
+  const hostpot::Knots &k = c->knots;
+  // src/lookup_m.F90:161-251 with the ranges of fetransmetal (:589-593) and potential_m (:185)
+  P.nlookup = nlookup;
+  P.r_min = 0.0; P.r_max = 6.0;
+  P.vp_range = P.r_max - P.r_min;
+  P.vp_reciprange = 1.0 / P.vp_range;
+  P.vp_recipnlkp = 1.0 / (double)nlookup;
+  P.vp_recipstep = (double)nlookup / P.vp_range;
+  P.rho_min = 0.0; P.rho_max = 10000.0;
+  P.e_range = P.rho_max - P.rho_min;
+  P.e_reciprange = 1.0 / P.e_range;
+  P.e_recipnlkp = 1.0 / (double)nlookup;
+  P.e_recipstep = (double)nlookup / P.e_range;
+  const size_t m = (size_t)nlookup + 1;
+  std::vector<double> t(6 * m);
+  for (size_t i = 0; i < m; i++) {
+    double r = P.r_min + (double)i * P.vp_range * P.vp_recipnlkp;
+    t[0 * m + i] = hostpot::vee(k, r);
+    t[1 * m + i] = hostpot::dvee(k, r);
+    t[2 * m + i] = hostpot::phi(k, r);
+    t[3 * m + i] = hostpot::dphi(k, r);
+    double rh = P.rho_min + (double)i * P.e_range * P.e_recipnlkp;
+    t[4 * m + i] = hostpot::emb(rh);
+    t[5 * m + i] = hostpot::demb(rh);
+  }
+  if (c->tabs) hipFree(c->tabs);
+  HIPCK(hipMalloc(&c->tabs, sizeof(double) * 6 * m));
+  HIPCK(hipMemcpy(c->tabs, t.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice));
+  P.tab_v = c->tabs; P.tab_dv = c->tabs + m; P.tab_p = c->tabs + 2 * m;
+  P.tab_dp = c->tabs + 3 * m; P.tab_e = c->tabs + 4 * m; P.tab_de = c->tabs + 5 * m;
+  P.use_lookup = 1;
+  return MDK_OK;
+}
+
+extern "C" int mdk_potential_constants(void *h, double *out) {
+  mdk_ctx *c = H(h);
+  if (!c || !c->pot_set || !out) return set_err(MDK_ERR_STATE, "potential not set");
+  out[0] = c->knots.bc1; out[1] = c->knots.bc2;
+  for (int i = 0; i < 4; i++) out[2 + i] = c->knots.sc[i];
+  out[6] = c->knots.pcr; out[7] = c->knots.pcv;
+  return MDK_OK;
+}
+
+extern "C" int mdk_eval_potential(void *h, int which, const double *x, double *y, int count) {
+  mdk_ctx *c = H(h);
+  if (!c || !c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
+  if (which < 0 || which > 5 || count < 0 || (count > 0 && (!x || !y))) return set_err(MDK_ERR_BAD_ARG, "bad eval args");
+  if (count == 0) return MDK_OK;
+  double *d = nullptr;
+  HIPCK(hipMalloc(&d, sizeof(double) * 2 * (size_t)count));
+  HIPCK(hipMemcpyAsync(d, x, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+  k_eval_pot<<<nblocks(count), MDK_BLOCK, 0, c->stream>>>(c->pot, which, d, d + count, count);
+  int rc = check_launch("k_eval_pot");
+  hipMemcpyAsync(y, d + count, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return rc;
+}
+
+static const int field_map[MDK_NFIELDS] = {F_X0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2,
+                                           F_FX, F_FY, F_FZ, F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2};
+
+extern "C" int mdk_upload(void *h, int field, const double *data) {
+  mdk_ctx *c = H(h);
+  if (!c || field < 0 || field >= MDK_NFIELDS || !data) return set_err(MDK_ERR_BAD_ARG, "bad upload");
+  const int n = c->n;
+  double *tmp = fld(c->atoms_alt, 0, n);  // staging in the alternate buffer
+  HIPCK(hipMemcpyAsync(tmp, data, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  k_scatter_d<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(fld(c->atoms, field_map[field], n), tmp, c->slot_of, n);
+  RCK(check_launch("k_scatter_d"));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (field <= MDK_Z0) c->list_valid = false;  // positions replaced: rebuild before the next force
+  return MDK_OK;
+}
+
+extern "C" int mdk_download(void *h, int field, double *data) {
+  mdk_ctx *c = H(h);
+  if (!c || field < 0 || field >= MDK_NFIELDS || !data) return set_err(MDK_ERR_BAD_ARG, "bad download");
+  const int n = c->n;
+  double *tmp = fld(c->atoms_alt, 0, n);
+  k_gather_d<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(tmp, fld(c->atoms, field_map[field], n), c->slot_of, n);
+  RCK(check_launch("k_gather_d"));
+  HIPCK(hipMemcpyAsync(data, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_upload_species(void *h, const int *zn) {
+  mdk_ctx *c = H(h);
+  if (!c || !zn) return set_err(MDK_ERR_BAD_ARG, "bad species upload");
+  const int n = c->n;
+  HIPCK(hipMemcpyAsync(c->zat_alt, zn, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  k_scatter_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat, c->zat_alt, c->slot_of, n);
+  RCK(check_launch("k_scatter_i"));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+static double *box_field(BoxState &s, int which) {
+  switch (which) {
+    case MDK_B0: return s.b0;
+    case MDK_B1: return s.b1;
+    case MDK_B2: return s.b2;
+    case MDK_TK: return s.tk;
+    case MDK_TC: return s.tc;
+    case MDK_TG: return s.tg;
+    case MDK_TP: return s.tp;
+    case MDK_TGINV: return s.tginv;
+    case MDK_TGDOT: return s.tgdot;
+    case MDK_TGID: return s.tgid;
+    case MDK_FB: return s.fb;
+    default: return nullptr;
+  }
+}
+
+extern "C" int mdk_set_tensor(void *h, int which, const double *t9) {
+  mdk_ctx *c = H(h);
+  if (!c || !t9 || which < 0 || which >= MDK_NTENSORS) return set_err(MDK_ERR_BAD_ARG, "bad tensor");
+  BoxState s;
+  RCK(read_box(c, &s));
+  memcpy(box_field(s, which), t9, 9 * sizeof(double));
+  if (which == MDK_B0) {
+    const double *b = s.b0;
+    s.vol = TT(b, 1, 1) * (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) +
+            TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
+            TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
+    c->list_valid = false;
+  }
+  HIPCK(hipMemcpyAsync(c->box + c->cur, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_get_tensor(void *h, int which, double *t9) {
+  mdk_ctx *c = H(h);
+  if (!c || !t9 || which < 0 || which >= MDK_NTENSORS) return set_err(MDK_ERR_BAD_ARG, "bad tensor");
+  BoxState s;
+  RCK(read_box(c, &s));
+  memcpy(t9, box_field(s, which), 9 * sizeof(double));
+  return MDK_OK;
+}
+
+extern "C" int mdk_get_scalars(void *h, double *out) {
+  mdk_ctx *c = H(h);
+  if (!c || !out) return set_err(MDK_ERR_BAD_ARG, "bad scalars");
+  BoxState s;
+  RCK(read_box(c, &s));
+  out[MDK_S_PE] = s.pe; out[MDK_S_KE] = s.ke; out[MDK_S_VOL] = s.vol;
+  out[MDK_S_S1] = s.s1; out[MDK_S_S2] = s.s2; out[MDK_S_S3] = s.s3;
+  out[MDK_S_TE] = s.te; out[MDK_S_H] = s.h; out[MDK_S_TH] = s.th; out[MDK_S_BOXKE] = s.boxke;
+  out[MDK_S_MAXDISP] = sqrt(fmax(s.maxdisp2, 0.0));
+  return MDK_OK;
+}
+
+extern "C" int mdk_set_ntc(void *h, int ntc) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->ntc = ntc;
+  return MDK_OK;
+}
+extern "C" int mdk_get_ntc(void *h, int *ntc) {
+  mdk_ctx *c = H(h);
+  if (!c || !ntc) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  *ntc = c->ntc;
+  return MDK_OK;
+}
+
+extern "C" int mdk_linkup(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  const int n = c->n;
+  k_ref_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, n,
+                                                      c->p.nlcx, c->p.nlcy, c->p.nlcz, c->ic);
+  RCK(check_launch("k_ref_cells"));
+  c->linked = true;
+  return MDK_OK;
+}
+
+extern "C" int mdk_relink(void *h) { return mdk_linkup(h); }
+
+extern "C" int mdk_get_linkcells(void *h, int *l0, int *link, int *ic) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  const int n = c->n, m = c->p.nlcx * c->p.nlcy * c->p.nlcz;
+  std::vector<int> ico(n);
+  int *tmp = c->cell_of;
+  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(tmp, c->ic, c->slot_of, n);
+  RCK(check_launch("k_gather_i"));
+  HIPCK(hipMemcpyAsync(ico.data(), tmp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (ic) memcpy(ic, ico.data(), sizeof(int) * n);
+  // chains in ascending atom order, closed (last -> first): linkup's layout
+  std::vector<int> first(m + 1, 0), last(m + 1, 0);
+  std::vector<int> lk(n + 1, 0);
+  for (int i = 1; i <= n; i++) {
+    int cc = ico[i - 1];
+    if (cc < 1 || cc > m) return set_err(MDK_ERR_STATE, "link cells not set (call linkup)");
+    if (last[cc]) lk[last[cc]] = i; else first[cc] = i;
+    last[cc] = i;
+  }
+  for (int cc = 1; cc <= m; cc++)
+    if (last[cc]) lk[last[cc]] = first[cc];
+  if (l0)
+    for (int cc = 1; cc <= m; cc++) l0[cc - 1] = first[cc];
+  if (link)
+    for (int i = 1; i <= n; i++) link[i - 1] = lk[i];
+  return MDK_OK;
+}
+
+extern "C" int mdk_update_neighbourlist(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (!c->linked) RCK(mdk_linkup(h));
+  const int n = c->n, m = c->p.nlcx * c->p.nlcy * c->p.nlcz;
+  // bin by reference cell (0-based key), in-cell order by original index
+  int *key = c->cell_of;
+  k_minus1<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->ic, key, n);
+  RCK(bin_atoms(c, key, m, c->orig));
+  HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
+  k_half_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, c->ic, c->orig, c->cell_start, c->members, n,
+      c->p.nlcx, c->p.nlcy, c->p.nlcz, c->box + c->cur, c->rnear2_exact, c->pbc, c->p.nnbrs,
+      c->numn_ref, c->nlist_ref, c->dflags);
+  RCK(check_launch("k_half_list"));
+  int fl[4];
+  HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (fl[1]) return set_err(MDK_ERR_NBR_OVERFLOW, "NEIGHBOURLIST: EXCESS NEIGHBOURS! (nnbrs=%d)", c->p.nnbrs);
+  return MDK_OK;
+}
+
+extern "C" int mdk_get_neighbourlist(void *h, int *numn, int *nlist) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (numn) HIPCK(hipMemcpy(numn, c->numn_ref, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+  if (nlist) HIPCK(hipMemcpy(nlist, c->nlist_ref, sizeof(int) * (size_t)c->n * c->p.nnbrs, hipMemcpyDeviceToHost));
+  return MDK_OK;
+}
+
+extern "C" int mdk_get_pairs(void *h, long *count, int *pi, int *pj, long cap) {
+  mdk_ctx *c = H(h);
+  if (!c || !count) return set_err(MDK_ERR_BAD_ARG, "null argument");
+  const int n = c->n, nn = c->p.nnbrs;
+  std::vector<int> numn(n), nl((size_t)n * nn);
+  RCK(mdk_get_neighbourlist(h, numn.data(), nl.data()));
+  long k = 0;
+  for (int i = 0; i < n; i++)
+    for (int q = 0; q < std::min(numn[i], nn); q++) {
+      if (pi && pj && k < cap) {
+        pi[k] = i;
+        pj[k] = nl[(size_t)i * nn + q] - 1;
+      }
+      k++;
+    }
+  *count = k;
+  return MDK_OK;
+}
+
+extern "C" int mdk_force(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  RCK(force_body(c, FIN_FORCE, false));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return check_device_status(c);
+}
+
+extern "C" int mdk_rhoset(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c || !c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
+  if (!c->list_valid) RCK(rebuild_full(c));
+  RCK(launch_density(c, 0));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_embed(void *h, double *pe_embed) {
+  mdk_ctx *c = H(h);
+  if (!c || !c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
+  if (!c->list_valid) RCK(rebuild_full(c));
+  RCK(launch_density(c, 1));
+  if (pe_embed) {
+    std::vector<double> part(c->nblk);
+    HIPCK(hipMemcpyAsync(part.data(), c->part_d, sizeof(double) * c->nblk, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *pe_embed = s;
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_kinten(void *h) { return mdk_force(h); }
+
+extern "C" int mdk_md_init(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  RCK(force_body(c, FIN_INIT, false));
+  RCK(launch_kick(c, 2, nullptr, nullptr));
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->last_rebuild_step = (int)c->step_counter;
+  return check_device_status(c);
+}
+
+extern "C" int mdk_velocityverlet(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  bool skin_due = skin_check(c);
+  RCK(launch_drift(c));
+  RCK(force_body(c, FIN_VV, skin_due));
+  // half kick without the thermodynamic sums' box part (tgid stays lagged)
+  int slot = (int)(c->step_counter & 3);
+  RCK(launch_kick(c, 0, nullptr, c->mon_dev + 2 * slot));
+  HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
+  c->step_counter++;
+  HIPCK(hipStreamSynchronize(c->stream));
+  return check_device_status(c);
+}
+
+extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  k_therm_box<<<1, 64, 0, c->stream>>>(c->box + c->cur, c->p.ivol);
+  RCK(check_launch("k_therm_box"));
+  TimedEv te;
+  tbegin(c, TC_KICK, te);
+  k_kick<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1,
+                                               nullptr, c->part_k);
+  tend(c, te);
+  RCK(check_launch("k_kick"));
+  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, c->box + c->cur, c->p.deltat,
+                                                  c->p.bdel2, c->p.press, nullptr, nullptr);
+  RCK(check_launch("k_finalize_kick"));
+  BoxState s;
+  RCK(read_box(c, &s));
+  if (s1) *s1 = s.s1;
+  if (s2) *s2 = s.s2;
+  if (s3) *s3 = s.s3;
+  if (s.status) return set_err(s.status, "singular box matrix in matrix_invert");
+  return MDK_OK;
+}
+
+extern "C" int mdk_md_steps(void *h, int nsteps, double *trace) {
+  mdk_ctx *c = H(h);
+  if (!c || nsteps < 0) return set_err(MDK_ERR_BAD_ARG, "bad md_steps arguments");
+  if (trace && nsteps > c->trace_cap) {
+    if (c->trace_dev) hipFree(c->trace_dev);
+    HIPCK(hipMalloc(&c->trace_dev, sizeof(double) * 6 * (size_t)nsteps));
+    c->trace_cap = nsteps;
+  }
+  for (int s = 0; s < nsteps; s++) {
+    RCK(md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true));
+    // relink when the reference's ntc counter wraps (moldin.F90:791)
+    if (c->ntc == 0) RCK(mdk_relink(h));
+  }
+  if (trace && nsteps > 0)
+    HIPCK(hipMemcpyAsync(trace, c->trace_dev, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return check_device_status(c);
+}
+
+extern "C" int mdk_stats(void *h, long *rebuilds, long *full_pairs) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (rebuilds) *rebuilds = c->rebuilds;
+  if (full_pairs) {
+    std::vector<int> nn(c->n);
+    HIPCK(hipMemcpy(nn.data(), c->numn, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+    long s = 0;
+    for (int v : nn) s += v;
+    *full_pairs = s;
+  }
+  return MDK_OK;
+}
+
+extern "C" int mdk_timing(void *h, int enable) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->timing = enable != 0;
+  if (enable) {
+    for (int k = 0; k < TC_NCLASS; k++) { c->t_ms[k] = 0.0; c->t_cnt[k] = 0; }
+    for (auto &e : c->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    c->evs.clear();
+  }
+  return MDK_OK;
+}
+
+extern "C" int mdk_timing_read(void *h, double *ms, long *launches, int nclasses) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  HIPCK(hipStreamSynchronize(c->stream));
+  for (auto &e : c->evs) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, e.a, e.b);
+    c->t_ms[e.cls] += t;
+    c->t_cnt[e.cls] += 1;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  c->evs.clear();
+  for (int k = 0; k < nclasses && k < TC_NCLASS; k++) {
+    if (ms) ms[k] = c->t_ms[k];
+    if (launches) launches[k] = c->t_cnt[k];
+  }
+  return MDK_OK;
+}
+
+extern "C" int mdk_synchronize(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}

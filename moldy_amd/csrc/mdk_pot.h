@@ -1,0 +1,228 @@
+// mdk_pot.h -- Fe-Fe Finnis-Sinclair potential on the device (gfx950).
+//
+// Functions of src/fetransmetal.F90 for the Fe-Fe pair (find_coeff factors are
+// exactly 1 for Fe-Fe, :831,937-940):
+//   r > 1.9 A         V  = sum_k a_k (r_k - r)^3 H(r_k - r)          (:133-139)
+//   0.9 < r <= 1.9 A  V  = exp(cubic Hermite of log V)  joining       (:511-551)
+//                          Biersack (0.9-1e-12) to the knot form (1.9+1e-12)
+//   r < 0.9 A         V  = Biersack-Ziegler universal screening        (:414-454)
+//   phi = sum_k A_k (R_k - r)^3 H(R_k - r) above its maximum r*, flat below (:256-265)
+//   F(rho) = -sqrt(max(rho,0)), F'(rho) = -1/(2 sqrt(rho)) for rho > 0 (:375-404)
+// and the uniform-grid linear-interpolation tables of src/lookup_m.F90:307-394.
+//
+// Everything is templated on the arithmetic type: double for the parity form,
+// float for the reduced-precision form (the caller accumulates in double).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define MDK_MAXV 8
+#define MDK_MAXP 4
+
+struct PotDev {
+  int nv, np, use_lookup, nlookup;
+  double av[MDK_MAXV], rv[MDK_MAXV], ap[MDK_MAXP], rp[MDK_MAXP];
+  double bc1, bc2;          // Biersack and knot-form join radii (REAL(4) 0.9 and 1.9)
+  double sc[4];             // spline_coeff(1:4): V, V' at bc1-eps and at bc2+eps
+  double phi_cut_r, phi_cut_v;
+  double bier_a, bier_zed;  // screening length (a REAL(4) value) and Z1 Z2 e^2/(4 pi eps0)
+  double r_int;             // outermost knot: V, V', phi, phi' vanish beyond it
+  // lookup tables (src/lookup_m.F90): [0,6] A for V,V',phi,phi'; rho in [0,1e4]
+  double r_min, r_max, vp_range, vp_reciprange, vp_recipnlkp, vp_recipstep;
+  double rho_min, rho_max, e_range, e_reciprange, e_recipnlkp, e_recipstep;
+  const double *tab_v, *tab_dv, *tab_p, *tab_dp, *tab_e, *tab_de;
+  // float copies of the knot data for the reduced-precision form
+  float avf[MDK_MAXV], rvf[MDK_MAXV], apf[MDK_MAXP], rpf[MDK_MAXP];
+  float bc1f, bc2f, phi_cut_rf, phi_cut_vf, r_intf;
+  // pre-derived join constants: log(V) and V'/V at both ends
+  double jx1, jv1, jx2, jv2;
+  float jx1f, jv1f, jx2f, jv2f, bier_af, bier_zedf;
+};
+
+template <typename T> __device__ __forceinline__ T mdk_exp(T x);
+template <> __device__ __forceinline__ double mdk_exp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float mdk_exp<float>(float x) { return __expf(x); }
+
+template <typename T> struct PotView;
+template <> struct PotView<double> {
+  const PotDev &P;
+  __device__ PotView(const PotDev &p) : P(p) {}
+  __device__ double av(int k) const { return P.av[k]; }
+  __device__ double rv(int k) const { return P.rv[k]; }
+  __device__ double ap(int k) const { return P.ap[k]; }
+  __device__ double rp(int k) const { return P.rp[k]; }
+  __device__ double bc1() const { return P.bc1; }
+  __device__ double bc2() const { return P.bc2; }
+  __device__ double pcr() const { return P.phi_cut_r; }
+  __device__ double pcv() const { return P.phi_cut_v; }
+  __device__ double jx1() const { return P.jx1; }
+  __device__ double jv1() const { return P.jv1; }
+  __device__ double jx2() const { return P.jx2; }
+  __device__ double jv2() const { return P.jv2; }
+  __device__ double ba() const { return P.bier_a; }
+  __device__ double bz() const { return P.bier_zed; }
+};
+template <> struct PotView<float> {
+  const PotDev &P;
+  __device__ PotView(const PotDev &p) : P(p) {}
+  __device__ float av(int k) const { return P.avf[k]; }
+  __device__ float rv(int k) const { return P.rvf[k]; }
+  __device__ float ap(int k) const { return P.apf[k]; }
+  __device__ float rp(int k) const { return P.rpf[k]; }
+  __device__ float bc1() const { return P.bc1f; }
+  __device__ float bc2() const { return P.bc2f; }
+  __device__ float pcr() const { return P.phi_cut_rf; }
+  __device__ float pcv() const { return P.phi_cut_vf; }
+  __device__ float jx1() const { return P.jx1f; }
+  __device__ float jv1() const { return P.jv1f; }
+  __device__ float jx2() const { return P.jx2f; }
+  __device__ float jv2() const { return P.jv2f; }
+  __device__ float ba() const { return P.bier_af; }
+  __device__ float bz() const { return P.bier_zedf; }
+};
+
+// Biersack-Ziegler V and V' (src/fetransmetal.F90:414-454)
+template <typename T>
+__device__ __noinline__ void mdk_biersack(const PotView<T> &Q, T r, T &v, T &dv) {
+  const T a = Q.ba(), z = Q.bz();
+  const T x = r / a, ir = T(1) / r;
+  const T e1 = mdk_exp<T>(-T(3.2) * x), e2 = mdk_exp<T>(-T(0.9423) * x);
+  const T e3 = mdk_exp<T>(-T(0.4029) * x), e4 = mdk_exp<T>(-T(0.2016) * x);
+  // The reference evaluates the value with REAL(4)-rounded literals.
+  const T ph = T((double)0.1818f) * mdk_exp<T>(-T((double)3.2f) * x) +
+               T((double)0.5099f) * mdk_exp<T>(-T((double)0.9423f) * x) +
+               T((double)0.2802f) * mdk_exp<T>(-T((double)0.4029f) * x) +
+               T((double)0.02817f) * mdk_exp<T>(-T((double)0.2016f) * x);
+  v = z * ph / r;
+  const T s = (ir + T(3.2) / a) * T(0.1818) * e1 + (ir + T(0.9423) / a) * T(0.5099) * e2 +
+              (ir + T(0.4029) / a) * T(0.2802) * e3 + (ir + T(0.2016) / a) * T(0.02817) * e4;
+  dv = -(z * ir) * s;
+}
+
+// log-space cubic Hermite join between bc1 and bc2 (src/fetransmetal.F90:511-551)
+template <typename T>
+__device__ __noinline__ void mdk_join(const PotView<T> &Q, T r, T &v, T &dv) {
+  const T a1 = Q.bc1(), a2 = Q.bc2(), h = a2 - a1;
+  const T t = (r - a1) / h;
+  const T t2 = t * t, t3 = t2 * t;
+  const T sp = (T(2) * t3 - T(3) * t2 + T(1)) * Q.jx1() + (t3 - T(2) * t2 + t) * Q.jv1() * h +
+               (-(T(2) * t3) + T(3) * t2) * Q.jx2() + (t3 - t2) * Q.jv2() * h;
+  T ds = (T(6) * t2 - T(6) * t) * Q.jx1() + (T(3) * t2 - T(4) * t + T(1)) * Q.jv1() * h +
+         (-(T(6) * t2) + T(6) * t) * Q.jx2() + (T(3) * t2 - T(2) * t) * Q.jv2() * h;
+  ds = ds / h;
+  const T e = mdk_exp<T>(sp);
+  v = e;
+  dv = ds * e;
+}
+
+// Pair function V and V' (exact mode).
+template <typename T>
+__device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
+  PotView<T> Q(P);
+  if (r > Q.bc2()) {
+    T s = T(0), d1 = T(0);
+#pragma unroll
+    for (int k = 0; k < MDK_MAXV; k++) {
+      if (k < P.nv) {
+        const T d = Q.rv(k) - r;
+        const T hd = d < T(0) ? T(0) : T(1);
+        const T ad2 = Q.av(k) * (d * d) * hd;
+        s = s + ad2 * d;
+        d1 = d1 - T(3) * ad2;
+      }
+    }
+    v = s;
+    dv = d1;
+  } else if (r > Q.bc1()) {
+    mdk_join<T>(Q, r, v, dv);
+  } else if (r < Q.bc1()) {
+    mdk_biersack<T>(Q, r, v, dv);
+  } else {
+    v = T(0);
+    dv = T(0);
+  }
+}
+
+// Density function phi (exact mode).
+template <typename T>
+__device__ __forceinline__ T mdk_phi(const PotDev &P, T r) {
+  PotView<T> Q(P);
+  if (r < Q.pcr()) return Q.pcv();
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < MDK_MAXP; k++) {
+    if (k < P.np) {
+      const T d = Q.rp(k) - r;
+      const T hd = d < T(0) ? T(0) : T(1);
+      s = s + Q.ap(k) * (d * d * d) * hd;
+    }
+  }
+  return s;
+}
+
+// Density derivative phi' (exact mode).
+template <typename T>
+__device__ __forceinline__ T mdk_dphi(const PotDev &P, T r) {
+  PotView<T> Q(P);
+  if (r < Q.pcr()) return T(0);
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < MDK_MAXP; k++) {
+    if (k < P.np) {
+      const T d = Q.rp(k) - r;
+      const T hd = d < T(0) ? T(0) : T(1);
+      s = s - T(3) * Q.ap(k) * (d * d) * hd;
+    }
+  }
+  return s;
+}
+
+// Linear-interpolation lookups in the reference's index arithmetic
+// (src/lookup_m.F90:307-394); no FMA so the result is the reference's.
+__device__ __forceinline__ double mdk_lk_vp(const PotDev &P, const double *t, double r) {
+#pragma clang fp contract(off)
+  if (r >= P.r_max) return t[P.nlookup];
+  int lo = (int)((double)P.nlookup * (r - P.r_min) * P.vp_reciprange);
+  lo = min(max(lo, 0), P.nlookup - 1);  // r < r_min (or NaN) would index outside the table
+  const double rl = P.r_min + P.vp_range * (double)lo * P.vp_recipnlkp;
+  const double dr = (r - rl) * P.vp_recipstep;
+  return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
+}
+__device__ __forceinline__ double mdk_lk_e(const PotDev &P, const double *t, double rho) {
+#pragma clang fp contract(off)
+  if (rho >= P.rho_max) return t[P.nlookup];
+  int lo = (int)((double)P.nlookup * (rho - P.rho_min) * P.e_reciprange);
+  lo = min(max(lo, 0), P.nlookup - 1);
+  const double rl = P.rho_min + P.e_range * (double)lo * P.e_recipnlkp;
+  const double dr = (rho - rl) * P.e_recipstep;
+  return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
+}
+
+// Dispatchers (potential_m.F90:288-417): exact or table.
+template <typename T>
+__device__ __forceinline__ void mdk_pair_vdv(const PotDev &P, T r, T &v, T &dv) {
+  if (P.use_lookup) {
+    v = (T)mdk_lk_vp(P, P.tab_v, (double)r);
+    dv = (T)mdk_lk_vp(P, P.tab_dv, (double)r);
+  } else {
+    mdk_vee<T>(P, r, v, dv);
+  }
+}
+template <typename T> __device__ __forceinline__ T mdk_pair_phi(const PotDev &P, T r) {
+  if (P.use_lookup) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
+  return mdk_phi<T>(P, r);
+}
+template <typename T> __device__ __forceinline__ T mdk_pair_dphi(const PotDev &P, T r) {
+  if (P.use_lookup) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
+  return mdk_dphi<T>(P, r);
+}
+// Embedding F and F' in FP64 (per atom, cheap).
+__device__ __forceinline__ void mdk_embed_f(const PotDev &P, double rho, double &f, double &df) {
+  if (P.use_lookup) {
+    f = mdk_lk_e(P, P.tab_e, rho);
+    df = mdk_lk_e(P, P.tab_de, rho);
+  } else {
+    f = -1.0 * sqrt(rho > 0.0 ? rho : 0.0);
+    df = 0.0;
+    if (rho > 0.0) df = -0.5 / sqrt(rho);
+  }
+}

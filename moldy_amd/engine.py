@@ -1,0 +1,320 @@
+"""ctypes binding of libmoldy_hip.so (the C ABI declared in include/moldy_hip.h).
+
+This is the Python mirror of the reference's Fortran module procedures for the
+hot path (linkcell_m, neighbourlist_m, potential_m, dynamics_m,
+parrinellorahman_m, quench_m::velocityverlet).  Host arrays are numpy arrays in
+the reference's atom order.  There is no CPU fallback: when the HIP library is
+missing or no GPU is visible every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import lattice
+from .potential import FE_POTENTIAL
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmoldy_hip.so")
+
+MDK_OK, MDK_ERR_HIP, MDK_ERR_NBR_OVERFLOW, MDK_ERR_BAD_ARG = 0, 1, 2, 3
+MDK_ERR_SINGULAR_BOX, MDK_ERR_STATE, MDK_ERR_SKIN, MDK_ERR_NO_DEVICE = 4, 5, 6, 7
+PREC_FP64, PREC_FP32 = 0, 1
+
+FIELDS = ["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2", "fx", "fy", "fz",
+          "rho", "afrho", "dafrho", "mass", "del2"]
+TENSORS = ["b0", "b1", "b2", "tk", "tc", "tg", "tp", "tginv", "tgdot", "tgid", "fb"]
+SCALARS = ["pe", "ke", "vol", "s1", "s2", "s3", "te", "h", "th", "boxke", "maxdisp"]
+
+# Every symbol include/moldy_hip.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "mdk_create", "mdk_destroy", "mdk_last_error", "mdk_set_potential", "mdk_set_lookup",
+    "mdk_potential_constants", "mdk_eval_potential", "mdk_upload", "mdk_download",
+    "mdk_upload_species", "mdk_set_tensor", "mdk_get_tensor", "mdk_get_scalars", "mdk_set_ntc",
+    "mdk_get_ntc", "mdk_linkup", "mdk_relink", "mdk_get_linkcells", "mdk_update_neighbourlist",
+    "mdk_get_neighbourlist", "mdk_get_pairs", "mdk_force", "mdk_rhoset", "mdk_embed",
+    "mdk_kinten", "mdk_md_init", "mdk_velocityverlet", "mdk_update_therm_avs", "mdk_md_steps",
+    "mdk_stats", "mdk_timing", "mdk_timing_read", "mdk_synchronize",
+]
+
+
+class MdkParams(C.Structure):
+    _fields_ = [("nm", C.c_int), ("ivol", C.c_int), ("nnbrs", C.c_int), ("ntcm", C.c_int),
+                ("nlcx", C.c_int), ("nlcy", C.c_int), ("nlcz", C.c_int),
+                ("precision", C.c_int), ("rebuild", C.c_int), ("max_full", C.c_int),
+                ("deltat", C.c_double), ("rnear", C.c_double), ("rskin", C.c_double),
+                ("press", C.c_double), ("bdel2", C.c_double)]
+
+
+class MdkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mdk error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the HIP library; raises when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libmoldy_hip.so not built at {path}: run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp, dp, ip, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+    sig = {
+        "mdk_create": [C.POINTER(C.c_void_p), C.POINTER(MdkParams)],
+        "mdk_destroy": [vp],
+        "mdk_set_potential": [vp, C.c_int, dp, dp, C.c_int, dp, dp],
+        "mdk_set_lookup": [vp, C.c_int],
+        "mdk_potential_constants": [vp, dp],
+        "mdk_eval_potential": [vp, C.c_int, dp, dp, C.c_int],
+        "mdk_upload": [vp, C.c_int, dp],
+        "mdk_download": [vp, C.c_int, dp],
+        "mdk_upload_species": [vp, ip],
+        "mdk_set_tensor": [vp, C.c_int, dp],
+        "mdk_get_tensor": [vp, C.c_int, dp],
+        "mdk_get_scalars": [vp, dp],
+        "mdk_set_ntc": [vp, C.c_int],
+        "mdk_get_ntc": [vp, ip],
+        "mdk_linkup": [vp], "mdk_relink": [vp],
+        "mdk_get_linkcells": [vp, ip, ip, ip],
+        "mdk_update_neighbourlist": [vp],
+        "mdk_get_neighbourlist": [vp, ip, ip],
+        "mdk_get_pairs": [vp, lp, ip, ip, C.c_long],
+        "mdk_force": [vp], "mdk_rhoset": [vp], "mdk_embed": [vp, dp], "mdk_kinten": [vp],
+        "mdk_md_init": [vp], "mdk_velocityverlet": [vp],
+        "mdk_update_therm_avs": [vp, dp, dp, dp],
+        "mdk_md_steps": [vp, C.c_int, dp],
+        "mdk_stats": [vp, lp, lp],
+        "mdk_timing": [vp, C.c_int],
+        "mdk_timing_read": [vp, dp, lp, C.c_int],
+        "mdk_synchronize": [vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.mdk_last_error.argtypes = []
+    L.mdk_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(rc):
+    if rc != MDK_OK:
+        raise MdkError(rc, load_library().mdk_last_error().decode(errors="replace"))
+
+
+FS_TO_TIME = lattice.FS_TO_TIME
+GPA_TO_PRESS = 1.0 / ((1.602176487e-19 / (1e-10 * 1e-10 * 1e-10)) / 1.0e9)
+
+
+def link_cells_for(b0: np.ndarray, rnear: float) -> tuple[int, int, int]:
+    """moldin.F90:252-257: int(b0(i,i)/rnear), at least 3."""
+    return tuple(max(3, int(b0[i, i] / rnear)) for i in range(3))
+
+
+class Engine:
+    """One simulated periodic cell on the GPU (reference semantics)."""
+
+    def __init__(self, sysdef: lattice.System, ivol: int = 1, rpad: float = 0.3,
+                 rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10, press_gpa: float = 0.0,
+                 boxmass: float = 50.0, precision: int = PREC_FP64, rebuild: int = 0,
+                 skin: float | None = None, max_full: int = 0, coeffs: dict | None = None,
+                 lookup: int = 0):
+        L = load_library()
+        self.n = sysdef.n
+        n = self.n
+        rcut = max(rcut, 6.0)                       # moldin.F90:226 with fetransmetal range 6
+        self.rnear = rcut + rpad
+        dt = sysdef.deltat_fs * FS_TO_TIME
+        bmass = n * boxmass
+        p = MdkParams()
+        p.nm, p.ivol, p.nnbrs, p.ntcm = n, ivol, nnbrs, ntcm
+        p.nlcx, p.nlcy, p.nlcz = link_cells_for(sysdef.b0, self.rnear)
+        p.precision, p.rebuild, p.max_full = precision, rebuild, max_full
+        p.deltat, p.rnear = dt, self.rnear
+        p.rskin = skin if skin is not None else rpad
+        p.press = press_gpa * GPA_TO_PRESS
+        p.bdel2 = dt * dt / (2.0 * bmass)
+        self.params = p
+        h = C.c_void_p()
+        _check(L.mdk_create(C.byref(h), C.byref(p)))
+        self.h = h
+        self.L = L
+        c = coeffs if coeffs is not None else FE_POTENTIAL
+        self.set_potential(c)
+        if lookup:
+            _check(L.mdk_set_lookup(h, int(lookup)))
+        s = np.ascontiguousarray(sysdef.s)
+        x1 = np.ascontiguousarray(sysdef.x1)
+        for k, nm in enumerate(("x0", "y0", "z0")):
+            self.upload(nm, s[:, k])
+        for k, nm in enumerate(("x1", "y1", "z1")):
+            self.upload(nm, x1[:, k])
+        zero = np.zeros(n)
+        for nm in ("x2", "y2", "z2"):
+            self.upload(nm, zero)
+        mass = np.ascontiguousarray(sysdef.mass, dtype=np.float64)
+        self.upload("mass", mass)
+        # moldin.F90:294-300 (0.01 is a REAL(4) literal)
+        del2 = np.where(mass > float(np.float32(0.01)), dt * dt / (2.0 * mass), 0.0)
+        self.upload("del2", del2)
+        _check(L.mdk_upload_species(h, _ip(np.ascontiguousarray(sysdef.z, dtype=np.int32))))
+        self.set_tensor("b0", sysdef.b0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mdk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- potential
+    def set_potential(self, c: dict):
+        av, rv, ap, rp = (np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp"))
+        _check(self.L.mdk_set_potential(self.h, len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp)))
+
+    def potential_constants(self) -> np.ndarray:
+        out = np.empty(8)
+        _check(self.L.mdk_potential_constants(self.h, _dp(out)))
+        return out
+
+    def eval_potential(self, which: str, x) -> np.ndarray:
+        k = ["vee", "dvee", "phi", "dphi", "emb", "demb"].index(which)
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        y = np.empty_like(x)
+        _check(self.L.mdk_eval_potential(self.h, k, _dp(x), _dp(y), len(x)))
+        return y
+
+    # --- state
+    def upload(self, name: str, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.n,)
+        _check(self.L.mdk_upload(self.h, FIELDS.index(name), _dp(v)))
+
+    def array(self, name: str) -> np.ndarray:
+        out = np.empty(self.n)
+        _check(self.L.mdk_download(self.h, FIELDS.index(name), _dp(out)))
+        return out
+
+    def set_tensor(self, name: str, t):
+        t = np.asfortranarray(np.asarray(t, dtype=np.float64)).ravel(order="F").copy()
+        _check(self.L.mdk_set_tensor(self.h, TENSORS.index(name), _dp(t)))
+
+    def tensor(self, name: str) -> np.ndarray:
+        out = np.empty(9)
+        _check(self.L.mdk_get_tensor(self.h, TENSORS.index(name), _dp(out)))
+        return out.reshape(3, 3, order="F")
+
+    def scalars(self) -> dict:
+        out = np.empty(len(SCALARS))
+        _check(self.L.mdk_get_scalars(self.h, _dp(out)))
+        return dict(zip(SCALARS, out.tolist()))
+
+    def scalar(self, name: str) -> float:
+        return self.scalars()[name]
+
+    @property
+    def ntc(self) -> int:
+        v = C.c_int()
+        _check(self.L.mdk_get_ntc(self.h, C.byref(v)))
+        return v.value
+
+    @ntc.setter
+    def ntc(self, v: int):
+        _check(self.L.mdk_set_ntc(self.h, int(v)))
+
+    # --- reference procedures
+    def call(self, name: str):
+        _check(getattr(self.L, "mdk_" + name)(self.h))
+
+    def linkup(self):
+        self.call("linkup")
+
+    def relink(self):
+        self.call("relink")
+
+    def update_neighbourlist(self):
+        self.call("update_neighbourlist")
+
+    def force(self):
+        self.call("force")
+
+    def md_init(self):
+        self.call("md_init")
+
+    def velocityverlet(self):
+        self.call("velocityverlet")
+
+    def update_therm_avs(self):
+        s1, s2, s3 = C.c_double(), C.c_double(), C.c_double()
+        _check(self.L.mdk_update_therm_avs(self.h, C.byref(s1), C.byref(s2), C.byref(s3)))
+        return s1.value, s2.value, s3.value
+
+    def md_steps(self, nsteps: int, trace: bool = True) -> np.ndarray | None:
+        tr = np.empty((max(nsteps, 1), 6)) if trace else None
+        _check(self.L.mdk_md_steps(self.h, int(nsteps), _dp(tr) if trace else None))
+        return tr[:nsteps] if trace else None
+
+    def linkcells(self):
+        m = self.params.nlcx * self.params.nlcy * self.params.nlcz
+        l0 = np.empty(m, dtype=np.int32)
+        link = np.empty(self.n, dtype=np.int32)
+        ic = np.empty(self.n, dtype=np.int32)
+        _check(self.L.mdk_get_linkcells(self.h, _ip(l0), _ip(link), _ip(ic)))
+        return l0, link, ic
+
+    def neighbourlist(self):
+        nn = self.params.nnbrs
+        numn = np.empty(self.n, dtype=np.int32)
+        nlist = np.empty(self.n * nn, dtype=np.int32)
+        _check(self.L.mdk_get_neighbourlist(self.h, _ip(numn), _ip(nlist)))
+        return numn, nlist.reshape(self.n, nn)
+
+    def pairs(self) -> np.ndarray:
+        cnt = C.c_long()
+        _check(self.L.mdk_get_pairs(self.h, C.byref(cnt), None, None, 0))
+        k = cnt.value
+        pi = np.empty(max(k, 1), dtype=np.int32)
+        pj = np.empty(max(k, 1), dtype=np.int32)
+        _check(self.L.mdk_get_pairs(self.h, C.byref(cnt), _ip(pi), _ip(pj), k))
+        return np.stack([pi[:k], pj[:k]], axis=1)
+
+    def stats(self) -> tuple[int, int]:
+        a, b = C.c_long(), C.c_long()
+        _check(self.L.mdk_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def timing(self, enable: bool):
+        _check(self.L.mdk_timing(self.h, 1 if enable else 0))
+
+    def timing_read(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, dtype=np.int64)
+        _check(self.L.mdk_timing_read(self.h, _dp(ms), cnt.ctypes.data_as(C.POINTER(C.c_long)), 5))
+        return dict(zip(["drift", "density", "force", "kick", "rebuild"], zip(ms.tolist(), cnt.tolist())))
+
+    def synchronize(self):
+        self.call("synchronize")
+
+    def forces_cart(self) -> np.ndarray:
+        f = np.stack([self.array("fx"), self.array("fy"), self.array("fz")], axis=1)
+        return f @ self.tensor("b0").T

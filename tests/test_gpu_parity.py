@@ -1,0 +1,274 @@
+"""GPU parity tests: libmoldy_hip.so (through its C ABI) against the oracle and
+the reference's golden vectors.
+
+Tolerances (stated per quantity):
+  * neighbour sets: bit-exact as unordered integer pair sets;
+  * FP64 form: forces, densities, F', virial within 1e-10 relative (norm-wise:
+    max|a-b| / max|b|); energies within 1e-12 relative;
+  * lookup-mode potential values: bit-exact;
+  * FP32 form: max force error <= 1e-5 eV/A against the FP64 oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden_system, load_golden, run_kwargs
+from moldy_amd import engine, lattice
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["c0_nve_2000", "npt_tric_1024", "hot_1024"]
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def make_engine(F, **kw):
+    args = run_kwargs(F)
+    args.update(kw)
+    if int(F["run_uselookup"]):
+        args.setdefault("lookup", 5000)
+    return engine.Engine(golden_system(F), **args)
+
+
+def test_gpu_potential_constants_and_functions(oracle):
+    F = load_golden("c0_nve_2000")
+    say("potential functions, exact mode")
+    eng = make_engine(F)
+    pot = oracle.Potential()
+    c = eng.potential_constants()
+    np.testing.assert_array_equal(c[2:6], pot.spline_coeff)
+    assert (c[6], c[7]) == pot.phi_cut
+    r = F["rgrid"]
+    for fn in ("vee", "dvee", "phi", "dphi"):
+        g = eng.eval_potential(fn, r)
+        ref = F[fn]
+        knot = r > float(np.float32(1.9))
+        assert rel(g[knot], ref[knot]) < 1e-13, fn
+        short = ~knot
+        err = np.abs(g[short] - ref[short]) / np.maximum(np.abs(ref[short]), 1e-300)
+        assert np.max(err) < 1e-11, (fn, np.max(err))
+        assert np.all(g[ref == 0.0] == 0.0), fn
+    for fn in ("emb", "demb"):
+        g = eng.eval_potential(fn, F["rhogrid"])
+        np.testing.assert_allclose(g, F[fn], rtol=1e-15, atol=0)
+    eng.close()
+
+
+def test_gpu_lookup_functions_bit_exact():
+    F = load_golden("lookup_2000")
+    say("potential functions, lookup mode")
+    eng = make_engine(F)
+    for fn in ("vee", "dvee", "phi", "dphi"):
+        np.testing.assert_array_equal(eng.eval_potential(fn, F["rgrid"]), F[fn], err_msg=fn)
+    for fn in ("emb", "demb"):
+        np.testing.assert_array_equal(eng.eval_potential(fn, F["rhogrid"]), F[fn], err_msg=fn)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", CASES + ["lookup_2000"])
+def test_gpu_reference_half_list_bit_exact(oracle, case):
+    F = load_golden(case)
+    say("half list", case)
+    eng = make_engine(F)
+    eng.linkup()
+    eng.update_neighbourlist()
+    pairs = eng.pairs()
+    assert len(pairs) == int(F["npairs0"])
+    assert oracle.pairset_digest(pairs) == str(F["pairs0_sha256"])
+    numn, nlist = eng.neighbourlist()
+    assert int(numn.sum()) == int(F["npairs0"])
+    # link cells as linkup leaves them
+    S = oracle.OracleSystem(golden_system(F), oracle.Potential(), **run_kwargs(F))
+    S.call("linkup")
+    l0, link, ic = eng.linkcells()
+    np.testing.assert_array_equal(ic, S.ic())
+    for c in range(len(l0)):  # closed chains visiting exactly the cell's atoms, ascending
+        if l0[c] == 0:
+            continue
+        members, i = [], l0[c]
+        while True:
+            members.append(i)
+            i = link[i - 1]
+            if i == l0[c]:
+                break
+        assert members == sorted(members)
+        assert np.all(ic[np.array(members) - 1] == c + 1)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", CASES + ["lookup_2000"])
+def test_gpu_first_force_matches_reference(case):
+    F = load_golden(case)
+    say("first force", case)
+    eng = make_engine(F)
+    eng.linkup()
+    eng.md_init()
+    for nm, key in [("rho", "rho0"), ("dafrho", "dafrho0"), ("afrho", "afrho0")]:
+        assert rel(eng.array(nm), F[key]) < 1e-10, nm
+    f = np.stack([eng.array("fx"), eng.array("fy"), eng.array("fz")], axis=1)
+    fr = np.stack([F["fx0"], F["fy0"], F["fz0"]], axis=1)
+    assert rel(f, fr) < 1e-10
+    x2 = np.stack([eng.array("x2"), eng.array("y2"), eng.array("z2")], axis=1)
+    assert rel(x2, np.stack([F["x20"], F["y20"], F["z20"]], axis=1)) < 1e-10
+    s = eng.scalars()
+    assert abs(s["pe"] - float(F["pe0"])) < 1e-12 * abs(float(F["pe0"]))
+    for nm in ("tp", "tk", "fb"):
+        assert rel(eng.tensor(nm), F[nm + "0"]) < 1e-10, nm
+    np.testing.assert_array_equal(eng.tensor("tc"), F["tc0"])
+    assert rel(eng.tensor("tgid"), F["tgid0"]) < 1e-10 or np.max(np.abs(F["tgid0"])) == 0
+    eng.close()
+
+
+@pytest.mark.parametrize("case", CASES + ["lookup_2000"])
+def test_gpu_md_trajectory_matches_reference(case):
+    F = load_golden(case)
+    say("trajectory", case)
+    eng = make_engine(F)
+    eng.linkup()
+    eng.md_init()
+    tr = eng.md_steps(int(F["run_nsteps"]))
+    ref = F["trace"]
+    assert np.max(np.abs(tr[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])) < 1e-11   # PE
+    assert np.max(np.abs(tr[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])) < 1e-10   # KE
+    assert np.max(np.abs(tr[:, 5] - ref[:, 4]) / np.abs(ref[:, 4])) < 1e-13   # volume
+    for nm in ("x0", "y0", "z0"):
+        assert np.max(np.abs(eng.array(nm) - F[nm + "_f"])) < 1e-13, nm
+    v = np.stack([eng.array(n) for n in ("x1", "y1", "z1")], axis=1)
+    vr = np.stack([F[n + "_f"] for n in ("x1", "y1", "z1")], axis=1)
+    assert rel(v, vr) < 1e-10
+    f = np.stack([eng.array(n) for n in ("fx", "fy", "fz")], axis=1)
+    fr = np.stack([F[n + "_f"] for n in ("fx", "fy", "fz")], axis=1)
+    assert rel(f, fr) < 1e-9
+    for nm in ("b0", "b1"):
+        assert np.max(np.abs(eng.tensor(nm) - F[nm + "_f"])) <= 1e-12 * np.max(np.abs(F["b0_f"])), nm
+    assert rel(eng.tensor("tp"), F["tp_f"]) < 1e-9
+    eng.close()
+
+
+def test_gpu_step_api_matches_md_steps():
+    """velocityverlet + update_therm_avs + relink (the Fortran call order)
+    gives the same state as the fused md_steps path."""
+    F = load_golden("npt_tric_1024")
+    say("step api vs fused")
+    a, b = make_engine(F), make_engine(F)
+    for e in (a, b):
+        e.linkup()
+        e.md_init()
+    tr = a.md_steps(6)
+    for k in range(6):
+        b.velocityverlet()
+        s1, s2, s3 = b.update_therm_avs()
+        if b.ntc == 0:
+            b.relink()
+        assert abs(b.scalar("ke") - tr[k, 1]) <= 1e-12 * tr[k, 1]
+    for nm in ("x0", "x1", "fx"):
+        np.testing.assert_array_equal(a.array(nm), b.array(nm))
+    a.close()
+    b.close()
+
+
+def test_gpu_fp32_form_force_error(oracle):
+    F = load_golden("c0_nve_2000")
+    say("fp32 form")
+    eng = make_engine(F, precision=engine.PREC_FP32)
+    eng.linkup()
+    eng.md_init()
+    f32 = eng.forces_cart()
+    S = oracle.OracleSystem(golden_system(F), oracle.Potential(), **run_kwargs(F))
+    S.call("linkup")
+    S.call("md_init")
+    f64 = S.forces_cart()
+    err = np.max(np.abs(f32 - f64))
+    say(f"fp32 max force error {err:.3e} eV/A, pe rel err "
+        f"{abs(eng.scalar('pe') - S.scalar('pe')) / abs(S.scalar('pe')):.3e}")
+    assert err <= 1e-5
+    assert abs(eng.scalar("pe") - S.scalar("pe")) < 1e-7 * abs(S.scalar("pe"))
+    assert rel(eng.tensor("tp"), S.tensor("tp")) < 1e-5
+    eng.close()
+
+
+def test_gpu_neighbour_overflow_reported():
+    sd = lattice.make_system(nc=6, sigma=0.05, seed=1)
+    say("overflow")
+    eng = engine.Engine(sd, nnbrs=10)
+    eng.linkup()
+    with pytest.raises(engine.MdkError) as ei:
+        eng.update_neighbourlist()
+    assert ei.value.code == engine.MDK_ERR_NBR_OVERFLOW
+    eng.close()
+
+
+def test_gpu_skin_rebuild_hot_liquid(oracle):
+    """Displacement-triggered rebuilds of the pruned full list reproduce a
+    reference run that rebuilds its (rnear) list every step."""
+    sd = lattice.make_system(nc=6, a=lattice.liquid_lattice_parameter(7.0), sigma=0.15,
+                             temp=2500.0, deltat_fs=0.5, seed=21)
+    say("skin rebuild")
+    eng = engine.Engine(sd, ivol=1, ntcm=1000, rebuild=1)
+    eng.linkup()
+    eng.md_init()
+    tr = eng.md_steps(60)
+    rebuilds, _ = eng.stats()
+    S = oracle.OracleSystem(sd, oracle.Potential(), ivol=1, ntcm=1)
+    S.call("linkup")
+    S.call("md_init")
+    pe = []
+    for _ in range(60):
+        S.call("md_step")
+        pe.append(S.scalar("pe"))
+    assert rebuilds >= 2
+    assert np.max(np.abs(tr[:, 0] - np.array(pe)) / np.abs(np.array(pe))) < 1e-9
+    x = eng.array("x0")
+    assert np.max(np.abs(x - S.array("x0"))) < 1e-11
+    eng.close()
+
+
+def test_gpu_large_perfect_lattice_closed_form(oracle):
+    """64^3 bcc cells (524,288 atoms): perfect-lattice energy per atom and zero
+    forces against the closed form."""
+    a = 2.8665
+    sd = lattice.make_system(nc=64, a=a, sigma=0.0, temp=0.0, seed=0)
+    say("large perfect lattice n=", sd.n)
+    eng = engine.Engine(sd, ivol=1)
+    eng.linkup()
+    eng.md_init()
+    pot = oracle.Potential()
+    r1, r2 = a * np.sqrt(3) / 2, a
+    V1, V2 = pot.f("vee", [r1, r2])
+    P1, P2 = pot.f("phi", [r1, r2])
+    e_atom = 0.5 * (8 * V1 + 6 * V2) - np.sqrt(8 * P1 + 6 * P2)
+    assert abs(eng.scalar("pe") / sd.n - e_atom) < 1e-11
+    assert np.max(np.abs(eng.forces_cart())) < 1e-10
+    eng.close()
+
+
+def test_gpu_large_random_properties():
+    """524,288 atoms, displaced: Newton's third law and translation invariance."""
+    sd = lattice.make_system(nc=64, a=2.8665, sigma=0.05, temp=300.0, seed=8)
+    say("large random n=", sd.n)
+    eng = engine.Engine(sd, ivol=0, press_gpa=0.0)
+    eng.linkup()
+    eng.md_init()
+    f = np.stack([eng.array(n) for n in ("fx", "fy", "fz")], axis=1)
+    pe = eng.scalar("pe")
+    assert np.all(np.abs(f.sum(axis=0)) < 1e-9 * np.abs(f).max() * np.sqrt(sd.n))
+    eng.close()
+    shifted = lattice.System(b0=sd.b0, s=sd.s + np.array([0.1234, -0.377, 0.05]), x1=sd.x1,
+                             mass=sd.mass, z=sd.z, deltat_fs=sd.deltat_fs)
+    eng2 = engine.Engine(shifted, ivol=0)
+    eng2.linkup()
+    eng2.md_init()
+    assert abs(eng2.scalar("pe") - pe) < 1e-11 * abs(pe)
+    f2 = np.stack([eng2.array(n) for n in ("fx", "fy", "fz")], axis=1)
+    assert rel(f2, f) < 1e-10
+    eng2.close()

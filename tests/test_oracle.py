@@ -1,0 +1,306 @@
+"""CPU tests that pin the oracle (oracle/moldy_oracle.c).
+
+Two independent anchors:
+  1. golden vectors written by the REFERENCE Fortran (oracle/_ref/ref_driver,
+     tests/golden/gen_golden.py): the oracle must reproduce them bit for bit;
+  2. what the mathematics fixes regardless of the reference: closed forms of
+     the knot functions, the Biersack-Ziegler form, -sqrt embedding, finite
+     differences (F = -grad E, tp = -dE/dB0), Newton's third law, the perfect
+     bcc lattice, brute-force neighbour enumeration, kinetic tensor.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_system, load_golden, run_kwargs
+from moldy_amd import lattice
+from moldy_amd.potential import FE_POTENTIAL
+
+CASES = ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024"]
+
+
+def _oracle_for(oracle, F):
+    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])))
+    S = oracle.OracleSystem(golden_system(F), pot, **run_kwargs(F))
+    return pot, S
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_bit_exact_vs_reference(oracle, case):
+    F = load_golden(case)
+    pot, S = _oracle_for(oracle, F)
+    S.call("linkup")
+    assert S.call("md_init") == 0
+    assert [S.int("nlcx"), S.int("nlcy"), S.int("nlcz")] == list(F["nlc"])
+    assert S.scalar("rnear") == float(F["rnear"])
+    assert S.scalar("deltat") == float(F["deltat"])
+    np.testing.assert_array_equal(S.numn(), F["numn0"].astype(np.int32))
+    assert len(S.pairs()) == int(F["npairs0"])
+    assert oracle.pairset_digest(S.pairs()) == str(F["pairs0_sha256"])
+    for nm, key in [("rho", "rho0"), ("afrho", "afrho0"), ("dafrho", "dafrho0"), ("fx", "fx0"),
+                    ("fy", "fy0"), ("fz", "fz0"), ("x2", "x20"), ("y2", "y20"), ("z2", "z20")]:
+        np.testing.assert_array_equal(S.array(nm), F[key], err_msg=nm)
+    assert S.scalar("pe") == float(F["pe0"])
+    for nm in ("tp", "tk", "tc", "fb", "tgid"):
+        np.testing.assert_array_equal(S.tensor(nm), F[nm + "0"], err_msg=nm)
+    np.testing.assert_array_equal(S.tensor("b2"), F["b20"])
+    tr = []
+    for _ in range(int(F["run_nsteps"])):
+        assert S.call("md_step") == 0
+        tr.append((S.scalar("pe"), S.scalar("ke"), S.scalar("s1"), S.scalar("s3"), S.scalar("vol"),
+                   S.int("ntc")))
+    np.testing.assert_array_equal(np.array(tr), F["trace"])
+    for nm in ("x0", "y0", "z0", "x1", "y1", "z1", "fx", "fy", "fz"):
+        np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
+    for nm in ("b0", "b1", "b2", "tp", "fb", "tgid", "tk"):
+        np.testing.assert_array_equal(S.tensor(nm), F[nm + "_f"], err_msg=nm)
+
+
+@pytest.mark.parametrize("case", ["c0_nve_2000", "lookup_2000"])
+def test_potential_functions_bit_exact_vs_reference(oracle, case):
+    F = load_golden(case)
+    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])))
+    for fn in ("vee", "dvee", "phi", "dphi"):
+        np.testing.assert_array_equal(pot.f(fn, F["rgrid"]), F[fn], err_msg=fn)
+    for fn in ("emb", "demb"):
+        np.testing.assert_array_equal(pot.f(fn, F["rhogrid"]), F[fn], err_msg=fn)
+
+
+def test_hot_fixture_reaches_short_range(oracle):
+    F = load_golden("hot_1024")
+    pr = oracle.bruteforce_pairs(F["b0"], F["s"], float(np.float32(1.9)))
+    assert len(pr) > 0, "fixture should exercise the Biersack/join region"
+
+
+# ---------------------------------------------------------------- closed forms
+def _knot_sum(a, rk, r, p):
+    d = rk[None, :] - r[:, None]
+    return (a[None, :] * np.where(d >= 0, d, 0.0) ** p).sum(axis=1)
+
+
+def test_knot_regions_closed_form(oracle):
+    pot = oracle.Potential()
+    c = FE_POTENTIAL
+    r = np.linspace(1.95, 4.2, 900)
+    np.testing.assert_allclose(pot.f("vee", r), _knot_sum(c["av"], c["rv"], r, 3), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(pot.f("dvee", r), -3 * _knot_sum(c["av"], c["rv"], r, 2), rtol=1e-12, atol=1e-13)
+    rs = pot.phi_cut[0]
+    rp = r[r > rs]
+    np.testing.assert_allclose(pot.f("phi", rp), _knot_sum(c["ap"], c["rp"], rp, 3), rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(pot.f("dphi", rp), -3 * _knot_sum(c["ap"], c["rp"], rp, 2), rtol=1e-12, atol=1e-13)
+    # identically zero beyond the outermost knots
+    far = np.linspace(3.72645, 7.0, 50)
+    assert np.all(pot.f("phi", far) == 0.0) and np.all(pot.f("dphi", far) == 0.0)
+    far_v = np.linspace(3.38247, 7.0, 50)
+    assert np.all(pot.f("vee", far_v) == 0.0) and np.all(pot.f("dvee", far_v) == 0.0)
+
+
+def test_phi_flat_cap_at_closed_form_maximum(oracle):
+    pot = oracle.Potential()
+    A1, A2 = FE_POTENTIAL["ap"]
+    R1, R2 = FE_POTENTIAL["rp"]
+    # dphi = 0  <=>  sqrt(A1)(R1-r) = sqrt(-A2)(R2-r)
+    r_star = (np.sqrt(A1) * R1 - np.sqrt(-A2) * R2) / (np.sqrt(A1) - np.sqrt(-A2))
+    rs, vs = pot.phi_cut
+    assert abs(rs - r_star) < 1e-12
+    assert abs(vs - _knot_sum(np.array([A1, A2]), np.array([R1, R2]), np.array([r_star]), 3)[0]) < 1e-12
+    below = np.linspace(0.1, rs * 0.999, 20)
+    assert np.all(pot.f("phi", below) == vs) and np.all(pot.f("dphi", below) == 0.0)
+
+
+def test_biersack_ziegler_closed_form(oracle):
+    pot = oracle.Potential()
+    # universal screening function (textbook ZBL coefficients); a from Z=26
+    e, eps_k = 1.602176487e-19, 8.98755178e9
+    a = 0.8854 * 0.529 / np.sqrt(2 * 26 ** 0.666666667)
+    r = np.linspace(0.2, 0.85, 30)
+    x = r / a
+    phi = (0.1818 * np.exp(-3.2 * x) + 0.5099 * np.exp(-0.9423 * x) + 0.2802 * np.exp(-0.4029 * x)
+           + 0.02817 * np.exp(-0.2016 * x))
+    v = 26 * 26 * eps_k * e / 1e-10 * phi / r
+    # the reference rounds its literals and the screening length to REAL(4)
+    np.testing.assert_allclose(pot.f("vee", r), v, rtol=2e-6)
+
+
+def test_derivatives_match_finite_differences(oracle):
+    pot = oracle.Potential()
+    h = 1e-6
+    for lo, hi in [(0.3, 0.85), (0.95, 1.85), (1.95, 3.6)]:
+        r = np.linspace(lo, hi, 40)
+        fd = (pot.f("vee", r + h) - pot.f("vee", r - h)) / (2 * h)
+        np.testing.assert_allclose(pot.f("dvee", r), fd, rtol=2e-6, atol=1e-6)
+    r = np.linspace(pot.phi_cut[0] + 0.01, 3.7, 40)
+    fd = (pot.f("phi", r + h) - pot.f("phi", r - h)) / (2 * h)
+    np.testing.assert_allclose(pot.f("dphi", r), fd, rtol=1e-6, atol=1e-7)
+
+
+def test_join_is_continuous(oracle):
+    pot = oracle.Potential()
+    for rc in (float(np.float32(0.9)), float(np.float32(1.9))):
+        lo, hi = rc - 1e-9, rc + 1e-9
+        for fn in ("vee", "dvee"):
+            a, b = pot.f(fn, [lo])[0], pot.f(fn, [hi])[0]
+            assert abs(a - b) <= 1e-6 * max(abs(a), abs(b)), (fn, rc, a, b)
+
+
+def test_embedding_closed_form(oracle):
+    pot = oracle.Potential()
+    rho = np.array([0.0, 1e-3, 1.0, 17.5, 26.0, 100.0])
+    np.testing.assert_array_equal(pot.f("emb", rho), -np.sqrt(rho))
+    d = pot.f("demb", rho)
+    assert d[0] == 0.0
+    np.testing.assert_allclose(d[1:], -0.5 / np.sqrt(rho[1:]), rtol=1e-15)
+
+
+def test_lookup_is_linear_interpolation_of_exact(oracle):
+    ex = oracle.Potential()
+    lk = oracle.Potential(lookup=True)
+    step = 6.0 / 5000
+    nodes = step * np.arange(1000, 4000, 37)
+    for fn in ("vee", "dvee", "phi", "dphi"):
+        np.testing.assert_allclose(lk.f(fn, nodes), ex.f(fn, nodes), rtol=1e-12, atol=1e-14)
+        mid = nodes + 0.5 * step
+        avg = 0.5 * (ex.f(fn, nodes) + ex.f(fn, nodes + step))
+        np.testing.assert_allclose(lk.f(fn, mid), avg, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024"])
+def test_cell_list_equals_bruteforce(oracle, case):
+    F = load_golden(case)
+    pot, S = _oracle_for(oracle, F)
+    S.call("linkup")
+    assert S.call("update_neighbourlist") == 0
+    brute = oracle.bruteforce_pairs(F["b0"], F["s"], S.scalar("rnear"), ivol=int(F["run_ivol"]))
+    if case == "c0_nve_2000":  # cells wider than rnear: every pair is found
+        assert np.array_equal(oracle.canonical_pairs(S.pairs()), oracle.canonical_pairs(brute))
+    else:  # sheared box: the reference grid may miss pairs; never invents one
+        a = {tuple(p) for p in oracle.canonical_pairs(S.pairs()).tolist()}
+        b = {tuple(p) for p in oracle.canonical_pairs(brute).tolist()}
+        assert a <= b
+        # every pair inside the interaction range is found
+        near = {tuple(p) for p in oracle.canonical_pairs(
+            oracle.bruteforce_pairs(F["b0"], F["s"], 3.72645 + 0.3)).tolist()}
+        assert near <= a
+
+
+def test_excess_neighbours_is_reported(oracle):
+    sd = lattice.make_system(nc=5, sigma=0.05, seed=1)
+    S = oracle.OracleSystem(sd, oracle.Potential(), nnbrs=10)
+    S.call("linkup")
+    assert S.call("update_neighbourlist") == 1
+
+
+def _pe(oracle, sd, pot):
+    S = oracle.OracleSystem(sd, pot, ivol=1)
+    S.call("linkup")
+    S.call("force")
+    return S
+
+
+@pytest.mark.parametrize("a", [2.8665, 2.80, 2.95])
+def test_perfect_bcc_closed_form(oracle, a):
+    sd = lattice.make_system(nc=6, a=a, sigma=0.0, temp=0.0, seed=0)
+    pot = oracle.Potential()
+    S = _pe(oracle, sd, pot)
+    r1, r2 = a * np.sqrt(3) / 2, a
+    V1, V2 = pot.f("vee", [r1, r2])
+    P1, P2 = pot.f("phi", [r1, r2])
+    dV1, dV2 = pot.f("dvee", [r1, r2])
+    dP1, dP2 = pot.f("dphi", [r1, r2])
+    rho = 8 * P1 + 6 * P2
+    e_atom = 0.5 * (8 * V1 + 6 * V2) - np.sqrt(rho)
+    assert abs(S.scalar("pe") / sd.n - e_atom) < 1e-12
+    np.testing.assert_allclose(S.array("rho"), rho, rtol=1e-14)
+    f = S.forces_cart()
+    assert np.max(np.abs(f)) < 1e-11
+    # tp(a,a) = -(N/2)(1/L) [8 r1/3 (V'+2F'phi')(r1) + 2 a (V'+2F'phi')(a)]
+    Fp = -0.5 / np.sqrt(rho)
+    L = 6 * a
+    tp_aa = -(sd.n / 2) / L * (8 * r1 / 3 * (dV1 + 2 * Fp * dP1) + 2 * a * (dV2 + 2 * Fp * dP2))
+    tp = S.tensor("tp")
+    # the sum cancels near equilibrium: tolerance relative to its terms' scale
+    scale = (sd.n / 2) / L * (8 * r1 / 3 * (abs(dV1) + abs(2 * Fp * dP1)) + 2 * a * (abs(dV2) + abs(2 * Fp * dP2)))
+    np.testing.assert_allclose(np.diag(tp), tp_aa, rtol=0, atol=1e-13 * scale)
+    assert np.max(np.abs(tp - np.diag(np.diag(tp)))) < 1e-13 * scale
+
+
+def test_force_is_minus_energy_gradient(oracle):
+    sd = lattice.make_system(nc=5, sigma=0.08, seed=11)
+    pot = oracle.Potential()
+    S = _pe(oracle, sd, pot)
+    f = S.forces_cart()
+    h = 1e-5
+    rng = np.random.default_rng(0)
+    binv = np.linalg.inv(sd.b0)
+    for k in rng.choice(sd.n, 4, replace=False):
+        for d in range(3):
+            e = []
+            for sgn in (1, -1):
+                s2 = sd.s.copy()
+                dr = np.zeros(3)
+                dr[d] = sgn * h
+                s2[k] += binv @ dr
+                sp = lattice.System(b0=sd.b0, s=s2, x1=sd.x1, mass=sd.mass, z=sd.z, deltat_fs=1.0)
+                e.append(_pe(oracle, sp, pot).scalar("pe"))
+            fd = -(e[0] - e[1]) / (2 * h)
+            assert abs(fd - f[k, d]) < 1e-6 * max(1.0, abs(f[k, d])), (k, d, fd, f[k, d])
+
+
+def test_newton_third_law(oracle):
+    sd = lattice.make_system(nc=6, sigma=0.1, seed=4)
+    S = _pe(oracle, sd, oracle.Potential())
+    tot = np.array([S.array(n).sum() for n in ("fx", "fy", "fz")])
+    scale = np.abs(S.array("fx")).max()
+    assert np.all(np.abs(tot) < 1e-11 * scale * sd.n)
+
+
+def test_virial_is_minus_box_derivative(oracle):
+    sd = lattice.make_system(nc=5, sigma=0.08, seed=12,
+                             strain=np.array([[0.01, 0.003, 0], [0.003, -0.01, 0.002], [0, 0.002, 0.0]]))
+    pot = oracle.Potential()
+    tp = _pe(oracle, sd, pot).tensor("tp")
+    h = 1e-6
+    for a_ in range(3):
+        for b_ in range(3):
+            e = []
+            for sgn in (1, -1):
+                b0 = sd.b0.copy()
+                b0[a_, b_] += sgn * h
+                sp = lattice.System(b0=b0, s=sd.s, x1=sd.x1, mass=sd.mass, z=sd.z, deltat_fs=1.0)
+                e.append(_pe(oracle, sp, pot).scalar("pe"))
+            fd = -(e[0] - e[1]) / (2 * h)
+            assert abs(fd - tp[a_, b_]) < 2e-5 * max(1.0, np.abs(tp).max()), (a_, b_, fd, tp[a_, b_])
+
+
+def test_kinetic_tensor_closed_form(oracle):
+    sd = lattice.make_system(nc=5, sigma=0.05, temp=500.0, seed=5)
+    S = _pe(oracle, sd, oracle.Potential())
+    dt = S.scalar("deltat")
+    v = sd.x1 @ sd.b0.T  # b0 . x1 per atom
+    tk = (sd.mass[:, None, None] * v[:, :, None] * sd.x1[:, None, :]).sum(axis=0) / dt ** 2
+    np.testing.assert_allclose(S.tensor("tk"), tk, rtol=1e-12, atol=1e-12 * np.abs(tk).max())
+
+
+def test_relink_matches_linkup(oracle):
+    sd = lattice.make_system(nc=6, sigma=0.05, temp=800.0, seed=6)
+    S = oracle.OracleSystem(sd, oracle.Potential(), ntcm=3)
+    S.call("linkup")
+    S.call("md_init")
+    for _ in range(25):
+        S.call("md_step")
+    S.call("relink")
+    ic_relinked = S.ic()
+    S.call("linkup")
+    np.testing.assert_array_equal(S.ic(), ic_relinked)
+
+
+def test_nve_energy_drift_small(oracle):
+    sd = lattice.make_system(nc=6, sigma=0.05, temp=300.0, seed=7)
+    S = oracle.OracleSystem(sd, oracle.Potential(), ntcm=10)
+    S.call("linkup")
+    S.call("md_init")
+    te = []
+    for _ in range(200):
+        S.call("md_step")
+        te.append(S.scalar("te"))
+    te = np.array(te)
+    assert np.max(np.abs(te - te[0])) / sd.n < 2e-4
