@@ -1,0 +1,192 @@
+#!/usr/bin/env python
+"""bench.py -- atom-steps/s of MOLDY's Fe force hot path on one MI355X.
+
+Workload (the headline configuration): bcc Fe, 64^3 conventional cells =
+524,288 atoms, a = 2.8665 A, seeded Gaussian displacements sigma = 0.05 A,
+Maxwell-Boltzmann velocities at 300 K, Parrinello-Rahman NPT at 0 GPa
+(box mass 50 amu/atom), dt = 1 fs, Finnis-Sinclair Fe potential.  A "step" is
+one MD step of the reference's moldin loop (velocity Verlet with force:
+neighbour maintenance, density + embedding, force + virial, PR box update,
+thermodynamic sums).  Inputs are resident in HBM before timing starts.
+
+value = atom-steps/s of the FP32-pair/FP64-accumulate form; the plain FP64
+form is run on the same input and reported beside it.  The CPU baseline is
+the reference Fortran (built from its own sources into oracle/_ref) timed on
+this host on a bounded sample of the same workload.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(seed=2026):
+    from moldy_amd import lattice
+    return lattice.make_system(nc=64, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=seed)
+
+
+def run_form(sd, precision, steps, warmup):
+    from moldy_amd import engine
+    eng = engine.Engine(sd, ivol=0, press_gpa=0.0, boxmass=50.0, ntcm=10, precision=precision,
+                        rebuild=1, rpad=0.3)
+    eng.linkup()
+    eng.md_init()
+    if warmup:
+        eng.md_steps(warmup, trace=False)
+    eng.synchronize()
+    reb0, _ = eng.stats()
+    eng.timing(True)
+    t0 = time.perf_counter()
+    eng.md_steps(steps, trace=False)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    kt = eng.timing_read()
+    eng.timing(False)
+    reb1, full_pairs = eng.stats()
+    res = {
+        "seconds": t1 - t0,
+        "ms_per_step": (t1 - t0) / steps * 1e3,
+        "atom_steps_per_s": sd.n * steps / (t1 - t0),
+        "kernels": {k: {"ms_total": v[0], "launches": v[1],
+                        "us_avg": (1e3 * v[0] / v[1]) if v[1] else None} for k, v in kt.items()},
+        "rebuilds_in_timed_region": reb1 - reb0,
+        "full_pairs_per_atom": full_pairs / sd.n,
+        "pe": eng.scalar("pe"),
+    }
+    return eng, res
+
+
+def cpu_baseline(sd, steps, threads):
+    """The reference Fortran (OpenMP) on the host cores, else the C port."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refrun
+    if os.path.exists(refrun.DRIVER_OMP):
+        secs = refrun.time_md_loop(sd, steps, threads, ivol=0, ntcm=10, press=0.0, boxmass=50.0,
+                                   timeout=900)
+        return {"value": sd.n * steps / secs, "unit": "atom-steps/s", "cores": threads,
+                "kind": "reference",
+                "sample": f"{sd.n} atoms x {steps} MD steps of the same NPT workload (one "
+                          f"nbrupdate=10 cycle), reference modules built with amdflang -O2 "
+                          f"-fopenmp, wall clock of the MD loop"}
+    import pyoracle as po
+    pot = po.Potential()
+    po.lib().orc_set_threads(threads)
+    S = po.OracleSystem(sd, pot, ivol=0, ntcm=10)
+    S.call("linkup")
+    S.call("md_init")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        S.call("md_step")
+    secs = time.perf_counter() - t0
+    return {"value": sd.n * steps / secs, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{sd.n} atoms x {steps} MD steps, C restatement with OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (the force path of one periodic cell "
+              "on one MI355X); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    from moldy_amd import engine
+
+    sd = workload()
+    log(f"workload n={sd.n} steps={args.steps} warmup={args.warmup}")
+    eng32, r32 = run_form(sd, engine.PREC_FP32, args.steps, args.warmup)
+    log(f"fp32 form: {r32['ms_per_step']:.4f} ms/step, {r32['atom_steps_per_s']:.4e} atom-steps/s")
+    f32 = eng32.forces_cart()
+    eng32.close()
+    r64 = None
+    force_err = None
+    if not args.no_fp64:
+        eng64, r64 = run_form(sd, engine.PREC_FP64, args.steps, args.warmup)
+        log(f"fp64 form: {r64['ms_per_step']:.4f} ms/step, {r64['atom_steps_per_s']:.4e} atom-steps/s")
+        eng64.close()
+        # FP32-form force error on this configuration: both forms restarted
+        # from the same input after md_init (FP64 form is oracle-verified).
+        e32 = engine.Engine(sd, ivol=0, precision=engine.PREC_FP32)
+        e32.linkup()
+        e32.md_init()
+        e64 = engine.Engine(sd, ivol=0, precision=engine.PREC_FP64)
+        e64.linkup()
+        e64.md_init()
+        force_err = float(np.max(np.abs(e32.forces_cart() - e64.forces_cart())))
+        pe_err_atom = abs(e32.scalar("pe") - e64.scalar("pe")) / sd.n
+        e32.close()
+        e64.close()
+        log(f"fp32 vs fp64 max force error {force_err:.3e} eV/A, pe error {pe_err_atom:.3e} eV/atom")
+    del f32
+
+    # roofline of the dominant kernel (force pass): algorithmic bytes per atom
+    # = positions (24) + F' (8) + neighbour indices (4 n_f) + forces (24)
+    nf = r32["full_pairs_per_atom"]
+    kf = r32["kernels"]["force"]
+    bytes_per_launch = sd.n * (56.0 + 4.0 * nf)
+    achieved = bytes_per_launch / (kf["us_avg"] * 1e-6) / 1e9 if kf["us_avg"] else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": None,
+                "kernel": "k_force<float>", "us_per_launch": kf["us_avg"],
+                "bytes_per_launch": bytes_per_launch}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(sd, args.cpu_steps, args.cpu_threads)
+        log(f"cpu baseline {cpu['value']:.4e} atom-steps/s on {cpu['cores']} cores ({cpu['kind']})")
+
+    line = {
+        "metric": "atom-steps/s, 0.5M-atom Fe EAM (FP32/FP64-acc vs FP64)",
+        "value": r32["atom_steps_per_s"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair/spline arithmetic, f64 accumulation and integration",
+        "data": "synthetic (seeded bcc Fe lattice, Gaussian displacements, MB velocities)",
+        "config": {"workload": "bcc Fe 64x64x64 cells = 524288 atoms, a=2.8665 A, sigma=0.05 A, "
+                               "300 K, Parrinello-Rahman NPT 0 GPa, dt=1 fs, Finnis-Sinclair Fe",
+                   "n_atoms": sd.n, "neighbour_skin_A": 0.3,
+                   "rebuilds_in_timed_region": r32["rebuilds_in_timed_region"],
+                   "full_list_neighbours_per_atom": nf},
+        "fp64_plain": None if r64 is None else {"value": r64["atom_steps_per_s"],
+                                                "ms_per_step": r64["ms_per_step"],
+                                                "kernels": r64["kernels"]},
+        "fp32_force_error_vs_fp64_eV_per_A": force_err,
+        "kernels": r32["kernels"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (r32["atom_steps_per_s"] / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -60,8 +60,13 @@ def build_fortran() -> str | None:
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(mk) or not os.path.exists(fc):
         return None
-    _run(["make", "-C", os.path.join(HERE, "fortran"), "FC=" + fc])
-    return os.path.join(LIBDIR, "moldy_fortran")
+    fdir = os.path.join(HERE, "fortran")
+    _run(["make", "-C", fdir, "FC=" + fc])
+    # the golden-vector driver, compiled unchanged against the drop-in modules
+    _run(["make", "-C", fdir, "FC=" + fc, "dropin",
+          "CALLER=" + os.path.join(ROOT, "oracle", "ref_driver.F90"),
+          "OUT=" + os.path.join(LIBDIR, "ref_driver_dropin")])
+    return os.path.join(LIBDIR, "libmoldy_fortran.a")
 
 
 def build_all(verbose: bool = True) -> None:

@@ -20,8 +20,10 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -195,6 +197,10 @@ struct mdk_ctx {
   double *atoms, *atoms_alt;
   int *zat, *zat_alt, *orig, *orig_alt, *slot_of;
   float *sref;             // fractional positions at the last full-list build
+  double4 *pos4;           // packed (x0, y0, z0, F') for the pair kernels
+  bool pos4_valid;
+  int tpa_density, tpa_force;  // lanes per atom of the pair kernels
+  bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
   // reference link cells (1-based cell per atom, sorted order)
   int *ic;
   // parity half list in reference layout: numn_ref[nm] and nlist_ref[nm*nnbrs] (orig order)
@@ -234,6 +240,7 @@ struct mdk_ctx {
   // timing
   bool timing;
   std::vector<TimedEv> evs;
+  std::vector<hipEvent_t> evpool;
   double t_ms[TC_NCLASS];
   long t_cnt[TC_NCLASS];
   bool linked;
@@ -406,28 +413,51 @@ __global__ void k_count(const int *key, int n, int *count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) atomicAdd(&count[key[i]], 1);
 }
-// exclusive scan of count[0..m) into start[0..m], start[m] = total; one block
-__global__ void k_scan1(const int *count, int *start, int m) {
-  __shared__ int sh[1024];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int chunk = (m + nt - 1) / nt;
-  const int b = t * chunk, e = min(m, b + chunk);
-  int s = 0;
-  for (int i = b; i < e; i++) s += count[i];
-  sh[t] = s;
+// exclusive scan of count[0..m) into start[0..m] (start[m] = total) in three
+// passes: per-block scans of SCAN_CHUNK elements, a scan of the block totals,
+// and the add-back.
+#define SCAN_CHUNK 2048
+__device__ __forceinline__ int block_exclusive_scan_1024(int v, int *sh, int &total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
   __syncthreads();
-  for (int off = 1; off < nt; off <<= 1) {
-    int v = (t >= off) ? sh[t - off] : 0;
+  for (int off = 1; off < 1024; off <<= 1) {
+    int u = (t >= off) ? sh[t - off] : 0;
     __syncthreads();
-    sh[t] += v;
+    sh[t] += u;
     __syncthreads();
   }
-  int run = sh[t] - s;
-  for (int i = b; i < e; i++) {
-    start[i] = run;
-    run += count[i];
+  total = sh[1023];
+  int ex = sh[t] - v;
+  __syncthreads();
+  return ex;
+}
+__global__ void __launch_bounds__(1024) k_scan_a(const int *count, int *start, int *btot, int m) {
+  __shared__ int sh[1024];
+  const int base = blockIdx.x * SCAN_CHUNK + 2 * threadIdx.x;
+  const int c0 = base < m ? count[base] : 0, c1 = base + 1 < m ? count[base + 1] : 0;
+  int total;
+  int ex = block_exclusive_scan_1024(c0 + c1, sh, total);
+  if (base < m) start[base] = ex;
+  if (base + 1 < m) start[base + 1] = ex + c0;
+  if (threadIdx.x == 0) btot[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(1024) k_scan_b(int *btot, int nb, int *start, int m) {
+  __shared__ int sh[1024];
+  int run = 0;
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    int t = b0 + threadIdx.x;
+    int v = t < nb ? btot[t] : 0;
+    int total;
+    int ex = block_exclusive_scan_1024(v, sh, total);
+    if (t < nb) btot[t] = run + ex;
+    run += total;
   }
-  if (t == nt - 1) start[m] = sh[t];
+  if (threadIdx.x == 0) start[m] = run;
+}
+__global__ void k_scan_c(int *start, const int *btot, int m) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) start[k] += btot[k / SCAN_CHUNK];
 }
 __global__ void k_fill(const int *key, int n, const int *start, int *fill, int *members) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -583,31 +613,42 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
 // ---------------------------------------------------------------------------
 // MD-step kernels
 // ---------------------------------------------------------------------------
-// velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2; one thread
-// copies the box state into the other buffer and advances b1, b0 there.
-__global__ void k_drift(double *atoms, int n, const BoxState *bin, BoxState *bout, int ivol,
-                        int copy_box) {
+// velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2 (also into the
+// packed copy); block 0 copies the box state into the other buffer, advancing
+// b1 and b0 there (one field per thread).
+__global__ void k_drift(double *atoms, double4 *pos4, int n, const BoxState *bin, BoxState *bout,
+                        int ivol) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
     const double *x1 = atoms + 3 * (size_t)n, *y1 = atoms + 4 * (size_t)n, *z1 = atoms + 5 * (size_t)n;
     const double *x2 = atoms + 6 * (size_t)n, *y2 = atoms + 7 * (size_t)n, *z2 = atoms + 8 * (size_t)n;
-    x0[i] = x0[i] + (x1[i] + x2[i]);
-    y0[i] = y0[i] + (y1[i] + y2[i]);
-    z0[i] = z0[i] + (z1[i] + z2[i]);
+    const double xn = x0[i] + (x1[i] + x2[i]);
+    const double yn = y0[i] + (y1[i] + y2[i]);
+    const double zn = z0[i] + (z1[i] + z2[i]);
+    x0[i] = xn;
+    y0[i] = yn;
+    z0[i] = zn;
+    pos4[i] = make_double4(xn, yn, zn, atoms[(size_t)F_DAFRHO * n + i]);
   }
-  if (copy_box && blockIdx.x == 0 && threadIdx.x == 0) {
-    BoxState s = *bin;
-    if (ivol == 4) {
-      TT(s.b1, 3, 3) = TT(s.b1, 3, 3) + TT(s.b2, 3, 3);
-      TT(s.b0, 3, 3) = TT(s.b0, 3, 3) + TT(s.b1, 3, 3);
+  if (blockIdx.x == 0) {
+    const int nw = (int)(sizeof(BoxState) / sizeof(double));
+    const double *src = (const double *)bin;
+    double *dst = (double *)bout;
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+      const int f = w - (int)(offsetof(BoxState, b0) / sizeof(double));
+      const int g = w - (int)(offsetof(BoxState, b1) / sizeof(double));
+      double val = src[w];
+      if (f >= 0 && f < 9) {  // b0 += b1 + b2 (component-wise, as the reference's two updates)
+        const bool upd = ivol < 1 || (ivol == 4 && f == 8);
+        if (upd) val = bin->b0[f] + (bin->b1[f] + bin->b2[f]);
+      } else if (g >= 0 && g < 9) {
+        const bool upd = ivol < 1 || (ivol == 4 && g == 8);
+        if (upd) val = bin->b1[g] + bin->b2[g];
+      }
+      dst[w] = val;
     }
-    if (ivol < 1) {
-      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
-      for (int k = 0; k < 9; k++) s.b0[k] = s.b0[k] + s.b1[k];
-    }
-    *bout = s;
   }
 }
 
@@ -626,46 +667,100 @@ __device__ __forceinline__ double pair_geom(const double *b, int pbc, double xi,
   return rx * rx + ry * ry + rz * rz;
 }
 
-// Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
-// (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list;
-// F, F' per atom; per-block embedding energy.
+// r and 1/r from r^2: hardware reciprocal square root refined by Newton steps
+// (two in FP64, one in FP32) instead of a square root followed by a division.
+__device__ __forceinline__ void r_and_inv(double r2, double &r, double &ri) {
+  double y = __builtin_amdgcn_rsq(r2);
+  const double h = 0.5 * r2;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  ri = y;
+  r = r2 * y;
+}
+__device__ __forceinline__ void r_and_inv(float r2, float &r, float &ri) {
+  float y = __builtin_amdgcn_rsqf(r2);
+  y = y * (1.5f - 0.5f * r2 * y * y);
+  ri = y;
+  r = r2 * y;
+}
+
+// Pair geometry: minimum-image fractional separation s (FP64) and the squared
+// Cartesian distance in the arithmetic type T (the FP32 form rounds B0.s).
 template <typename T>
-__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, int n, const int *nbr,
-                                                       const int *numn, const BoxState *bs,
-                                                       PotDev P, int pbc, int do_embed,
-                                                       double *part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
-  double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
-         *daf = atoms + (size_t)F_DAFRHO * n;
+__device__ __forceinline__ T pair_r2(const double *b, int pbc, const double4 &pi, const double4 &pj,
+                                     double &dx, double &dy, double &dz) {
+  dx = pi.x - pj.x;
+  dy = pi.y - pj.y;
+  dz = pi.z - pj.z;
+  min_image(pbc, dx, dy, dz);
+  const double rx = b[0] * dx + b[3] * dy + b[6] * dz;
+  const double ry = b[1] * dx + b[4] * dy + b[7] * dz;
+  const double rz = b[2] * dx + b[5] * dy + b[8] * dz;
+  if (sizeof(T) == 8) return (T)(rx * rx + ry * ry + rz * rz);
+  const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+  return (T)(fx * fx + fy * fy + fz * fz);
+}
+
+template <int TPA>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = TPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// positions (and the current F') packed for one 32-byte gather per neighbour
+__global__ void k_pack(const double *atoms, int n, double4 *pos4) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  pos4[i] = make_double4(atoms[i], atoms[(size_t)n + i], atoms[2 * (size_t)n + i],
+                         atoms[(size_t)F_DAFRHO * n + i]);
+}
+
+// Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
+// (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
+// with TPA lanes per atom; F, F' per atom; per-block embedding energy.  F' is
+// also stored in pos4[i].w for the force pass.
+template <typename T, int TPA>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
+                                                       const int *__restrict__ nbr,
+                                                       const int *__restrict__ numn,
+                                                       const BoxState *bs, PotDev P, int pbc,
+                                                       int do_embed, double *part) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / TPA, l = gt % TPA;
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
-  double pe[1] = {0.0};
+  double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
   if (i < n) {
-    const double xi = x0[i], yi = y0[i], zi = z0[i];
+    const double4 pi = pos4[i];
     const int nn = numn[i];
-    double acc = 0.0;
-    T accf = T(0);
-    for (int k = 0; k < nn; k++) {
+    for (int k = l; k < nn; k += TPA) {
       const int j = nbr[(size_t)k * n + i];
-      double dx, dy, dz, rx, ry, rz;
-      double r2 = pair_geom<false>(b, pbc, xi, yi, zi, x0[j], y0[j], z0[j], dx, dy, dz, rx, ry, rz);
-      T r = sizeof(T) == 8 ? (T)sqrt(r2) : (T)sqrtf((float)r2);
-      T ph = mdk_pair_phi<T>(P, r);
-      if (sizeof(T) == 8)
-        acc += (double)ph;
-      else
-        accf += ph;
+      const double4 pj = pos4[j];
+      double dx, dy, dz;
+      const T r2 = pair_r2<T>(b, pbc, pi, pj, dx, dy, dz);
+      T r, ri;
+      r_and_inv(r2, r, ri);
+      acc += (double)mdk_pair_phi<T>(P, r);
     }
-    if (sizeof(T) != 8) acc = (double)accf;
+  }
+  acc = group_sum<TPA>(acc);
+  double pe[1] = {0.0};
+  if (i < n && l == 0) {
+    double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
+           *daf = atoms + (size_t)F_DAFRHO * n;
     rho[i] = acc;
-    if (do_embed && acc > 1e-40) {
-      double f, df;
-      mdk_embed_f(P, acc, f, df);
-      af[i] = f;
-      daf[i] = df;
-      pe[0] = f;
+    if (do_embed) {
+      double dfi = daf[i];
+      if (acc > 1e-40) {  // src/potential_m.F90:161 leaves stale values below
+        double f;
+        mdk_embed_f(P, acc, f, dfi);
+        af[i] = f;
+        daf[i] = dfi;
+        pe[0] = f;
+      }
+      pos4[i].w = dfi;
     }
   }
   if (do_embed) block_store<1, 0>(pe, part);
@@ -673,62 +768,71 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, int n, con
 
 // Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
 // fp = -(V' + phi'(F'_i + F'_j))/r;  f_i = sum_j s_ij fp;  the virial is
-// accumulated as W = 1/2 sum s s^T fp (tp = B0 W in the finishing block);
-// the pair energy as 1/2 sum V.
-template <typename T>
-__global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const int *zat, int n,
-                                                     const int *nbr, const int *numn,
+// accumulated as W = 1/2 sum s s^T fp (tp = B0 W in the finishing block) and
+// the pair energy as 1/2 sum V.  TPA lanes per atom; MASSCHK keeps the
+// reference's exclusion of pairs of immobile atoms from the virial.
+template <typename T, int TPA, bool MASSCHK>
+__global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
+                                                     const int *zat, int n,
+                                                     const int *__restrict__ nbr,
+                                                     const int *__restrict__ numn,
                                                      const BoxState *bs, PotDev P, int pbc,
                                                      double *part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
-  const double *x1 = atoms + (size_t)F_X1 * n, *y1 = atoms + (size_t)F_Y1 * n, *z1 = atoms + (size_t)F_Z1 * n;
-  double *fx = atoms + (size_t)F_FX * n, *fy = atoms + (size_t)F_FY * n, *fz = atoms + (size_t)F_FZ * n;
-  const double *daf = atoms + (size_t)F_DAFRHO * n, *mass = atoms + (size_t)F_MASS * n;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / TPA, l = gt % TPA;
+  const double *mass = atoms + (size_t)F_MASS * n;
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   double v[NPART_FORCE];
 #pragma unroll
   for (int k = 0; k < NPART_FORCE; k++) v[k] = 0.0;
+  double fxi = 0.0, fyi = 0.0, fzi = 0.0;
   if (i < n) {
-    const double xi = x0[i], yi = y0[i], zi = z0[i];
-    const double dfi = daf[i], mi = mass[i];
+    const double4 pi = pos4[i];
+    const T dfi = (T)pi.w;
+    const double mi = MASSCHK ? mass[i] : 0.0;
     const int nn = numn[i];
-    double fxi = 0.0, fyi = 0.0, fzi = 0.0, pe2 = 0.0;
-    double wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
-    for (int k = 0; k < nn; k++) {
+    double pe2 = 0.0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
+    for (int k = l; k < nn; k += TPA) {
       const int j = nbr[(size_t)k * n + i];
-      double dx, dy, dz, rx, ry, rz;
-      double r2 = pair_geom<false>(b, pbc, xi, yi, zi, x0[j], y0[j], z0[j], dx, dy, dz, rx, ry, rz);
-      T r = sizeof(T) == 8 ? (T)sqrt(r2) : (T)sqrtf((float)r2);
-      T rr = T(1) / r;
+      const double4 pj = pos4[j];
+      double dx, dy, dz;
+      const T r2 = pair_r2<T>(b, pbc, pi, pj, dx, dy, dz);
+      T r, rr;
+      r_and_inv(r2, r, rr);
       T vv, dv;
       mdk_pair_vdv<T>(P, r, vv, dv);
-      T dph = mdk_pair_dphi<T>(P, r);
-      T fp = -dv * rr + (-T(1) * (dph * (T)dfi + dph * (T)daf[j]) * rr);
+      const T dph = mdk_pair_dphi<T>(P, r);
+      const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj.w) * rr);
       const double fpd = (double)fp;
       pe2 += 0.5 * (double)vv;
       fxi += dx * fpd;
       fyi += dy * fpd;
       fzi += dz * fpd;
-      if (mi + mass[j] > (double)0.1f) {
+      if (!MASSCHK || mi + mass[j] > (double)0.1f) {
         const double h = 0.5 * fpd;
         wxx += dx * dx * h; wyy += dy * dy * h; wzz += dz * dz * h;
         wxy += dx * dy * h; wxz += dx * dz * h; wyz += dy * dz * h;
       }
     }
-    fx[i] = fxi;
-    fy[i] = fyi;
-    fz[i] = fzi;
     v[0] = pe2;
     v[1] = wxx; v[2] = wyy; v[3] = wzz; v[4] = wxy; v[5] = wxz; v[6] = wyz;
-    if (zat[i] != 0) {  // kinten
-      const double a1 = x1[i], a2 = y1[i], a3 = z1[i];
+  }
+  fxi = group_sum<TPA>(fxi);
+  fyi = group_sum<TPA>(fyi);
+  fzi = group_sum<TPA>(fzi);
+  if (i < n && l == 0) {
+    atoms[(size_t)F_FX * n + i] = fxi;
+    atoms[(size_t)F_FY * n + i] = fyi;
+    atoms[(size_t)F_FZ * n + i] = fzi;
+    if (zat[i] != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
+      const double mi = mass[i];
+      const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
+                   a3 = atoms[(size_t)F_Z1 * n + i];
       const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
       const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
       const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
-      // tk(a,b) += x1_b * mv_a, stored column-major from v[10]
       v[10] = a1 * mvx; v[11] = a1 * mvy; v[12] = a1 * mvz;
       v[13] = a2 * mvx; v[14] = a2 * mvy; v[15] = a2 * mvz;
       v[16] = a3 * mvx; v[17] = a3 * mvy; v[18] = a3 * mvz;
@@ -976,12 +1080,22 @@ __global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int 
 // ---------------------------------------------------------------------------
 static inline int nblocks(long n) { return (int)((n + MDK_BLOCK - 1) / MDK_BLOCK); }
 
+static hipEvent_t pooled_event(mdk_ctx *c) {
+  hipEvent_t e;
+  if (!c->evpool.empty()) {
+    e = c->evpool.back();
+    c->evpool.pop_back();
+  } else {
+    hipEventCreate(&e);
+  }
+  return e;
+}
 static void tbegin(mdk_ctx *c, int cls, TimedEv &te) {
   te.cls = -1;
   if (!c->timing) return;
   te.cls = cls;
-  hipEventCreate(&te.a);
-  hipEventCreate(&te.b);
+  te.a = pooled_event(c);
+  te.b = pooled_event(c);
   hipEventRecord(te.a, c->stream);
 }
 static void tend(mdk_ctx *c, TimedEv &te) {
@@ -995,8 +1109,8 @@ static void tend(mdk_ctx *c, TimedEv &te) {
       hipEventElapsedTime(&ms, e.a, e.b);
       c->t_ms[e.cls] += ms;
       c->t_cnt[e.cls] += 1;
-      hipEventDestroy(e.a);
-      hipEventDestroy(e.b);
+      c->evpool.push_back(e.a);
+      c->evpool.push_back(e.b);
     }
     c->evs.clear();
   }
@@ -1059,13 +1173,18 @@ static int bin_atoms(mdk_ctx *c, const int *key, int m, const int *sortkey) {
   const int n = c->n;
   RCK(ensure_cells(c, m));
   HIPCK(hipMemsetAsync(c->cell_count, 0, sizeof(int) * m, c->stream));
-  HIPCK(hipMemsetAsync(c->cell_fill, 0, sizeof(int) * m, c->stream));
   k_count<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(key, n, c->cell_count);
-  k_scan1<<<1, 1024, 0, c->stream>>>(c->cell_count, c->cell_start, m);
+  const int nb = (m + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  k_scan_a<<<nb, 1024, 0, c->stream>>>(c->cell_count, c->cell_start, c->cell_fill, m);
+  k_scan_b<<<1, 1024, 0, c->stream>>>(c->cell_fill, nb, c->cell_start, m);
+  k_scan_c<<<nblocks(m), MDK_BLOCK, 0, c->stream>>>(c->cell_start, c->cell_fill, m);
+  HIPCK(hipMemsetAsync(c->cell_fill, 0, sizeof(int) * m, c->stream));
   k_fill<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(key, n, c->cell_start, c->cell_fill, c->members);
   k_sort_cells<<<nblocks(m), MDK_BLOCK, 0, c->stream>>>(c->cell_start, m, c->members, sortkey);
   return check_launch("bin_atoms");
 }
+
+static int ensure_pos4(mdk_ctx *c);
 
 // Full-list rebuild: fine grid from the current box, re-sort atoms by cell,
 // build the list; grows the per-atom capacity and retries on overflow.
@@ -1090,6 +1209,8 @@ static int rebuild_full(mdk_ctx *c) {
   std::swap(c->atoms, c->atoms_alt);
   std::swap(c->zat, c->zat_alt);
   std::swap(c->orig, c->orig_alt);
+  c->pos4_valid = false;
+  RCK(ensure_pos4(c));
   // reference cells move with the atoms
   k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
   HIPCK(hipMemcpyAsync(c->ic, c->zat_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1122,15 +1243,53 @@ static int rebuild_full(mdk_ctx *c) {
   return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
 }
 
+template <typename T, int TPA>
+static void density_t(mdk_ctx *c, int do_embed) {
+  k_density<T, TPA><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->pbc, do_embed, c->part_d);
+}
+template <typename T, int TPA, bool MC>
+static void force_t(mdk_ctx *c) {
+  k_force<T, TPA, MC><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->pbc, c->part_f);
+}
+template <typename T>
+static void density_dispatch(mdk_ctx *c, int do_embed) {
+  switch (c->tpa_density) {
+    case 1: density_t<T, 1>(c, do_embed); break;
+    case 2: density_t<T, 2>(c, do_embed); break;
+    case 8: density_t<T, 8>(c, do_embed); break;
+    default: density_t<T, 4>(c, do_embed); break;
+  }
+}
+template <typename T, bool MC>
+static void force_dispatch(mdk_ctx *c) {
+  switch (c->tpa_force) {
+    case 1: force_t<T, 1, MC>(c); break;
+    case 2: force_t<T, 2, MC>(c); break;
+    case 8: force_t<T, 8, MC>(c); break;
+    default: force_t<T, 4, MC>(c); break;
+  }
+}
+
+static int ensure_pos4(mdk_ctx *c) {
+  if (c->pos4_valid) return MDK_OK;
+  k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->pos4);
+  c->pos4_valid = true;
+  return check_launch("k_pack");
+}
+
+// per-block partial counts must cover every launch shape
+static int block_count(mdk_ctx *c, int tpa) { return nblocks((long)c->n * tpa); }
+
 static int launch_density(mdk_ctx *c, int do_embed) {
+  RCK(ensure_pos4(c));
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
   if (c->p.precision == MDK_PREC_FP32)
-    k_density<float><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->nbr, c->numn, c->box + c->cur,
-                                                          c->pot, c->pbc, do_embed, c->part_d);
+    density_dispatch<float>(c, do_embed);
   else
-    k_density<double><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->nbr, c->numn, c->box + c->cur,
-                                                           c->pot, c->pbc, do_embed, c->part_d);
+    density_dispatch<double>(c, do_embed);
   tend(c, te);
   return check_launch("k_density");
 }
@@ -1138,15 +1297,15 @@ static int launch_density(mdk_ctx *c, int do_embed) {
 static int launch_force(mdk_ctx *c, int fin_mode) {
   TimedEv te;
   tbegin(c, TC_FORCE, te);
-  if (c->p.precision == MDK_PREC_FP32)
-    k_force<float><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->nbr, c->numn,
-                                                        c->box + c->cur, c->pot, c->pbc, c->part_f);
-  else
-    k_force<double><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->nbr, c->numn,
-                                                         c->box + c->cur, c->pot, c->pbc, c->part_f);
+  const bool f32 = c->p.precision == MDK_PREC_FP32;
+  if (f32 && c->masschk) force_dispatch<float, true>(c);
+  else if (f32) force_dispatch<float, false>(c);
+  else if (c->masschk) force_dispatch<double, true>(c);
+  else force_dispatch<double, false>(c);
   tend(c, te);
   RCK(check_launch("k_force"));
-  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_d, c->nblk, c->part_f, c->nblk, c->box + c->cur,
+  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_d, block_count(c, c->tpa_density), c->part_f,
+                                                   block_count(c, c->tpa_force), c->box + c->cur,
                                                    fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
                                                    c->p.deltat, 1);
   return check_launch("k_finalize_force");
@@ -1181,7 +1340,9 @@ static int launch_drift(mdk_ctx *c) {
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
-  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->box + c->cur, c->box + nxt, c->p.ivol, 1);
+  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->n, c->box + c->cur, c->box + nxt,
+                                                c->p.ivol);
+  c->pos4_valid = true;
   tend(c, te);
   c->cur = nxt;
   return check_launch("k_drift");
@@ -1237,6 +1398,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   memset(mem, 0, sizeof(mdk_ctx));
   mdk_ctx *c = (mdk_ctx *)mem;
   new (&c->evs) std::vector<TimedEv>();
+  new (&c->evpool) std::vector<hipEvent_t>();
   c->p = *p;
   c->n = p->nm;
   const int n = c->n;
@@ -1248,6 +1410,10 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   }
   hipGetDevice(&c->device);
   c->maxn = p->max_full > 0 ? p->max_full : 64;
+  c->tpa_density = 4;
+  c->tpa_force = 4;
+  if (const char *e = getenv("MDK_TPA_DENSITY")) c->tpa_density = atoi(e);
+  if (const char *e = getenv("MDK_TPA_FORCE")) c->tpa_force = atoi(e);
   c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
   c->rnear2_exact = near_threshold(p->rnear);
   c->nblk = nblocks(n);
@@ -1280,8 +1446,9 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->members, sizeof(int) * n);
   ALLOC(c->dflags, sizeof(int) * 4);
   ALLOC(c->box, sizeof(BoxState) * 2);
-  ALLOC(c->part_d, sizeof(double) * (size_t)c->nblk);
-  ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * NPART_FORCE);
+  ALLOC(c->pos4, sizeof(double4) * (size_t)n);
+  ALLOC(c->part_d, sizeof(double) * (size_t)nblocks((long)n * 8));
+  ALLOC(c->part_f, sizeof(double) * (size_t)nblocks((long)n * 8) * NPART_FORCE);
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
   if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 8, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->mon_dev, c->mon_host, 0) != hipSuccess) {
@@ -1314,7 +1481,7 @@ extern "C" int mdk_destroy(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return MDK_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
+  void *ptrs[] = {c->pos4, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
                   c->members, c->dflags, c->box, c->part_d, c->part_f, c->part_k, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev};
@@ -1327,8 +1494,10 @@ extern "C" int mdk_destroy(void *h) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
   }
+  for (auto e : c->evpool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   c->evs.~vector();
+  c->evpool.~vector();
   ::operator delete(c);
   return MDK_OK;
 }
@@ -1458,6 +1627,13 @@ extern "C" int mdk_upload(void *h, int field, const double *data) {
   RCK(check_launch("k_scatter_d"));
   HIPCK(hipStreamSynchronize(c->stream));
   if (field <= MDK_Z0) c->list_valid = false;  // positions replaced: rebuild before the next force
+  if (field <= MDK_Z0 || field == MDK_DAFRHO) c->pos4_valid = false;
+  if (field == MDK_MASS) {
+    // the per-pair test m_i + m_j > 0.1 (REAL(4) literal) can only fail with a mass <= 0.05
+    bool any = false;
+    for (int i = 0; i < n; i++) any = any || !(data[i] > 0.05);
+    c->masschk = any;
+  }
   return MDK_OK;
 }
 
@@ -1767,7 +1943,8 @@ extern "C" int mdk_timing(void *h, int enable) {
   c->timing = enable != 0;
   if (enable) {
     for (int k = 0; k < TC_NCLASS; k++) { c->t_ms[k] = 0.0; c->t_cnt[k] = 0; }
-    for (auto &e : c->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+    hipStreamSynchronize(c->stream);
+    for (auto &e : c->evs) { c->evpool.push_back(e.a); c->evpool.push_back(e.b); }
     c->evs.clear();
   }
   return MDK_OK;
@@ -1782,8 +1959,8 @@ extern "C" int mdk_timing_read(void *h, double *ms, long *launches, int nclasses
     hipEventElapsedTime(&t, e.a, e.b);
     c->t_ms[e.cls] += t;
     c->t_cnt[e.cls] += 1;
-    hipEventDestroy(e.a);
-    hipEventDestroy(e.b);
+    c->evpool.push_back(e.a);
+    c->evpool.push_back(e.b);
   }
   c->evs.clear();
   for (int k = 0; k < nclasses && k < TC_NCLASS; k++) {

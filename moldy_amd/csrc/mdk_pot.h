@@ -80,10 +80,15 @@ template <> struct PotView<float> {
   __device__ float bz() const { return P.bier_zedf; }
 };
 
-// Biersack-Ziegler V and V' (src/fetransmetal.F90:414-454)
+// Biersack-Ziegler V and V' (src/fetransmetal.F90:414-454).  The rare short-range
+// paths take scalars by value so the potential struct never has its address
+// taken (which would force a per-thread scratch copy of it).
+template <typename T> struct VdV {
+  T v, dv;
+};
+
 template <typename T>
-__device__ __noinline__ void mdk_biersack(const PotView<T> &Q, T r, T &v, T &dv) {
-  const T a = Q.ba(), z = Q.bz();
+__device__ __noinline__ VdV<T> mdk_biersack(T a, T z, T r) {
   const T x = r / a, ir = T(1) / r;
   const T e1 = mdk_exp<T>(-T(3.2) * x), e2 = mdk_exp<T>(-T(0.9423) * x);
   const T e3 = mdk_exp<T>(-T(0.4029) * x), e4 = mdk_exp<T>(-T(0.2016) * x);
@@ -92,26 +97,25 @@ __device__ __noinline__ void mdk_biersack(const PotView<T> &Q, T r, T &v, T &dv)
                T((double)0.5099f) * mdk_exp<T>(-T((double)0.9423f) * x) +
                T((double)0.2802f) * mdk_exp<T>(-T((double)0.4029f) * x) +
                T((double)0.02817f) * mdk_exp<T>(-T((double)0.2016f) * x);
-  v = z * ph / r;
   const T s = (ir + T(3.2) / a) * T(0.1818) * e1 + (ir + T(0.9423) / a) * T(0.5099) * e2 +
               (ir + T(0.4029) / a) * T(0.2802) * e3 + (ir + T(0.2016) / a) * T(0.02817) * e4;
-  dv = -(z * ir) * s;
+  return VdV<T>{z * ph / r, -(z * ir) * s};
 }
 
-// log-space cubic Hermite join between bc1 and bc2 (src/fetransmetal.F90:511-551)
+// log-space cubic Hermite join between a1 and a2 (src/fetransmetal.F90:511-551);
+// x1 = log V(a1), v1 = V'/V(a1), x2 = log V(a2), v2 = V'/V(a2)
 template <typename T>
-__device__ __noinline__ void mdk_join(const PotView<T> &Q, T r, T &v, T &dv) {
-  const T a1 = Q.bc1(), a2 = Q.bc2(), h = a2 - a1;
+__device__ __noinline__ VdV<T> mdk_join(T a1, T a2, T x1, T v1, T x2, T v2, T r) {
+  const T h = a2 - a1;
   const T t = (r - a1) / h;
   const T t2 = t * t, t3 = t2 * t;
-  const T sp = (T(2) * t3 - T(3) * t2 + T(1)) * Q.jx1() + (t3 - T(2) * t2 + t) * Q.jv1() * h +
-               (-(T(2) * t3) + T(3) * t2) * Q.jx2() + (t3 - t2) * Q.jv2() * h;
-  T ds = (T(6) * t2 - T(6) * t) * Q.jx1() + (T(3) * t2 - T(4) * t + T(1)) * Q.jv1() * h +
-         (-(T(6) * t2) + T(6) * t) * Q.jx2() + (T(3) * t2 - T(2) * t) * Q.jv2() * h;
+  const T sp = (T(2) * t3 - T(3) * t2 + T(1)) * x1 + (t3 - T(2) * t2 + t) * v1 * h +
+               (-(T(2) * t3) + T(3) * t2) * x2 + (t3 - t2) * v2 * h;
+  T ds = (T(6) * t2 - T(6) * t) * x1 + (T(3) * t2 - T(4) * t + T(1)) * v1 * h +
+         (-(T(6) * t2) + T(6) * t) * x2 + (T(3) * t2 - T(2) * t) * v2 * h;
   ds = ds / h;
   const T e = mdk_exp<T>(sp);
-  v = e;
-  dv = ds * e;
+  return VdV<T>{e, ds * e};
 }
 
 // Pair function V and V' (exact mode).
@@ -133,9 +137,13 @@ __device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
     v = s;
     dv = d1;
   } else if (r > Q.bc1()) {
-    mdk_join<T>(Q, r, v, dv);
+    VdV<T> q = mdk_join<T>(Q.bc1(), Q.bc2(), Q.jx1(), Q.jv1(), Q.jx2(), Q.jv2(), r);
+    v = q.v;
+    dv = q.dv;
   } else if (r < Q.bc1()) {
-    mdk_biersack<T>(Q, r, v, dv);
+    VdV<T> q = mdk_biersack<T>(Q.ba(), Q.bz(), r);
+    v = q.v;
+    dv = q.dv;
   } else {
     v = T(0);
     dv = T(0);

@@ -1,0 +1,71 @@
+"""Runs the reference driver (oracle/_ref/ref_driver[_omp]) on a system --
+TEST INFRASTRUCTURE (golden-vector generation and the CPU baseline).
+
+The driver reads the potential file "pot_026_026.in" from its working
+directory; the package's own copy (same AVTF layout and numbers) is linked
+there, so the GPU box does not need the reference sources.
+"""
+from __future__ import annotations
+
+import os
+import re
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+DRIVER_OMP = os.path.join(HERE, "_ref", "ref_driver_omp")
+POT_FILE = os.path.join(ROOT, "moldy_amd", "data", "pot_026_026.in")
+
+
+def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press, boxmass,
+                rgrid=(), rhogrid=()):
+    n = sysdef.n
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<7i", n, ivol, ntcm, nsteps, uselookup, nnbrs, flags))
+        fh.write(struct.pack("<4d", sysdef.deltat_fs, rpad, press, boxmass))
+        fh.write(np.asarray(sysdef.b0, dtype="<f8").ravel(order="F").tobytes())
+        for a in (sysdef.s[:, 0], sysdef.s[:, 1], sysdef.s[:, 2], sysdef.x1[:, 0],
+                  sysdef.x1[:, 1], sysdef.x1[:, 2], sysdef.mass):
+            fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        fh.write(np.ascontiguousarray(sysdef.z, dtype="<i4").tobytes())
+        fh.write(struct.pack("<i", len(rgrid)))
+        fh.write(np.asarray(rgrid, dtype="<f8").tobytes())
+        fh.write(struct.pack("<i", len(rhogrid)))
+        fh.write(np.asarray(rhogrid, dtype="<f8").tobytes())
+
+
+def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3, press=0.0,
+        boxmass=50.0, flags=0, rgrid=(), rhogrid=(), driver=DRIVER, pot_file=POT_FILE,
+        threads=None, timeout=None):
+    """Runs the driver; returns (stdout, output bytes or None)."""
+    with tempfile.TemporaryDirectory() as td:
+        os.symlink(pot_file, os.path.join(td, "pot_026_026.in"))
+        inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        write_input(inp, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press,
+                    boxmass, rgrid, rhogrid)
+        env = dict(os.environ)
+        if threads is not None:
+            env["OMP_NUM_THREADS"] = str(threads)
+            env.setdefault("OMP_PROC_BIND", "close")
+            env.setdefault("OMP_PLACES", "cores")
+        res = subprocess.run([driver, inp, outp], cwd=td, capture_output=True, text=True, env=env,
+                             timeout=timeout)
+        if res.returncode != 0:
+            raise RuntimeError(f"reference driver failed ({res.returncode}): "
+                               f"{res.stdout[-2000:]} {res.stderr[-2000:]}")
+        data = None if flags & 1 else open(outp, "rb").read()
+        return res.stdout, data
+
+
+def time_md_loop(sysdef, nsteps, threads, **kw) -> float:
+    """Seconds spent in the driver's MD loop (set-up, first force excluded)."""
+    out, _ = run(sysdef, nsteps=nsteps, flags=1, driver=DRIVER_OMP, threads=threads, **kw)
+    m = re.search(r"REF_TIMING steps=(\d+) seconds=\s*([0-9.eE+-]+)", out)
+    if not m:
+        raise RuntimeError("no timing line from the reference driver: " + out[-500:])
+    return float(m.group(2))

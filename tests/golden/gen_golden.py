@@ -11,10 +11,7 @@ built driver):
 from __future__ import annotations
 
 import os
-import struct
-import subprocess
 import sys
-import tempfile
 
 import numpy as np
 
@@ -43,23 +40,6 @@ def rho_grid() -> np.ndarray:
     return np.concatenate([np.linspace(0.0, 60.0, 601), [1e-41, 1e-40, -1.0, 9999.0, 10000.0, 12000.0]])
 
 
-def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press, boxmass,
-                rg, rhog):
-    n = sysdef.n
-    with open(path, "wb") as fh:
-        fh.write(struct.pack("<7i", n, ivol, ntcm, nsteps, uselookup, nnbrs, flags))
-        fh.write(struct.pack("<4d", sysdef.deltat_fs, rpad, press, boxmass))
-        fh.write(np.asarray(sysdef.b0, dtype="<f8").ravel(order="F").tobytes())
-        for a in (sysdef.s[:, 0], sysdef.s[:, 1], sysdef.s[:, 2], sysdef.x1[:, 0],
-                  sysdef.x1[:, 1], sysdef.x1[:, 2], sysdef.mass):
-            fh.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
-        fh.write(np.ascontiguousarray(sysdef.z, dtype="<i4").tobytes())
-        fh.write(struct.pack("<i", len(rg)))
-        fh.write(np.asarray(rg, dtype="<f8").tobytes())
-        fh.write(struct.pack("<i", len(rhog)))
-        fh.write(np.asarray(rhog, dtype="<f8").tobytes())
-
-
 class Reader:
     def __init__(self, buf):
         self.b = buf
@@ -77,20 +57,13 @@ class Reader:
 
 
 def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
-                  press=0.0, boxmass=50.0, flags=0, driver=DRIVER):
+                  press=0.0, boxmass=50.0, flags=0, driver=DRIVER, pot_file=REF_POT):
+    import refrun
     rg, rhog = r_grid(), rho_grid()
     n = sysdef.n
-    with tempfile.TemporaryDirectory() as td:
-        os.symlink(REF_POT, os.path.join(td, "pot_026_026.in"))
-        inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        write_input(inp, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press,
-                    boxmass, rg, rhog)
-        res = subprocess.run([driver, inp, outp], cwd=td, capture_output=True, text=True)
-        if res.returncode != 0:
-            raise RuntimeError(f"ref_driver failed: {res.stdout[-2000:]} {res.stderr[-2000:]}")
-        if flags & 1:
-            return res.stdout
-        buf = open(outp, "rb").read()
+    _, buf = refrun.run(sysdef, ivol=ivol, ntcm=ntcm, nsteps=nsteps, uselookup=uselookup,
+                        nnbrs=nnbrs, rpad=rpad, press=press, boxmass=boxmass, flags=flags,
+                        rgrid=rg, rhogrid=rhog, driver=driver, pot_file=pot_file)
     R = Reader(buf)
     d = {}
     d["nlc"] = R.i4(3)

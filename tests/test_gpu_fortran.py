@@ -1,0 +1,59 @@
+"""The Fortran drop-in: the driver that produced the golden vectors
+(oracle/ref_driver.F90, a caller of the reference's module interfaces) is
+compiled UNCHANGED against moldy_amd/fortran (same module names and argument
+lists, kernels behind the C ABI) and must reproduce the reference's outputs.
+Tolerances as in test_gpu_parity: pair sets exact, forces/densities 1e-10
+relative (norm-wise), energies 1e-12 relative; lookup-mode potential values
+bit-exact."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_system, load_golden
+
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+pytestmark = pytest.mark.gpu
+
+DROPIN = os.path.join(ROOT, "moldy_amd", "lib", "ref_driver_dropin")
+
+
+def rel(a, b):
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / (den if den > 0 else 1.0))
+
+
+@pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024"])
+def test_fortran_dropin_matches_reference(oracle, case):
+    import gen_golden
+    import refrun
+    if not os.path.exists(DROPIN):
+        pytest.fail("drop-in driver not built (moldy_amd.build.build_fortran)")
+    F = load_golden(case)
+    print("[gpu-test] fortran drop-in", case, flush=True)
+    kw = {k[4:]: (int(F[k]) if F[k].dtype.kind == "i" else float(F[k])) for k in F.files
+          if k.startswith("run_")}
+    d = gen_golden.run_reference(golden_system(F), driver=DROPIN, pot_file=refrun.POT_FILE, **kw)
+    assert oracle.pairset_digest(d["pairs0"]) == str(F["pairs0_sha256"])
+    for k in ("rho0", "dafrho0", "fx0", "fy0", "fz0", "x20"):
+        assert rel(d[k], F[k]) < 1e-10, k
+    assert abs(d["pe0"] - float(F["pe0"])) < 1e-12 * abs(float(F["pe0"]))
+    for k in ("tp0", "tk0", "fb0"):
+        assert rel(d[k], F[k]) < 1e-10, k
+    tr = d["trace"]
+    assert np.max(np.abs(tr[:, 0] - F["trace"][:, 0]) / np.abs(F["trace"][:, 0])) < 1e-11
+    assert np.max(np.abs(tr[:, 1] - F["trace"][:, 1]) / np.abs(F["trace"][:, 1])) < 1e-10
+    assert np.array_equal(tr[:, 5], F["trace"][:, 5])  # ntc bookkeeping
+    for k in ("x0_f", "y0_f", "z0_f"):
+        assert np.max(np.abs(d[k] - F[k])) < 1e-13, k
+    for k in ("x1_f", "fx_f"):
+        assert rel(d[k], F[k]) < 1e-9, k
+    assert rel(d["b0_f"], F["b0_f"]) < 1e-14
+    if "rgrid" in F:
+        exact_cmp = bool(int(F["run_uselookup"]))
+        for k in ("vee", "dvee", "phi", "dphi"):
+            if exact_cmp:
+                np.testing.assert_array_equal(d[k], F[k], err_msg=k)
+            else:
+                assert rel(d[k][F["rgrid"] > 1.9], F[k][F["rgrid"] > 1.9]) < 1e-13, k

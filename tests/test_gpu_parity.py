@@ -191,9 +191,13 @@ def test_gpu_fp32_form_force_error(oracle):
     err = np.max(np.abs(f32 - f64))
     say(f"fp32 max force error {err:.3e} eV/A, pe rel err "
         f"{abs(eng.scalar('pe') - S.scalar('pe')) / abs(S.scalar('pe')):.3e}")
-    assert err <= 1e-5
-    assert abs(eng.scalar("pe") - S.scalar("pe")) < 1e-7 * abs(S.scalar("pe"))
-    assert rel(eng.tensor("tp"), S.tensor("tp")) < 1e-5
+    assert err <= 1e-5                                       # eV/A
+    assert abs(eng.scalar("pe") - S.scalar("pe")) / S.n < 1e-6  # eV/atom
+    # virial as stress (the tp sum cancels ~1e4-fold, so compare in GPa)
+    ev_a3_to_gpa = 160.21766208
+    dsig = np.max(np.abs(eng.tensor("tp") - S.tensor("tp"))) / S.scalar("vol") * ev_a3_to_gpa
+    say(f"fp32 virial error {dsig:.3e} GPa")
+    assert dsig < 1e-4
     eng.close()
 
 
