@@ -197,7 +197,8 @@ struct mdk_ctx {
   double *atoms, *atoms_alt;
   int *zat, *zat_alt, *orig, *orig_alt, *slot_of;
   float *sref;             // fractional positions at the last full-list build
-  double4 *pos4;           // packed (x0, y0, z0, F') for the pair kernels
+  double4 *pos4;           // packed (B0 (s - img), F') for the pair kernels
+  int *img;                // integer image floor(s) fixed at the last full-list build
   bool pos4_valid;
   int tpa_density, tpa_force;  // lanes per atom of the pair kernels
   bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
@@ -220,6 +221,7 @@ struct mdk_ctx {
   int cur;
   // partials
   double *part_d, *part_f, *part_k;
+  double *stage_d, *stage_f, *stage_k;  // first-stage reductions of the partials
   int nblk;
   // host mapped monitor ring: [slot][0]=maxdisp2,[1]=maxstep2
   double *mon_host, *mon_dev;
@@ -262,7 +264,9 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Block-reduce K sums (and the last KM entries as maxima) into part[blockIdx.x*K+k].
+// Block-reduce K sums (and the last KM entries as maxima) into
+// part[k*gridDim.x + blockIdx.x] (value-major, so the finishing block's reads
+// of one value across all blocks are contiguous).
 template <int K, int KM>
 __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
   __shared__ double sh[MDK_BLOCK / 64][K];
@@ -278,11 +282,11 @@ __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
     const int k = threadIdx.x;
     double s = sh[0][k];
     for (int q = 1; q < MDK_BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
-    part[(size_t)blockIdx.x * K + k] = s;
+    part[(size_t)k * gridDim.x + blockIdx.x] = s;
   }
 }
 
-// Fixed-order reduction of nb block partials inside one block.
+// Fixed-order reduction of nb block partials (value-major) inside one block.
 template <int K, int KM>
 __device__ __forceinline__ void block_final(const double *part, int nb, double (&out)[K]) {
   __shared__ double sh[MDK_BLOCK / 64][K];
@@ -292,7 +296,7 @@ __device__ __forceinline__ void block_final(const double *part, int nb, double (
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      double x = part[(size_t)b * K + k];
+      double x = part[(size_t)k * nb + b];
       v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
     }
   }
@@ -311,6 +315,37 @@ __device__ __forceinline__ void block_final(const double *part, int nb, double (
     out[k] = s;
   }
   __syncthreads();
+}
+
+// First reduction stage: block b folds partial columns [b*chunk, (b+1)*chunk)
+// of the value-major array `in` (nb columns) into out[k*gridDim.x + b], in a
+// fixed order, so the finishing block reads only gridDim.x columns.
+template <int K, int KM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_reduce_stage(const double *in, int nb, double *out,
+                                                            int chunk) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double x = in[(size_t)k * nb + b];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+  block_store<K, KM>(v, out);
+}
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2; remapping physical block b to logical tile
+// start(b%8) + b/8 gives every XCD one contiguous range of (cell-sorted)
+// atoms, so its neighbour gathers stay inside its own L2.
+#define MDK_NXCD 8
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / MDK_NXCD, r = nb % MDK_NXCD, x = b % MDK_NXCD;
+  return x * q + min(x, r) + b / MDK_NXCD;
 }
 
 __device__ __forceinline__ void min_image(int pbc, double &dx, double &dy, double &dz) {
@@ -363,6 +398,54 @@ __device__ double dev_volume(const double *b) {
   return TT(b, 1, 1) * (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) +
          TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
          TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
+}
+
+
+// Image shifts of a neighbour entry: bits 27..31 of nbr[] hold
+// code = (sx+1) + 3 (sy+1) + 9 (sz+1) with s in {-1,0,1}^3 such that the
+// minimum-image separation is (s'_i - s'_j) + (sx,sy,sz), s' = s - img being
+// fractional positions with the integer image img fixed at the last rebuild.
+#define NBR_IDX_MASK 0x07FFFFFF
+#define NBR_CODE_SHIFT 27
+// Neighbour k of atom i lives at nbr[((k/4) n + i) 4 + k%4]: four consecutive
+// entries of one atom form one 16-byte load, and consecutive atoms' quads are
+// contiguous (coalesced dwordx4 loads).
+__device__ __forceinline__ size_t nbr_at(int k, int i, int n) {
+  return (((size_t)(k >> 2) * n + i) << 2) + (k & 3);
+}
+#define NBR_NULL_CODE 31
+__device__ __forceinline__ void shift_table(const double *b, double (*S)[3]) {
+  // codes 27..31 place the partner 1e6 A away: a null entry that pads a quad
+  // and contributes exactly zero to every sum
+  for (int c = threadIdx.x; c < 32; c += blockDim.x) {
+    if (c >= 27) {
+      S[c][0] = 1.0e6;
+      S[c][1] = 0.0;
+      S[c][2] = 0.0;
+      continue;
+    }
+    const double sx = (double)(c % 3 - 1), sy = (double)((c / 3) % 3 - 1), sz = (double)(c / 9 - 1);
+    S[c][0] = b[0] * sx + b[3] * sy + b[6] * sz;
+    S[c][1] = b[1] * sx + b[4] * sy + b[7] * sz;
+    S[c][2] = b[2] * sx + b[5] * sy + b[8] * sz;
+  }
+}
+// inverse of a 3x3 column-major matrix (cofactors / determinant)
+__device__ __forceinline__ void inv3(const double *b, double *bi) {
+  const double c00 = TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2);
+  const double c01 = TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 2, 1) * TT(b, 3, 3);
+  const double c02 = TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 2, 2) * TT(b, 3, 1);
+  const double det = TT(b, 1, 1) * c00 + TT(b, 1, 2) * c01 + TT(b, 1, 3) * c02;
+  const double id = 1.0 / det;
+  TT(bi, 1, 1) = c00 * id;
+  TT(bi, 2, 1) = c01 * id;
+  TT(bi, 3, 1) = c02 * id;
+  TT(bi, 1, 2) = (TT(b, 1, 3) * TT(b, 3, 2) - TT(b, 1, 2) * TT(b, 3, 3)) * id;
+  TT(bi, 2, 2) = (TT(b, 1, 1) * TT(b, 3, 3) - TT(b, 1, 3) * TT(b, 3, 1)) * id;
+  TT(bi, 3, 2) = (TT(b, 1, 2) * TT(b, 3, 1) - TT(b, 1, 1) * TT(b, 3, 2)) * id;
+  TT(bi, 1, 3) = (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 1, 3) * TT(b, 2, 2)) * id;
+  TT(bi, 2, 3) = (TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3)) * id;
+  TT(bi, 3, 3) = (TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 1, 2) * TT(b, 2, 1)) * id;
 }
 
 // ---------------------------------------------------------------------------
@@ -553,6 +636,28 @@ __global__ void k_half_list(const double *x0, const double *y0, const double *z0
   if (cnt > nnbrs) atomicOr(&flags[1], 1);
 }
 
+// Morton (Z-order) key of a fine-grid cell: consecutive keys form compact 3-D
+// blocks, so atoms sorted by key have spatially local neighbourhoods.
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x030000FF;
+  v = (v | (v << 8)) & 0x0300F00F;
+  v = (v | (v << 4)) & 0x030C30C3;
+  v = (v | (v << 2)) & 0x09249249;
+  return v;
+}
+__device__ __forceinline__ unsigned compact3(unsigned v) {
+  v &= 0x09249249;
+  v = (v | (v >> 2)) & 0x030C30C3;
+  v = (v | (v >> 4)) & 0x0300F00F;
+  v = (v | (v >> 8)) & 0x030000FF;
+  v = (v | (v >> 16)) & 0x000003FF;
+  return v;
+}
+__device__ __forceinline__ int morton3(int x, int y, int z) {
+  return (int)(spread3((unsigned)x) | (spread3((unsigned)y) << 1) | (spread3((unsigned)z) << 2));
+}
+
 // fine-grid cell of each atom (fractional, folded into [0,1))
 __global__ void k_fast_cells(const double *x0, const double *y0, const double *z0, int n, int gx,
                              int gy, int gz, int *cell) {
@@ -562,41 +667,60 @@ __global__ void k_fast_cells(const double *x0, const double *y0, const double *z
   int cx = min(max((int)(fx * gx), 0), gx - 1);
   int cy = min(max((int)(fy * gy), 0), gy - 1);
   int cz = min(max((int)(fz * gz), 0), gz - 1);
-  cell[i] = cx + gx * (cy + gy * cz);
+  cell[i] = morton3(cx, cy, cz);
 }
 
 // Full neighbour list at r_list over the fine grid; atoms are already sorted by
-// cell so cell c holds slots [cstart[c], cstart[c+1]).  nbr is column-major
-// nbr[k*n + i] for coalesced loads.
+// cell so cell c holds slots [cstart[c], cstart[c+1]).  Positions are the
+// image-fixed Cartesian ones of pos4; the image shift of a candidate follows
+// from the wrap of its cell (or from rounding in a dimension with one cell).
+// nbr is column-major nbr[k*n + i] (index | shift code << 27).
 __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
-    const double *x0, const double *y0, const double *z0, const int *cell, const int *cstart,
-    int n, int gx, int gy, int gz, const BoxState *bs, double rl2, int pbc, int maxn, int *nbr,
-    int *numn, float *sref, int *flags) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
+    const int *cstart, int n, int gx, int gy, int gz, const BoxState *bs, double rl2, int pbc,
+    int maxn, int *nbr, int *numn, float *sref, int *flags) {
+  __shared__ double S[32][3];
+  double b[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  shift_table(b, S);
+  __syncthreads();
+  int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *b = bs->b0;
-  const int c = cell[i];
-  const int cx = c % gx, cy = (c / gx) % gy, cz = c / (gx * gy);
-  const int sx = gx >= 3 ? 1 : 0, sy = gy >= 3 ? 1 : 0, sz = gz >= 3 ? 1 : 0;
-  const double xi = x0[i], yi = y0[i], zi = z0[i];
+  const unsigned c = (unsigned)cell[i];
+  const int cx = (int)compact3(c), cy = (int)compact3(c >> 1), cz = (int)compact3(c >> 2);
+  const int sxr = gx >= 3 ? 1 : 0, syr = gy >= 3 ? 1 : 0, szr = gz >= 3 ? 1 : 0;
+  const double4 pi = pos4[i];
+  const double sxi = atoms[i] - img[3 * i], syi = atoms[(size_t)n + i] - img[3 * i + 1],
+               szi = atoms[2 * (size_t)n + i] - img[3 * i + 2];
   int cnt = 0;
-  for (int dz = -sz; dz <= sz; dz++) {
-    int jz = (cz + dz + gz) % gz;
-    for (int dy = -sy; dy <= sy; dy++) {
-      int jy = (cy + dy + gy) % gy;
-      for (int dx = -sx; dx <= sx; dx++) {
-        int jx = (cx + dx + gx) % gx;
-        int cj = jx + gx * (jy + gy * jz);
+  for (int dz = -szr; dz <= szr; dz++) {
+    int kz = cz + dz, wz = 0;
+    if (kz < 0) { kz += gz; wz = 1; } else if (kz >= gz) { kz -= gz; wz = -1; }
+    for (int dy = -syr; dy <= syr; dy++) {
+      int ky = cy + dy, wy = 0;
+      if (ky < 0) { ky += gy; wy = 1; } else if (ky >= gy) { ky -= gy; wy = -1; }
+      for (int dx = -sxr; dx <= sxr; dx++) {
+        int kx = cx + dx, wx = 0;
+        if (kx < 0) { kx += gx; wx = 1; } else if (kx >= gx) { kx -= gx; wx = -1; }
+        const int cj = morton3(kx, ky, kz);
         for (int j = cstart[cj]; j < cstart[cj + 1]; j++) {
           if (j == i) continue;
-          double ddx = xi - x0[j], ddy = yi - y0[j], ddz = zi - z0[j];
-          min_image(pbc, ddx, ddy, ddz);
-          double rx = b[0] * ddx + b[3] * ddy + b[6] * ddz;
-          double ry = b[1] * ddx + b[4] * ddy + b[7] * ddz;
-          double rz = b[2] * ddx + b[5] * ddy + b[8] * ddz;
-          double r2 = rx * rx + ry * ry + rz * rz;
+          int tx = wx, ty = wy, tz = wz;
+          if (!sxr && (pbc & 1)) tx = -(int)round(sxi - (atoms[j] - img[3 * j]));
+          if (!syr && (pbc & 2)) ty = -(int)round(syi - (atoms[(size_t)n + j] - img[3 * j + 1]));
+          if (!szr && (pbc & 4)) tz = -(int)round(szi - (atoms[2 * (size_t)n + j] - img[3 * j + 2]));
+          if (!(pbc & 1)) tx = 0;
+          if (!(pbc & 2)) ty = 0;
+          if (!(pbc & 4)) tz = 0;
+          const int code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+          const double4 pj = pos4[j];
+          const double rx = pi.x - pj.x + S[code][0];
+          const double ry = pi.y - pj.y + S[code][1];
+          const double rz = pi.z - pj.z + S[code][2];
+          const double r2 = rx * rx + ry * ry + rz * rz;
           if (r2 < rl2) {
-            if (cnt < maxn) nbr[(size_t)cnt * n + i] = j;
+            if (cnt < maxn) nbr[nbr_at(cnt, i, n)] = j | (code << NBR_CODE_SHIFT);
             cnt++;
           }
         }
@@ -605,9 +729,11 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
   }
   numn[i] = min(cnt, maxn);
   if (cnt > maxn) atomicMax(&flags[0], cnt);
-  sref[3 * i + 0] = (float)xi;
-  sref[3 * i + 1] = (float)yi;
-  sref[3 * i + 2] = (float)zi;
+  for (int k = cnt; k < ((cnt + 3) & ~3) && k < maxn; k++)  // pad the last quad with null entries
+    nbr[nbr_at(k, i, n)] = i | (NBR_NULL_CODE << NBR_CODE_SHIFT);
+  sref[3 * i + 0] = (float)atoms[i];
+  sref[3 * i + 1] = (float)atoms[(size_t)n + i];
+  sref[3 * i + 2] = (float)atoms[2 * (size_t)n + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -616,10 +742,10 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
 // velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2 (also into the
 // packed copy); block 0 copies the box state into the other buffer, advancing
 // b1 and b0 there (one field per thread).
-__global__ void k_drift(double *atoms, double4 *pos4, int n, const BoxState *bin, BoxState *bout,
-                        int ivol) {
+__global__ void k_drift(double *atoms, double4 *pos4, const int *img, int n, const BoxState *bin,
+                        BoxState *bout, int ivol) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i < n) {
     double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
     const double *x1 = atoms + 3 * (size_t)n, *y1 = atoms + 4 * (size_t)n, *z1 = atoms + 5 * (size_t)n;
@@ -630,7 +756,15 @@ __global__ void k_drift(double *atoms, double4 *pos4, int n, const BoxState *bin
     x0[i] = xn;
     y0[i] = yn;
     z0[i] = zn;
-    pos4[i] = make_double4(xn, yn, zn, atoms[(size_t)F_DAFRHO * n + i]);
+    // the box after this step's update (computed redundantly per thread)
+    double b[9];
+    for (int k = 0; k < 9; k++) {
+      const bool upd = ivol < 1 || (ivol == 4 && k == 8);
+      b[k] = upd ? bin->b0[k] + (bin->b1[k] + bin->b2[k]) : bin->b0[k];
+    }
+    const double sx = xn - img[3 * i], sy = yn - img[3 * i + 1], sz = zn - img[3 * i + 2];
+    pos4[i] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
+                           b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
   }
   if (blockIdx.x == 0) {
     const int nw = (int)(sizeof(BoxState) / sizeof(double));
@@ -684,21 +818,15 @@ __device__ __forceinline__ void r_and_inv(float r2, float &r, float &ri) {
   r = r2 * y;
 }
 
-// Pair geometry: minimum-image fractional separation s (FP64) and the squared
-// Cartesian distance in the arithmetic type T (the FP32 form rounds B0.s).
+// Pair geometry: Cartesian minimum-image separation R'_i - R'_j + B0 shift (FP64)
+// and its square in the arithmetic type T (the FP32 form rounds the components).
 template <typename T>
-__device__ __forceinline__ T pair_r2(const double *b, int pbc, const double4 &pi, const double4 &pj,
-                                     double &dx, double &dy, double &dz) {
-  dx = pi.x - pj.x;
-  dy = pi.y - pj.y;
-  dz = pi.z - pj.z;
-  min_image(pbc, dx, dy, dz);
-  const double rx = b[0] * dx + b[3] * dy + b[6] * dz;
-  const double ry = b[1] * dx + b[4] * dy + b[7] * dz;
-  const double rz = b[2] * dx + b[5] * dy + b[8] * dz;
-  if (sizeof(T) == 8) return (T)(rx * rx + ry * ry + rz * rz);
-  const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-  return (T)(fx * fx + fy * fy + fz * fz);
+__device__ __forceinline__ T pair_r2(const double (*S)[3], const double4 &pi, const double4 &pj,
+                                     int code, T &rx, T &ry, T &rz) {
+  rx = (T)(pi.x - pj.x + S[code][0]);
+  ry = (T)(pi.y - pj.y + S[code][1]);
+  rz = (T)(pi.z - pj.z + S[code][2]);
+  return rx * rx + ry * ry + rz * rz;
 }
 
 template <int TPA>
@@ -708,41 +836,59 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-// positions (and the current F') packed for one 32-byte gather per neighbour
-__global__ void k_pack(const double *atoms, int n, double4 *pos4) {
+// Cartesian image-fixed positions R' = B0 (s - img) and the current F', packed
+// for one 32-byte gather per neighbour.  with_img recomputes img = floor(s).
+__global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n, int with_img,
+                       double4 *pos4) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  pos4[i] = make_double4(atoms[i], atoms[(size_t)n + i], atoms[2 * (size_t)n + i],
-                         atoms[(size_t)F_DAFRHO * n + i]);
+  const double *b = bs->b0;
+  const double x = atoms[i], y = atoms[(size_t)n + i], z = atoms[2 * (size_t)n + i];
+  if (with_img) {
+    img[3 * i] = (int)floor(x);
+    img[3 * i + 1] = (int)floor(y);
+    img[3 * i + 2] = (int)floor(z);
+  }
+  const double sx = x - img[3 * i], sy = y - img[3 * i + 1], sz = z - img[3 * i + 2];
+  pos4[i] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
+                         b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
 }
 
 // Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
 // (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
 // with TPA lanes per atom; F, F' per atom; per-block embedding energy.  F' is
 // also stored in pos4[i].w for the force pass.
-template <typename T, int TPA>
+template <typename T, int TPA, bool LOOKUP>
 __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
                                                        const int *__restrict__ nbr,
                                                        const int *__restrict__ numn,
-                                                       const BoxState *bs, PotDev P, int pbc,
+                                                       const BoxState *bs, PotDev P,
                                                        int do_embed, double *part) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double S[32][3];
+  shift_table(bs->b0, S);
+  __syncthreads();
+  const int gt = xcd_tile() * blockDim.x + threadIdx.x;
   const int i = gt / TPA, l = gt % TPA;
-  double b[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
   if (i < n) {
     const double4 pi = pos4[i];
     const int nn = numn[i];
-    for (int k = l; k < nn; k += TPA) {
-      const int j = nbr[(size_t)k * n + i];
-      const double4 pj = pos4[j];
-      double dx, dy, dz;
-      const T r2 = pair_r2<T>(b, pbc, pi, pj, dx, dy, dz);
-      T r, ri;
-      r_and_inv(r2, r, ri);
-      acc += (double)mdk_pair_phi<T>(P, r);
+    // lane l of the atom's TPA lanes takes quads l, l+TPA, ...: one 16-byte
+    // index load and four independent gathers in flight per iteration
+    for (int q = l; 4 * q < nn; q += TPA) {  // quads are padded with null entries
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
+      double4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        T rx, ry, rz;
+        const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
+        T r, ri;
+        r_and_inv(r2, r, ri);
+        acc += (double)mdk_pair_phi<T, LOOKUP>(P, r);
+      }
     }
   }
   acc = group_sum<TPA>(acc);
@@ -767,53 +913,65 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
 }
 
 // Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
-// fp = -(V' + phi'(F'_i + F'_j))/r;  f_i = sum_j s_ij fp;  the virial is
-// accumulated as W = 1/2 sum s s^T fp (tp = B0 W in the finishing block) and
-// the pair energy as 1/2 sum V.  TPA lanes per atom; MASSCHK keeps the
+// fp = -(V' + phi'(F'_i + F'_j))/r;  F_i = sum_j r_ij fp (Cartesian) and
+// f_i = B0^-1 F_i;  the virial is accumulated as W = 1/2 sum r r^T fp (tp =
+// W B0^-T in the finishing block, i.e. sum r_a s_b fp) and the pair energy as
+// 1/2 sum V.  TPA lanes per atom; MASSCHK keeps the
 // reference's exclusion of pairs of immobile atoms from the virial.
-template <typename T, int TPA, bool MASSCHK>
+template <typename T, int TPA, bool MASSCHK, bool LOOKUP>
 __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
                                                      const int *zat, int n,
                                                      const int *__restrict__ nbr,
                                                      const int *__restrict__ numn,
-                                                     const BoxState *bs, PotDev P, int pbc,
+                                                     const BoxState *bs, PotDev P,
                                                      double *part) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gt / TPA, l = gt % TPA;
-  const double *mass = atoms + (size_t)F_MASS * n;
+  __shared__ double S[32][3];
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  shift_table(b, S);
+  __syncthreads();
+  const int gt = xcd_tile() * blockDim.x + threadIdx.x;
+  const int i = gt / TPA, l = gt % TPA;
+  const double *mass = atoms + (size_t)F_MASS * n;
   double v[NPART_FORCE];
 #pragma unroll
   for (int k = 0; k < NPART_FORCE; k++) v[k] = 0.0;
-  double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+  double fxi = 0.0, fyi = 0.0, fzi = 0.0;  // Cartesian force on i
   if (i < n) {
     const double4 pi = pos4[i];
     const T dfi = (T)pi.w;
     const double mi = MASSCHK ? mass[i] : 0.0;
     const int nn = numn[i];
     double pe2 = 0.0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
-    for (int k = l; k < nn; k += TPA) {
-      const int j = nbr[(size_t)k * n + i];
-      const double4 pj = pos4[j];
-      double dx, dy, dz;
-      const T r2 = pair_r2<T>(b, pbc, pi, pj, dx, dy, dz);
-      T r, rr;
-      r_and_inv(r2, r, rr);
-      T vv, dv;
-      mdk_pair_vdv<T>(P, r, vv, dv);
-      const T dph = mdk_pair_dphi<T>(P, r);
-      const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj.w) * rr);
-      const double fpd = (double)fp;
-      pe2 += 0.5 * (double)vv;
-      fxi += dx * fpd;
-      fyi += dy * fpd;
-      fzi += dz * fpd;
-      if (!MASSCHK || mi + mass[j] > (double)0.1f) {
-        const double h = 0.5 * fpd;
-        wxx += dx * dx * h; wyy += dy * dy * h; wzz += dz * dz * h;
-        wxy += dx * dy * h; wxz += dx * dz * h; wyz += dy * dz * h;
+    for (int q = l; 4 * q < nn; q += TPA) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
+      double4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = (int)(e[u] & NBR_IDX_MASK);
+        T rx, ry, rz;
+        const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
+        T r, rr;
+        r_and_inv(r2, r, rr);
+        T vv, dv;
+        mdk_pair_vdv<T, LOOKUP>(P, r, vv, dv);
+        const T dph = mdk_pair_dphi<T, LOOKUP>(P, r);
+        const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj[u].w) * rr);
+        const double fpd = (double)fp;
+        const double dx = (double)rx, dy = (double)ry, dz = (double)rz;
+        pe2 += 0.5 * (double)vv;
+        fxi += dx * fpd;
+        fyi += dy * fpd;
+        fzi += dz * fpd;
+        if (!MASSCHK || mi + mass[j] > (double)0.1f) {
+          const double h = 0.5 * fpd;
+          wxx += dx * dx * h; wyy += dy * dy * h; wzz += dz * dz * h;
+          wxy += dx * dy * h; wxz += dx * dz * h; wyz += dy * dz * h;
+        }
       }
     }
     v[0] = pe2;
@@ -823,9 +981,12 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
   fyi = group_sum<TPA>(fyi);
   fzi = group_sum<TPA>(fzi);
   if (i < n && l == 0) {
-    atoms[(size_t)F_FX * n + i] = fxi;
-    atoms[(size_t)F_FY * n + i] = fyi;
-    atoms[(size_t)F_FZ * n + i] = fzi;
+    // fractional force f = B0^-1 F (the reference's fx = sum s fp, Cartesian = B0 f)
+    double bi[9];
+    inv3(b, bi);
+    atoms[(size_t)F_FX * n + i] = bi[0] * fxi + bi[3] * fyi + bi[6] * fzi;
+    atoms[(size_t)F_FY * n + i] = bi[1] * fxi + bi[4] * fyi + bi[7] * fzi;
+    atoms[(size_t)F_FZ * n + i] = bi[2] * fxi + bi[5] * fyi + bi[8] * fzi;
     if (zat[i] != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
       const double mi = mass[i];
       const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
@@ -871,68 +1032,103 @@ __device__ void therm_box(BoxState &s, int ivol) {
 
 // Finishing block of force: pe, tp, tk, tc, fb (dynamics_m.F90:365-394),
 // then per mode the velocity-Verlet box kick (quench_m.F90:595-604), the
-// initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part.
+// initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part
+// (dynamics_m.F90:672-700).  Threads 0..8 each own one tensor element (i,j);
+// phases are separated by barriers; only changed fields are written back.
 __global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pd, int nbd,
                                                               const double *pf, int nbf,
                                                               BoxState *bs, int mode, int ivol,
                                                               double press, double bdel2,
                                                               double dt, int have_emb) {
-#pragma clang fp contract(off)
   double e[1] = {0.0};
   if (have_emb) block_final<1, 0>(pd, nbd, e);
   double v[NPART_FORCE];
   block_final<NPART_FORCE, 0>(pf, nbf, v);
-  if (threadIdx.x != 0) return;
-  BoxState s = *bs;
-  if (have_emb) s.pe_emb = e[0];
-  s.pe = s.pe_emb + v[0];
-  const double *b = s.b0;
-  // W (symmetric) -> tp = B0 W
-  double W[9];
-  TT(W, 1, 1) = v[1]; TT(W, 2, 2) = v[2]; TT(W, 3, 3) = v[3];
-  TT(W, 1, 2) = TT(W, 2, 1) = v[4];
-  TT(W, 1, 3) = TT(W, 3, 1) = v[5];
-  TT(W, 2, 3) = TT(W, 3, 2) = v[6];
-  for (int a = 1; a <= 3; a++)
-    for (int c = 1; c <= 3; c++)
-      TT(s.tp, a, c) = TT(b, a, 1) * TT(W, 1, c) + TT(b, a, 2) * TT(W, 2, c) + TT(b, a, 3) * TT(W, 3, c);
-  for (int k = 0; k < 9; k++) s.tk[k] = v[10 + k] / (dt * dt);
-  TT(s.tc, 1, 1) = TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2);
-  TT(s.tc, 2, 1) = TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3);
-  TT(s.tc, 3, 1) = TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3);
-  TT(s.tc, 1, 2) = TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 3, 3) * TT(b, 2, 1);
-  TT(s.tc, 2, 2) = TT(b, 3, 3) * TT(b, 1, 1) - TT(b, 1, 3) * TT(b, 3, 1);
-  TT(s.tc, 3, 2) = TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3);
-  TT(s.tc, 1, 3) = TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 3, 1) * TT(b, 2, 2);
-  TT(s.tc, 2, 3) = TT(b, 3, 1) * TT(b, 1, 2) - TT(b, 1, 1) * TT(b, 3, 2);
-  TT(s.tc, 3, 3) = TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 2, 1) * TT(b, 1, 2);
-  // kinten's tgdot and force's tg
-  for (int k = 0; k < 9; k++) { s.tgdot[k] = 0.0; s.tg[k] = 0.0; }
-  for (int i = 1; i <= 3; i++)
-    for (int k = 1; k <= 3; k++)
-      for (int j = 1; j <= 3; j++) {
-        TT(s.tgdot, i, j) = TT(s.tgdot, i, j) + TT(b, k, i) * TT(s.b1, k, j) + TT(s.b1, k, i) * TT(b, k, j);
-        TT(s.tg, i, j) = TT(s.tg, i, j) + TT(b, k, i) * TT(b, k, j);
-      }
-  for (int k = 0; k < 9; k++) s.fb[k] = s.tk[k] + s.tp[k] - press * s.tc[k];
-  s.vol = dev_volume(b);
-  if (mode == FIN_VV || mode == FIN_VV_THERM) {
-    for (int k = 0; k < 9; k++) s.tgid_lag[k] = s.tgid[k];
-    if (ivol < 1) {
-      for (int k = 0; k < 9; k++) s.b2[k] = bdel2 * s.fb[k];
-      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
-    }
-    if (ivol == 4) {
-      TT(s.b2, 3, 3) = bdel2 * TT(s.fb, 3, 3);
-      for (int k = 0; k < 9; k++) s.b1[k] = s.b1[k] + s.b2[k];
-    }
-    if (mode == FIN_VV_THERM) therm_box(s, ivol);
-  } else if (mode == FIN_INIT) {
-    dev_invert(s.tg, s.tginv, &s.status);
-    dev_matmul(s.tginv, s.tgdot, s.tgid);
-    for (int k = 0; k < 9; k++) s.b2[k] = bdel2 * s.fb[k];
+  __shared__ double b0[9], b1[9], tg[9], tgd[9], tgi[9];
+  const int t = threadIdx.x;
+  const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
+  if (t < 9) {
+    b0[t] = bs->b0[t];
+    b1[t] = bs->b1[t];
   }
-  *bs = s;
+  __syncthreads();
+  if (t == 0) {
+    if (have_emb) bs->pe_emb = e[0];
+    bs->pe = (have_emb ? e[0] : bs->pe_emb) + v[0];
+    bs->vol = dev_volume(b0);
+  }
+  if (t < 9) {
+    double bi[9];
+    inv3(b0, bi);
+    // W = 1/2 sum r r^T fp (symmetric) -> tp(a,c) = sum_k W(a,k) B0^-1(c,k)
+    const double W[9] = {v[1], v[4], v[5], v[4], v[2], v[6], v[5], v[6], v[3]};
+    const double tpv = TT(W, ii, 1) * TT(bi, jj, 1) + TT(W, ii, 2) * TT(bi, jj, 2) + TT(W, ii, 3) * TT(bi, jj, 3);
+    const double tkv = v[10 + t] / (dt * dt);
+    // tc = cofactor matrix of b0 (dynamics_m.F90:365-373)
+    const int i1 = ii % 3 + 1, i2 = (ii + 1) % 3 + 1, j1 = jj % 3 + 1, j2 = (jj + 1) % 3 + 1;
+    const double tcv = TT(b0, i1, j1) * TT(b0, i2, j2) - TT(b0, i1, j2) * TT(b0, i2, j1);  // cofactor C(ii,jj)
+    double tgv = 0.0, tgdv = 0.0;
+    for (int k = 1; k <= 3; k++) {
+      tgv += TT(b0, k, ii) * TT(b0, k, jj);
+      tgdv += TT(b0, k, ii) * TT(b1, k, jj) + TT(b1, k, ii) * TT(b0, k, jj);
+    }
+    const double fbv = tkv + tpv - press * tcv;
+    bs->tp[t] = tpv;
+    bs->tk[t] = tkv;
+    bs->tc[t] = tcv;
+    bs->fb[t] = fbv;
+    tg[t] = tgv;
+    tgd[t] = tgdv;
+    if (mode == FIN_FORCE || mode == FIN_INIT) {
+      bs->tg[t] = tgv;
+      bs->tgdot[t] = tgdv;  // kinten's tgdot
+    }
+    if (mode == FIN_VV || mode == FIN_VV_THERM) {
+      bs->tgid_lag[t] = bs->tgid[t];
+      const bool upd = ivol < 1 || (ivol == 4 && t == 8);
+      double b2v = bs->b2[t];
+      if (upd) b2v = bdel2 * fbv;
+      bs->b2[t] = b2v;
+      if (upd) b1[t] = b1[t] + b2v;
+      bs->b1[t] = b1[t];
+    } else if (mode == FIN_INIT) {
+      bs->b2[t] = bdel2 * fbv;
+    }
+  }
+  __syncthreads();
+  if (mode == FIN_VV_THERM || mode == FIN_INIT) {
+    // update_therm_avs box part (VV_THERM) or moldin's first tgid (INIT)
+    if (t < 9 && mode == FIN_VV_THERM) {
+      double tgdv = 0.0;
+      if (ivol != 1)
+        for (int k = 1; k <= 3; k++) tgdv += TT(b1, k, ii) * TT(b0, k, jj) + TT(b0, k, ii) * TT(b1, k, jj);
+      tgd[t] = tgdv;
+      bs->tg[t] = tg[t];
+      bs->tgdot[t] = tgdv;
+    }
+    if (t < 9) {
+      // inverse of tg, element (ii,jj): cofactor(jj,ii)/det
+      const int a1 = jj % 3 + 1, a2 = (jj + 1) % 3 + 1, c1 = ii % 3 + 1, c2 = (ii + 1) % 3 + 1;
+      const double det = TT(tg, 1, 1) * (TT(tg, 2, 2) * TT(tg, 3, 3) - TT(tg, 2, 3) * TT(tg, 3, 2)) +
+                         TT(tg, 2, 1) * (TT(tg, 3, 2) * TT(tg, 1, 3) - TT(tg, 1, 2) * TT(tg, 3, 3)) +
+                         TT(tg, 3, 1) * (TT(tg, 1, 2) * TT(tg, 2, 3) - TT(tg, 2, 2) * TT(tg, 1, 3));
+      if (fabs(det) == 0.0) bs->status = MDK_ERR_SINGULAR_BOX;
+      tgi[t] = (TT(tg, a1, c1) * TT(tg, a2, c2) - TT(tg, a1, c2) * TT(tg, a2, c1)) / det;
+      bs->tginv[t] = tgi[t];
+    }
+    __syncthreads();
+    if (t < 9) {
+      double g = 0.0;
+      for (int k = 1; k <= 3; k++) g += TT(tgi, ii, k) * TT(tgd, k, jj);
+      bs->tgid[t] = g;
+      if (mode == FIN_VV_THERM) {
+        // cofactor(b0)/vol arranged as update_therm_avs's b0_XXdet ... (dynamics_m.F90:692-700)
+        const double vol = dev_volume(b0);
+        const int r1 = ii % 3 + 1, r2 = (ii + 1) % 3 + 1, q1 = jj % 3 + 1, q2 = (jj + 1) % 3 + 1;
+        bs->binv[t] = (TT(b0, r1, q1) * TT(b0, r2, q2) - TT(b0, r1, q2) * TT(b0, r2, q1)) / vol;
+      }
+    }
+  }
 }
 
 // update_therm_avs box part only (stand-alone entry point)
@@ -951,7 +1147,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
                                                     const BoxState *bs, int ivol, int mode,
                                                     const float *sref, double *part) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     double *x0p = atoms, *y0p = atoms + n, *z0p = atoms + 2 * (size_t)n;
@@ -1065,11 +1261,12 @@ __global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int 
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
   double r = x[k], v = 0.0, dv = 0.0;
+  const bool lk = P.use_lookup != 0;
   switch (which) {
-    case 0: mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
-    case 1: mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
-    case 2: y[k] = mdk_pair_phi<double>(P, r); break;
-    case 3: y[k] = mdk_pair_dphi<double>(P, r); break;
+    case 0: if (lk) mdk_pair_vdv<double, true>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
+    case 1: if (lk) mdk_pair_vdv<double, true>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
+    case 2: y[k] = lk ? mdk_pair_phi<double, true>(P, r) : mdk_pair_phi<double>(P, r); break;
+    case 3: y[k] = lk ? mdk_pair_dphi<double, true>(P, r) : mdk_pair_dphi<double>(P, r); break;
     case 4: mdk_embed_f(P, r, v, dv); y[k] = v; break;
     case 5: mdk_embed_f(P, r, v, dv); y[k] = dv; break;
   }
@@ -1149,7 +1346,7 @@ static void plan_grid(mdk_ctx *c, const double *b, int g[3]) {
     double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
     int nc = (int)floor(width / c->r_list);
     if (nc < 3) nc = 1;  // too few cells for a distinct 27-cell stencil: one cell in this dim
-    if (nc > 1024) nc = 1024;
+    if (nc > 256) nc = 256;
     g[d] = nc;
   }
 }
@@ -1197,7 +1394,9 @@ static int rebuild_full(mdk_ctx *c) {
   RCK(read_box(c, &bs));
   int g[3];
   plan_grid(c, bs.b0, g);
-  const int m = g[0] * g[1] * g[2];
+  int p2 = 1;
+  while (p2 < std::max(g[0], std::max(g[1], g[2]))) p2 <<= 1;
+  const int m = p2 * p2 * p2;  // Morton key space
   c->grid[0] = g[0]; c->grid[1] = g[1]; c->grid[2] = g[2];
   k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
                                                        n, g[0], g[1], g[2], c->cell_of);
@@ -1209,8 +1408,9 @@ static int rebuild_full(mdk_ctx *c) {
   std::swap(c->atoms, c->atoms_alt);
   std::swap(c->zat, c->zat_alt);
   std::swap(c->orig, c->orig_alt);
-  c->pos4_valid = false;
-  RCK(ensure_pos4(c));
+  // images fixed for this list's lifetime; Cartesian packed positions
+  k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4);
+  c->pos4_valid = true;
   // reference cells move with the atoms
   k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
   HIPCK(hipMemcpyAsync(c->ic, c->zat_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1220,8 +1420,8 @@ static int rebuild_full(mdk_ctx *c) {
   for (int attempt = 0; attempt < 4; attempt++) {
     HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
     k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, c->cell_of, c->cell_start, n, g[0], g[1], g[2],
-        c->box + c->cur, rl2, c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+        c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
+        c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
     RCK(check_launch("k_full_list"));
     int fl[4];
     HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
@@ -1233,7 +1433,7 @@ static int rebuild_full(mdk_ctx *c) {
       return MDK_OK;
     }
     // grow capacity to the observed maximum (+25%) and retry
-    int newmax = (int)(fl[0] * 1.25) + 8;
+    int newmax = ((int)(fl[0] * 1.25) + 8 + 3) & ~3;
     HIPCK(hipFree(c->nbr));
     HIPCK(hipMalloc(&c->nbr, sizeof(int) * (size_t)newmax * n));
     c->maxn = newmax;
@@ -1243,40 +1443,52 @@ static int rebuild_full(mdk_ctx *c) {
   return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
 }
 
-template <typename T, int TPA>
+template <typename T, int TPA, bool LK>
 static void density_t(mdk_ctx *c, int do_embed) {
-  k_density<T, TPA><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->pbc, do_embed, c->part_d);
+  k_density<T, TPA, LK><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, c->part_d);
 }
-template <typename T, int TPA, bool MC>
+template <typename T, int TPA, bool MC, bool LK>
 static void force_t(mdk_ctx *c) {
-  k_force<T, TPA, MC><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->pbc, c->part_f);
+  k_force<T, TPA, MC, LK><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->part_f);
 }
-template <typename T>
+template <typename T, bool LK>
 static void density_dispatch(mdk_ctx *c, int do_embed) {
   switch (c->tpa_density) {
-    case 1: density_t<T, 1>(c, do_embed); break;
-    case 2: density_t<T, 2>(c, do_embed); break;
-    case 8: density_t<T, 8>(c, do_embed); break;
-    default: density_t<T, 4>(c, do_embed); break;
+    case 1: density_t<T, 1, LK>(c, do_embed); break;
+    case 4: density_t<T, 4, LK>(c, do_embed); break;
+    default: density_t<T, 2, LK>(c, do_embed); break;
   }
 }
-template <typename T, bool MC>
+template <typename T, bool MC, bool LK>
 static void force_dispatch(mdk_ctx *c) {
   switch (c->tpa_force) {
-    case 1: force_t<T, 1, MC>(c); break;
-    case 2: force_t<T, 2, MC>(c); break;
-    case 8: force_t<T, 8, MC>(c); break;
-    default: force_t<T, 4, MC>(c); break;
+    case 2: force_t<T, 2, MC, LK>(c); break;
+    case 4: force_t<T, 4, MC, LK>(c); break;
+    default: force_t<T, 1, MC, LK>(c); break;
   }
 }
 
 static int ensure_pos4(mdk_ctx *c) {
   if (c->pos4_valid) return MDK_OK;
-  k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->n, c->pos4);
+  k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, c->n, 0, c->pos4);
   c->pos4_valid = true;
   return check_launch("k_pack");
+}
+
+// two-stage fixed-order reduction: returns the array/column count the
+// finishing block must read
+template <int K, int KM>
+static const double *stage_reduce(mdk_ctx *c, const double *part, int nb, double *stage, int *nb_out) {
+  if (nb <= 128) {
+    *nb_out = nb;
+    return part;
+  }
+  const int chunk = 64, p = (nb + chunk - 1) / chunk;
+  k_reduce_stage<K, KM><<<p, MDK_BLOCK, 0, c->stream>>>(part, nb, stage, chunk);
+  *nb_out = p;
+  return stage;
 }
 
 // per-block partial counts must cover every launch shape
@@ -1286,10 +1498,12 @@ static int launch_density(mdk_ctx *c, int do_embed) {
   RCK(ensure_pos4(c));
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
-  if (c->p.precision == MDK_PREC_FP32)
-    density_dispatch<float>(c, do_embed);
-  else
-    density_dispatch<double>(c, do_embed);
+  const bool lk = c->pot.use_lookup != 0;
+  if (c->p.precision == MDK_PREC_FP32) {
+    if (lk) density_dispatch<float, true>(c, do_embed); else density_dispatch<float, false>(c, do_embed);
+  } else {
+    if (lk) density_dispatch<double, true>(c, do_embed); else density_dispatch<double, false>(c, do_embed);
+  }
   tend(c, te);
   return check_launch("k_density");
 }
@@ -1297,17 +1511,21 @@ static int launch_density(mdk_ctx *c, int do_embed) {
 static int launch_force(mdk_ctx *c, int fin_mode) {
   TimedEv te;
   tbegin(c, TC_FORCE, te);
-  const bool f32 = c->p.precision == MDK_PREC_FP32;
-  if (f32 && c->masschk) force_dispatch<float, true>(c);
-  else if (f32) force_dispatch<float, false>(c);
-  else if (c->masschk) force_dispatch<double, true>(c);
-  else force_dispatch<double, false>(c);
+  const bool f32 = c->p.precision == MDK_PREC_FP32, lk = c->pot.use_lookup != 0, mc = c->masschk;
+  if (f32) {
+    if (mc) { if (lk) force_dispatch<float, true, true>(c); else force_dispatch<float, true, false>(c); }
+    else { if (lk) force_dispatch<float, false, true>(c); else force_dispatch<float, false, false>(c); }
+  } else {
+    if (mc) { if (lk) force_dispatch<double, true, true>(c); else force_dispatch<double, true, false>(c); }
+    else { if (lk) force_dispatch<double, false, true>(c); else force_dispatch<double, false, false>(c); }
+  }
   tend(c, te);
   RCK(check_launch("k_force"));
-  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_d, block_count(c, c->tpa_density), c->part_f,
-                                                   block_count(c, c->tpa_force), c->box + c->cur,
-                                                   fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
-                                                   c->p.deltat, 1);
+  int nbd, nbf;
+  const double *pd = stage_reduce<1, 0>(c, c->part_d, block_count(c, c->tpa_density), c->stage_d, &nbd);
+  const double *pf = stage_reduce<NPART_FORCE, 0>(c, c->part_f, block_count(c, c->tpa_force), c->stage_f, &nbf);
+  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(pd, nbd, pf, nbf, c->box + c->cur, fin_mode, c->p.ivol,
+                                                   c->p.press, c->p.bdel2, c->p.deltat, 1);
   return check_launch("k_finalize_force");
 }
 
@@ -1331,8 +1549,10 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon)
   tend(c, te);
   RCK(check_launch("k_kick"));
   if (mode == 2) return MDK_OK;
-  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, c->box + c->cur, c->p.deltat,
-                                                  c->p.bdel2, c->p.press, trace_row_dev, mon);
+  int nbk;
+  const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
+  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
+                                                  c->p.press, trace_row_dev, mon);
   return check_launch("k_finalize_kick");
 }
 
@@ -1340,8 +1560,8 @@ static int launch_drift(mdk_ctx *c) {
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
-  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->n, c->box + c->cur, c->box + nxt,
-                                                c->p.ivol);
+  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->img, c->n, c->box + c->cur,
+                                                c->box + nxt, c->p.ivol);
   c->pos4_valid = true;
   tend(c, te);
   c->cur = nxt;
@@ -1409,9 +1629,9 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
     default: c->pbc = 0; break;
   }
   hipGetDevice(&c->device);
-  c->maxn = p->max_full > 0 ? p->max_full : 64;
-  c->tpa_density = 4;
-  c->tpa_force = 4;
+  c->maxn = ((p->max_full > 0 ? p->max_full : 64) + 3) & ~3;
+  c->tpa_density = 2;
+  c->tpa_force = p->precision == MDK_PREC_FP32 ? 1 : 2;
   if (const char *e = getenv("MDK_TPA_DENSITY")) c->tpa_density = atoi(e);
   if (const char *e = getenv("MDK_TPA_FORCE")) c->tpa_force = atoi(e);
   c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
@@ -1447,9 +1667,13 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->dflags, sizeof(int) * 4);
   ALLOC(c->box, sizeof(BoxState) * 2);
   ALLOC(c->pos4, sizeof(double4) * (size_t)n);
+  ALLOC(c->img, sizeof(int) * 3 * (size_t)n);
   ALLOC(c->part_d, sizeof(double) * (size_t)nblocks((long)n * 8));
   ALLOC(c->part_f, sizeof(double) * (size_t)nblocks((long)n * 8) * NPART_FORCE);
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
+  ALLOC(c->stage_d, sizeof(double) * (size_t)(nblocks((long)n * 8) / 64 + 1));
+  ALLOC(c->stage_f, sizeof(double) * (size_t)(nblocks((long)n * 8) / 64 + 1) * NPART_FORCE);
+  ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
   if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 8, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->mon_dev, c->mon_host, 0) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "mapped host buffer failed");
@@ -1460,6 +1684,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   hipMemsetAsync(c->zat, 0, sizeof(int) * n, c->stream);
   hipMemsetAsync(c->box, 0, sizeof(BoxState) * 2, c->stream);
   hipMemsetAsync(c->ic, 0, sizeof(int) * n, c->stream);
+  hipMemsetAsync(c->img, 0, sizeof(int) * 3 * (size_t)n, c->stream);
   hipMemsetAsync(c->numn_ref, 0, sizeof(int) * n, c->stream);
   hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream);
   k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->orig, n);
@@ -1481,9 +1706,9 @@ extern "C" int mdk_destroy(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return MDK_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->pos4, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
+  void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
-                  c->members, c->dflags, c->box, c->part_d, c->part_f, c->part_k, c->cell_count,
+                  c->members, c->dflags, c->box, c->part_d, c->part_f, c->part_k, c->stage_d, c->stage_f, c->stage_k, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -1689,6 +1914,7 @@ extern "C" int mdk_set_tensor(void *h, int which, const double *t9) {
             TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
             TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
     c->list_valid = false;
+    c->pos4_valid = false;
   }
   HIPCK(hipMemcpyAsync(c->box + c->cur, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
@@ -1843,8 +2069,9 @@ extern "C" int mdk_embed(void *h, double *pe_embed) {
   if (!c->list_valid) RCK(rebuild_full(c));
   RCK(launch_density(c, 1));
   if (pe_embed) {
-    std::vector<double> part(c->nblk);
-    HIPCK(hipMemcpyAsync(part.data(), c->part_d, sizeof(double) * c->nblk, hipMemcpyDeviceToHost, c->stream));
+    const int nb = block_count(c, c->tpa_density);
+    std::vector<double> part(nb);
+    HIPCK(hipMemcpyAsync(part.data(), c->part_d, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     double s = 0.0;
     for (double v : part) s += v;
@@ -1892,8 +2119,10 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
                                                nullptr, c->part_k);
   tend(c, te);
   RCK(check_launch("k_kick"));
-  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, c->box + c->cur, c->p.deltat,
-                                                  c->p.bdel2, c->p.press, nullptr, nullptr);
+  int nbk;
+  const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
+  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
+                                                  c->p.press, nullptr, nullptr);
   RCK(check_launch("k_finalize_kick"));
   BoxState s;
   RCK(read_box(c, &s));

@@ -118,70 +118,67 @@ __device__ __noinline__ VdV<T> mdk_join(T a1, T a2, T x1, T v1, T x2, T v2, T r)
   return VdV<T>{e, ds * e};
 }
 
-// Pair function V and V' (exact mode).
+// Pair function V and V' (exact mode): the knot form is evaluated branch-free;
+// the short-range forms (r <= 1.9 A, rare) are a divergent call.
 template <typename T>
 __device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
   PotView<T> Q(P);
-  if (r > Q.bc2()) {
-    T s = T(0), d1 = T(0);
+  T s = T(0), d1 = T(0);
 #pragma unroll
-    for (int k = 0; k < MDK_MAXV; k++) {
-      if (k < P.nv) {
-        const T d = Q.rv(k) - r;
-        const T hd = d < T(0) ? T(0) : T(1);
-        const T ad2 = Q.av(k) * (d * d) * hd;
-        s = s + ad2 * d;
-        d1 = d1 - T(3) * ad2;
-      }
+  for (int k = 0; k < MDK_MAXV; k++) {
+    if (k < P.nv) {
+      const T d = Q.rv(k) - r;
+      const T ad2 = d < T(0) ? T(0) : Q.av(k) * (d * d);
+      s = s + ad2 * d;
+      d1 = d1 - T(3) * ad2;
     }
-    v = s;
-    dv = d1;
-  } else if (r > Q.bc1()) {
-    VdV<T> q = mdk_join<T>(Q.bc1(), Q.bc2(), Q.jx1(), Q.jv1(), Q.jx2(), Q.jv2(), r);
-    v = q.v;
-    dv = q.dv;
-  } else if (r < Q.bc1()) {
-    VdV<T> q = mdk_biersack<T>(Q.ba(), Q.bz(), r);
-    v = q.v;
-    dv = q.dv;
-  } else {
-    v = T(0);
-    dv = T(0);
+  }
+  v = s;
+  dv = d1;
+  if (!(r > Q.bc2())) {
+    if (r > Q.bc1()) {
+      VdV<T> q = mdk_join<T>(Q.bc1(), Q.bc2(), Q.jx1(), Q.jv1(), Q.jx2(), Q.jv2(), r);
+      v = q.v;
+      dv = q.dv;
+    } else if (r < Q.bc1()) {
+      VdV<T> q = mdk_biersack<T>(Q.ba(), Q.bz(), r);
+      v = q.v;
+      dv = q.dv;
+    } else {
+      v = T(0);
+      dv = T(0);
+    }
   }
 }
 
-// Density function phi (exact mode).
+// Density function phi (exact mode), flat below its maximum r*.
 template <typename T>
 __device__ __forceinline__ T mdk_phi(const PotDev &P, T r) {
   PotView<T> Q(P);
-  if (r < Q.pcr()) return Q.pcv();
   T s = T(0);
 #pragma unroll
   for (int k = 0; k < MDK_MAXP; k++) {
     if (k < P.np) {
       const T d = Q.rp(k) - r;
-      const T hd = d < T(0) ? T(0) : T(1);
-      s = s + Q.ap(k) * (d * d * d) * hd;
+      s = s + (d < T(0) ? T(0) : Q.ap(k) * (d * d * d));
     }
   }
-  return s;
+  return r < Q.pcr() ? Q.pcv() : s;
 }
 
-// Density derivative phi' (exact mode).
+// Density derivative phi' (exact mode), zero below r*.
 template <typename T>
 __device__ __forceinline__ T mdk_dphi(const PotDev &P, T r) {
   PotView<T> Q(P);
-  if (r < Q.pcr()) return T(0);
   T s = T(0);
 #pragma unroll
   for (int k = 0; k < MDK_MAXP; k++) {
     if (k < P.np) {
       const T d = Q.rp(k) - r;
-      const T hd = d < T(0) ? T(0) : T(1);
-      s = s - T(3) * Q.ap(k) * (d * d) * hd;
+      s = s - (d < T(0) ? T(0) : T(3) * Q.ap(k) * (d * d));
     }
   }
-  return s;
+  return r < Q.pcr() ? T(0) : s;
 }
 
 // Linear-interpolation lookups in the reference's index arithmetic
@@ -205,22 +202,25 @@ __device__ __forceinline__ double mdk_lk_e(const PotDev &P, const double *t, dou
   return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
 }
 
-// Dispatchers (potential_m.F90:288-417): exact or table.
-template <typename T>
+// Dispatchers (potential_m.F90:288-417): exact or table (a compile-time choice
+// in the pair kernels).
+template <typename T, bool LOOKUP = false>
 __device__ __forceinline__ void mdk_pair_vdv(const PotDev &P, T r, T &v, T &dv) {
-  if (P.use_lookup) {
+  if (LOOKUP) {
     v = (T)mdk_lk_vp(P, P.tab_v, (double)r);
     dv = (T)mdk_lk_vp(P, P.tab_dv, (double)r);
   } else {
     mdk_vee<T>(P, r, v, dv);
   }
 }
-template <typename T> __device__ __forceinline__ T mdk_pair_phi(const PotDev &P, T r) {
-  if (P.use_lookup) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
+template <typename T, bool LOOKUP = false>
+__device__ __forceinline__ T mdk_pair_phi(const PotDev &P, T r) {
+  if (LOOKUP) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
   return mdk_phi<T>(P, r);
 }
-template <typename T> __device__ __forceinline__ T mdk_pair_dphi(const PotDev &P, T r) {
-  if (P.use_lookup) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
+template <typename T, bool LOOKUP = false>
+__device__ __forceinline__ T mdk_pair_dphi(const PotDev &P, T r) {
+  if (LOOKUP) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
   return mdk_dphi<T>(P, r);
 }
 // Embedding F and F' in FP64 (per atom, cheap).
