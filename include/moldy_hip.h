@@ -156,6 +156,33 @@ int mdk_timing(void *h, int enable);
 int mdk_timing_read(void *h, double *ms, long *launches, int nclasses);
 int mdk_synchronize(void *h);
 
+/* ---------------------------------------------------------------------------
+ * Batched evaluation: energies, forces and the 3x3 virial of many independent
+ * periodic cells of uneven size in ONE persistent launch (replayed as a
+ * hipGraph).  Each cell is evaluated like force() in src/dynamics_m.F90:135-401
+ * (densities, embedding, pair + embedding forces, virial) with the Fe-Fe
+ * potential; the reference evaluates one cell per program run.
+ * ------------------------------------------------------------------------- */
+/* natoms[ncell]; boxes 9 doubles per cell (column-major, columns = cell
+ * vectors); s = 3 fractional coordinates per atom, cells concatenated. */
+int mdk_batch_create(void **handle, int ncell, const int *natoms, const double *boxes,
+                     const double *s, int precision);
+int mdk_batch_destroy(void *handle);
+const char *mdk_batch_last_error(void);
+int mdk_batch_set_potential(void *h, int nv, const double *av, const double *rv, int np,
+                            const double *ap, const double *rp);
+/* block in {64,128,256,512}, workgroups per CU, hipGraph replay on/off */
+int mdk_batch_config(void *h, int block, int wg_per_cu, int use_graph);
+int mdk_batch_evaluate(void *h);
+int mdk_batch_synchronize(void *h);
+/* energy[ncell] eV; virial[6*ncell] = sum over pairs r r^T fp (xx yy zz xy xz yz),
+ * stress = -virial/volume; forces[3*natom] Cartesian eV/A; rho[natom]; any may be NULL */
+int mdk_batch_results(void *h, double *energy, double *virial, double *forces, double *rho);
+/* autotuner over block size and workgroups per CU (best of reps replays) */
+int mdk_batch_tune(void *h, int reps, int *best_block, int *best_wgpc, double *best_ms);
+int mdk_batch_time(void *h, int reps, double *ms_avg);
+int mdk_batch_info(void *h, int *natom, int *block, int *wg_per_cu, long *pairs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,9 @@ EXPORTS = [
     "mdk_get_neighbourlist", "mdk_get_pairs", "mdk_force", "mdk_rhoset", "mdk_embed",
     "mdk_kinten", "mdk_md_init", "mdk_velocityverlet", "mdk_update_therm_avs", "mdk_md_steps",
     "mdk_stats", "mdk_timing", "mdk_timing_read", "mdk_synchronize",
+    "mdk_batch_create", "mdk_batch_destroy", "mdk_batch_last_error", "mdk_batch_set_potential",
+    "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
+    "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
 ]
 
 
@@ -95,12 +98,24 @@ def load_library(path: str = LIB_PATH):
         "mdk_timing_read": [vp, dp, lp, C.c_int],
         "mdk_synchronize": [vp],
     }
+    sig.update({
+        "mdk_batch_create": [C.POINTER(C.c_void_p), C.c_int, ip, dp, dp, C.c_int],
+        "mdk_batch_destroy": [vp],
+        "mdk_batch_set_potential": [vp, C.c_int, dp, dp, C.c_int, dp, dp],
+        "mdk_batch_config": [vp, C.c_int, C.c_int, C.c_int],
+        "mdk_batch_evaluate": [vp], "mdk_batch_synchronize": [vp],
+        "mdk_batch_results": [vp, dp, dp, dp, dp],
+        "mdk_batch_tune": [vp, C.c_int, ip, ip, dp],
+        "mdk_batch_time": [vp, C.c_int, dp],
+        "mdk_batch_info": [vp, ip, ip, ip, lp],
+    })
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
-    L.mdk_last_error.argtypes = []
-    L.mdk_last_error.restype = C.c_char_p
+    for nm in ("mdk_last_error", "mdk_batch_last_error"):
+        getattr(L, nm).argtypes = []
+        getattr(L, nm).restype = C.c_char_p
     _lib = L
     return L
 
@@ -318,3 +333,79 @@ class Engine:
     def forces_cart(self) -> np.ndarray:
         f = np.stack([self.array("fx"), self.array("fy"), self.array("fz")], axis=1)
         return f @ self.tensor("b0").T
+
+
+def _bcheck(rc):
+    if rc != MDK_OK:
+        raise MdkError(rc, load_library().mdk_batch_last_error().decode(errors="replace"))
+
+
+class BatchEngine:
+    """Many independent periodic cells evaluated in one persistent launch."""
+
+    def __init__(self, cells: list, precision: int = PREC_FP32, coeffs: dict | None = None):
+        L = load_library()
+        self.L = L
+        self.ncell = len(cells)
+        self.natoms = np.array([c.n for c in cells], dtype=np.int32)
+        self.boxes = np.concatenate([np.asarray(c.b0, dtype=np.float64).ravel(order="F") for c in cells])
+        s = np.ascontiguousarray(np.concatenate([c.s for c in cells]), dtype=np.float64)
+        self.natom = int(self.natoms.sum())
+        h = C.c_void_p()
+        _bcheck(L.mdk_batch_create(C.byref(h), self.ncell, _ip(self.natoms), _dp(self.boxes), _dp(s),
+                                   int(precision)))
+        self.h = h
+        self.set_potential(coeffs if coeffs is not None else FE_POTENTIAL)
+
+    def set_potential(self, c: dict):
+        av, rv, ap, rp = (np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp"))
+        _bcheck(self.L.mdk_batch_set_potential(self.h, len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp)))
+
+    def config(self, block=256, wg_per_cu=4, graph=True):
+        _bcheck(self.L.mdk_batch_config(self.h, block, wg_per_cu, 1 if graph else 0))
+
+    def evaluate(self):
+        _bcheck(self.L.mdk_batch_evaluate(self.h))
+
+    def synchronize(self):
+        _bcheck(self.L.mdk_batch_synchronize(self.h))
+
+    def results(self):
+        e = np.empty(self.ncell)
+        w = np.empty(6 * self.ncell)
+        f = np.empty(3 * self.natom)
+        rho = np.empty(self.natom)
+        _bcheck(self.L.mdk_batch_results(self.h, _dp(e), _dp(w), _dp(f), _dp(rho)))
+        return e, w.reshape(self.ncell, 6), f.reshape(self.natom, 3), rho
+
+    def tune(self, reps=3):
+        b, g, ms = C.c_int(), C.c_int(), C.c_double()
+        _bcheck(self.L.mdk_batch_tune(self.h, reps, C.byref(b), C.byref(g), C.byref(ms)))
+        return b.value, g.value, ms.value
+
+    def time(self, reps=10) -> float:
+        ms = C.c_double()
+        _bcheck(self.L.mdk_batch_time(self.h, reps, C.byref(ms)))
+        return ms.value
+
+    def info(self):
+        n, b, g, p = C.c_int(), C.c_int(), C.c_int(), C.c_long()
+        _bcheck(self.L.mdk_batch_info(self.h, C.byref(n), C.byref(b), C.byref(g), C.byref(p)))
+        return {"natom": n.value, "block": b.value, "wg_per_cu": g.value, "pairs": p.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mdk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def virial_to_stress(w6: np.ndarray, b0: np.ndarray) -> np.ndarray:
+    """stress = -W/V (the reference's efs.out sigma, src/dynamics_m.F90:610-615)."""
+    W = np.array([[w6[0], w6[3], w6[4]], [w6[3], w6[1], w6[5]], [w6[4], w6[5], w6[2]]])
+    return -W / abs(np.linalg.det(b0))

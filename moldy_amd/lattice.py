@@ -103,3 +103,26 @@ def liquid_lattice_parameter(density_g_cm3: float) -> float:
     avogadro = 6.02214179e23
     vol_cm3 = 2.0 * FE_MASS / avogadro / density_g_cm3
     return (vol_cm3 * 1e24) ** (1.0 / 3.0)
+
+
+def make_batch(ncell: int = 4096, seed: int = 7, nmin: int = 3, nmax: int = 10,
+               sigma: float = 0.1, max_strain: float = 0.03, max_vacancy: float = 0.02,
+               a_bcc: float = A_FE) -> list[System]:
+    """Independent periodic Fe cells of uneven size: bcc or fcc (probability 1/2)
+    n^3 supercells with n uniform in [nmin, nmax], fcc at the bcc volume per atom
+    (a_fcc = a_bcc 2^(1/3)), a random symmetric strain with each of the six
+    components uniform in [-max_strain, max_strain], a random vacancy fraction
+    uniform in [0, max_vacancy] and Gaussian displacements sigma (Angstrom)."""
+    rng = np.random.default_rng(seed)
+    cells = []
+    for c in range(ncell):
+        n = int(rng.integers(nmin, nmax + 1))
+        fcc = bool(rng.random() < 0.5)
+        e = rng.uniform(-max_strain, max_strain, 6)
+        strain = np.array([[e[0], e[3], e[4]], [e[3], e[1], e[5]], [e[4], e[5], e[2]]])
+        vac = float(rng.uniform(0.0, max_vacancy))
+        cells.append(make_system(nc=n, a=a_bcc * 2 ** (1 / 3) if fcc else a_bcc,
+                                 basis=FCC_BASIS if fcc else BCC_BASIS, sigma=sigma, temp=0.0,
+                                 seed=int(rng.integers(1 << 31)), strain=strain,
+                                 vacancy_frac=vac))
+    return cells

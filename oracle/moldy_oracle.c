@@ -1006,3 +1006,53 @@ long orc_bruteforce_pairs(int n, const double *b0, const double *x, const double
     }
   return c;
 }
+
+/* Brute-force single-cell evaluation with the reference's per-pair arithmetic
+ * (src/dynamics_m.F90:235-346, src/potential_m.F90:148-169); used to check the
+ * batched engine on cells too small for the reference's 6.3 A link cells. */
+int orc_eval_cell(const orc_pot *p, int n, const double *b0, const double *s, double rcut,
+                  double *pe_out, double *fcart, double *rho, double *w9) {
+  double *x = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *fx = (double *)calloc((size_t)3 * n, sizeof(double));
+  double *df = (double *)calloc((size_t)n, sizeof(double));
+  if (!x || !fx || !df) { free(x); free(fx); free(df); return -1; }
+  for (int i = 0; i < n; i++) { x[i] = s[3 * i]; x[n + i] = s[3 * i + 1]; x[2 * n + i] = s[3 * i + 2]; }
+  for (int i = 0; i < n; i++) rho[i] = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double dx = x[i] - x[j], dy = x[n + i] - x[n + j], dz = x[2 * n + i] - x[2 * n + j];
+      double r = realsep(b0, 1, &dx, &dy, &dz);
+      if (r >= rcut) continue;
+      rho[j] = rho[j] + orc_phi(p, r);
+      rho[i] = rho[i] + orc_phi(p, r);
+    }
+  double pe = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (rho[i] <= 1e-40) continue;
+    pe = pe + orc_emb(p, rho[i]);
+    df[i] = orc_demb(p, rho[i]);
+  }
+  for (int k = 0; k < 9; k++) w9[k] = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double dx = x[i] - x[j], dy = x[n + i] - x[n + j], dz = x[2 * n + i] - x[2 * n + j];
+      double r = realsep(b0, 1, &dx, &dy, &dz);
+      if (r >= rcut) continue;
+      double rr = 1.0 / r;
+      pe = pe + orc_vee(p, r);
+      double fp = -orc_dvee(p, r) * rr + (-1.0 * (orc_dphi(p, r) * df[i] + orc_dphi(p, r) * df[j]) * rr);
+      double rv[3];
+      rv[0] = T(b0, 1, 1) * dx + T(b0, 1, 2) * dy + T(b0, 1, 3) * dz;
+      rv[1] = T(b0, 2, 1) * dx + T(b0, 2, 2) * dy + T(b0, 2, 3) * dz;
+      rv[2] = T(b0, 3, 1) * dx + T(b0, 3, 2) * dy + T(b0, 3, 3) * dz;
+      for (int a = 0; a < 3; a++) {
+        fx[3 * i + a] += rv[a] * fp;
+        fx[3 * j + a] -= rv[a] * fp;
+        for (int c = 0; c < 3; c++) w9[c * 3 + a] += rv[a] * rv[c] * fp;
+      }
+    }
+  for (int k = 0; k < 3 * n; k++) fcart[k] = fx[k];
+  *pe_out = pe;
+  free(x); free(fx); free(df);
+  return 0;
+}

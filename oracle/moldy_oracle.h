@@ -118,6 +118,13 @@ long orc_bruteforce_pairs(int n, const double *b0, const double *x, const double
                           const double *z, int ivol, double rcut,
                           int *pi, int *pj, long cap);
 
+/* One periodic cell evaluated like force() (src/dynamics_m.F90:235-346) over a
+ * brute-force minimum-image half list at cutoff rcut: energy (pe), Cartesian
+ * forces B0.f (3n, row-major), rho (n) and W = sum over pairs r r^T fp (9,
+ * column-major) -- the batched engine's outputs.  ivol selects the periodicity. */
+int orc_eval_cell(const orc_pot *p, int n, const double *b0, const double *s, double rcut,
+                  double *pe, double *fcart, double *rho, double *w9);
+
 /* Set OpenMP-free single-thread evaluation (default) or threaded pair loops
  * for the CPU baseline; threads<=1 keeps the reference's serial order. */
 void orc_set_threads(int threads);

@@ -81,6 +81,7 @@ def lib():
                                            ip, ip, C.c_long]
         L.orc_bruteforce_pairs.restype = C.c_long
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_eval_cell.argtypes = [C.POINTER(OrcPot), C.c_int, dp, dp, C.c_double, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -237,3 +238,16 @@ def pairset_digest(p: np.ndarray) -> str:
     import hashlib
     q = canonical_pairs(p).astype("<i8")
     return hashlib.sha256(q.tobytes()).hexdigest()
+
+
+def eval_cell(pot: Potential, sysdef, rcut: float = 3.72645 + 1e-9):
+    """(pe, Cartesian forces (n,3), rho (n,), W (3,3)) of one periodic cell, brute force."""
+    n = sysdef.n
+    b = np.asfortranarray(sysdef.b0).ravel(order="F").copy()
+    s = np.ascontiguousarray(sysdef.s, dtype=np.float64)
+    pe = C.c_double()
+    f = np.empty(3 * n)
+    rho = np.empty(n)
+    w = np.empty(9)
+    lib().orc_eval_cell(C.byref(pot.s), n, _dp(b), _dp(s), rcut, C.byref(pe), _dp(f), _dp(rho), _dp(w))
+    return pe.value, f.reshape(n, 3), rho, w.reshape(3, 3, order="F")
