@@ -183,6 +183,26 @@ int mdk_batch_tune(void *h, int reps, int *best_block, int *best_wgpc, double *b
 int mdk_batch_time(void *h, int reps, double *ms_avg);
 int mdk_batch_info(void *h, int *natom, int *block, int *wg_per_cu, long *pairs);
 
+/* ---------------------------------------------------------------------------
+ * Adjoint of the batched evaluation with respect to the knot parameters of the
+ * Fe-Fe pair and density functions (src/fetransmetal.F90:133-139, 256-265;
+ * coefficients of src/potentials/pot_026_026.in): gradient order a_0..a_{nv-1},
+ * r_0..r_{nv-1}, A_0..A_{np-1}, R_0..R_{np-1} (2 nv + 2 np doubles; 16 for Fe).
+ * The set_up constants (log-space join at 1.9 A, flat density cap below its
+ * maximum) are differentiated with the knots.  No reference counterpart: the
+ * reference fits no potentials.
+ * ------------------------------------------------------------------------- */
+/* loss targets: e_target[ncell] eV, f_target[3*natom] Cartesian eV/A (caller order) */
+int mdk_batch_set_targets(void *h, const double *e_target, const double *f_target);
+/* loss = sum_c (E_c-E_t,c)^2/N_c^2 + (0.1/natom) sum_i |F_i-F_t,i|^2 and its gradient,
+ * one persistent launch (forward, seeds, densities recomputed, adjoint) */
+int mdk_batch_loss_grad(void *h, double *loss, double *grad);
+/* vector-Jacobian product grad = sum_c dE[c] dE_c/dtheta + sum_i dF_i . dF_i/dtheta at
+ * the last forward evaluation (its neighbour list is reused, densities recomputed) */
+int mdk_batch_vjp(void *h, const double *dE, const double *dF, double *grad);
+/* average ms of reps loss + gradient launches */
+int mdk_batch_time_loss_grad(void *h, int reps, double *ms_avg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,8 +49,7 @@ int berr(int code, const char *fmt, ...) {
 #define NB_CODE_SHIFT 27
 
 struct BatchArgs {
-  int ncell;
-  int maxnb;
+  int ncell, maxnb, ngrad;
   const int *order;      // cells, largest first
   const int *cell_off;   // ncell+1 atom offsets
   const double *box;     // 9 per cell, column-major
@@ -70,12 +69,36 @@ struct BatchArgs {
   int *counter;
   int *flags;            // [0] max neighbour count seen when over capacity
   double rc2;
+  // adjoint
+  const double *e_in;    // per cell: dL/dE_c (vector-Jacobian product) or target E_c (loss)
+  const double *f_in;    // 3 per caller atom: dL/dF_i (product) or target F_i (loss)
+  double *gvec;          // 3 per sorted atom: the force seed dL/dF_i
+  double *out_g;         // NGRAD_SLOTS per cell: gradient partials
+  double *out_loss;      // per cell loss
+  double f_weight;       // loss weight of sum_i |F_i - F_t,i|^2 (0.1 / total atoms)
 };
+
+// gradient slots per cell: a_k, r_k (MDK_MAXV each), A_k, R_k (MDK_MAXP each)
+#define NGRAD_SLOTS (2 * MDK_MAXV + 2 * MDK_MAXP)
 
 __device__ __forceinline__ double bwave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// Fixed-order sum over the workgroup (lanes, then waves); every thread gets the
+// total.  red must hold BLOCK/64 doubles; the call contains two barriers.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  v = bwave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = red[0];
+#pragma unroll
+  for (int q = 1; q < BLOCK / 64; q++) s += red[q];
+  __syncthreads();
+  return s;
 }
 
 __device__ __forceinline__ void r_inv(double r2, double &r, double &ri) {
@@ -97,168 +120,452 @@ __global__ void k_batch_reset(int *counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// One cell as seen by its workgroup.
+struct CellView {
+  int c, a0, nc, nx, ny, nz;
+  const int *sst;
+  int *nbr;
+  double b[9];
+};
+
+// Takes the next cell from the queue (largest first); returns false when the
+// queue is exhausted (uniform over the workgroup).
+__device__ __forceinline__ bool next_cell(const BatchArgs &A, int *sh_cell, CellView &V) {
+  if (threadIdx.x == 0) *sh_cell = atomicAdd(A.counter, 1);
+  __syncthreads();
+  const int q = *sh_cell;
+  __syncthreads();
+  if (q >= A.ncell) return false;
+  V.c = A.order[q];
+  V.a0 = A.cell_off[V.c];
+  V.nc = A.cell_off[V.c + 1] - V.a0;
+  const double *B = A.box + 9 * (size_t)V.c;
+#pragma unroll
+  for (int k = 0; k < 9; k++) V.b[k] = B[k];
+  V.nx = A.nsub[3 * V.c];
+  V.ny = A.nsub[3 * V.c + 1];
+  V.nz = A.nsub[3 * V.c + 2];
+  V.sst = A.sub_start + A.sub_off[V.c];
+  V.nbr = A.nbr + (size_t)V.a0 * A.maxnb;
+  return true;
+}
+
+// image shift table of the cell (codes 27..31 never occur) and Cartesian
+// positions; the caller places a barrier after it
+template <int BLOCK>
+__device__ void phase_positions(const BatchArgs &A, const CellView &V, double (*S)[3]) {
+  const int t = threadIdx.x;
+  for (int k = t; k < 32; k += BLOCK) {
+    if (k >= 27) {
+      S[k][0] = 1.0e6; S[k][1] = 0.0; S[k][2] = 0.0;
+    } else {
+      const double sx = (double)(k % 3 - 1), sy = (double)((k / 3) % 3 - 1), sz = (double)(k / 9 - 1);
+      S[k][0] = V.b[0] * sx + V.b[3] * sy + V.b[6] * sz;
+      S[k][1] = V.b[1] * sx + V.b[4] * sy + V.b[7] * sz;
+      S[k][2] = V.b[2] * sx + V.b[5] * sy + V.b[8] * sz;
+    }
+  }
+  for (int il = t; il < V.nc; il += BLOCK) {
+    const size_t a = (size_t)V.a0 + il;
+    const double sx = A.s[3 * a], sy = A.s[3 * a + 1], sz = A.s[3 * a + 2];
+    A.pos4[a] = make_double4(V.b[0] * sx + V.b[3] * sy + V.b[6] * sz,
+                             V.b[1] * sx + V.b[4] * sy + V.b[7] * sz,
+                             V.b[2] * sx + V.b[5] * sy + V.b[8] * sz, 0.0);
+  }
+}
+
+// full neighbour list within the cell and its periodic images: the 3^3
+// sub-cell stencil in dimensions with >= 3 sub-cells (image from the wrap),
+// rounding of the fractional separation in dimensions with one sub-cell
+template <int BLOCK>
+__device__ void phase_list(const BatchArgs &A, const CellView &V, const double (*S)[3]) {
+  const int nx = V.nx, ny = V.ny, nz = V.nz, nc = V.nc;
+  const int rx_ = nx >= 3 ? 1 : 0, ry_ = ny >= 3 ? 1 : 0, rz_ = nz >= 3 ? 1 : 0;
+  for (int il = threadIdx.x; il < nc; il += BLOCK) {
+    const size_t a = (size_t)V.a0 + il;
+    const int sc = A.subc[a];
+    const int cx = sc % nx, cy = (sc / nx) % ny, cz = sc / (nx * ny);
+    const double4 pi = A.pos4[a];
+    const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
+    int cnt = 0;
+    for (int dz = -rz_; dz <= rz_; dz++) {
+      int kz = cz + dz, wz = 0;
+      if (kz < 0) { kz += nz; wz = 1; } else if (kz >= nz) { kz -= nz; wz = -1; }
+      for (int dy = -ry_; dy <= ry_; dy++) {
+        int ky = cy + dy, wy = 0;
+        if (ky < 0) { ky += ny; wy = 1; } else if (ky >= ny) { ky -= ny; wy = -1; }
+        for (int dx = -rx_; dx <= rx_; dx++) {
+          int kx = cx + dx, wx = 0;
+          if (kx < 0) { kx += nx; wx = 1; } else if (kx >= nx) { kx -= nx; wx = -1; }
+          const int cj = kx + nx * (ky + ny * kz);
+          for (int jl = V.sst[cj]; jl < V.sst[cj + 1]; jl++) {
+            if (jl == il) continue;
+            const size_t aj = (size_t)V.a0 + jl;
+            int tx = wx, ty = wy, tz = wz;
+            if (!rx_) tx = -(int)round(sxi - A.s[3 * aj]);
+            if (!ry_) ty = -(int)round(syi - A.s[3 * aj + 1]);
+            if (!rz_) tz = -(int)round(szi - A.s[3 * aj + 2]);
+            const int code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+            const double4 pj = A.pos4[aj];
+            const double ex = pi.x - pj.x + S[code][0], ey = pi.y - pj.y + S[code][1],
+                         ez = pi.z - pj.z + S[code][2];
+            if (ex * ex + ey * ey + ez * ez < A.rc2) {
+              if (cnt < A.maxnb) V.nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
+              cnt++;
+            }
+          }
+        }
+      }
+    }
+    A.numn[a] = min(cnt, A.maxnb);
+    if (cnt > A.maxnb) atomicMax(&A.flags[0], cnt);
+  }
+}
+
+// separation of the k-th neighbour of local atom il (arithmetic type T)
+template <typename T>
+__device__ __forceinline__ double4 nb_sep(const BatchArgs &A, const CellView &V, const double (*S)[3],
+                                          const double4 &pi, int il, int k, T &dx, T &dy, T &dz,
+                                          int &jl) {
+  const unsigned e = (unsigned)V.nbr[(size_t)k * V.nc + il];
+  jl = (int)(e & NB_IDX_MASK);
+  const double4 pj = A.pos4[V.a0 + jl];
+  const int code = (int)(e >> NB_CODE_SHIFT);
+  dx = (T)(pi.x - pj.x + S[code][0]);
+  dy = (T)(pi.y - pj.y + S[code][1]);
+  dz = (T)(pi.z - pj.z + S[code][2]);
+  return pj;
+}
+
+// densities and embedding (src/dynamics_m.F90:497-579, src/potential_m.F90:148-169):
+// rho (FP64 sum), F' into pos4.w; returns this thread's embedding energy
+template <typename T, int BLOCK>
+__device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellView &V,
+                                const double (*S)[3], bool store_rho) {
+  double e_emb = 0.0;
+  for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
+    const size_t a = (size_t)V.a0 + il;
+    const double4 pi = A.pos4[a];
+    const int nn = A.numn[a];
+    double acc = 0.0;
+    for (int k = 0; k < nn; k++) {
+      T dx, dy, dz, r, ri;
+      int jl;
+      nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
+      r_inv(dx * dx + dy * dy + dz * dz, r, ri);
+      acc += (double)mdk_phi<T>(P, r);
+    }
+    if (store_rho) A.rho[a] = acc;
+    double f = 0.0, df = 0.0;
+    if (acc > 1e-40) {
+      mdk_embed_f(P, acc, f, df);
+      e_emb += f;
+    }
+    A.pos4[a].w = df;
+  }
+  return e_emb;
+}
+
+// forces and virial (src/dynamics_m.F90:235-346).  With LOSS, also the force
+// seed dL/dF_i = 2 f_weight (F_i - F_t,i) into gvec and this thread's share of
+// sum |F_i - F_t,i|^2 in *fsq.
+template <typename T, int BLOCK, bool LOSS>
+__device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView &V,
+                            const double (*S)[3], double &e_pair, double *w, double *fsq) {
+  for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
+    const size_t a = (size_t)V.a0 + il;
+    const double4 pi = A.pos4[a];
+    const T dfi = (T)pi.w;
+    const int nn = A.numn[a];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int k = 0; k < nn; k++) {
+      T dx, dy, dz, r, rr;
+      int jl;
+      const double4 pj = nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
+      r_inv(dx * dx + dy * dy + dz * dz, r, rr);
+      T vv, dv;
+      mdk_vee<T>(P, r, vv, dv);
+      const T dph = mdk_dphi<T>(P, r);
+      const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj.w) * rr);
+      const double fpd = (double)fp, ddx = (double)dx, ddy = (double)dy, ddz = (double)dz;
+      e_pair += 0.5 * (double)vv;
+      fx += ddx * fpd;
+      fy += ddy * fpd;
+      fz += ddz * fpd;
+      const double h = 0.5 * fpd;
+      w[0] += ddx * ddx * h; w[1] += ddy * ddy * h; w[2] += ddz * ddz * h;
+      w[3] += ddx * ddy * h; w[4] += ddx * ddz * h; w[5] += ddy * ddz * h;
+    }
+    const size_t u = (size_t)A.perm[a];
+    A.out_f[3 * u] = fx;
+    A.out_f[3 * u + 1] = fy;
+    A.out_f[3 * u + 2] = fz;
+    if (LOSS) {
+      const double ex = fx - A.f_in[3 * u], ey = fy - A.f_in[3 * u + 1], ez = fz - A.f_in[3 * u + 2];
+      *fsq += ex * ex + ey * ey + ez * ez;
+      const double g = 2.0 * A.f_weight;
+      A.gvec[3 * a] = g * ex;
+      A.gvec[3 * a + 1] = g * ey;
+      A.gvec[3 * a + 2] = g * ez;
+    }
+  }
+}
+
+// Adjoint of one cell.  With seeds e_c = dL/dE_c and G_i = dL/dF_i, F_i the
+// force on atom i, r_ij = R_i - R_j and u_ij = (G_i - G_j) . r_ij / r:
+//   dL/dtheta = sum_i sum_j { e_c [ V_theta/2 + F'_i phi_theta ]
+//                             - [ V'_theta + (F'_i + F'_j) phi'_theta ] u_ij / 2
+//                             - lambda_i phi_theta },
+//   lambda_i = F''(rho_i) Q_i,  Q_i = sum_j phi'(r_ij) u_ij,  F'' = -2 F'^3,
+// over the full list (every pair from both ends).  The lambda term is
+// accumulated as lambda_i d(rho_i)/dtheta after the neighbour loop, so one
+// pass over the list suffices.  F' comes from pos4.w, recomputed by the
+// caller's density phase; G from gvec.
+template <typename T, int BLOCK>
+__device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev &D,
+                              const CellView &V, const double (*S)[3], double ec, double *ga,
+                              double *gr, double *gA, double *gR) {
+  PotView<T> Q(P);
+  for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
+    const size_t a = (size_t)V.a0 + il;
+    const double4 pi = A.pos4[a];
+    const double gix = A.gvec[3 * a], giy = A.gvec[3 * a + 1], giz = A.gvec[3 * a + 2];
+    const T fpi = (T)pi.w;
+    const int nn = A.numn[a];
+    double q = 0.0, drA[MDK_MAXP], drR[MDK_MAXP];
+#pragma unroll
+    for (int m = 0; m < MDK_MAXP; m++) { drA[m] = 0.0; drR[m] = 0.0; }
+    for (int k = 0; k < nn; k++) {
+      T dx, dy, dz, r, ri;
+      int jl;
+      const double4 pj = nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
+      r_inv(dx * dx + dy * dy + dz * dz, r, ri);
+      const size_t aj = (size_t)V.a0 + jl;
+      const T u = ((T)(gix - A.gvec[3 * aj]) * dx + (T)(giy - A.gvec[3 * aj + 1]) * dy +
+                   (T)(giz - A.gvec[3 * aj + 2]) * dz) * ri;
+      const T dph = mdk_dphi<T>(P, r);
+      q += (double)(dph * u);
+      const T cv = T(0.5) * (T)ec, cvp = T(-0.5) * u;
+      const T cp = (T)ec * fpi, cpp = T(-0.5) * (fpi + (T)pj.w) * u;
+      // pair parameters
+      if (r > Q.bc2()) {
+#pragma unroll
+        for (int m = 0; m < MDK_MAXV; m++) {
+          if (m < P.nv) {
+            const T d = Q.rv(m) - r;
+            if (d > T(0)) {
+              const T d2 = d * d;
+              ga[m] += (double)(cv * d2 * d - T(3) * cvp * d2);
+              gr[m] += (double)(Q.av(m) * (T(3) * cv * d2 - T(6) * cvp * d));
+            }
+          }
+        }
+      } else if (r > Q.bc1()) {
+        // log-space Hermite join: only x2 and v2 depend on the knots
+        const double h = P.bc2 - P.bc1, t = ((double)r - P.bc1) / h;
+        const double t2 = t * t, t3 = t2 * t;
+        const double sp = (2 * t3 - 3 * t2 + 1) * P.jx1 + (t3 - 2 * t2 + t) * P.jv1 * h +
+                          (-2 * t3 + 3 * t2) * P.jx2 + (t3 - t2) * P.jv2 * h;
+        const double ds = ((6 * t2 - 6 * t) * P.jx1 + (3 * t2 - 4 * t + 1) * P.jv1 * h +
+                           (-6 * t2 + 6 * t) * P.jx2 + (3 * t2 - 2 * t) * P.jv2 * h) / h;
+        const double ve = exp(sp), vpe = ds * ve;
+        const double h01 = -2 * t3 + 3 * t2, h11 = (t3 - t2) * h;
+        const double d01 = (-6 * t2 + 6 * t) / h, d11 = 3 * t2 - 2 * t;
+#pragma unroll
+        for (int m = 0; m < MDK_MAXV; m++) {
+          if (m < P.nv) {
+            const double sa = h01 * D.xa[m] + h11 * D.ya[m], sr = h01 * D.xr[m] + h11 * D.yr[m];
+            const double pa = d01 * D.xa[m] + d11 * D.ya[m], pr = d01 * D.xr[m] + d11 * D.yr[m];
+            ga[m] += (double)cv * (ve * sa) + (double)cvp * (pa * ve + vpe * sa);
+            gr[m] += (double)cv * (ve * sr) + (double)cvp * (pr * ve + vpe * sr);
+          }
+        }
+      }
+      // density parameters
+      if (r < Q.pcr()) {
+#pragma unroll
+        for (int m = 0; m < MDK_MAXP; m++) {
+          if (m < P.np) {
+            gA[m] += (double)cp * D.cpa[m];
+            gR[m] += (double)cp * D.cpr[m];
+            drA[m] += D.cpa[m];
+            drR[m] += D.cpr[m];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < MDK_MAXP; m++) {
+          if (m < P.np) {
+            const T d = Q.rp(m) - r;
+            if (d > T(0)) {
+              const T d2 = d * d, pa = d2 * d, pr = T(3) * Q.ap(m) * d2;
+              gA[m] += (double)(cp * pa - T(3) * cpp * d2);
+              gR[m] += (double)(cp * pr - T(6) * cpp * Q.ap(m) * d);
+              drA[m] += (double)pa;
+              drR[m] += (double)pr;
+            }
+          }
+        }
+      }
+    }
+    const double lam = -2.0 * pi.w * pi.w * pi.w * q;  // F''(rho_i) Q_i
+#pragma unroll
+    for (int m = 0; m < MDK_MAXP; m++) {
+      gA[m] -= lam * drA[m];
+      gR[m] -= lam * drR[m];
+    }
+  }
+}
+
+// cell energy/virial reduction and output (fixed order: lanes, then waves);
+// returns the cell energy to every thread
+template <int BLOCK>
+__device__ double store_cell(const BatchArgs &A, const CellView &V, double e, const double *w,
+                             double (*red)[7]) {
+  const int t = threadIdx.x;
+  double v[7] = {e, w[0], w[1], w[2], w[3], w[4], w[5]};
+#pragma unroll
+  for (int k = 0; k < 7; k++) v[k] = bwave_sum(v[k]);
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; k++) red[t >> 6][k] = v[k];
+  }
+  __syncthreads();
+  double etot = red[0][0];
+  for (int q2 = 1; q2 < BLOCK / 64; q2++) etot += red[q2][0];
+  if (t < 7) {
+    double sum = red[0][t];
+    for (int q2 = 1; q2 < BLOCK / 64; q2++) sum += red[q2][t];
+    if (t == 0) A.out_e[V.c] = sum;
+    else A.out_w[6 * (size_t)V.c + t - 1] = sum;
+  }
+  __syncthreads();
+  return etot;
+}
+
+// Forward: energies, forces and virials of every cell.
 template <typename T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_batch_eval(BatchArgs A, PotDev P) {
   __shared__ double S[32][3];
   __shared__ double red[BLOCK / 64][7];
   __shared__ int sh_cell;
-  const int t = threadIdx.x;
-  for (;;) {
-    if (t == 0) sh_cell = atomicAdd(A.counter, 1);
+  CellView V;
+  while (next_cell(A, &sh_cell, V)) {
+    phase_positions<BLOCK>(A, V, S);
     __syncthreads();
-    const int q = sh_cell;
+    phase_list<BLOCK>(A, V, S);
     __syncthreads();
-    if (q >= A.ncell) break;  // uniform over the workgroup: every wave exits
-    const int c = A.order[q];
-    const int a0 = A.cell_off[c], nc = A.cell_off[c + 1] - a0;
-    const double *B = A.box + 9 * (size_t)c;
-    double b[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) b[k] = B[k];
-    const int nx = A.nsub[3 * c], ny = A.nsub[3 * c + 1], nz = A.nsub[3 * c + 2];
-    const int *sst = A.sub_start + A.sub_off[c];
-    int *nbr = A.nbr + (size_t)a0 * A.maxnb;
-    // image shift table of this cell; codes 27..31 never occur here
-    for (int k = t; k < 32; k += BLOCK) {
-      if (k >= 27) {
-        S[k][0] = 1.0e6; S[k][1] = 0.0; S[k][2] = 0.0;
-      } else {
-        const double sx = (double)(k % 3 - 1), sy = (double)((k / 3) % 3 - 1), sz = (double)(k / 9 - 1);
-        S[k][0] = b[0] * sx + b[3] * sy + b[6] * sz;
-        S[k][1] = b[1] * sx + b[4] * sy + b[7] * sz;
-        S[k][2] = b[2] * sx + b[5] * sy + b[8] * sz;
-      }
-    }
-    // phase 1: Cartesian positions
-    for (int il = t; il < nc; il += BLOCK) {
-      const size_t a = (size_t)a0 + il;
-      const double sx = A.s[3 * a], sy = A.s[3 * a + 1], sz = A.s[3 * a + 2];
-      A.pos4[a] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
-                               b[2] * sx + b[5] * sy + b[8] * sz, 0.0);
-    }
+    const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
     __syncthreads();
-    // phase 2: full neighbour list within the cell and its periodic images
-    const int rx_ = nx >= 3 ? 1 : 0, ry_ = ny >= 3 ? 1 : 0, rz_ = nz >= 3 ? 1 : 0;
-    for (int il = t; il < nc; il += BLOCK) {
-      const size_t a = (size_t)a0 + il;
-      const int sc = A.subc[a];
-      const int cx = sc % nx, cy = (sc / nx) % ny, cz = sc / (nx * ny);
-      const double4 pi = A.pos4[a];
-      const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
-      int cnt = 0;
-      for (int dz = -rz_; dz <= rz_; dz++) {
-        int kz = cz + dz, wz = 0;
-        if (kz < 0) { kz += nz; wz = 1; } else if (kz >= nz) { kz -= nz; wz = -1; }
-        for (int dy = -ry_; dy <= ry_; dy++) {
-          int ky = cy + dy, wy = 0;
-          if (ky < 0) { ky += ny; wy = 1; } else if (ky >= ny) { ky -= ny; wy = -1; }
-          for (int dx = -rx_; dx <= rx_; dx++) {
-            int kx = cx + dx, wx = 0;
-            if (kx < 0) { kx += nx; wx = 1; } else if (kx >= nx) { kx -= nx; wx = -1; }
-            const int cj = kx + nx * (ky + ny * kz);
-            for (int jl = sst[cj]; jl < sst[cj + 1]; jl++) {
-              if (jl == il) continue;
-              const size_t aj = (size_t)a0 + jl;
-              int tx = wx, ty = wy, tz = wz;
-              if (!rx_) tx = -(int)round(sxi - A.s[3 * aj]);
-              if (!ry_) ty = -(int)round(syi - A.s[3 * aj + 1]);
-              if (!rz_) tz = -(int)round(szi - A.s[3 * aj + 2]);
-              const int code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
-              const double4 pj = A.pos4[aj];
-              const double ex = pi.x - pj.x + S[code][0], ey = pi.y - pj.y + S[code][1],
-                           ez = pi.z - pj.z + S[code][2];
-              if (ex * ex + ey * ey + ez * ez < A.rc2) {
-                if (cnt < A.maxnb) nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
-                cnt++;
-              }
-            }
-          }
-        }
-      }
-      A.numn[a] = min(cnt, A.maxnb);
-      if (cnt > A.maxnb) atomicMax(&A.flags[0], cnt);
-    }
-    __syncthreads();
-    // phase 3: densities and embedding (src/dynamics_m.F90:497-579, src/potential_m.F90:148-169)
-    double e_emb = 0.0;
-    for (int il = t; il < nc; il += BLOCK) {
-      const size_t a = (size_t)a0 + il;
-      const double4 pi = A.pos4[a];
-      const int nn = A.numn[a];
-      double acc = 0.0;
-      for (int k = 0; k < nn; k++) {
-        const unsigned e = (unsigned)nbr[(size_t)k * nc + il];
-        const double4 pj = A.pos4[a0 + (e & NB_IDX_MASK)];
-        const int code = (int)(e >> NB_CODE_SHIFT);
-        const T dx = (T)(pi.x - pj.x + S[code][0]), dy = (T)(pi.y - pj.y + S[code][1]),
-                dz = (T)(pi.z - pj.z + S[code][2]);
-        T r, ri;
-        r_inv(dx * dx + dy * dy + dz * dz, r, ri);
-        acc += (double)mdk_phi<T>(P, r);
-      }
-      A.rho[a] = acc;
-      double f = 0.0, df = 0.0;
-      if (acc > 1e-40) {
-        mdk_embed_f(P, acc, f, df);
-        e_emb += f;
-      }
-      A.pos4[a].w = df;
-    }
-    __syncthreads();
-    // phase 4: forces and virial (src/dynamics_m.F90:235-346)
     double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
-    for (int il = t; il < nc; il += BLOCK) {
-      const size_t a = (size_t)a0 + il;
-      const double4 pi = A.pos4[a];
-      const T dfi = (T)pi.w;
-      const int nn = A.numn[a];
-      double fx = 0.0, fy = 0.0, fz = 0.0;
-      for (int k = 0; k < nn; k++) {
-        const unsigned e = (unsigned)nbr[(size_t)k * nc + il];
-        const double4 pj = A.pos4[a0 + (e & NB_IDX_MASK)];
-        const int code = (int)(e >> NB_CODE_SHIFT);
-        const T dx = (T)(pi.x - pj.x + S[code][0]), dy = (T)(pi.y - pj.y + S[code][1]),
-                dz = (T)(pi.z - pj.z + S[code][2]);
-        T r, rr;
-        r_inv(dx * dx + dy * dy + dz * dz, r, rr);
-        T vv, dv;
-        mdk_vee<T>(P, r, vv, dv);
-        const T dph = mdk_dphi<T>(P, r);
-        const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj.w) * rr);
-        const double fpd = (double)fp, ddx = (double)dx, ddy = (double)dy, ddz = (double)dz;
-        e_pair += 0.5 * (double)vv;
-        fx += ddx * fpd;
-        fy += ddy * fpd;
-        fz += ddz * fpd;
-        const double h = 0.5 * fpd;
-        w[0] += ddx * ddx * h; w[1] += ddy * ddy * h; w[2] += ddz * ddz * h;
-        w[3] += ddx * ddy * h; w[4] += ddx * ddz * h; w[5] += ddy * ddz * h;
-      }
-      const size_t u = (size_t)A.perm[a];
-      A.out_f[3 * u] = fx;
-      A.out_f[3 * u + 1] = fy;
-      A.out_f[3 * u + 2] = fz;
-    }
-    // per-cell reduction (fixed order: lanes, then waves)
-    double v[7] = {e_emb + e_pair, w[0], w[1], w[2], w[3], w[4], w[5]};
-#pragma unroll
-    for (int k = 0; k < 7; k++) v[k] = bwave_sum(v[k]);
-    if ((t & 63) == 0) {
-#pragma unroll
-      for (int k = 0; k < 7; k++) red[t >> 6][k] = v[k];
-    }
-    __syncthreads();
-    if (t < 7) {
-      double sum = red[0][t];
-      for (int q2 = 1; q2 < BLOCK / 64; q2++) sum += red[q2][t];
-      if (t == 0) A.out_e[c] = sum;
-      else A.out_w[6 * (size_t)c + t - 1] = sum;
-    }
-    __syncthreads();
+    phase_force<T, BLOCK, false>(A, P, V, S, e_pair, w, nullptr);
+    store_cell<BLOCK>(A, V, e_emb + e_pair, w, red);
   }
 }
 
+// per-cell gradient partials (fixed order) into out_g[c*NGRAD_SLOTS + slot],
+// slots packed as a_0..a_{nv-1}, r_0.., A_0..A_{np-1}, R_0..
+template <int BLOCK>
+__device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &V, const double *ga,
+                           const double *gr, const double *gA, const double *gR, double *red) {
+  double *out = A.out_g + (size_t)V.c * NGRAD_SLOTS;
+#pragma unroll
+  for (int m = 0; m < MDK_MAXV; m++) {
+    if (m < P.nv) {
+      const double sa = block_sum<BLOCK>(ga[m], red);
+      const double sr = block_sum<BLOCK>(gr[m], red);
+      if (threadIdx.x == 0) {
+        out[m] = sa;
+        out[P.nv + m] = sr;
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < MDK_MAXP; m++) {
+    if (m < P.np) {
+      const double sa = block_sum<BLOCK>(gA[m], red);
+      const double sr = block_sum<BLOCK>(gR[m], red);
+      if (threadIdx.x == 0) {
+        out[2 * P.nv + m] = sa;
+        out[2 * P.nv + P.np + m] = sr;
+      }
+    }
+  }
+}
+
+// Loss and its gradient in one pass per cell:
+//   L_c = (E_c - E_t,c)^2 / N_c^2 + f_weight sum_i |F_i - F_t,i|^2,
+// forward phases, seeds e_c = 2 (E_c - E_t,c) / N_c^2 and G_i, then the
+// densities are recomputed and the adjoint phase runs.
+// VJP=true instead takes the seeds from e_in / f_in and reuses the neighbour
+// list of the last forward launch.
+template <typename T, int BLOCK, bool VJP>
+__global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
+  __shared__ double S[32][3];
+  __shared__ double red[BLOCK / 64][7];
+  __shared__ double red1[BLOCK / 64];
+  __shared__ int sh_cell;
+  CellView V;
+  while (next_cell(A, &sh_cell, V)) {
+    phase_positions<BLOCK>(A, V, S);
+    double ec;
+    if (VJP) {
+      for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
+        const size_t a = (size_t)V.a0 + il, u = (size_t)A.perm[a];
+        A.gvec[3 * a] = A.f_in[3 * u];
+        A.gvec[3 * a + 1] = A.f_in[3 * u + 1];
+        A.gvec[3 * a + 2] = A.f_in[3 * u + 2];
+      }
+      ec = A.e_in[V.c];
+      __syncthreads();
+    } else {
+      __syncthreads();
+      phase_list<BLOCK>(A, V, S);
+      __syncthreads();
+      const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
+      __syncthreads();
+      double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0}, fsq = 0.0;
+      phase_force<T, BLOCK, true>(A, P, V, S, e_pair, w, &fsq);
+      const double fs = block_sum<BLOCK>(fsq, red1);
+      const double ecell = store_cell<BLOCK>(A, V, e_emb + e_pair, w, red);
+      const double inv_n2 = 1.0 / ((double)V.nc * (double)V.nc);
+      const double de = ecell - A.e_in[V.c];
+      ec = 2.0 * de * inv_n2;
+      if (threadIdx.x == 0) A.out_loss[V.c] = de * de * inv_n2 + A.f_weight * fs;
+    }
+    // backward: densities recomputed from the positions (F' into pos4.w)
+    phase_density<T, BLOCK>(A, P, V, S, false);
+    __syncthreads();
+    double ga[MDK_MAXV], gr[MDK_MAXV], gA[MDK_MAXP], gR[MDK_MAXP];
+#pragma unroll
+    for (int m = 0; m < MDK_MAXV; m++) { ga[m] = 0.0; gr[m] = 0.0; }
+#pragma unroll
+    for (int m = 0; m < MDK_MAXP; m++) { gA[m] = 0.0; gR[m] = 0.0; }
+    phase_adjoint<T, BLOCK>(A, P, D, V, S, ec, ga, gr, gA, gR);
+    store_grad<BLOCK>(A, P, V, ga, gr, gA, gR, red1);
+  }
+}
+
+// Sums the per-cell partials over cells in a fixed order: block k reduces
+// slot k (k < nslot) of out_g, block nslot the per-cell losses.
+__global__ void __launch_bounds__(256) k_grad_final(const double *out_g, const double *out_loss,
+                                                    int ncell, int nslot, double *res) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < ncell; c += 256)
+    s += k < nslot ? out_g[(size_t)c * NGRAD_SLOTS + k] : out_loss[c];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) res[k] = s;
+}
 }  // namespace
+
+// One captured launch sequence (forward, or loss + gradient).
+struct GraphSlot {
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+  bool valid;
+};
 
 struct mdk_batch {
   int ncell, natom, precision, maxnb;
@@ -267,16 +574,18 @@ struct mdk_batch {
   int *order, *cell_off, *nsub, *sub_off, *sub_start, *subc, *perm, *numn, *nbr, *counter, *flags;
   double *box, *s, *rho, *out_e, *out_w, *out_f;
   double4 *pos4;
+  // adjoint buffers: targets, product seeds, force seeds (sorted order),
+  // per-cell gradient and loss partials, reduced results
+  double *e_tgt, *f_tgt, *e_seed, *f_seed, *gvec, *out_g, *out_loss, *res;
   std::vector<int> h_cell_off;
   std::vector<double> h_box;
   PotDev pot;
+  GradDev gd;
   hostpot::Knots knots;
-  bool pot_set;
+  bool pot_set, targets_set, list_valid;
   int block, wg_per_cu;
   bool use_graph;
-  hipGraph_t graph;
-  hipGraphExec_t gexec;
-  bool graph_valid;
+  GraphSlot g_eval, g_grad;
   double rc;
   double min_width;  // narrowest perpendicular cell width: the minimum image needs 2 rc below it
 };
@@ -293,6 +602,17 @@ void launch_eval(int block, int grid, hipStream_t st, const BatchArgs &A, const 
   }
 }
 
+template <typename T, bool VJP>
+void launch_grad(int block, int grid, hipStream_t st, const BatchArgs &A, const PotDev &P,
+                 const GradDev &D) {
+  switch (block) {
+    case 64: k_batch_grad<T, 64, VJP><<<grid, 64, 0, st>>>(A, P, D); break;
+    case 128: k_batch_grad<T, 128, VJP><<<grid, 128, 0, st>>>(A, P, D); break;
+    case 512: k_batch_grad<T, 512, VJP><<<grid, 512, 0, st>>>(A, P, D); break;
+    default: k_batch_grad<T, 256, VJP><<<grid, 256, 0, st>>>(A, P, D); break;
+  }
+}
+
 BatchArgs args_of(mdk_batch *b) {
   BatchArgs A;
   A.ncell = b->ncell; A.maxnb = b->maxnb; A.order = b->order; A.cell_off = b->cell_off;
@@ -300,44 +620,86 @@ BatchArgs args_of(mdk_batch *b) {
   A.subc = b->subc; A.s = b->s; A.perm = b->perm; A.pos4 = b->pos4; A.nbr = b->nbr;
   A.numn = b->numn; A.rho = b->rho; A.out_e = b->out_e; A.out_w = b->out_w; A.out_f = b->out_f;
   A.counter = b->counter; A.flags = b->flags; A.rc2 = b->rc * b->rc;
+  A.ngrad = 2 * b->knots.nv + 2 * b->knots.np;
+  A.e_in = b->e_tgt; A.f_in = b->f_tgt; A.gvec = b->gvec; A.out_g = b->out_g;
+  A.out_loss = b->out_loss; A.f_weight = 0.1 / (double)b->natom;
   return A;
 }
 
-int enqueue(mdk_batch *b) {
-  k_batch_reset<<<1, 64, 0, b->stream>>>(b->counter);
-  const int grid = std::max(1, std::min(b->ncell, b->ncu * b->wg_per_cu));
-  if (b->precision == MDK_PREC_FP64)
-    launch_eval<double>(b->block, grid, b->stream, args_of(b), b->pot);
-  else
-    launch_eval<float>(b->block, grid, b->stream, args_of(b), b->pot);
+int grid_of(const mdk_batch *b) { return std::max(1, std::min(b->ncell, b->ncu * b->wg_per_cu)); }
+
+int launch_status() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return berr(MDK_ERR_HIP, "batch launch: %s", hipGetErrorString(e));
   return MDK_OK;
 }
 
-void drop_graph(mdk_batch *b) {
-  if (b->graph_valid) {
-    hipGraphExecDestroy(b->gexec);
-    hipGraphDestroy(b->graph);
-    b->graph_valid = false;
+int enqueue(mdk_batch *b) {
+  k_batch_reset<<<1, 64, 0, b->stream>>>(b->counter);
+  if (b->precision == MDK_PREC_FP64)
+    launch_eval<double>(b->block, grid_of(b), b->stream, args_of(b), b->pot);
+  else
+    launch_eval<float>(b->block, grid_of(b), b->stream, args_of(b), b->pot);
+  return launch_status();
+}
+
+// loss + gradient (vjp = false, from the targets) or the vector-Jacobian
+// product with the seeds e_seed/f_seed (vjp = true), then the fixed-order
+// reduction over cells into res[0..ngrad) (and res[ngrad] = loss)
+int enqueue_grad(mdk_batch *b, bool vjp) {
+  BatchArgs A = args_of(b);
+  if (vjp) {
+    A.e_in = b->e_seed;
+    A.f_in = b->f_seed;
+  }
+  k_batch_reset<<<1, 64, 0, b->stream>>>(b->counter);
+  const int grid = grid_of(b);
+  if (b->precision == MDK_PREC_FP64) {
+    if (vjp) launch_grad<double, true>(b->block, grid, b->stream, A, b->pot, b->gd);
+    else launch_grad<double, false>(b->block, grid, b->stream, A, b->pot, b->gd);
+  } else {
+    if (vjp) launch_grad<float, true>(b->block, grid, b->stream, A, b->pot, b->gd);
+    else launch_grad<float, false>(b->block, grid, b->stream, A, b->pot, b->gd);
+  }
+  k_grad_final<<<A.ngrad + (vjp ? 0 : 1), 256, 0, b->stream>>>(b->out_g, b->out_loss, b->ncell,
+                                                               A.ngrad, b->res);
+  return launch_status();
+}
+
+void drop_slot(GraphSlot &g) {
+  if (g.valid) {
+    hipGraphExecDestroy(g.gexec);
+    hipGraphDestroy(g.graph);
+    g.valid = false;
   }
 }
 
-int capture(mdk_batch *b) {
-  drop_graph(b);
+void drop_graph(mdk_batch *b) {
+  drop_slot(b->g_eval);
+  drop_slot(b->g_grad);
+}
+
+// Captures the launches of enqueue (grad = false) or of the loss + gradient
+// (grad = true) into a graph slot.
+int capture(mdk_batch *b, bool grad) {
+  GraphSlot &gs = grad ? b->g_grad : b->g_eval;
+  drop_slot(gs);
   BCK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue(b);
+  int rc = grad ? enqueue_grad(b, false) : enqueue(b);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (rc != MDK_OK) return rc;
+  if (rc != MDK_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
   if (e != hipSuccess) return berr(MDK_ERR_HIP, "capture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&gs.gexec, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     hipGraphDestroy(g);
     return berr(MDK_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
   }
-  b->graph = g;
-  b->graph_valid = true;
+  gs.graph = g;
+  gs.valid = true;
   return MDK_OK;
 }
 
@@ -352,7 +714,8 @@ extern "C" int mdk_batch_destroy(void *h) {
   drop_graph(b);
   void *p[] = {b->order, b->cell_off, b->nsub, b->sub_off, b->sub_start, b->subc, b->perm,
                b->numn, b->nbr, b->counter, b->flags, b->box, b->s, b->rho, b->out_e, b->out_w,
-               b->out_f, b->pos4};
+               b->out_f, b->pos4, b->e_tgt, b->f_tgt, b->e_seed, b->f_seed, b->gvec, b->out_g,
+               b->out_loss, b->res};
   for (void *q : p)
     if (q) hipFree(q);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -473,6 +836,14 @@ extern "C" int mdk_batch_create(void **handle, int ncell, const int *natoms, con
   BALLOC(b->out_e, sizeof(double) * ncell);
   BALLOC(b->out_w, sizeof(double) * 6 * (size_t)ncell);
   BALLOC(b->out_f, sizeof(double) * 3 * (size_t)natom);
+  BALLOC(b->e_tgt, sizeof(double) * ncell);
+  BALLOC(b->f_tgt, sizeof(double) * 3 * (size_t)natom);
+  BALLOC(b->e_seed, sizeof(double) * ncell);
+  BALLOC(b->f_seed, sizeof(double) * 3 * (size_t)natom);
+  BALLOC(b->gvec, sizeof(double) * 3 * (size_t)natom);
+  BALLOC(b->out_g, sizeof(double) * NGRAD_SLOTS * (size_t)ncell);
+  BALLOC(b->out_loss, sizeof(double) * ncell);
+  BALLOC(b->res, sizeof(double) * (NGRAD_SLOTS + 1));
   BALLOC(b->counter, sizeof(int));
   BALLOC(b->flags, sizeof(int) * 4);
   hipMemcpy(b->order, horder.data(), sizeof(int) * ncell, hipMemcpyHostToDevice);
@@ -508,12 +879,14 @@ extern "C" int mdk_batch_set_potential(void *h, int nv, const double *av, const 
   for (int i = 0; i < np; i++) { k.ap[i] = ap[i]; k.rp[i] = rp[i]; }
   hostpot::set_up(k);
   const double rint = hostpot::fill_potdev(b->pot, k);
+  hostpot::grad_consts(k, b->gd);
   if (rint > 4.5) return berr(MDK_ERR_BAD_ARG, "knots beyond 4.5 A are not supported by the batch grid");
   if (2.0 * rint >= b->min_width)
     return berr(MDK_ERR_BAD_ARG, "a cell is narrower (%.3f A) than twice the interaction range", b->min_width);
   b->rc = rint;
   b->pot_set = true;
-  drop_graph(b);  // the potential is a kernel argument of the captured launch
+  b->list_valid = false;
+  drop_graph(b);  // the potential is a kernel argument of the captured launches
   return MDK_OK;
 }
 
@@ -533,15 +906,16 @@ extern "C" int mdk_batch_evaluate(void *h) {
   mdk_batch *b = (mdk_batch *)h;
   if (!b || !b->pot_set) return berr(MDK_ERR_STATE, "batch potential not set");
   if (b->use_graph) {
-    if (!b->graph_valid) {
-      int rc = capture(b);
+    if (!b->g_eval.valid) {
+      int rc = capture(b, false);
       if (rc != MDK_OK) return rc;
     }
-    BCK(hipGraphLaunch(b->gexec, b->stream));
+    BCK(hipGraphLaunch(b->g_eval.gexec, b->stream));
   } else {
     int rc = enqueue(b);
     if (rc != MDK_OK) return rc;
   }
+  b->list_valid = true;
   return MDK_OK;
 }
 
@@ -663,4 +1037,105 @@ extern "C" int mdk_batch_info(void *h, int *natom, int *block, int *wg_per_cu, l
     *pairs = s;
   }
   return MDK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: d(loss)/d(knot parameters).  Gradient order: a_0..a_{nv-1},
+// r_0..r_{nv-1}, A_0..A_{np-1}, R_0..R_{np-1} (2 nv + 2 np values).
+// ---------------------------------------------------------------------------
+
+// Targets of the loss: e_target[ncell] (eV) and f_target[3*natom] (Cartesian
+// eV/A, the caller's atom order).
+extern "C" int mdk_batch_set_targets(void *h, const double *e_target, const double *f_target) {
+  mdk_batch *b = (mdk_batch *)h;
+  if (!b || !e_target || !f_target) return berr(MDK_ERR_BAD_ARG, "bad arguments");
+  BCK(hipMemcpyAsync(b->e_tgt, e_target, sizeof(double) * b->ncell, hipMemcpyHostToDevice, b->stream));
+  BCK(hipMemcpyAsync(b->f_tgt, f_target, sizeof(double) * 3 * (size_t)b->natom, hipMemcpyHostToDevice,
+                     b->stream));
+  BCK(hipStreamSynchronize(b->stream));
+  b->targets_set = true;
+  return MDK_OK;
+}
+
+namespace {
+int read_res(mdk_batch *b, int n, double *out) {
+  int rc = mdk_batch_synchronize(b);
+  if (rc != MDK_OK) return rc;
+  BCK(hipMemcpy(out, b->res, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return MDK_OK;
+}
+int enqueue_loss_grad(mdk_batch *b) {
+  if (!b->pot_set || !b->targets_set) return berr(MDK_ERR_STATE, "batch potential or targets not set");
+  if (b->use_graph) {
+    if (!b->g_grad.valid) {
+      int rc = capture(b, true);
+      if (rc != MDK_OK) return rc;
+    }
+    BCK(hipGraphLaunch(b->g_grad.gexec, b->stream));
+  } else {
+    int rc = enqueue_grad(b, false);
+    if (rc != MDK_OK) return rc;
+  }
+  b->list_valid = true;
+  return MDK_OK;
+}
+}  // namespace
+
+// loss = sum_c (E_c - E_t,c)^2 / N_c^2 + (0.1 / natom) sum_i |F_i - F_t,i|^2 and
+// grad = d(loss)/d(knot parameters), in one persistent launch (forward, seeds,
+// densities recomputed, adjoint) plus a fixed-order reduction over cells.
+// Energies and forces of the forward part are left for mdk_batch_results.
+extern "C" int mdk_batch_loss_grad(void *h, double *loss, double *grad) {
+  mdk_batch *b = (mdk_batch *)h;
+  if (!b) return berr(MDK_ERR_BAD_ARG, "null handle");
+  int rc = enqueue_loss_grad(b);
+  if (rc != MDK_OK) return rc;
+  const int ng = 2 * b->knots.nv + 2 * b->knots.np;
+  double r[NGRAD_SLOTS + 1];
+  rc = read_res(b, ng + 1, r);
+  if (rc != MDK_OK) return rc;
+  if (grad) memcpy(grad, r, sizeof(double) * ng);
+  if (loss) *loss = r[ng];
+  return MDK_OK;
+}
+
+// Vector-Jacobian product: grad = sum_c dE[c] dE_c/dtheta + sum_i dF_i . dF_i/dtheta
+// for seeds dE[ncell] and dF[3*natom] (Cartesian, caller's order), at the
+// positions and potential of the last forward evaluation (mdk_batch_evaluate or
+// mdk_batch_loss_grad; its neighbour list is reused, densities are recomputed).
+extern "C" int mdk_batch_vjp(void *h, const double *dE, const double *dF, double *grad) {
+  mdk_batch *b = (mdk_batch *)h;
+  if (!b || !dE || !dF || !grad) return berr(MDK_ERR_BAD_ARG, "bad arguments");
+  if (!b->pot_set || !b->list_valid)
+    return berr(MDK_ERR_STATE, "mdk_batch_vjp needs a forward evaluation with the current potential");
+  BCK(hipMemcpyAsync(b->e_seed, dE, sizeof(double) * b->ncell, hipMemcpyHostToDevice, b->stream));
+  BCK(hipMemcpyAsync(b->f_seed, dF, sizeof(double) * 3 * (size_t)b->natom, hipMemcpyHostToDevice,
+                     b->stream));
+  int rc = enqueue_grad(b, true);
+  if (rc != MDK_OK) return rc;
+  return read_res(b, 2 * b->knots.nv + 2 * b->knots.np, grad);
+}
+
+// Average time of `reps` loss + gradient launches (HIP events on the batch stream), ms.
+extern "C" int mdk_batch_time_loss_grad(void *h, int reps, double *ms_avg) {
+  mdk_batch *b = (mdk_batch *)h;
+  if (!b || reps < 1) return berr(MDK_ERR_BAD_ARG, "bad arguments");
+  hipEvent_t e0, e1;
+  BCK(hipEventCreate(&e0));
+  BCK(hipEventCreate(&e1));
+  int rc = enqueue_loss_grad(b);
+  if (rc != MDK_OK) return rc;
+  BCK(hipEventRecord(e0, b->stream));
+  for (int r = 0; r < reps; r++) {
+    rc = enqueue_loss_grad(b);
+    if (rc != MDK_OK) return rc;
+  }
+  BCK(hipEventRecord(e1, b->stream));
+  BCK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  BCK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (ms_avg) *ms_avg = ms / reps;
+  return mdk_batch_synchronize(h);
 }

@@ -149,5 +149,32 @@ static double fill_potdev(PotDev &P, const Knots &k) {
   P.bier_af = (float)P.bier_a; P.bier_zedf = (float)P.bier_zed;
   return rint;
 }
+
+// Sensitivities of the set_up-derived constants to the knot parameters, for
+// the adjoint.  The join (0.9 < r <= 1.9 A) depends on the knots only through
+// x2 = log V(b2) and v2 = V'(b2)/V(b2) at b2 = 1.9 + eps: X = dx2/dtheta and
+// Y = dv2/dtheta for every pair parameter (a_k, r_k).  Below the maximum r* of
+// phi the density is the constant phi(r*); since phi'(r*) = 0 its derivative is
+// the partial derivative of the knot sum at r* (cpa = d/dA_k, cpr = d/dR_k).
+static void grad_consts(const Knots &k, GradDev &D) {
+  memset(&D, 0, sizeof(D));
+  const double eps = (double)0.000000000001f;
+  const double b = k.bc2 + eps;
+  const double V = knot_v(k, b), Vp = knot_dv(k, b);
+  for (int m = 0; m < k.nv; m++) {
+    const double d = k.rv[m] - b, H = h_step(d);
+    const double dva = d * d * d * H, dvr = 3.0 * k.av[m] * d * d * H;
+    const double dpa = -3.0 * d * d * H, dpr = -6.0 * k.av[m] * d * H;
+    D.xa[m] = dva / V;
+    D.xr[m] = dvr / V;
+    D.ya[m] = dpa / V - Vp * dva / (V * V);
+    D.yr[m] = dpr / V - Vp * dvr / (V * V);
+  }
+  for (int m = 0; m < k.np; m++) {
+    const double d = k.rp[m] - k.pcr, H = h_step(d);
+    D.cpa[m] = d * d * d * H;
+    D.cpr[m] = 3.0 * k.ap[m] * d * d * H;
+  }
+}
 }  // namespace hostpot
 

@@ -38,6 +38,14 @@ struct PotDev {
   float jx1f, jv1f, jx2f, jv2f, bier_af, bier_zedf;
 };
 
+// Knot-parameter sensitivities of the set_up constants (filled on the host):
+// join end values x2 = log V(1.9+eps), v2 = V'/V there, per pair parameter;
+// flat-cap value phi(r*) per density parameter.
+struct GradDev {
+  double xa[MDK_MAXV], xr[MDK_MAXV], ya[MDK_MAXV], yr[MDK_MAXV];
+  double cpa[MDK_MAXP], cpr[MDK_MAXP];
+};
+
 template <typename T> __device__ __forceinline__ T mdk_exp(T x);
 template <> __device__ __forceinline__ double mdk_exp<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float mdk_exp<float>(float x) { return __expf(x); }

@@ -40,6 +40,7 @@ EXPORTS = [
     "mdk_batch_create", "mdk_batch_destroy", "mdk_batch_last_error", "mdk_batch_set_potential",
     "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
+    "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
 ]
 
 
@@ -108,6 +109,10 @@ def load_library(path: str = LIB_PATH):
         "mdk_batch_tune": [vp, C.c_int, ip, ip, dp],
         "mdk_batch_time": [vp, C.c_int, dp],
         "mdk_batch_info": [vp, ip, ip, ip, lp],
+        "mdk_batch_set_targets": [vp, dp, dp],
+        "mdk_batch_loss_grad": [vp, dp, dp],
+        "mdk_batch_vjp": [vp, dp, dp, dp],
+        "mdk_batch_time_loss_grad": [vp, C.c_int, dp],
     })
     for name, args in sig.items():
         f = getattr(L, name)
@@ -360,6 +365,7 @@ class BatchEngine:
     def set_potential(self, c: dict):
         av, rv, ap, rp = (np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp"))
         _bcheck(self.L.mdk_batch_set_potential(self.h, len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp)))
+        self.ngrad = 2 * len(av) + 2 * len(ap)
 
     def config(self, block=256, wg_per_cu=4, graph=True):
         _bcheck(self.L.mdk_batch_config(self.h, block, wg_per_cu, 1 if graph else 0))
@@ -386,6 +392,32 @@ class BatchEngine:
     def time(self, reps=10) -> float:
         ms = C.c_double()
         _bcheck(self.L.mdk_batch_time(self.h, reps, C.byref(ms)))
+        return ms.value
+
+    def set_targets(self, e_target, f_target):
+        """Loss targets: per-cell energies (eV) and Cartesian forces (natom, 3)."""
+        self._e_t = np.ascontiguousarray(e_target, dtype=np.float64).reshape(self.ncell)
+        self._f_t = np.ascontiguousarray(f_target, dtype=np.float64).reshape(3 * self.natom)
+        _bcheck(self.L.mdk_batch_set_targets(self.h, _dp(self._e_t), _dp(self._f_t)))
+
+    def loss_grad(self):
+        """(loss, d loss / d theta) with theta = (a_k, r_k, A_k, R_k)."""
+        loss = C.c_double()
+        g = np.empty(self.ngrad)
+        _bcheck(self.L.mdk_batch_loss_grad(self.h, C.byref(loss), _dp(g)))
+        return loss.value, g
+
+    def vjp(self, dE, dF):
+        """sum_c dE_c dE_c/dtheta + sum_i dF_i . dF_i/dtheta at the last forward evaluation."""
+        de = np.ascontiguousarray(dE, dtype=np.float64).reshape(self.ncell)
+        df = np.ascontiguousarray(dF, dtype=np.float64).reshape(3 * self.natom)
+        g = np.empty(self.ngrad)
+        _bcheck(self.L.mdk_batch_vjp(self.h, _dp(de), _dp(df), _dp(g)))
+        return g
+
+    def time_loss_grad(self, reps=10) -> float:
+        ms = C.c_double()
+        _bcheck(self.L.mdk_batch_time_loss_grad(self.h, reps, C.byref(ms)))
         return ms.value
 
     def info(self):

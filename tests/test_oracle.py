@@ -304,3 +304,25 @@ def test_nve_energy_drift_small(oracle):
         te.append(S.scalar("te"))
     te = np.array(te)
     assert np.max(np.abs(te - te[0])) / sd.n < 2e-4
+
+
+@pytest.mark.parametrize("kind", ["golden_tric", "fcc_strained"])
+def test_eval_cell_matches_force_path(oracle, kind):
+    """orc_eval_cell (the batched engine's checker: brute-force half pairs,
+    Cartesian outputs) against orc_force, which the reference's own output pins
+    bit for bit: same pe, rho, Cartesian forces B0.f, and W B0^-T == tp."""
+    if kind == "golden_tric":
+        sd = golden_system(load_golden("npt_tric_1024"))
+    else:
+        e = np.array([[0.02, 0.01, -0.015], [0.01, -0.025, 0.005], [-0.015, 0.005, 0.01]])
+        sd = lattice.make_system(nc=4, a=2.8665 * 2 ** (1 / 3), basis=lattice.FCC_BASIS, sigma=0.1,
+                                 temp=0.0, seed=21, strain=e, vacancy_frac=0.02)
+    pot = oracle.Potential()
+    S = _pe(oracle, sd, pot)
+    pe, f, rho, w = oracle.eval_cell(pot, sd)
+    assert abs(pe - S.scalar("pe")) < 1e-12 * abs(pe)
+    fr = S.forces_cart()
+    assert np.max(np.abs(f - fr)) < 1e-10 * np.max(np.abs(fr))
+    np.testing.assert_allclose(rho, S.array("rho"), rtol=1e-12)
+    tp = S.tensor("tp")
+    np.testing.assert_allclose(w @ np.linalg.inv(sd.b0).T, tp, rtol=0, atol=1e-10 * np.abs(tp).max())
