@@ -108,6 +108,39 @@ def batch_legs(reps):
     }
 
 
+def copy_calibration():
+    """Plain device copies of known size (1 GiB read + 1 GiB written per
+    dispatch, 3 dispatches each of an elementwise kernel and of the runtime's
+    copy kernel), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to scale
+    those counters to bytes."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": 2 * 4 * (1 << 28)}))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_bcc524288_fp32_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per force launch from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (scaled by the copy calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for ln in fh:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(sd, steps, threads):
     """The reference Fortran (OpenMP) on the host cores, else the C port."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -144,7 +177,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the FETCH_SIZE/WRITE_SIZE calibration copies")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the force path of one periodic cell "
               "on one MI355X); --gpus must be 1", file=sys.stderr)
@@ -185,7 +223,7 @@ def main():
     bytes_per_launch = sd.n * (56.0 + 4.0 * nf)
     achieved = bytes_per_launch / (kf["us_avg"] * 1e-6) / 1e9 if kf["us_avg"] else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": None,
+                "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": measured_traffic(),
                 "kernel": "k_force<float>", "us_per_launch": kf["us_avg"],
                 "bytes_per_launch": bytes_per_launch}
 
