@@ -716,6 +716,21 @@ __device__ __forceinline__ T pair_r2(const double (*S)[3], const double4 &pi, co
   return rx * rx + ry * ry + rz * rz;
 }
 
+// The same with the FP64 components kept (for FP64 accumulation of forces and
+// virial); the arithmetic type T sees them rounded.
+template <typename T>
+__device__ __forceinline__ T pair_sep(const double (*S)[3], const double4 &pi, const double4 &pj,
+                                      int code, double &dx, double &dy, double &dz, T &rx, T &ry,
+                                      T &rz) {
+  dx = pi.x - pj.x + S[code][0];
+  dy = pi.y - pj.y + S[code][1];
+  dz = pi.z - pj.z + S[code][2];
+  rx = (T)dx;
+  ry = (T)dy;
+  rz = (T)dz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
 template <int TPA>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -745,7 +760,7 @@ __global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n,
 // (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
 // with TPA lanes per atom; F, F' per atom; per-block embedding energy.  F' is
 // also stored in pos4[i].w for the force pass.
-template <typename T, int TPA, bool LOOKUP>
+template <typename T, int TPA, int POTM>
 __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
                                                        const int *__restrict__ nbr,
                                                        const int *__restrict__ numn,
@@ -774,7 +789,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
         const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
         T r, ri;
         r_and_inv(r2, r, ri);
-        acc += (double)mdk_pair_phi<T, LOOKUP>(P, r);
+        acc += (double)mdk_pair_phi<T, POTM>(P, r);
       }
     }
   }
@@ -805,7 +820,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
 // W B0^-T in the finishing block, i.e. sum r_a s_b fp) and the pair energy as
 // 1/2 sum V.  TPA lanes per atom; MASSCHK keeps the
 // reference's exclusion of pairs of immobile atoms from the virial.
-template <typename T, int TPA, bool MASSCHK, bool LOOKUP>
+template <typename T, int TPA, bool MASSCHK, int POTM>
 __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
                                                      const int *zat, int n,
                                                      const int *__restrict__ nbr,
@@ -830,6 +845,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
     const T dfi = (T)pi.w;
     const double mi = MASSCHK ? mass[i] : 0.0;
     const int nn = numn[i];
+    // sums of V and of r r^T fp; the factor 1/2 of both is applied once below
     double pe2 = 0.0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
     for (int q = l; 4 * q < nn; q += TPA) {
       const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
@@ -840,29 +856,30 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int j = (int)(e[u] & NBR_IDX_MASK);
+        double dx, dy, dz;
         T rx, ry, rz;
-        const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
+        const T r2 = pair_sep<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), dx, dy, dz, rx, ry, rz);
         T r, rr;
         r_and_inv(r2, r, rr);
         T vv, dv;
-        mdk_pair_vdv<T, LOOKUP>(P, r, vv, dv);
-        const T dph = mdk_pair_dphi<T, LOOKUP>(P, r);
+        mdk_pair_vdv<T, POTM>(P, r, vv, dv);
+        const T dph = mdk_pair_dphi<T, POTM>(P, r);
         const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj[u].w) * rr);
         const double fpd = (double)fp;
-        const double dx = (double)rx, dy = (double)ry, dz = (double)rz;
-        pe2 += 0.5 * (double)vv;
-        fxi += dx * fpd;
-        fyi += dy * fpd;
-        fzi += dz * fpd;
+        const double gx = dx * fpd, gy = dy * fpd, gz = dz * fpd;
+        pe2 += (double)vv;
+        fxi += gx;
+        fyi += gy;
+        fzi += gz;
         if (!MASSCHK || mi + mass[j] > (double)0.1f) {
-          const double h = 0.5 * fpd;
-          wxx += dx * dx * h; wyy += dy * dy * h; wzz += dz * dz * h;
-          wxy += dx * dy * h; wxz += dx * dz * h; wyz += dy * dz * h;
+          wxx += dx * gx; wyy += dy * gy; wzz += dz * gz;
+          wxy += dx * gy; wxz += dx * gz; wyz += dy * gz;
         }
       }
     }
-    v[0] = pe2;
-    v[1] = wxx; v[2] = wyy; v[3] = wzz; v[4] = wxy; v[5] = wxz; v[6] = wyz;
+    v[0] = 0.5 * pe2;
+    v[1] = 0.5 * wxx; v[2] = 0.5 * wyy; v[3] = 0.5 * wzz;
+    v[4] = 0.5 * wxy; v[5] = 0.5 * wxz; v[6] = 0.5 * wyz;
   }
   fxi = group_sum<TPA>(fxi);
   fyi = group_sum<TPA>(fyi);
@@ -1150,10 +1167,10 @@ __global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int 
   double r = x[k], v = 0.0, dv = 0.0;
   const bool lk = P.use_lookup != 0;
   switch (which) {
-    case 0: if (lk) mdk_pair_vdv<double, true>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
-    case 1: if (lk) mdk_pair_vdv<double, true>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
-    case 2: y[k] = lk ? mdk_pair_phi<double, true>(P, r) : mdk_pair_phi<double>(P, r); break;
-    case 3: y[k] = lk ? mdk_pair_dphi<double, true>(P, r) : mdk_pair_dphi<double>(P, r); break;
+    case 0: if (lk) mdk_pair_vdv<double, POT_TABLE>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
+    case 1: if (lk) mdk_pair_vdv<double, POT_TABLE>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
+    case 2: y[k] = lk ? mdk_pair_phi<double, POT_TABLE>(P, r) : mdk_pair_phi<double>(P, r); break;
+    case 3: y[k] = lk ? mdk_pair_dphi<double, POT_TABLE>(P, r) : mdk_pair_dphi<double>(P, r); break;
     case 4: mdk_embed_f(P, r, v, dv); y[k] = v; break;
     case 5: mdk_embed_f(P, r, v, dv); y[k] = dv; break;
   }
@@ -1330,30 +1347,45 @@ static int rebuild_full(mdk_ctx *c) {
   return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
 }
 
-template <typename T, int TPA, bool LK>
+template <typename T, int TPA, int POTM>
 static void density_t(mdk_ctx *c, int do_embed) {
-  k_density<T, TPA, LK><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+  k_density<T, TPA, POTM><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
       c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, c->part_d);
 }
-template <typename T, int TPA, bool MC, bool LK>
+template <typename T, int TPA, bool MC, int POTM>
 static void force_t(mdk_ctx *c) {
-  k_force<T, TPA, MC, LK><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
+  k_force<T, TPA, MC, POTM><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
       c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->part_f);
 }
-template <typename T, bool LK>
+template <typename T, int POTM>
 static void density_dispatch(mdk_ctx *c, int do_embed) {
-  switch (c->tpa_density) {
-    case 1: density_t<T, 1, LK>(c, do_embed); break;
-    case 4: density_t<T, 4, LK>(c, do_embed); break;
-    default: density_t<T, 2, LK>(c, do_embed); break;
+  if (c->tpa_density == 1) density_t<T, 1, POTM>(c, do_embed);
+  else density_t<T, 2, POTM>(c, do_embed);
+}
+template <typename T, bool MC, int POTM>
+static void force_dispatch(mdk_ctx *c) {
+  if (c->tpa_force == 2) force_t<T, 2, MC, POTM>(c);
+  else force_t<T, 1, MC, POTM>(c);
+}
+// lookup tables, the Fe-Fe knot counts, or any knot count
+static int pot_mode(const mdk_ctx *c) {
+  if (c->pot.use_lookup) return POT_TABLE;
+  return (c->pot.nv == MDK_FE_NV && c->pot.np == MDK_FE_NP) ? POT_FE : POT_KNOTS;
+}
+template <typename T>
+static void density_pot(mdk_ctx *c, int do_embed) {
+  switch (pot_mode(c)) {
+    case POT_TABLE: density_dispatch<T, POT_TABLE>(c, do_embed); break;
+    case POT_FE: density_dispatch<T, POT_FE>(c, do_embed); break;
+    default: density_dispatch<T, POT_KNOTS>(c, do_embed); break;
   }
 }
-template <typename T, bool MC, bool LK>
-static void force_dispatch(mdk_ctx *c) {
-  switch (c->tpa_force) {
-    case 2: force_t<T, 2, MC, LK>(c); break;
-    case 4: force_t<T, 4, MC, LK>(c); break;
-    default: force_t<T, 1, MC, LK>(c); break;
+template <typename T, bool MC>
+static void force_pot(mdk_ctx *c) {
+  switch (pot_mode(c)) {
+    case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c); break;
+    case POT_FE: force_dispatch<T, MC, POT_FE>(c); break;
+    default: force_dispatch<T, MC, POT_KNOTS>(c); break;
   }
 }
 
@@ -1385,12 +1417,8 @@ static int launch_density(mdk_ctx *c, int do_embed) {
   RCK(ensure_pos4(c));
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
-  const bool lk = c->pot.use_lookup != 0;
-  if (c->p.precision == MDK_PREC_FP32) {
-    if (lk) density_dispatch<float, true>(c, do_embed); else density_dispatch<float, false>(c, do_embed);
-  } else {
-    if (lk) density_dispatch<double, true>(c, do_embed); else density_dispatch<double, false>(c, do_embed);
-  }
+  if (c->p.precision == MDK_PREC_FP32) density_pot<float>(c, do_embed);
+  else density_pot<double>(c, do_embed);
   tend(c, te);
   return check_launch("k_density");
 }
@@ -1398,13 +1426,11 @@ static int launch_density(mdk_ctx *c, int do_embed) {
 static int launch_force(mdk_ctx *c, int fin_mode) {
   TimedEv te;
   tbegin(c, TC_FORCE, te);
-  const bool f32 = c->p.precision == MDK_PREC_FP32, lk = c->pot.use_lookup != 0, mc = c->masschk;
+  const bool f32 = c->p.precision == MDK_PREC_FP32, mc = c->masschk;
   if (f32) {
-    if (mc) { if (lk) force_dispatch<float, true, true>(c); else force_dispatch<float, true, false>(c); }
-    else { if (lk) force_dispatch<float, false, true>(c); else force_dispatch<float, false, false>(c); }
+    if (mc) force_pot<float, true>(c); else force_pot<float, false>(c);
   } else {
-    if (mc) { if (lk) force_dispatch<double, true, true>(c); else force_dispatch<double, true, false>(c); }
-    else { if (lk) force_dispatch<double, false, true>(c); else force_dispatch<double, false, false>(c); }
+    if (mc) force_pot<double, true>(c); else force_pot<double, false>(c);
   }
   tend(c, te);
   RCK(check_launch("k_force"));
@@ -1521,6 +1547,8 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   c->tpa_force = p->precision == MDK_PREC_FP32 ? 1 : 2;
   if (const char *e = getenv("MDK_TPA_DENSITY")) c->tpa_density = atoi(e);
   if (const char *e = getenv("MDK_TPA_FORCE")) c->tpa_force = atoi(e);
+  c->tpa_density = c->tpa_density == 1 ? 1 : 2;  // compiled lane counts
+  c->tpa_force = c->tpa_force == 2 ? 2 : 1;
   c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
   c->rnear2_exact = near_threshold(p->rnear);
   c->nblk = nblocks(n);

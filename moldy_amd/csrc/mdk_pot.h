@@ -126,20 +126,44 @@ __device__ __noinline__ VdV<T> mdk_join(T a1, T a2, T x1, T v1, T x2, T v2, T r)
   return VdV<T>{e, ds * e};
 }
 
-// Pair function V and V' (exact mode): the knot form is evaluated branch-free;
-// the short-range forms (r <= 1.9 A, rare) are a divergent call.
-template <typename T>
-__device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
+// Knot counts.  The pair kernels are compiled for the Fe-Fe potential's 6 pair
+// and 2 density knots (POT_FE) and for the padded capacity (POT_KNOTS, any
+// count up to MDK_MAXV/MDK_MAXP: unused knots are zero, so they add nothing
+// for r > 0); POT_TABLE selects the lookup tables.
+enum { POT_TABLE = 0, POT_KNOTS = 1, POT_FE = 2 };
+#define MDK_FE_NV 6
+#define MDK_FE_NP 2
+template <int POTM> struct KnotCount {
+  static constexpr int nv = POTM == POT_FE ? MDK_FE_NV : MDK_MAXV;
+  static constexpr int np = POTM == POT_FE ? MDK_FE_NP : MDK_MAXP;
+};
+
+// (r_k - r) H(r_k - r) as max(r_k - r, 0)
+template <typename T> __device__ __forceinline__ T mdk_pos(T d) { return d > T(0) ? d : T(0); }
+
+// s + a*b with the product rounded first in FP64 (the reference's density knot
+// sums are not contracted: at the density maximum the sum cancels exactly), a
+// fused multiply-add in FP32.
+__device__ __forceinline__ double knot_acc(double s, double a, double b) {
+#pragma clang fp contract(off)
+  return s + a * b;
+}
+__device__ __forceinline__ float knot_acc(float s, float a, float b) { return s + a * b; }
+
+// Pair function V and V' (exact mode) over NV knots: the knot form is
+// evaluated branch-free (6 VALU operations per knot, in the operation order
+// of the reference's knot sum); the short-range forms
+// (r <= 1.9 A, rare) are a divergent call.
+template <typename T, int NV>
+__device__ __forceinline__ void mdk_vee_n(const PotDev &P, T r, T &v, T &dv) {
   PotView<T> Q(P);
   T s = T(0), d1 = T(0);
 #pragma unroll
-  for (int k = 0; k < MDK_MAXV; k++) {
-    if (k < P.nv) {
-      const T d = Q.rv(k) - r;
-      const T ad2 = d < T(0) ? T(0) : Q.av(k) * (d * d);
-      s = s + ad2 * d;
-      d1 = d1 - T(3) * ad2;
-    }
+  for (int k = 0; k < NV; k++) {
+    const T d = mdk_pos<T>(Q.rv(k) - r);
+    const T ad2 = Q.av(k) * (d * d);
+    s = s + ad2 * d;
+    d1 = d1 - T(3) * ad2;
   }
   v = s;
   dv = d1;
@@ -160,34 +184,40 @@ __device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
 }
 
 // Density function phi (exact mode), flat below its maximum r*.
-template <typename T>
-__device__ __forceinline__ T mdk_phi(const PotDev &P, T r) {
+template <typename T, int NP>
+__device__ __forceinline__ T mdk_phi_n(const PotDev &P, T r) {
   PotView<T> Q(P);
   T s = T(0);
 #pragma unroll
-  for (int k = 0; k < MDK_MAXP; k++) {
-    if (k < P.np) {
-      const T d = Q.rp(k) - r;
-      s = s + (d < T(0) ? T(0) : Q.ap(k) * (d * d * d));
-    }
+  for (int k = 0; k < NP; k++) {
+    const T d = mdk_pos<T>(Q.rp(k) - r);
+    s = knot_acc(s, Q.ap(k), d * d * d);
   }
   return r < Q.pcr() ? Q.pcv() : s;
 }
 
 // Density derivative phi' (exact mode), zero below r*.
-template <typename T>
-__device__ __forceinline__ T mdk_dphi(const PotDev &P, T r) {
+template <typename T, int NP>
+__device__ __forceinline__ T mdk_dphi_n(const PotDev &P, T r) {
   PotView<T> Q(P);
   T s = T(0);
 #pragma unroll
-  for (int k = 0; k < MDK_MAXP; k++) {
-    if (k < P.np) {
-      const T d = Q.rp(k) - r;
-      s = s - (d < T(0) ? T(0) : T(3) * Q.ap(k) * (d * d));
-    }
+  for (int k = 0; k < NP; k++) {
+    const T d = mdk_pos<T>(Q.rp(k) - r);
+    s = knot_acc(s, T(-3) * Q.ap(k), d * d);
   }
   return r < Q.pcr() ? T(0) : s;
 }
+
+// Padded-capacity forms (any knot count up to MDK_MAXV / MDK_MAXP).
+template <typename T>
+__device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
+  mdk_vee_n<T, MDK_MAXV>(P, r, v, dv);
+}
+template <typename T>
+__device__ __forceinline__ T mdk_phi(const PotDev &P, T r) { return mdk_phi_n<T, MDK_MAXP>(P, r); }
+template <typename T>
+__device__ __forceinline__ T mdk_dphi(const PotDev &P, T r) { return mdk_dphi_n<T, MDK_MAXP>(P, r); }
 
 // Linear-interpolation lookups in the reference's index arithmetic
 // (src/lookup_m.F90:307-394); no FMA so the result is the reference's.
@@ -210,26 +240,26 @@ __device__ __forceinline__ double mdk_lk_e(const PotDev &P, const double *t, dou
   return t[lo] * (1.0 - dr) + t[lo + 1] * dr;
 }
 
-// Dispatchers (potential_m.F90:288-417): exact or table (a compile-time choice
-// in the pair kernels).
-template <typename T, bool LOOKUP = false>
+// Dispatchers (potential_m.F90:288-417): tables or knots, a compile-time
+// choice in the pair kernels.
+template <typename T, int POTM = POT_KNOTS>
 __device__ __forceinline__ void mdk_pair_vdv(const PotDev &P, T r, T &v, T &dv) {
-  if (LOOKUP) {
+  if (POTM == POT_TABLE) {
     v = (T)mdk_lk_vp(P, P.tab_v, (double)r);
     dv = (T)mdk_lk_vp(P, P.tab_dv, (double)r);
   } else {
-    mdk_vee<T>(P, r, v, dv);
+    mdk_vee_n<T, KnotCount<POTM>::nv>(P, r, v, dv);
   }
 }
-template <typename T, bool LOOKUP = false>
+template <typename T, int POTM = POT_KNOTS>
 __device__ __forceinline__ T mdk_pair_phi(const PotDev &P, T r) {
-  if (LOOKUP) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
-  return mdk_phi<T>(P, r);
+  if (POTM == POT_TABLE) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
+  return mdk_phi_n<T, KnotCount<POTM>::np>(P, r);
 }
-template <typename T, bool LOOKUP = false>
+template <typename T, int POTM = POT_KNOTS>
 __device__ __forceinline__ T mdk_pair_dphi(const PotDev &P, T r) {
-  if (LOOKUP) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
-  return mdk_dphi<T>(P, r);
+  if (POTM == POT_TABLE) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
+  return mdk_dphi_n<T, KnotCount<POTM>::np>(P, r);
 }
 // Embedding F and F' in FP64 (per atom, cheap).
 __device__ __forceinline__ void mdk_embed_f(const PotDev &P, double rho, double &f, double &df) {
