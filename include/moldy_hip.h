@@ -149,10 +149,16 @@ int mdk_md_steps(void *h, int nsteps, double *trace);
  * pair count (sum over atoms, each pair counted twice). */
 int mdk_stats(void *h, long *rebuilds, long *full_pairs);
 
-/* Timing: per-kernel-class HIP-event accumulators on the handle's stream.
- * enable=1 starts recording; mdk_timing_read returns total milliseconds and
- * launch counts for classes {0 drift, 1 density, 2 force, 3 kick, 4 rebuild}. */
-int mdk_timing(void *h, int enable);
+/* Timing: per-kernel-class HIP-event accumulators on the handle's stream for
+ * classes {0 drift, 1 density, 2 force, 3 kick, 4 rebuild}.  mask selects the
+ * classes to record (bit k = class k; MDK_TIMING_ALL every class, 0 stops);
+ * each recorded launch adds two events to the stream.  mdk_timing_read
+ * returns total milliseconds and launch counts per class. */
+#define MDK_TIMING_ALL 0x1F
+int mdk_timing(void *h, int mask);
+/* record only every `every`-th launch of each class (default 1): an event pair
+ * idles the GPU for a few microseconds, so sampling bounds the overhead */
+int mdk_timing_every(void *h, int every);
 int mdk_timing_read(void *h, double *ms, long *launches, int nclasses);
 int mdk_synchronize(void *h);
 

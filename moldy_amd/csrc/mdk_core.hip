@@ -74,6 +74,8 @@ struct TimedEv {
   hipEvent_t a, b;
 };
 
+#define MON_STRIDE 16
+
 struct mdk_ctx {
   mdk_params p;
   int n;
@@ -87,7 +89,6 @@ struct mdk_ctx {
   double4 *pos4;           // packed (B0 (s - img), F') for the pair kernels
   int *img;                // integer image floor(s) fixed at the last full-list build
   bool pos4_valid;
-  int tpa_density, tpa_force;  // lanes per atom of the pair kernels
   bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
   // reference link cells (1-based cell per atom, sorted order)
   int *ic;
@@ -107,12 +108,20 @@ struct mdk_ctx {
   BoxState *box;
   int cur;
   // partials
-  double *part_d, *part_f, *part_k;
-  double *stage_d, *stage_f, *stage_k;  // first-stage reductions of the partials
+  // per-block partials, value-major: force pass (NPART_FORCE values) plus the
+  // density pass's embedding energy as value NPART_FORCE; half kick
+  double *part_f, *part_k;
+  double *stage_f, *stage_k;  // first-stage reductions of the partials
   int nblk;
-  // host mapped monitor ring: [slot][0]=maxdisp2,[1]=maxstep2
+  // host-mapped monitor ring, MON_STRIDE doubles per step slot (4 slots):
+  // [0] maxdisp2, [1] maxstep2, [2..10] B0 after the step, [11] full-list
+  // overflow flag, [12] grid-too-coarse flag
   double *mon_host, *mon_dev;
   hipEvent_t mon_ev[4];
+  double b0_lag[9];        // B0 of a completed step (from the monitor ring)
+  long b0_lag_step;        // its step index, -1 when none
+  int lag_error;           // MDK_ERR_* seen in a monitor sample
+  bool drifted;            // the next step's drift was fused into the last kick
   long step_counter;
   // trace buffer on device
   double *trace_dev;
@@ -127,7 +136,9 @@ struct mdk_ctx {
   double r_list, skin;
   int ntc;
   // timing
-  bool timing;
+  int timing_mask;          // kernel classes recorded with HIP events
+  int timing_every;         // record one launch in this many per class
+  long t_seen[TC_NCLASS];
   std::vector<TimedEv> evs;
   std::vector<hipEvent_t> evpool;
   double t_ms[TC_NCLASS];
@@ -154,9 +165,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // Block-reduce K sums (and the last KM entries as maxima) into
 // part[k*gridDim.x + blockIdx.x] (value-major, so the finishing block's reads
 // of one value across all blocks are contiguous).
-template <int K, int KM>
+template <int K, int KM, int BLOCK = MDK_BLOCK>
 __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
-  __shared__ double sh[MDK_BLOCK / 64][K];
+  __shared__ double sh[BLOCK / 64][K];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
@@ -168,15 +179,15 @@ __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
   if (threadIdx.x < K) {
     const int k = threadIdx.x;
     double s = sh[0][k];
-    for (int q = 1; q < MDK_BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
+    for (int q = 1; q < BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
     part[(size_t)k * gridDim.x + blockIdx.x] = s;
   }
 }
 
 // Fixed-order reduction of nb block partials (value-major) inside one block.
-template <int K, int KM>
+template <int K, int KM, int BLOCK = MDK_BLOCK>
 __device__ __forceinline__ void block_final(const double *part, int nb, double (&out)[K]) {
-  __shared__ double sh[MDK_BLOCK / 64][K];
+  __shared__ double sh[BLOCK / 64][K];
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = 0.0;
@@ -207,22 +218,6 @@ __device__ __forceinline__ void block_final(const double *part, int nb, double (
 // First reduction stage: block b folds partial columns [b*chunk, (b+1)*chunk)
 // of the value-major array `in` (nb columns) into out[k*gridDim.x + b], in a
 // fixed order, so the finishing block reads only gridDim.x columns.
-template <int K, int KM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_reduce_stage(const double *in, int nb, double *out,
-                                                            int chunk) {
-  double v[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
-  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const double x = in[(size_t)k * nb + b];
-      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
-    }
-  }
-  block_store<K, KM>(v, out);
-}
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs, each with its own L2; remapping physical block b to logical tile
@@ -317,6 +312,23 @@ __device__ __forceinline__ void shift_table(const double *b, double (*S)[3]) {
     S[c][2] = b[2] * sx + b[5] * sy + b[8] * sz;
   }
 }
+template <int K, int KM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_reduce_stage(const double *in, int nb, double *out,
+                                                            int chunk) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = 0.0;
+  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double x = in[(size_t)k * nb + b];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+  block_store<K, KM>(v, out);
+}
+
 // inverse of a 3x3 column-major matrix (cofactors / determinant)
 __device__ __forceinline__ void inv3(const double *b, double *bi) {
   const double c00 = TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2);
@@ -572,6 +584,17 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   shift_table(b, S);
   __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // cells of the current box must be at least r_list wide where the stencil spans them
+    const int gg[3] = {gx, gy, gz};
+    const double vol = fabs(dev_volume(b));
+    for (int d = 0; d < 3; d++) {
+      const double *u = b + 3 * ((d + 1) % 3), *w = b + 3 * ((d + 2) % 3);
+      const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+      const double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
+      if (gg[d] >= 3 && width / gg[d] < sqrt(rl2)) atomicMax(&flags[2], 1);
+    }
+  }
   int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned c = (unsigned)cell[i];
@@ -626,6 +649,29 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
 // ---------------------------------------------------------------------------
 // MD-step kernels
 // ---------------------------------------------------------------------------
+// Box part of the drift, by one workgroup: copies the box state into the other
+// buffer with b0 += b1 + b2 and b1 += b2 (src/quench_m.F90:574-589; one field
+// per thread).
+__device__ void drift_box(const BoxState *bin, BoxState *bout, int ivol) {
+#pragma clang fp contract(off)
+  const int nw = (int)(sizeof(BoxState) / sizeof(double));
+  const double *src = (const double *)bin;
+  double *dst = (double *)bout;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    const int f = w - (int)(offsetof(BoxState, b0) / sizeof(double));
+    const int g = w - (int)(offsetof(BoxState, b1) / sizeof(double));
+    double val = src[w];
+    if (f >= 0 && f < 9) {  // b0 += b1 + b2 (component-wise, as the reference's two updates)
+      const bool upd = ivol < 1 || (ivol == 4 && f == 8);
+      if (upd) val = bin->b0[f] + (bin->b1[f] + bin->b2[f]);
+    } else if (g >= 0 && g < 9) {
+      const bool upd = ivol < 1 || (ivol == 4 && g == 8);
+      if (upd) val = bin->b1[g] + bin->b2[g];
+    }
+    dst[w] = val;
+  }
+}
+
 // velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2 (also into the
 // packed copy); block 0 copies the box state into the other buffer, advancing
 // b1 and b0 there (one field per thread).
@@ -653,24 +699,7 @@ __global__ void k_drift(double *atoms, double4 *pos4, const int *img, int n, con
     pos4[i] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
                            b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
   }
-  if (blockIdx.x == 0) {
-    const int nw = (int)(sizeof(BoxState) / sizeof(double));
-    const double *src = (const double *)bin;
-    double *dst = (double *)bout;
-    for (int w = threadIdx.x; w < nw; w += blockDim.x) {
-      const int f = w - (int)(offsetof(BoxState, b0) / sizeof(double));
-      const int g = w - (int)(offsetof(BoxState, b1) / sizeof(double));
-      double val = src[w];
-      if (f >= 0 && f < 9) {  // b0 += b1 + b2 (component-wise, as the reference's two updates)
-        const bool upd = ivol < 1 || (ivol == 4 && f == 8);
-        if (upd) val = bin->b0[f] + (bin->b1[f] + bin->b2[f]);
-      } else if (g >= 0 && g < 9) {
-        const bool upd = ivol < 1 || (ivol == 4 && g == 8);
-        if (upd) val = bin->b1[g] + bin->b2[g];
-      }
-      dst[w] = val;
-    }
-  }
+  if (blockIdx.x == 0) drift_box(bin, bout, ivol);
 }
 
 // geometry of pair (i,j): fractional separation (minimum image) and Cartesian
@@ -731,12 +760,6 @@ __device__ __forceinline__ T pair_sep(const double (*S)[3], const double4 &pi, c
   return rx * rx + ry * ry + rz * rz;
 }
 
-template <int TPA>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int o = TPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // Cartesian image-fixed positions R' = B0 (s - img) and the current F', packed
 // for one 32-byte gather per neighbour.  with_img recomputes img = floor(s).
@@ -758,9 +781,9 @@ __global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n,
 
 // Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
 // (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
-// with TPA lanes per atom; F, F' per atom; per-block embedding energy.  F' is
+// one lane per atom; F, F' per atom; per-block embedding energy.  F' is
 // also stored in pos4[i].w for the force pass.
-template <typename T, int TPA, int POTM>
+template <typename T, int POTM>
 __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
                                                        const int *__restrict__ nbr,
                                                        const int *__restrict__ numn,
@@ -769,15 +792,14 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
   __shared__ double S[32][3];
   shift_table(bs->b0, S);
   __syncthreads();
-  const int gt = xcd_tile() * blockDim.x + threadIdx.x;
-  const int i = gt / TPA, l = gt % TPA;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
   if (i < n) {
     const double4 pi = pos4[i];
     const int nn = numn[i];
-    // lane l of the atom's TPA lanes takes quads l, l+TPA, ...: one 16-byte
-    // index load and four independent gathers in flight per iteration
-    for (int q = l; 4 * q < nn; q += TPA) {  // quads are padded with null entries
+    // one 16-byte index load and four independent gathers per quad (quads
+    // are padded with null entries)
+    for (int q = 0; 4 * q < nn; q++) {
       const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       double4 pj[4];
@@ -793,9 +815,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
       }
     }
   }
-  acc = group_sum<TPA>(acc);
   double pe[1] = {0.0};
-  if (i < n && l == 0) {
+  if (i < n) {
     double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
            *daf = atoms + (size_t)F_DAFRHO * n;
     rho[i] = acc;
@@ -818,9 +839,9 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
 // fp = -(V' + phi'(F'_i + F'_j))/r;  F_i = sum_j r_ij fp (Cartesian) and
 // f_i = B0^-1 F_i;  the virial is accumulated as W = 1/2 sum r r^T fp (tp =
 // W B0^-T in the finishing block, i.e. sum r_a s_b fp) and the pair energy as
-// 1/2 sum V.  TPA lanes per atom; MASSCHK keeps the
+// 1/2 sum V.  One lane per atom; MASSCHK keeps the
 // reference's exclusion of pairs of immobile atoms from the virial.
-template <typename T, int TPA, bool MASSCHK, int POTM>
+template <typename T, bool MASSCHK, int POTM>
 __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
                                                      const int *zat, int n,
                                                      const int *__restrict__ nbr,
@@ -833,8 +854,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   shift_table(b, S);
   __syncthreads();
-  const int gt = xcd_tile() * blockDim.x + threadIdx.x;
-  const int i = gt / TPA, l = gt % TPA;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
   const double *mass = atoms + (size_t)F_MASS * n;
   double v[NPART_FORCE];
 #pragma unroll
@@ -847,7 +867,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
     const int nn = numn[i];
     // sums of V and of r r^T fp; the factor 1/2 of both is applied once below
     double pe2 = 0.0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
-    for (int q = l; 4 * q < nn; q += TPA) {
+    for (int q = 0; 4 * q < nn; q++) {
       const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       double4 pj[4];
@@ -855,10 +875,11 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
       for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int j = (int)(e[u] & NBR_IDX_MASK);
+        const unsigned eu = e[u];
+        const int j = (int)(eu & NBR_IDX_MASK);
         double dx, dy, dz;
         T rx, ry, rz;
-        const T r2 = pair_sep<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), dx, dy, dz, rx, ry, rz);
+        const T r2 = pair_sep<T>(S, pi, pj[u], (int)(eu >> NBR_CODE_SHIFT), dx, dy, dz, rx, ry, rz);
         T r, rr;
         r_and_inv(r2, r, rr);
         T vv, dv;
@@ -881,10 +902,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
     v[1] = 0.5 * wxx; v[2] = 0.5 * wyy; v[3] = 0.5 * wzz;
     v[4] = 0.5 * wxy; v[5] = 0.5 * wxz; v[6] = 0.5 * wyz;
   }
-  fxi = group_sum<TPA>(fxi);
-  fyi = group_sum<TPA>(fyi);
-  fzi = group_sum<TPA>(fzi);
-  if (i < n && l == 0) {
+  if (i < n) {
     // fractional force f = B0^-1 F (the reference's fx = sum s fp, Cartesian = B0 f)
     double bi[9];
     inv3(b, bi);
@@ -939,35 +957,43 @@ __device__ void therm_box(BoxState &s, int ivol) {
 // initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part
 // (dynamics_m.F90:672-700).  Threads 0..8 each own one tensor element (i,j);
 // phases are separated by barriers; only changed fields are written back.
-__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pd, int nbd,
-                                                              const double *pf, int nbf,
-                                                              BoxState *bs, int mode, int ivol,
-                                                              double press, double bdel2,
-                                                              double dt, int have_emb) {
-  double e[1] = {0.0};
-  if (have_emb) block_final<1, 0>(pd, nbd, e);
-  double v[NPART_FORCE];
-  block_final<NPART_FORCE, 0>(pf, nbf, v);
-  __shared__ double b0[9], b1[9], tg[9], tgd[9], tgi[9];
+// Finishing block of force().  The per-block partials are in one value-major
+// array: values 0..18 from the force pass (NPART_FORCE) and value 19, the
+// embedding energy, from the density pass; nb blocks' worth (after an
+// optional first stage).
+__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pf, int nb, BoxState *bs,
+                                                              int mode, int ivol, double press,
+                                                              double bdel2, double dt) {
+  double v[NPART_FORCE + 1];
+  block_final<NPART_FORCE + 1, 0>(pf, nb, v);
+  // small matrices live in LDS: the nine tensor threads index them with their
+  // own (ii,jj), which private arrays would turn into scratch accesses
+  __shared__ double b0[9], b1[9], tg[9], tgd[9], tgi[9], sv[NPART_FORCE], sbi[9];
   const int t = threadIdx.x;
   const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
   if (t < 9) {
     b0[t] = bs->b0[t];
     b1[t] = bs->b1[t];
   }
+  if (t == 0)
+    for (int k = 0; k < NPART_FORCE; k++) sv[k] = v[k];
   __syncthreads();
   if (t == 0) {
-    if (have_emb) bs->pe_emb = e[0];
-    bs->pe = (have_emb ? e[0] : bs->pe_emb) + v[0];
+    bs->pe_emb = v[NPART_FORCE];
+    bs->pe = v[NPART_FORCE] + v[0];
     bs->vol = dev_volume(b0);
+    inv3(b0, sbi);
   }
+  __syncthreads();
   if (t < 9) {
-    double bi[9];
-    inv3(b0, bi);
-    // W = 1/2 sum r r^T fp (symmetric) -> tp(a,c) = sum_k W(a,k) B0^-1(c,k)
-    const double W[9] = {v[1], v[4], v[5], v[4], v[2], v[6], v[5], v[6], v[3]};
-    const double tpv = TT(W, ii, 1) * TT(bi, jj, 1) + TT(W, ii, 2) * TT(bi, jj, 2) + TT(W, ii, 3) * TT(bi, jj, 3);
-    const double tkv = v[10 + t] / (dt * dt);
+    // W = 1/2 sum r r^T fp (symmetric; sv[1..6] = xx yy zz xy xz yz) ->
+    // tp(a,c) = sum_k W(a,k) B0^-1(c,k)
+    double tpv = 0.0;
+    for (int k = 1; k <= 3; k++) {
+      const int w = ii == k ? ii : (ii + k == 3 ? 4 : (ii + k == 4 ? 5 : 6));
+      tpv += sv[w] * TT(sbi, jj, k);
+    }
+    const double tkv = sv[10 + t] / (dt * dt);
     // tc = cofactor matrix of b0 (dynamics_m.F90:365-373)
     const int i1 = ii % 3 + 1, i2 = (ii + 1) % 3 + 1, j1 = jj % 3 + 1, j2 = (jj + 1) % 3 + 1;
     const double tcv = TT(b0, i1, j1) * TT(b0, i2, j2) - TT(b0, i1, j2) * TT(b0, i2, j1);  // cofactor C(ii,jj)
@@ -1047,9 +1073,15 @@ __global__ void k_therm_box(BoxState *bs, int ivol) {
 // Velocity half-kick (src/quench_m.F90:604-622) fused with the per-atom sums of
 // update_therm_avs (src/dynamics_m.F90:702-732) and the list-skin monitor.
 // mode 0: VV kick + sums; 1: sums only; 2: initial accelerations (moldin.F90:461-472)
+// DRIFT (mode 0 only) fuses the next step's drift: x0 += x1 + x2 with the new
+// velocities, packed positions with the advanced box, and block 0 writes the
+// advanced box state into bout -- what k_drift would do at the start of the
+// next step.
+template <bool DRIFT>
 __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *zat, int n,
-                                                    const BoxState *bs, int ivol, int mode,
-                                                    const float *sref, double *part) {
+                                                    BoxState *bs, int ivol, int mode,
+                                                    const float *sref, double *part, double4 *pos4,
+                                                    const int *img, BoxState *bout) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1110,9 +1142,10 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
       double rz = TT(b, 3, 1) * x1 + TT(b, 3, 2) * y1 + TT(b, 3, 3) * z1;
       v[2] = m * (rx * rx + ry * ry + rz * rz);
     }
+    const double x0 = x0p[i], y0 = y0p[i], z0 = z0p[i];
     if (sref) {
-      double ex = x0p[i] - (double)sref[3 * i], ey = y0p[i] - (double)sref[3 * i + 1],
-             ez = z0p[i] - (double)sref[3 * i + 2];
+      double ex = x0 - (double)sref[3 * i], ey = y0 - (double)sref[3 * i + 1],
+             ez = z0 - (double)sref[3 * i + 2];
       double qx = b[0] * ex + b[3] * ey + b[6] * ez, qy = b[1] * ex + b[4] * ey + b[7] * ez,
              qz = b[2] * ex + b[5] * ey + b[8] * ez;
       v[3] = qx * qx + qy * qy + qz * qz;
@@ -1122,41 +1155,69 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
       qz = b[2] * sx + b[5] * sy + b[8] * sz;
       v[4] = qx * qx + qy * qy + qz * qz;
     }
+    if (DRIFT) {
+      const double xn = x0 + (x1 + x2), yn = y0 + (y1 + y2), zn = z0 + (z1 + z2);
+      x0p[i] = xn;
+      y0p[i] = yn;
+      z0p[i] = zn;
+      double bn[9];
+      for (int k = 0; k < 9; k++) {
+        const bool upd = ivol < 1 || (ivol == 4 && k == 8);
+        bn[k] = upd ? bs->b0[k] + (bs->b1[k] + bs->b2[k]) : bs->b0[k];
+      }
+      const double sx = xn - img[3 * i], sy = yn - img[3 * i + 1], sz = zn - img[3 * i + 2];
+      pos4[i] = make_double4(bn[0] * sx + bn[3] * sy + bn[6] * sz, bn[1] * sx + bn[4] * sy + bn[7] * sz,
+                             bn[2] * sx + bn[5] * sy + bn[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
+    }
   }
+  if (DRIFT && blockIdx.x == 0) drift_box(bs, bout, ivol);
   block_store<NPART_KICK, 2>(v, part);
 }
 
 // Finishing block of the kick: s1,s2,s3, KE/TE/H/TH (moldin.F90:763-790),
 // trace row and the host-visible skin monitor.
+// Finishing block of the half kick: KE, box KE, TE, H, TH, the trace row and
+// the monitor sample (thread 0 after the block reduction of the partials).
 __global__ void __launch_bounds__(MDK_BLOCK) k_finalize_kick(const double *pk, int nb, BoxState *bs,
                                                              double dt, double bdel2, double press,
-                                                             double *trace_row, double *mon) {
+                                                             double *trace_row, double *mon,
+                                                             const int *dflags) {
 #pragma clang fp contract(off)
   double v[NPART_KICK];
   block_final<NPART_KICK, 2>(pk, nb, v);
   if (threadIdx.x != 0) return;
-  BoxState s = *bs;
-  s.s1 = v[0];
-  s.s2 = v[1];
-  s.s3 = v[2];
-  s.maxdisp2 = v[3];
-  s.maxstep2 = v[4];
-  s.ke = s.s3 / (dt * dt) / 2.0;
+  // only the fields this kernel changes are read and written
+  const double pe = bs->pe, vol = bs->vol;
+  double b1[9], b0[9];
+  for (int k = 0; k < 9; k++) {
+    b1[k] = bs->b1[k];
+    b0[k] = bs->b0[k];
+  }
+  const double ke = v[2] / (dt * dt) / 2.0;
   double bk = 0.0;
   for (int i = 1; i <= 3; i++)
-    for (int j = 1; j <= 3; j++) bk = bk + (TT(s.b1, i, j) * TT(s.b1, i, j)) / bdel2;
-  s.boxke = bk;
-  s.te = s.ke + s.pe;
-  s.h = s.te + press * s.vol;
-  s.th = s.h + s.boxke;
-  *bs = s;
+    for (int j = 1; j <= 3; j++) bk = bk + (TT(b1, i, j) * TT(b1, i, j)) / bdel2;
+  const double te = ke + pe, hh = te + press * vol, th = hh + bk;
+  bs->s1 = v[0];
+  bs->s2 = v[1];
+  bs->s3 = v[2];
+  bs->maxdisp2 = v[3];
+  bs->maxstep2 = v[4];
+  bs->ke = ke;
+  bs->boxke = bk;
+  bs->te = te;
+  bs->h = hh;
+  bs->th = th;
   if (trace_row) {
-    trace_row[0] = s.pe; trace_row[1] = s.ke; trace_row[2] = s.te;
-    trace_row[3] = s.h; trace_row[4] = s.th; trace_row[5] = s.vol;
+    trace_row[0] = pe; trace_row[1] = ke; trace_row[2] = te;
+    trace_row[3] = hh; trace_row[4] = th; trace_row[5] = vol;
   }
   if (mon) {
     mon[0] = v[3];
     mon[1] = v[4];
+    for (int k = 0; k < 9; k++) mon[2 + k] = b0[k];
+    mon[11] = (double)dflags[0];
+    mon[12] = (double)dflags[2];
   }
 }
 
@@ -1193,7 +1254,8 @@ static hipEvent_t pooled_event(mdk_ctx *c) {
 }
 static void tbegin(mdk_ctx *c, int cls, TimedEv &te) {
   te.cls = -1;
-  if (!c->timing) return;
+  if (!((c->timing_mask >> cls) & 1)) return;
+  if (c->timing_every > 1 && (c->t_seen[cls]++ % c->timing_every) != 0) return;
   te.cls = cls;
   te.a = pooled_event(c);
   te.b = pooled_event(c);
@@ -1239,7 +1301,7 @@ static int read_box(mdk_ctx *c, BoxState *out) {
 }
 
 // Fine grid from the current box: cells at least r_list wide (perpendicular width)
-static void plan_grid(mdk_ctx *c, const double *b, int g[3]) {
+static void plan_grid(mdk_ctx *c, const double *b, int g[3], double margin) {
   double vol = fabs(TT(b, 1, 1) * (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) +
                     TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
                     TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3)));
@@ -1248,7 +1310,7 @@ static void plan_grid(mdk_ctx *c, const double *b, int g[3]) {
     const double *u = b + 3 * e1, *w = b + 3 * e2;
     double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
     double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
-    int nc = (int)floor(width / c->r_list);
+    int nc = (int)floor(width / (c->r_list * margin));
     if (nc < 3) nc = 1;  // too few cells for a distinct 27-cell stencil: one cell in this dim
     if (nc > 256) nc = 256;
     g[d] = nc;
@@ -1294,10 +1356,21 @@ static int rebuild_full(mdk_ctx *c) {
   c->list_valid = false;
   TimedEv te;
   tbegin(c, TC_REBUILD, te);
-  BoxState bs;
-  RCK(read_box(c, &bs));
+  // Grid from the box.  Inside the MD loop the box of a completed step comes
+  // from the monitor ring (no host round trip; cells planned 1 % wider than
+  // the list radius cover the box change of a few steps, and the list kernel
+  // flags any cell that is still too narrow); otherwise read synchronously.
+  const bool lagged = c->rebuilds > 0 && c->b0_lag_step >= 0 && c->step_counter - c->b0_lag_step <= 4;
+  double b0[9];
+  if (lagged) {
+    memcpy(b0, c->b0_lag, sizeof(b0));
+  } else {
+    BoxState bs;
+    RCK(read_box(c, &bs));
+    memcpy(b0, bs.b0, sizeof(b0));
+  }
   int g[3];
-  plan_grid(c, bs.b0, g);
+  plan_grid(c, b0, g, lagged ? 1.01 : 1.0);
   int p2 = 1;
   while (p2 < std::max(g[0], std::max(g[1], g[2]))) p2 <<= 1;
   const int m = p2 * p2 * p2;  // Morton key space
@@ -1327,9 +1400,17 @@ static int rebuild_full(mdk_ctx *c) {
         c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
         c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
     RCK(check_launch("k_full_list"));
+    if (lagged) {
+      // overflow and cell-width flags reach the host through the monitor ring
+      c->list_valid = true;
+      c->rebuilds++;
+      tend(c, te);
+      return MDK_OK;
+    }
     int fl[4];
     HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
+    if (fl[2] != 0) return set_err(MDK_ERR_STATE, "list cells narrower than the list radius");
     if (fl[0] == 0) {
       c->list_valid = true;
       c->rebuilds++;
@@ -1347,25 +1428,17 @@ static int rebuild_full(mdk_ctx *c) {
   return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
 }
 
-template <typename T, int TPA, int POTM>
-static void density_t(mdk_ctx *c, int do_embed) {
-  k_density<T, TPA, POTM><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, c->part_d);
-}
-template <typename T, int TPA, bool MC, int POTM>
-static void force_t(mdk_ctx *c) {
-  k_force<T, TPA, MC, POTM><<<nblocks((long)c->n * TPA), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->part_f);
-}
 template <typename T, int POTM>
 static void density_dispatch(mdk_ctx *c, int do_embed) {
-  if (c->tpa_density == 1) density_t<T, 1, POTM>(c, do_embed);
-  else density_t<T, 2, POTM>(c, do_embed);
+  // embedding-energy partials are value NPART_FORCE of the force pass's array
+  k_density<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed,
+      c->part_f + (size_t)NPART_FORCE * nblocks(c->n));
 }
 template <typename T, bool MC, int POTM>
-static void force_dispatch(mdk_ctx *c) {
-  if (c->tpa_force == 2) force_t<T, 2, MC, POTM>(c);
-  else force_t<T, 1, MC, POTM>(c);
+static void force_dispatch(mdk_ctx *c, int fin_mode) {
+  k_force<T, MC, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->part_f);
 }
 // lookup tables, the Fe-Fe knot counts, or any knot count
 static int pot_mode(const mdk_ctx *c) {
@@ -1381,11 +1454,11 @@ static void density_pot(mdk_ctx *c, int do_embed) {
   }
 }
 template <typename T, bool MC>
-static void force_pot(mdk_ctx *c) {
+static void force_pot(mdk_ctx *c, int fin_mode) {
   switch (pot_mode(c)) {
-    case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c); break;
-    case POT_FE: force_dispatch<T, MC, POT_FE>(c); break;
-    default: force_dispatch<T, MC, POT_KNOTS>(c); break;
+    case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c, fin_mode); break;
+    case POT_FE: force_dispatch<T, MC, POT_FE>(c, fin_mode); break;
+    default: force_dispatch<T, MC, POT_KNOTS>(c, fin_mode); break;
   }
 }
 
@@ -1396,8 +1469,9 @@ static int ensure_pos4(mdk_ctx *c) {
   return check_launch("k_pack");
 }
 
-// two-stage fixed-order reduction: returns the array/column count the
-// finishing block must read
+// Two-stage fixed-order reduction: more than 128 per-block partials are first
+// folded in chunks of 64 by a parallel stage; returns the array and the count
+// the finishing block reads.
 template <int K, int KM>
 static const double *stage_reduce(mdk_ctx *c, const double *part, int nb, double *stage, int *nb_out) {
   if (nb <= 128) {
@@ -1410,8 +1484,7 @@ static const double *stage_reduce(mdk_ctx *c, const double *part, int nb, double
   return stage;
 }
 
-// per-block partial counts must cover every launch shape
-static int block_count(mdk_ctx *c, int tpa) { return nblocks((long)c->n * tpa); }
+static int block_count(mdk_ctx *c) { return nblocks(c->n); }
 
 static int launch_density(mdk_ctx *c, int do_embed) {
   RCK(ensure_pos4(c));
@@ -1428,17 +1501,16 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   tbegin(c, TC_FORCE, te);
   const bool f32 = c->p.precision == MDK_PREC_FP32, mc = c->masschk;
   if (f32) {
-    if (mc) force_pot<float, true>(c); else force_pot<float, false>(c);
+    if (mc) force_pot<float, true>(c, fin_mode); else force_pot<float, false>(c, fin_mode);
   } else {
-    if (mc) force_pot<double, true>(c); else force_pot<double, false>(c);
+    if (mc) force_pot<double, true>(c, fin_mode); else force_pot<double, false>(c, fin_mode);
   }
   tend(c, te);
   RCK(check_launch("k_force"));
-  int nbd, nbf;
-  const double *pd = stage_reduce<1, 0>(c, c->part_d, block_count(c, c->tpa_density), c->stage_d, &nbd);
-  const double *pf = stage_reduce<NPART_FORCE, 0>(c, c->part_f, block_count(c, c->tpa_force), c->stage_f, &nbf);
-  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(pd, nbd, pf, nbf, c->box + c->cur, fin_mode, c->p.ivol,
-                                                   c->p.press, c->p.bdel2, c->p.deltat, 1);
+  int nbf;
+  const double *pf = stage_reduce<NPART_FORCE + 1, 0>(c, c->part_f, block_count(c), c->stage_f, &nbf);
+  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(pf, nbf, c->box + c->cur, fin_mode, c->p.ivol, c->p.press,
+                                                   c->p.bdel2, c->p.deltat);
   return check_launch("k_finalize_force");
 }
 
@@ -1454,22 +1526,44 @@ static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
   return launch_force(c, fin_mode);
 }
 
-static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon) {
+// Half kick (mode 0), sums only (1) or initial accelerations (2).  With
+// fuse_drift (mode 0) the kernel also performs the next step's drift; the
+// finishing block then completes this step's sums on the current box copy and
+// the next md_step skips its drift.
+static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon, bool fuse_drift = false) {
   TimedEv te;
   tbegin(c, TC_KICK, te);
-  k_kick<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode,
-                                               c->sref, c->part_k);
+  const int nxt = 1 - c->cur;
+  if (fuse_drift && mode == 0)
+    k_kick<true><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, c->pos4, c->img,
+        c->box + nxt);
+  else
+    k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, nullptr, nullptr,
+        nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
-  if (mode == 2) return MDK_OK;
-  int nbk;
-  const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
-  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
-                                                  c->p.press, trace_row_dev, mon);
-  return check_launch("k_finalize_kick");
+  if (mode != 2) {
+    int nbk;
+    const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
+    k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
+                                                    c->p.press, trace_row_dev, mon, c->dflags);
+    RCK(check_launch("k_finalize_kick"));
+  }
+  if (fuse_drift && mode == 0) {
+    c->pos4_valid = true;
+    c->cur = nxt;
+    c->drifted = true;
+  }
+  return MDK_OK;
 }
 
 static int launch_drift(mdk_ctx *c) {
+  if (c->drifted) {  // performed by the previous step's fused kick
+    c->drifted = false;
+    return MDK_OK;
+  }
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
@@ -1481,28 +1575,52 @@ static int launch_drift(mdk_ctx *c) {
   return check_launch("k_drift");
 }
 
+// Reads the monitor sample of step s-2 (its kernels have completed or the
+// host waits for them): box for the next grid plan, list/grid error flags.
+static const double *poll_monitor(mdk_ctx *c) {
+  const long s = c->step_counter;
+  if (s < 2) return nullptr;
+  const long probe = s - 2;
+  const int slot = (int)(probe & 3);
+  hipEventSynchronize(c->mon_ev[slot]);
+  const double *m = c->mon_host + MON_STRIDE * slot;
+  for (int k = 0; k < 9; k++) c->b0_lag[k] = m[2 + k];
+  c->b0_lag_step = probe;
+  if (m[11] > 0.0) c->lag_error = MDK_ERR_NBR_OVERFLOW;
+  if (m[12] > 0.0) c->lag_error = MDK_ERR_STATE;
+  return m;
+}
+
 // Decide from a completed monitor sample whether the next force evaluation
 // could see an atom displaced by more than half the skin since the last build.
 static bool skin_check(mdk_ctx *c) {
+  const double *m = poll_monitor(c);
   if (!c->list_valid) return true;
-  long s = c->step_counter;
-  if (s - c->last_rebuild_step < 3) return false;
-  long probe = s - 2;
-  int slot = (int)(probe & 3);
-  hipEventSynchronize(c->mon_ev[slot]);
-  double disp = sqrt(fmax(c->mon_host[2 * slot], 0.0));
-  double stp = sqrt(fmax(c->mon_host[2 * slot + 1], 0.0));
+  if (c->step_counter - c->last_rebuild_step < 3 || !m) return false;
+  double disp = sqrt(fmax(m[0], 0.0));
+  double stp = sqrt(fmax(m[1], 0.0));
   return disp + 3.0 * stp >= 0.5 * c->skin;
 }
 
-static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm) {
+static int lag_status(mdk_ctx *c) {
+  if (c->lag_error == MDK_ERR_NBR_OVERFLOW)
+    return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (capacity %d)", c->maxn);
+  if (c->lag_error) return set_err(c->lag_error, "list cells narrower than the list radius");
+  return MDK_OK;
+}
+
+// One moldin step.  allow_fuse lets the closing kick also perform the next
+// step's drift, except when relink follows this step (it must see the
+// positions of this step).
+static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool allow_fuse = false) {
   bool skin_due = skin_check(c);
+  RCK(lag_status(c));
   long before = c->rebuilds;
   RCK(launch_drift(c));
   RCK(force_body(c, fused_therm ? FIN_VV_THERM : FIN_VV, skin_due));
   if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
   int slot = (int)(c->step_counter & 3);
-  RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + 2 * slot));
+  RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + MON_STRIDE * slot, allow_fuse && c->ntc != 0));
   HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
   c->step_counter++;
   return MDK_OK;
@@ -1543,12 +1661,6 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   }
   hipGetDevice(&c->device);
   c->maxn = ((p->max_full > 0 ? p->max_full : 64) + 3) & ~3;
-  c->tpa_density = 2;
-  c->tpa_force = p->precision == MDK_PREC_FP32 ? 1 : 2;
-  if (const char *e = getenv("MDK_TPA_DENSITY")) c->tpa_density = atoi(e);
-  if (const char *e = getenv("MDK_TPA_FORCE")) c->tpa_force = atoi(e);
-  c->tpa_density = c->tpa_density == 1 ? 1 : 2;  // compiled lane counts
-  c->tpa_force = c->tpa_force == 2 ? 2 : 1;
   c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
   c->rnear2_exact = near_threshold(p->rnear);
   c->nblk = nblocks(n);
@@ -1583,13 +1695,11 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->box, sizeof(BoxState) * 2);
   ALLOC(c->pos4, sizeof(double4) * (size_t)n);
   ALLOC(c->img, sizeof(int) * 3 * (size_t)n);
-  ALLOC(c->part_d, sizeof(double) * (size_t)nblocks((long)n * 8));
-  ALLOC(c->part_f, sizeof(double) * (size_t)nblocks((long)n * 8) * NPART_FORCE);
+  ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * (NPART_FORCE + 1));
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
-  ALLOC(c->stage_d, sizeof(double) * (size_t)(nblocks((long)n * 8) / 64 + 1));
-  ALLOC(c->stage_f, sizeof(double) * (size_t)(nblocks((long)n * 8) / 64 + 1) * NPART_FORCE);
+  ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * (NPART_FORCE + 1));
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
-  if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 8, hipHostMallocMapped) != hipSuccess ||
+  if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 4 * MON_STRIDE, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->mon_dev, c->mon_host, 0) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "mapped host buffer failed");
     goto fail;
@@ -1604,7 +1714,8 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream);
   k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->orig, n);
   k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->slot_of, n);
-  memset(c->mon_host, 0, sizeof(double) * 8);
+  memset(c->mon_host, 0, sizeof(double) * 4 * MON_STRIDE);
+  c->b0_lag_step = -1;
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "init sync failed");
     goto fail;
@@ -1623,7 +1734,7 @@ extern "C" int mdk_destroy(void *h) {
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
-                  c->members, c->dflags, c->box, c->part_d, c->part_f, c->part_k, c->stage_d, c->stage_f, c->stage_k, c->cell_count,
+                  c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -1961,9 +2072,10 @@ extern "C" int mdk_embed(void *h, double *pe_embed) {
   if (!c->list_valid) RCK(rebuild_full(c));
   RCK(launch_density(c, 1));
   if (pe_embed) {
-    const int nb = block_count(c, c->tpa_density);
+    const int nb = block_count(c);
     std::vector<double> part(nb);
-    HIPCK(hipMemcpyAsync(part.data(), c->part_d, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(part.data(), c->part_f + (size_t)NPART_FORCE * nb, sizeof(double) * nb,
+                         hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     double s = 0.0;
     for (double v : part) s += v;
@@ -1993,10 +2105,12 @@ extern "C" int mdk_velocityverlet(void *h) {
   RCK(force_body(c, FIN_VV, skin_due));
   // half kick without the thermodynamic sums' box part (tgid stays lagged)
   int slot = (int)(c->step_counter & 3);
-  RCK(launch_kick(c, 0, nullptr, c->mon_dev + 2 * slot));
+  RCK(launch_kick(c, 0, nullptr, c->mon_dev + MON_STRIDE * slot));
   HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
   c->step_counter++;
   HIPCK(hipStreamSynchronize(c->stream));
+  poll_monitor(c);
+  RCK(lag_status(c));
   return check_device_status(c);
 }
 
@@ -2007,14 +2121,14 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
   RCK(check_launch("k_therm_box"));
   TimedEv te;
   tbegin(c, TC_KICK, te);
-  k_kick<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1,
-                                               nullptr, c->part_k);
+  k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
   int nbk;
   const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
   k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
-                                                  c->p.press, nullptr, nullptr);
+                                                  c->p.press, nullptr, nullptr, c->dflags);
   RCK(check_launch("k_finalize_kick"));
   BoxState s;
   RCK(read_box(c, &s));
@@ -2034,13 +2148,19 @@ extern "C" int mdk_md_steps(void *h, int nsteps, double *trace) {
     c->trace_cap = nsteps;
   }
   for (int s = 0; s < nsteps; s++) {
-    RCK(md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true));
+    RCK(md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true, s + 1 < nsteps));
     // relink when the reference's ntc counter wraps (moldin.F90:791)
     if (c->ntc == 0) RCK(mdk_relink(h));
   }
   if (trace && nsteps > 0)
     HIPCK(hipMemcpyAsync(trace, c->trace_dev, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  if (c->step_counter > 0) {  // flags of the last steps, not yet polled
+    const double *m = c->mon_host + MON_STRIDE * (int)((c->step_counter - 1) & 3);
+    if (m[11] > 0.0) c->lag_error = MDK_ERR_NBR_OVERFLOW;
+    if (m[12] > 0.0) c->lag_error = MDK_ERR_STATE;
+  }
+  RCK(lag_status(c));
   return check_device_status(c);
 }
 
@@ -2058,16 +2178,24 @@ extern "C" int mdk_stats(void *h, long *rebuilds, long *full_pairs) {
   return MDK_OK;
 }
 
-extern "C" int mdk_timing(void *h, int enable) {
+extern "C" int mdk_timing(void *h, int mask) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
-  c->timing = enable != 0;
-  if (enable) {
+  c->timing_mask = mask & ((1 << TC_NCLASS) - 1);
+  for (int k = 0; k < TC_NCLASS; k++) c->t_seen[k] = 0;
+  if (c->timing_mask) {
     for (int k = 0; k < TC_NCLASS; k++) { c->t_ms[k] = 0.0; c->t_cnt[k] = 0; }
     hipStreamSynchronize(c->stream);
     for (auto &e : c->evs) { c->evpool.push_back(e.a); c->evpool.push_back(e.b); }
     c->evs.clear();
   }
+  return MDK_OK;
+}
+
+extern "C" int mdk_timing_every(void *h, int every) {
+  mdk_ctx *c = H(h);
+  if (!c || every < 1) return set_err(MDK_ERR_BAD_ARG, "bad timing stride");
+  c->timing_every = every;
   return MDK_OK;
 }
 

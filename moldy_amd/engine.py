@@ -40,7 +40,7 @@ EXPORTS = [
     "mdk_batch_create", "mdk_batch_destroy", "mdk_batch_last_error", "mdk_batch_set_potential",
     "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
-    "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
+    "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
 ]
 
 
@@ -95,7 +95,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_update_therm_avs": [vp, dp, dp, dp],
         "mdk_md_steps": [vp, C.c_int, dp],
         "mdk_stats": [vp, lp, lp],
-        "mdk_timing": [vp, C.c_int],
+        "mdk_timing": [vp, C.c_int], "mdk_timing_every": [vp, C.c_int],
         "mdk_timing_read": [vp, dp, lp, C.c_int],
         "mdk_synchronize": [vp],
     }
@@ -323,8 +323,19 @@ class Engine:
         _check(self.L.mdk_stats(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def timing(self, enable: bool):
-        _check(self.L.mdk_timing(self.h, 1 if enable else 0))
+    TIMING_CLASSES = ("drift", "density", "force", "kick", "rebuild")
+
+    def timing(self, enable: bool, classes=None, every: int = 1):
+        """Record HIP events around the kernels of the given classes (all by
+        default; two events per recorded launch, one launch in `every` per
+        class) or stop recording."""
+        _check(self.L.mdk_timing_every(self.h, int(every)))
+        mask = 0
+        if enable:
+            names = self.TIMING_CLASSES if classes is None else classes
+            for nm in names:
+                mask |= 1 << self.TIMING_CLASSES.index(nm)
+        _check(self.L.mdk_timing(self.h, mask))
 
     def timing_read(self):
         ms = np.zeros(5)
