@@ -101,6 +101,8 @@ struct mdk_ctx {
   int ncell_cap;
   int grid[3];
   int *dflags;             // [0] full-list overflow, [1] half-list overflow, [2] grid too coarse
+  // 2x2x2-cell blocks of the list build (Morton key / 8), for grid lblock_g
+  int *lblocks, nlblocks, lblocks_cap, lblock_g[3];
   long rebuilds;
   bool list_valid;
   int last_rebuild_step;
@@ -569,34 +571,25 @@ __global__ void k_fast_cells(const double *x0, const double *y0, const double *z
   cell[i] = morton3(cx, cy, cz);
 }
 
-// Full neighbour list at r_list over the fine grid; atoms are already sorted by
-// cell so cell c holds slots [cstart[c], cstart[c+1]).  Positions are the
-// image-fixed Cartesian ones of pos4; the image shift of a candidate follows
-// from the wrap of its cell (or from rounding in a dimension with one cell).
-// nbr is column-major nbr[k*n + i] (index | shift code << 27).
-__global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
-    const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
-    const int *cstart, int n, int gx, int gy, int gz, const BoxState *bs, double rl2, int pbc,
-    int maxn, int *nbr, int *numn, float *sref, int *flags) {
-  __shared__ double S[32][3];
-  double b[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
-  shift_table(b, S);
-  __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // cells of the current box must be at least r_list wide where the stencil spans them
-    const int gg[3] = {gx, gy, gz};
-    const double vol = fabs(dev_volume(b));
-    for (int d = 0; d < 3; d++) {
-      const double *u = b + 3 * ((d + 1) % 3), *w = b + 3 * ((d + 2) % 3);
-      const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
-      const double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
-      if (gg[d] >= 3 && width / gg[d] < sqrt(rl2)) atomicMax(&flags[2], 1);
-    }
+// Flags (flags[2]) a grid whose cells are narrower than the list radius in
+// the current box (perpendicular widths) in any dimension the stencil spans.
+__device__ void check_cell_width(const double *b, int gx, int gy, int gz, double rl2, int *flags) {
+  const int gg[3] = {gx, gy, gz};
+  const double vol = fabs(dev_volume(b));
+  for (int d = 0; d < 3; d++) {
+    const double *u = b + 3 * ((d + 1) % 3), *w = b + 3 * ((d + 2) % 3);
+    const double cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+    const double width = vol / sqrt(cx * cx + cy * cy + cz * cz);
+    if (gg[d] >= 3 && width / gg[d] < sqrt(rl2)) atomicMax(&flags[2], 1);
   }
-  int i = xcd_tile() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+}
+
+// Neighbour list of atom i over its 27 stencil cells, gathering candidates
+// from global memory (any grid; rounding images in dimensions with < 3 cells).
+__device__ void list_one_atom(int i, const double4 *__restrict__ pos4, const double *atoms, const int *img,
+                              const int *cell, const int *cstart, int n, int gx, int gy, int gz,
+                              const double (*S)[3], double rl2, int pbc, int maxn, int *nbr, int *numn,
+                              float *sref, int *flags) {
   const unsigned c = (unsigned)cell[i];
   const int cx = (int)compact3(c), cy = (int)compact3(c >> 1), cz = (int)compact3(c >> 2);
   const int sxr = gx >= 3 ? 1 : 0, syr = gy >= 3 ? 1 : 0, szr = gz >= 3 ? 1 : 0;
@@ -644,6 +637,119 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
   sref[3 * i + 0] = (float)atoms[i];
   sref[3 * i + 1] = (float)atoms[(size_t)n + i];
   sref[3 * i + 2] = (float)atoms[2 * (size_t)n + i];
+}
+
+// Full neighbour list at r_list over the fine grid; atoms are already sorted by
+// cell so cell c holds slots [cstart[c], cstart[c+1]).  Positions are the
+// image-fixed Cartesian ones of pos4; the image shift of a candidate follows
+// from the wrap of its cell (or from rounding in a dimension with one cell).
+// nbr is column-major nbr[k*n + i] (index | shift code << 27).
+__global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
+    const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
+    const int *cstart, int n, int gx, int gy, int gz, const BoxState *bs, double rl2, int pbc,
+    int maxn, int *nbr, int *numn, float *sref, int *flags) {
+  __shared__ double S[32][3];
+  double b[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  shift_table(b, S);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) check_cell_width(b, gx, gy, gz, rl2, flags);
+  int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, pbc, maxn, nbr, numn, sref, flags);
+}
+
+// Full neighbour list by 2x2x2-cell blocks (8 consecutive Morton keys, so one
+// contiguous range of atoms), one wave per block.  The wave stages the atoms of
+// the 4x4x4 cells around the block in LDS -- positions relative to the block's
+// first atom, shifted by the periodic image through which the block sees them,
+// in FP32 (the 1e-6 A rounding is immaterial against the skin), with their
+// list entries (index | image code).  Every lane then tests the same staged
+// candidates at the same time (broadcast LDS reads, four per iteration) instead
+// of gathering them from global memory.  Fully periodic boxes with >= 4 cells
+// per dimension (no staged cell occurs twice); a block whose neighbourhood
+// exceeds LIST_CAP atoms uses the global-gather loop.  blocks[] lists the
+// Morton key / 8 of every block that holds grid cells.
+#define LIST_CAP 512
+__global__ void __launch_bounds__(64) k_full_list_blk(
+    const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
+    const int *__restrict__ cstart, const int *blocks, int n, int gx, int gy, int gz, const BoxState *bs,
+    double rl2, int maxn, int *nbr, int *numn, float *sref, int *flags) {
+  __shared__ double S[32][3];
+  __shared__ float4 cpos[LIST_CAP + 3];  // +3: the unrolled loop reads up to 3 slots past the end
+  __shared__ int cent[LIST_CAP + 3];
+  const int lane = threadIdx.x;
+  shift_table(bs->b0, S);
+  if (blockIdx.x == 0 && lane == 0) check_cell_width(bs->b0, gx, gy, gz, rl2, flags);
+  const unsigned base = (unsigned)blocks[xcd_tile()] << 3;
+  const int x0 = 2 * (int)compact3(base >> 3), y0 = 2 * (int)compact3(base >> 4), z0 = 2 * (int)compact3(base >> 5);
+  const int a0 = cstart[base], a1 = cstart[base + 8];
+  if (a1 <= a0) return;  // uniform: no atoms in this block
+  // staged cell h = hx + 4 hy + 16 hz at (x0-1+hx, y0-1+hy, z0-1+hz): one per lane
+  int x = x0 - 1 + (lane & 3), y = y0 - 1 + ((lane >> 2) & 3), z = z0 - 1 + (lane >> 4);
+  int wx = 0, wy = 0, wz = 0;
+  if (x < 0) { x += gx; wx = 1; } else if (x >= gx) { x -= gx; wx = -1; }
+  if (y < 0) { y += gy; wy = 1; } else if (y >= gy) { y -= gy; wy = -1; }
+  if (z < 0) { z += gz; wz = 1; } else if (z >= gz) { z -= gz; wz = -1; }
+  const int key = morton3(x, y, z), code = (wx + 1) + 3 * (wy + 1) + 9 * (wz + 1);
+  const int cs = cstart[key], cnt = cstart[key + 1] - cs;
+  int off = cnt;  // exclusive prefix sum over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(off, o, 64);
+    if (lane >= o) off += u;
+  }
+  const int total = __shfl(off, 63, 64);
+  off -= cnt;
+  __syncthreads();  // shift table
+  if (total > LIST_CAP) {
+    for (int i = a0 + lane; i < a1; i += 64)
+      list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, 7, maxn, nbr, numn, sref, flags);
+    return;
+  }
+  const double4 ref = pos4[a0];
+  for (int k = 0; k < cnt; k++) {
+    const double4 p = pos4[cs + k];
+    cpos[off + k] = make_float4((float)(p.x - S[code][0] - ref.x), (float)(p.y - S[code][1] - ref.y),
+                                (float)(p.z - S[code][2] - ref.z), 0.f);
+    cent[off + k] = (cs + k) | (code << NBR_CODE_SHIFT);
+  }
+  if (lane < 3) {  // far-away fillers for the unrolled tail (never within the list radius)
+    cpos[total + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, 0.f);
+    cent[total + lane] = NBR_NULL_CODE << NBR_CODE_SHIFT;
+  }
+  __syncthreads();
+  const float rl2f = (float)rl2;
+  for (int i = a0 + lane; i < a1; i += 64) {
+    const double4 pd = pos4[i];
+    const float px = (float)(pd.x - ref.x), py = (float)(pd.y - ref.y), pz = (float)(pd.z - ref.z);
+    int m = 0;
+    for (int c = 0; c < total; c += 4) {
+      float4 q[4];
+      int e[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        q[u] = cpos[c + u];
+        e[u] = cent[c + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const float rx = px - q[u].x, ry = py - q[u].y, rz = pz - q[u].z;
+        if (rx * rx + ry * ry + rz * rz < rl2f && (e[u] & NBR_IDX_MASK) != i) {
+          if (m < maxn) nbr[nbr_at(m, i, n)] = e[u];
+          m++;
+        }
+      }
+    }
+    numn[i] = min(m, maxn);
+    if (m > maxn) atomicMax(&flags[0], m);
+    for (int k = m; k < ((m + 3) & ~3) && k < maxn; k++)  // pad the last quad with null entries
+      nbr[nbr_at(k, i, n)] = i | (NBR_NULL_CODE << NBR_CODE_SHIFT);
+    sref[3 * i + 0] = (float)atoms[i];
+    sref[3 * i + 1] = (float)atoms[(size_t)n + i];
+    sref[3 * i + 2] = (float)atoms[2 * (size_t)n + i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -964,17 +1070,21 @@ __device__ void therm_box(BoxState &s, int ivol) {
 __global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pf, int nb, BoxState *bs,
                                                               int mode, int ivol, double press,
                                                               double bdel2, double dt) {
-  double v[NPART_FORCE + 1];
-  block_final<NPART_FORCE + 1, 0>(pf, nb, v);
   // small matrices live in LDS: the nine tensor threads index them with their
-  // own (ii,jj), which private arrays would turn into scratch accesses
-  __shared__ double b0[9], b1[9], tg[9], tgd[9], tgi[9], sv[NPART_FORCE], sbi[9];
+  // own (ii,jj), which private arrays would turn into scratch accesses.  The
+  // box fields are fetched before the partial sums so the two memory round
+  // trips overlap.
+  __shared__ double b0[9], b1[9], b2[9], tgid0[9], tg[9], tgd[9], tgi[9], sv[NPART_FORCE], sbi[9];
   const int t = threadIdx.x;
   const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
   if (t < 9) {
     b0[t] = bs->b0[t];
     b1[t] = bs->b1[t];
+    b2[t] = bs->b2[t];
+    tgid0[t] = bs->tgid[t];
   }
+  double v[NPART_FORCE + 1];
+  block_final<NPART_FORCE + 1, 0>(pf, nb, v);  // its barriers also publish the box fields
   if (t == 0)
     for (int k = 0; k < NPART_FORCE; k++) sv[k] = v[k];
   __syncthreads();
@@ -1014,9 +1124,9 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pf, 
       bs->tgdot[t] = tgdv;  // kinten's tgdot
     }
     if (mode == FIN_VV || mode == FIN_VV_THERM) {
-      bs->tgid_lag[t] = bs->tgid[t];
+      bs->tgid_lag[t] = tgid0[t];
       const bool upd = ivol < 1 || (ivol == 4 && t == 8);
-      double b2v = bs->b2[t];
+      double b2v = b2[t];
       if (upd) b2v = bdel2 * fbv;
       bs->b2[t] = b2v;
       if (upd) b1[t] = b1[t] + b2v;
@@ -1351,6 +1461,37 @@ static int ensure_pos4(mdk_ctx *c);
 
 // Full-list rebuild: fine grid from the current box, re-sort atoms by cell,
 // build the list; grows the per-atom capacity and retries on overflow.
+// Morton-ordered list of the 2x2x2-cell blocks that hold cells of grid g
+// (recomputed when g changes).
+static int plan_list_blocks(mdk_ctx *c, const int g[3]) {
+  if (c->lblocks && g[0] == c->lblock_g[0] && g[1] == c->lblock_g[1] && g[2] == c->lblock_g[2]) return MDK_OK;
+  const int bx = (g[0] + 1) / 2, by = (g[1] + 1) / 2, bz = (g[2] + 1) / 2;
+  std::vector<int> keys;
+  keys.reserve((size_t)bx * by * bz);
+  for (int z = 0; z < bz; z++)
+    for (int y = 0; y < by; y++)
+      for (int x = 0; x < bx; x++) {
+        unsigned k = 0;
+        for (int bit = 0; bit < 10; bit++) {
+          k |= (unsigned)((x >> bit) & 1) << (3 * bit);
+          k |= (unsigned)((y >> bit) & 1) << (3 * bit + 1);
+          k |= (unsigned)((z >> bit) & 1) << (3 * bit + 2);
+        }
+        keys.push_back((int)k);
+      }
+  std::sort(keys.begin(), keys.end());
+  const int nb = (int)keys.size();
+  if (nb > c->lblocks_cap) {
+    if (c->lblocks) HIPCK(hipFree(c->lblocks));
+    HIPCK(hipMalloc(&c->lblocks, sizeof(int) * nb));
+    c->lblocks_cap = nb;
+  }
+  HIPCK(hipMemcpy(c->lblocks, keys.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+  c->nlblocks = nb;
+  c->lblock_g[0] = g[0]; c->lblock_g[1] = g[1]; c->lblock_g[2] = g[2];
+  return MDK_OK;
+}
+
 static int rebuild_full(mdk_ctx *c) {
   const int n = c->n;
   c->list_valid = false;
@@ -1390,16 +1531,26 @@ static int rebuild_full(mdk_ctx *c) {
   c->pos4_valid = true;
   // reference cells move with the atoms
   k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
-  HIPCK(hipMemcpyAsync(c->ic, c->zat_alt, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+  std::swap(c->ic, c->zat_alt);  // zat_alt is free scratch after the swap above
   k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
                                                        n, g[0], g[1], g[2], c->cell_of);
   const double rl2 = c->r_list * c->r_list;
+  // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per dimension
+  const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4;
+  if (blk) RCK(plan_list_blocks(c, g));
   for (int attempt = 0; attempt < 4; attempt++) {
     HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
-    k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
-        c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
-        c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
-    RCK(check_launch("k_full_list"));
+    if (blk) {
+      k_full_list_blk<<<c->nlblocks, 64, 0, c->stream>>>(
+          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, c->lblocks, n, g[0], g[1], g[2], c->box + c->cur,
+          rl2, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+      RCK(check_launch("k_full_list_blk"));
+    } else {
+      k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
+          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
+          c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+      RCK(check_launch("k_full_list"));
+    }
     if (lagged) {
       // overflow and cell-width flags reach the host through the monitor ring
       c->list_valid = true;
@@ -1735,7 +1886,7 @@ extern "C" int mdk_destroy(void *h) {
   void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->cell_count,
-                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev};
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
