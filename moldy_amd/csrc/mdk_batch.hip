@@ -26,6 +26,7 @@
 #include "../../include/moldy_hip.h"
 #include "mdk_hostpot.h"
 #include "mdk_internal.h"
+#include "mdk_wave.h"
 
 namespace {
 
@@ -81,17 +82,12 @@ struct BatchArgs {
 // gradient slots per cell: a_k, r_k (MDK_MAXV each), A_k, R_k (MDK_MAXP each)
 #define NGRAD_SLOTS (2 * MDK_MAXV + 2 * MDK_MAXP)
 
-__device__ __forceinline__ double bwave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // Fixed-order sum over the workgroup (lanes, then waves); every thread gets the
 // total.  red must hold BLOCK/64 doubles; the call contains two barriers.
 template <int BLOCK>
 __device__ __forceinline__ double block_sum(double v, double *red) {
-  v = bwave_sum(v);
+  v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = red[0];
@@ -425,7 +421,7 @@ __device__ double store_cell(const BatchArgs &A, const CellView &V, double e, co
   const int t = threadIdx.x;
   double v[7] = {e, w[0], w[1], w[2], w[3], w[4], w[5]};
 #pragma unroll
-  for (int k = 0; k < 7; k++) v[k] = bwave_sum(v[k]);
+  for (int k = 0; k < 7; k++) v[k] = wave_sum(v[k]);
   if ((t & 63) == 0) {
 #pragma unroll
     for (int k = 0; k < 7; k++) red[t >> 6][k] = v[k];

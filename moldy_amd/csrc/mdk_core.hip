@@ -32,6 +32,7 @@
 
 #include "../../include/moldy_hip.h"
 #include "mdk_internal.h"
+#include "mdk_wave.h"
 
 // ---------------------------------------------------------------------------
 // error handling
@@ -114,6 +115,7 @@ struct mdk_ctx {
   // density pass's embedding energy as value NPART_FORCE; half kick
   double *part_f, *part_k;
   double *stage_f, *stage_k;  // first-stage reductions of the partials
+  unsigned *fin_cnt;          // arrival counters of the finishing kernels (force, kick), zeroed
   int nblk;
   // host-mapped monitor ring, MON_STRIDE doubles per step slot (4 slots):
   // [0] maxdisp2, [1] maxstep2, [2..10] B0 after the step, [11] full-list
@@ -153,16 +155,6 @@ static inline double *fld(double *base, int f, int n) { return base + (size_t)f 
 // ---------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 // Block-reduce K sums (and the last KM entries as maxima) into
 // part[k*gridDim.x + blockIdx.x] (value-major, so the finishing block's reads
@@ -216,10 +208,6 @@ __device__ __forceinline__ void block_final(const double *part, int nb, double (
   }
   __syncthreads();
 }
-
-// First reduction stage: block b folds partial columns [b*chunk, (b+1)*chunk)
-// of the value-major array `in` (nb columns) into out[k*gridDim.x + b], in a
-// fixed order, so the finishing block reads only gridDim.x columns.
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs, each with its own L2; remapping physical block b to logical tile
@@ -314,23 +302,6 @@ __device__ __forceinline__ void shift_table(const double *b, double (*S)[3]) {
     S[c][2] = b[2] * sx + b[5] * sy + b[8] * sz;
   }
 }
-template <int K, int KM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_reduce_stage(const double *in, int nb, double *out,
-                                                            int chunk) {
-  double v[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  const int b0 = blockIdx.x * chunk, b1 = min(nb, b0 + chunk);
-  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const double x = in[(size_t)k * nb + b];
-      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
-    }
-  }
-  block_store<K, KM>(v, out);
-}
-
 // inverse of a 3x3 column-major matrix (cofactors / determinant)
 __device__ __forceinline__ void inv3(const double *b, double *bi) {
   const double c00 = TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2);
@@ -1063,13 +1034,57 @@ __device__ void therm_box(BoxState &s, int ivol) {
 // initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part
 // (dynamics_m.F90:672-700).  Threads 0..8 each own one tensor element (i,j);
 // phases are separated by barriers; only changed fields are written back.
-// Finishing block of force().  The per-block partials are in one value-major
+// Stage-and-finish reduction for the finishing kernels, launched as p =
+// ceil(nb / 64) blocks of one wave (MDK_FIN_BLOCK).  Block b folds per-block
+// partials [64 b, 64 b + 64) lane-wise and across the wave in a fixed order;
+// lane 0 writes the K results to stage[k*p + b] and increments the arrival
+// counter with one agent-scope release/acquire, so each block makes a single
+// L2 write-back and invalidate.  The last block to arrive folds the p stage
+// columns the same way into out[] (every lane receives them), resets the
+// counter and returns true; the others return false.
+#define MDK_FIN_BLOCK 64
+template <int K, int KM>
+__device__ __forceinline__ void fold_columns(const double *part, int nb, int b0, int b1, double (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? 0.0 : -HUGE_VAL;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += MDK_FIN_BLOCK) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double x = part[(size_t)k * nb + b];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
+}
+
+template <int K, int KM>
+__device__ bool stage_and_finish(const double *part, int nb, double *stage, unsigned *cnt, double (&out)[K]) {
+  const int p = gridDim.x, b0 = blockIdx.x * 64;
+  double v[K];
+  fold_columns<K, KM>(part, nb, b0, min(nb, b0 + 64), v);
+  int last = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) stage[(size_t)k * p + blockIdx.x] = v[k];
+    const unsigned prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = prev + 1 == (unsigned)p;
+  }
+  // one wave: lane 0's verdict to all lanes, which are past lane 0's acquire
+  if (!__builtin_amdgcn_readfirstlane(last)) return false;
+  fold_columns<K, KM>(stage, p, 0, p, out);
+  if (threadIdx.x == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Finishing kernel of force().  The per-block partials are in one value-major
 // array: values 0..18 from the force pass (NPART_FORCE) and value 19, the
-// embedding energy, from the density pass; nb blocks' worth (after an
-// optional first stage).
-__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pf, int nb, BoxState *bs,
-                                                              int mode, int ivol, double press,
-                                                              double bdel2, double dt) {
+// embedding energy, from the density pass (nb blocks); reduced by
+// stage_and_finish, then the last block completes force().
+__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *pf, int nb, double *stage,
+                                                                  unsigned *cnt, BoxState *bs, int mode,
+                                                                  int ivol, double press, double bdel2,
+                                                                  double dt) {
   // small matrices live in LDS: the nine tensor threads index them with their
   // own (ii,jj), which private arrays would turn into scratch accesses.  The
   // box fields are fetched before the partial sums so the two memory round
@@ -1084,7 +1099,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_finalize_force(const double *pf, 
     tgid0[t] = bs->tgid[t];
   }
   double v[NPART_FORCE + 1];
-  block_final<NPART_FORCE + 1, 0>(pf, nb, v);  // its barriers also publish the box fields
+  if (!stage_and_finish<NPART_FORCE + 1, 0>(pf, nb, stage, cnt, v)) return;
   if (t == 0)
     for (int k = 0; k < NPART_FORCE; k++) sv[k] = v[k];
   __syncthreads();
@@ -1284,17 +1299,17 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
   block_store<NPART_KICK, 2>(v, part);
 }
 
-// Finishing block of the kick: s1,s2,s3, KE/TE/H/TH (moldin.F90:763-790),
-// trace row and the host-visible skin monitor.
-// Finishing block of the half kick: KE, box KE, TE, H, TH, the trace row and
-// the monitor sample (thread 0 after the block reduction of the partials).
-__global__ void __launch_bounds__(MDK_BLOCK) k_finalize_kick(const double *pk, int nb, BoxState *bs,
-                                                             double dt, double bdel2, double press,
-                                                             double *trace_row, double *mon,
-                                                             const int *dflags) {
+// Finishing kernel of the half kick: s1,s2,s3, KE, box KE, TE, H, TH
+// (moldin.F90:763-790), the trace row and the host-visible skin monitor
+// (thread 0 of the last block, after stage_and_finish).
+__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_kick(const double *pk, int nb, double *stage,
+                                                                 unsigned *cnt, BoxState *bs, double dt,
+                                                                 double bdel2, double press,
+                                                                 double *trace_row, double *mon,
+                                                                 const int *dflags) {
 #pragma clang fp contract(off)
   double v[NPART_KICK];
-  block_final<NPART_KICK, 2>(pk, nb, v);
+  if (!stage_and_finish<NPART_KICK, 2>(pk, nb, stage, cnt, v)) return;
   if (threadIdx.x != 0) return;
   // only the fields this kernel changes are read and written
   const double pe = bs->pe, vol = bs->vol;
@@ -1620,21 +1635,6 @@ static int ensure_pos4(mdk_ctx *c) {
   return check_launch("k_pack");
 }
 
-// Two-stage fixed-order reduction: more than 128 per-block partials are first
-// folded in chunks of 64 by a parallel stage; returns the array and the count
-// the finishing block reads.
-template <int K, int KM>
-static const double *stage_reduce(mdk_ctx *c, const double *part, int nb, double *stage, int *nb_out) {
-  if (nb <= 128) {
-    *nb_out = nb;
-    return part;
-  }
-  const int chunk = 64, p = (nb + chunk - 1) / chunk;
-  k_reduce_stage<K, KM><<<p, MDK_BLOCK, 0, c->stream>>>(part, nb, stage, chunk);
-  *nb_out = p;
-  return stage;
-}
-
 static int block_count(mdk_ctx *c) { return nblocks(c->n); }
 
 static int launch_density(mdk_ctx *c, int do_embed) {
@@ -1658,10 +1658,10 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   }
   tend(c, te);
   RCK(check_launch("k_force"));
-  int nbf;
-  const double *pf = stage_reduce<NPART_FORCE + 1, 0>(c, c->part_f, block_count(c), c->stage_f, &nbf);
-  k_finalize_force<<<1, MDK_BLOCK, 0, c->stream>>>(pf, nbf, c->box + c->cur, fin_mode, c->p.ivol, c->p.press,
-                                                   c->p.bdel2, c->p.deltat);
+  const int nb = block_count(c);
+  k_finalize_force<<<(nb + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_f, nb, c->stage_f, c->fin_cnt,
+                                                                c->box + c->cur, fin_mode, c->p.ivol,
+                                                                c->p.press, c->p.bdel2, c->p.deltat);
   return check_launch("k_finalize_force");
 }
 
@@ -1696,10 +1696,9 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   tend(c, te);
   RCK(check_launch("k_kick"));
   if (mode != 2) {
-    int nbk;
-    const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
-    k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
-                                                    c->p.press, trace_row_dev, mon, c->dflags);
+    k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
+        c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
+        trace_row_dev, mon, c->dflags);
     RCK(check_launch("k_finalize_kick"));
   }
   if (fuse_drift && mode == 0) {
@@ -1850,6 +1849,11 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
   ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * (NPART_FORCE + 1));
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
+  ALLOC(c->fin_cnt, sizeof(unsigned) * 2);
+  if (hipMemset(c->fin_cnt, 0, sizeof(unsigned) * 2) != hipSuccess) {
+    rc = set_err(MDK_ERR_HIP, "counter initialisation failed");
+    goto fail;
+  }
   if (hipHostMalloc((void **)&c->mon_host, sizeof(double) * 4 * MON_STRIDE, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->mon_dev, c->mon_host, 0) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "mapped host buffer failed");
@@ -1885,7 +1889,7 @@ extern "C" int mdk_destroy(void *h) {
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
-                  c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->cell_count,
+                  c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -2276,10 +2280,9 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
       c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
-  int nbk;
-  const double *pk = stage_reduce<NPART_KICK, 2>(c, c->part_k, c->nblk, c->stage_k, &nbk);
-  k_finalize_kick<<<1, MDK_BLOCK, 0, c->stream>>>(pk, nbk, c->box + c->cur, c->p.deltat, c->p.bdel2,
-                                                  c->p.press, nullptr, nullptr, c->dflags);
+  k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
+      c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
+      nullptr, nullptr, c->dflags);
   RCK(check_launch("k_finalize_kick"));
   BoxState s;
   RCK(read_box(c, &s));
