@@ -98,6 +98,8 @@ struct mdk_ctx {
   // full list
   int maxn;
   int *nbr, *numn;
+  int *nact;  // per atom: list slots the force pass visits (set by the density pass)
+  bool part_due;  // the next density pass reorders the new list (in-range partners first)
   int *cell_of, *cell_count, *cell_start, *cell_fill, *members;
   int ncell_cap;
   int grid[3];
@@ -856,42 +858,11 @@ __global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n,
                          b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
 }
 
-// Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
-// (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
-// one lane per atom; F, F' per atom; per-block embedding energy.  F' is
-// also stored in pos4[i].w for the force pass.
-template <typename T, int POTM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
-                                                       const int *__restrict__ nbr,
-                                                       const int *__restrict__ numn,
-                                                       const BoxState *bs, PotDev P,
-                                                       int do_embed, double *part) {
-  __shared__ double S[32][3];
-  shift_table(bs->b0, S);
-  __syncthreads();
-  const int i = xcd_tile() * blockDim.x + threadIdx.x;
-  double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
-  if (i < n) {
-    const double4 pi = pos4[i];
-    const int nn = numn[i];
-    // one 16-byte index load and four independent gathers per quad (quads
-    // are padded with null entries)
-    for (int q = 0; 4 * q < nn; q++) {
-      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
-      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
-      double4 pj[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        T rx, ry, rz;
-        const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
-        T r, ri;
-        r_and_inv(r2, r, ri);
-        acc += (double)mdk_pair_phi<T, POTM>(P, r);
-      }
-    }
-  }
+// Per-atom end of the density pass: rho, and with do_embed the embedding
+// energy F(rho), F'(rho) (also into the packed positions for the force pass)
+// and the block's embedding-energy partial.
+__device__ __forceinline__ void density_finish(double *atoms, double4 *pos4, int n, int i, double acc,
+                                               const PotDev &P, int do_embed, double *part) {
   double pe[1] = {0.0};
   if (i < n) {
     double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
@@ -912,6 +883,128 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
   if (do_embed) block_store<1, 0>(pe, part);
 }
 
+// Appends entry e as slot m of atom i's list through a register quad; every
+// fourth entry stores the quad with one 16-byte write.
+__device__ __forceinline__ void quad_put(int *nbr, int i, int n, int e, int &m, int4 &qd) {
+  const int sl = m & 3;
+  qd.x = sl == 0 ? e : qd.x;
+  qd.y = sl == 1 ? e : qd.y;
+  qd.z = sl == 2 ? e : qd.z;
+  qd.w = sl == 3 ? e : qd.w;
+  if (sl == 3) *(int4 *)(nbr + nbr_at(m - 3, i, n)) = qd;
+  m++;
+}
+
+// Density pass (rhoset, src/dynamics_m.F90:497-579) fused with embed
+// (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
+// one lane per atom; F, F' per atom; per-block embedding energy.  F' is
+// also stored in pos4[i].w for the force pass.
+template <typename T, int POTM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
+                                                       const int *__restrict__ nbr,
+                                                       const int *__restrict__ numn,
+                                                       const BoxState *bs, PotDev P,
+                                                       int do_embed, double *part, int *nact) {
+  __shared__ double S[32][3];
+  shift_table(bs->b0, S);
+  __syncthreads();
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
+  // list slots up to the end of the last quad holding a partner in range: with
+  // a margin of 1e-4 A the force pass's own r of every later pair is beyond the
+  // outermost knot, where all pair terms are exactly zero
+  const T rc = (T)P.r_int + (T)1e-4, rc2 = rc * rc;
+  int na = 0;
+  if (i < n) {
+    const double4 pi = pos4[i];
+    const int nn = numn[i];
+    // one 16-byte index load and four independent gathers per quad (quads
+    // are padded with null entries)
+    for (int q = 0; 4 * q < nn; q++) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
+      double4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
+      T mn = rc2;  // one min chain per quad keeps the register footprint of the loop
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        T rx, ry, rz;
+        const T r2 = pair_r2<T>(S, pi, pj[u], (int)(e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
+        T r, ri;
+        r_and_inv(r2, r, ri);
+        acc += (double)mdk_pair_phi<T, POTM>(P, r);
+        mn = fmin(mn, r2);
+      }
+      na = mn < rc2 ? 4 * q + 4 : na;
+    }
+    nact[i] = na;
+  }
+  density_finish(atoms, pos4, n, i, acc, P, do_embed, part);
+}
+
+// Density pass of the first step on a new list: as k_density, and it also
+// reorders each atom's list in place -- the partners in range now (r^2 < rc2,
+// the test k_density uses for nact) first, in list order, then the others --
+// so that the force pass of this and the following steps stops after the
+// in-range part.  Out-of-range entries wait in LDS (DENS_OBUF per lane, more
+// stay in place, which costs only efficiency).  Each completed quad is written
+// after its slots have been read (an atom's list belongs to its lane alone).
+#define DENS_OBUF 16
+template <typename T, int POTM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, double4 *pos4, int n, int *nbr,
+                                                            const int *__restrict__ numn,
+                                                            const BoxState *bs, PotDev P, int do_embed,
+                                                            double *part, int *nact) {
+  __shared__ double S[32][3];
+  __shared__ int obuf[DENS_OBUF][MDK_BLOCK];
+  shift_table(bs->b0, S);
+  __syncthreads();
+  const int t = threadIdx.x, i = xcd_tile() * blockDim.x + t;
+  double acc = 0.0;
+  const T rc = (T)P.r_int + (T)1e-4, rc2 = rc * rc;
+  if (i < n) {
+    const double4 pi = pos4[i];
+    const int nn = numn[i];
+    int m = 0, no = 0, last = 0;  // entries written back, entries held, end of the last in-range one
+    int4 qd = make_int4(0, 0, 0, 0);
+    for (int q = 0; 4 * q < nn; q++) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const int e[4] = {e4.x, e4.y, e4.z, e4.w};
+      double4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = pos4[(unsigned)e[u] & NBR_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        T rx, ry, rz;
+        const T r2 = pair_r2<T>(S, pi, pj[u], (int)((unsigned)e[u] >> NBR_CODE_SHIFT), rx, ry, rz);
+        T r, ri;
+        r_and_inv(r2, r, ri);
+        acc += (double)mdk_pair_phi<T, POTM>(P, r);
+        if (4 * q + u < nn) {
+          const bool in = r2 < rc2;
+          if (in || no == DENS_OBUF) {
+            quad_put(nbr, i, n, e[u], m, qd);
+            if (in) last = m;
+          } else {
+            obuf[no++][t] = e[u];
+          }
+        }
+      }
+    }
+    for (int u = 0; u < no; u++) quad_put(nbr, i, n, obuf[u][t], m, qd);
+    if (m & 3) {  // the last quad's null entries
+      const int nul = i | (NBR_NULL_CODE << NBR_CODE_SHIFT), sl = m & 3;
+      qd.y = sl <= 1 ? nul : qd.y;
+      qd.z = sl <= 2 ? nul : qd.z;
+      qd.w = nul;
+      *(int4 *)(nbr + nbr_at(m & ~3, i, n)) = qd;
+    }
+    nact[i] = (last + 3) & ~3;
+  }
+  density_finish(atoms, pos4, n, i, acc, P, do_embed, part);
+}
+
 // Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
 // fp = -(V' + phi'(F'_i + F'_j))/r;  F_i = sum_j r_ij fp (Cartesian) and
 // f_i = B0^-1 F_i;  the virial is accumulated as W = 1/2 sum r r^T fp (tp =
@@ -922,7 +1015,7 @@ template <typename T, bool MASSCHK, int POTM>
 __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
                                                      const int *zat, int n,
                                                      const int *__restrict__ nbr,
-                                                     const int *__restrict__ numn,
+                                                     const int *__restrict__ nvisit,
                                                      const BoxState *bs, PotDev P,
                                                      double *part) {
   __shared__ double S[32][3];
@@ -941,7 +1034,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
     const double4 pi = pos4[i];
     const T dfi = (T)pi.w;
     const double mi = MASSCHK ? mass[i] : 0.0;
-    const int nn = numn[i];
+    const int nn = nvisit[i];  // list slots that can hold a partner in range
     // sums of V and of r r^T fp; the factor 1/2 of both is applied once below
     double pe2 = 0.0, wxx = 0, wyy = 0, wzz = 0, wxy = 0, wxz = 0, wyz = 0;
     for (int q = 0; 4 * q < nn; q++) {
@@ -1569,6 +1662,7 @@ static int rebuild_full(mdk_ctx *c) {
     if (lagged) {
       // overflow and cell-width flags reach the host through the monitor ring
       c->list_valid = true;
+      c->part_due = true;
       c->rebuilds++;
       tend(c, te);
       return MDK_OK;
@@ -1579,6 +1673,7 @@ static int rebuild_full(mdk_ctx *c) {
     if (fl[2] != 0) return set_err(MDK_ERR_STATE, "list cells narrower than the list radius");
     if (fl[0] == 0) {
       c->list_valid = true;
+      c->part_due = true;
       c->rebuilds++;
       tend(c, te);
       return MDK_OK;
@@ -1597,14 +1692,21 @@ static int rebuild_full(mdk_ctx *c) {
 template <typename T, int POTM>
 static void density_dispatch(mdk_ctx *c, int do_embed) {
   // embedding-energy partials are value NPART_FORCE of the force pass's array
-  k_density<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed,
-      c->part_f + (size_t)NPART_FORCE * nblocks(c->n));
+  double *pe = c->part_f + (size_t)NPART_FORCE * nblocks(c->n);
+  if (c->part_due)
+    k_density_part<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, pe, c->nact);
+  else
+    k_density<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, pe, c->nact);
+  c->part_due = false;
 }
 template <typename T, bool MC, int POTM>
 static void force_dispatch(mdk_ctx *c, int fin_mode) {
+  // tabulated pair terms interpolate across the range edge: those use the whole list
   k_force<T, MC, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->zat, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, c->part_f);
+      c->atoms, c->pos4, c->zat, c->n, c->nbr, POTM == POT_TABLE ? c->numn : c->nact, c->box + c->cur, c->pot,
+      c->part_f);
 }
 // lookup tables, the Fe-Fe knot counts, or any knot count
 static int pot_mode(const mdk_ctx *c) {
@@ -1839,6 +1941,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->nlist_ref, sizeof(int) * (size_t)n * p->nnbrs);
   ALLOC(c->nbr, sizeof(int) * (size_t)c->maxn * n);
   ALLOC(c->numn, sizeof(int) * n);
+  ALLOC(c->nact, sizeof(int) * n);
   ALLOC(c->cell_of, sizeof(int) * n);
   ALLOC(c->members, sizeof(int) * n);
   ALLOC(c->dflags, sizeof(int) * 4);
@@ -1867,6 +1970,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   hipMemsetAsync(c->img, 0, sizeof(int) * 3 * (size_t)n, c->stream);
   hipMemsetAsync(c->numn_ref, 0, sizeof(int) * n, c->stream);
   hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream);
+  hipMemsetAsync(c->nact, 0, sizeof(int) * n, c->stream);
   k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->orig, n);
   k_iota<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->slot_of, n);
   memset(c->mon_host, 0, sizeof(double) * 4 * MON_STRIDE);
@@ -1888,7 +1992,7 @@ extern "C" int mdk_destroy(void *h) {
   if (!c) return MDK_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
-                  c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->cell_of,
+                  c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks};
   for (void *q : ptrs)
