@@ -667,14 +667,16 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
   if (z < 0) { z += gz; wz = 1; } else if (z >= gz) { z -= gz; wz = -1; }
   const int key = morton3(x, y, z), code = (wx + 1) + 3 * (wy + 1) + 9 * (wz + 1);
   const int cs = cstart[key], cnt = cstart[key + 1] - cs;
-  int off = cnt;  // exclusive prefix sum over the wave
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(off, o, 64);
-    if (lane >= o) off += u;
+  // staged in ascending cell key, i.e. ascending atom index (atoms are sorted
+  // by cell key): every list comes out sorted by partner index, so the lanes
+  // of a wave gather nearby partners at the same list slot.  The 64 cells are
+  // distinct (>= 4 cells per dimension); off = atoms in cells of smaller key.
+  int off = 0, total = 0;
+  for (int l = 0; l < 64; l++) {
+    const int kl = __shfl(key, l, 64), cl = __shfl(cnt, l, 64);
+    off += kl < key ? cl : 0;
+    total += cl;
   }
-  const int total = __shfl(off, 63, 64);
-  off -= cnt;
   __syncthreads();  // shift table
   if (total > LIST_CAP) {
     for (int i = a0 + lane; i < a1; i += 64)
@@ -1019,10 +1021,13 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
                                                      const BoxState *bs, PotDev P,
                                                      double *part) {
   __shared__ double S[32][3];
+  __shared__ float vtab[8][8];  // interval cubics of the pair function (FP32 Fe-Fe form)
+  constexpr bool CUBIC = sizeof(T) == 4 && POTM == POT_FE;
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   shift_table(b, S);
+  if (CUBIC && threadIdx.x < 64) vtab[threadIdx.x >> 3][threadIdx.x & 7] = P.vcub[threadIdx.x >> 3][threadIdx.x & 7];
   __syncthreads();
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   const double *mass = atoms + (size_t)F_MASS * n;
@@ -1053,7 +1058,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
         T r, rr;
         r_and_inv(r2, r, rr);
         T vv, dv;
-        mdk_pair_vdv<T, POTM>(P, r, vv, dv);
+        if constexpr (CUBIC) mdk_vee_fe_cubic(P, vtab, r, vv, dv);
+        else mdk_pair_vdv<T, POTM>(P, r, vv, dv);
         const T dph = mdk_pair_dphi<T, POTM>(P, r);
         const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj[u].w) * rr);
         const double fpd = (double)fp;

@@ -147,6 +147,24 @@ static double fill_potdev(PotDev &P, const Knots &k) {
   P.phi_cut_vf = (float)P.phi_cut_v; P.r_intf = (float)rint;
   P.jx1f = (float)P.jx1; P.jv1f = (float)P.jv1; P.jx2f = (float)P.jx2; P.jv2f = (float)P.jv2;
   P.bier_af = (float)P.bier_a; P.bier_zedf = (float)P.bier_zed;
+  // interval cubics of the pair function (used by the FP32 form of 6-knot sets)
+  int ord[MDK_MAXV];
+  for (int i = 0; i < k.nv; i++) ord[i] = i;
+  std::sort(ord, ord + k.nv, [&](int a, int b) { return k.rv[a] > k.rv[b]; });
+  for (int i = 0; i < 8; i++) P.vknf[i] = i < k.nv ? (float)k.rv[ord[i]] : -1.0f;
+  for (int j = 0; j <= k.nv && j < 8; j++) {
+    const double u = j > 0 ? k.rv[ord[j - 1]] : (k.nv > 0 ? k.rv[ord[0]] : 0.0);
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    for (int i = 0; i < j; i++) {
+      const double a = k.av[ord[i]], d = k.rv[ord[i]] - u;
+      c0 += a * d * d * d;
+      c1 += 3.0 * a * d * d;
+      c2 += 3.0 * a * d;
+      c3 += a;
+    }
+    const double row[8] = {u, c0, c1, c2, c3, 2.0 * c2, 3.0 * c3, 0.0};
+    for (int m = 0; m < 8; m++) P.vcub[j][m] = (float)row[m];
+  }
   return rint;
 }
 

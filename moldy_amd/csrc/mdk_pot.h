@@ -36,6 +36,11 @@ struct PotDev {
   // pre-derived join constants: log(V) and V'/V at both ends
   double jx1, jv1, jx2, jv2;
   float jx1f, jv1f, jx2f, jv2f, bier_af, bier_zedf;
+  // FP32 pair function of a 6-knot set as one cubic per knot interval: vknf
+  // holds the knots in descending order, vcub[j] the interval with j knots
+  // beyond r as (u, c0, c1, c2 | c3, 2 c2, 3 c3, 0), u the upper end
+  float vknf[8];
+  float vcub[8][8];
 };
 
 // Knot-parameter sensitivities of the set_up constants (filled on the host):
@@ -207,6 +212,38 @@ __device__ __forceinline__ T mdk_dphi_n(const PotDev &P, T r) {
     s = knot_acc(s, T(-3) * Q.ap(k), d * d);
   }
   return r < Q.pcr() ? T(0) : s;
+}
+
+// FP32 pair function V, V' of a 6-knot set (POT_FE) from the interval cubics:
+// j = number of knots beyond r picks the interval, x = u_j - r >= 0, and
+// V = c0 + x (c1 + x (c2 + x c3)) = sum_k a_k (r_k - r)^3 over the knots
+// beyond r (c0..c3 = sum a d^3, 3 sum a d^2, 3 sum a d, sum a with d = r_k - u_j,
+// formed in FP64 on the host), V' = -(c1 + x (2 c2 + 3 c3 x)).  Horner in the
+// local variable is about twice as accurate as the FP32 knot sum and costs
+// 19 VALU operations instead of 36.  tab is the coefficient table in LDS.
+__device__ __forceinline__ void mdk_vee_fe_cubic(const PotDev &P, const float (*tab)[8], float r, float &v,
+                                                 float &dv) {
+  int j = 0;
+#pragma unroll
+  for (int k = 0; k < MDK_FE_NV; k++) j += r < P.vknf[k] ? 1 : 0;
+  const float4 a = *(const float4 *)tab[j], b = *(const float4 *)(tab[j] + 4);
+  const float x = a.x - r;
+  v = a.y + x * (a.z + x * (a.w + x * b.x));
+  dv = -(a.z + x * (b.y + x * b.z));
+  if (!(r > P.bc2f)) {
+    if (r > P.bc1f) {
+      VdV<float> q = mdk_join<float>(P.bc1f, P.bc2f, P.jx1f, P.jv1f, P.jx2f, P.jv2f, r);
+      v = q.v;
+      dv = q.dv;
+    } else if (r < P.bc1f) {
+      VdV<float> q = mdk_biersack<float>(P.bier_af, P.bier_zedf, r);
+      v = q.v;
+      dv = q.dv;
+    } else {
+      v = 0.f;
+      dv = 0.f;
+    }
+  }
 }
 
 // Padded-capacity forms (any knot count up to MDK_MAXV / MDK_MAXP).
