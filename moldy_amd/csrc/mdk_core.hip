@@ -77,6 +77,12 @@ struct TimedEv {
 
 #define MON_STRIDE 16
 
+// list grid for cell-relative packing (gx == 0: absolute packing)
+struct PackGeo {
+  int gx, gy, gz;
+  double ix, iy, iz;  // 1/g
+};
+
 struct mdk_ctx {
   mdk_params p;
   int n;
@@ -87,7 +93,7 @@ struct mdk_ctx {
   double *atoms, *atoms_alt;
   int *zat, *zat_alt, *orig, *orig_alt, *slot_of;
   float *sref;             // fractional positions at the last full-list build
-  double4 *pos4;           // packed (B0 (s - img), F') for the pair kernels
+  double4 *pos4;           // packed positions (absolute or cell-relative) and F' for the pair kernels
   int *img;                // integer image floor(s) fixed at the last full-list build
   bool pos4_valid;
   bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
@@ -99,6 +105,8 @@ struct mdk_ctx {
   int maxn;
   int *nbr, *numn;
   int *nact;  // per atom: list slots the force pass visits (set by the density pass)
+  bool rel;     // positions packed cell-relative (block list) rather than absolute
+  float4 *p4f;  // FP32 copy of the packed positions (FP32 form only)
   bool part_due;  // the next density pass reorders the new list (in-range partners first)
   int *cell_of, *cell_count, *cell_start, *cell_fill, *members;
   int ncell_cap;
@@ -153,6 +161,15 @@ struct mdk_ctx {
 };
 
 static inline double *fld(double *base, int f, int n) { return base + (size_t)f * n; }
+static inline PackGeo pack_geo(const mdk_ctx *c) {
+  PackGeo pg = {0, 0, 0, 0.0, 0.0, 0.0};
+  if (c->rel) {
+    pg.gx = c->grid[0]; pg.gy = c->grid[1]; pg.gz = c->grid[2];
+    pg.ix = 1.0 / pg.gx; pg.iy = 1.0 / pg.gy; pg.iz = 1.0 / pg.gz;
+  }
+  return pg;
+}
+static inline float4 *p4f_of(const mdk_ctx *c) { return c->p.precision == MDK_PREC_FP32 ? c->p4f : nullptr; }
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -275,10 +292,17 @@ __device__ double dev_volume(const double *b) {
 }
 
 
-// Image shifts of a neighbour entry: bits 27..31 of nbr[] hold
-// code = (sx+1) + 3 (sy+1) + 9 (sz+1) with s in {-1,0,1}^3 such that the
-// minimum-image separation is (s'_i - s'_j) + (sx,sy,sz), s' = s - img being
-// fractional positions with the integer image img fixed at the last rebuild.
+// A neighbour entry of nbr[] is j | code << 27.  The pair separation is
+// r_ij = R_i - R_j + D[code] with R the packed positions (pos4):
+//  * absolute packing (legacy list, any box): R = B0 (s - img) with the integer
+//    image img fixed at the last rebuild, code = image shift (sx+1) + 3 (sy+1)
+//    + 9 (sz+1) with s in {-1,0,1}^3, D = B0 (sx, sy, sz);
+//  * cell-relative packing (block list): R = B0 (s - img - f) with f the
+//    fractional centre of the atom's list cell, code = offset (dx+1) + 3 (dy+1)
+//    + 9 (dz+1) of j's list cell (unwrapped) from i's, D = -B0 (dx/gx, dy/gy,
+//    dz/gz).  |R| stays below a cell width, so an FP32 copy of R loses little
+//    and one 16-byte gather carries a partner in the FP32 form.
+// Codes 27..31 place the partner 1e6 A away: null entries that pad quads.
 #define NBR_IDX_MASK 0x07FFFFFF
 #define NBR_CODE_SHIFT 27
 // Neighbour k of atom i lives at nbr[((k/4) n + i) 4 + k%4]: four consecutive
@@ -288,9 +312,7 @@ __device__ __forceinline__ size_t nbr_at(int k, int i, int n) {
   return (((size_t)(k >> 2) * n + i) << 2) + (k & 3);
 }
 #define NBR_NULL_CODE 31
-__device__ __forceinline__ void shift_table(const double *b, double (*S)[3]) {
-  // codes 27..31 place the partner 1e6 A away: a null entry that pads a quad
-  // and contributes exactly zero to every sum
+__device__ __forceinline__ void shift_table(const double *b, PackGeo pg, double (*S)[3]) {
   for (int c = threadIdx.x; c < 32; c += blockDim.x) {
     if (c >= 27) {
       S[c][0] = 1.0e6;
@@ -298,7 +320,12 @@ __device__ __forceinline__ void shift_table(const double *b, double (*S)[3]) {
       S[c][2] = 0.0;
       continue;
     }
-    const double sx = (double)(c % 3 - 1), sy = (double)((c / 3) % 3 - 1), sz = (double)(c / 9 - 1);
+    double sx = (double)(c % 3 - 1), sy = (double)((c / 3) % 3 - 1), sz = (double)(c / 9 - 1);
+    if (pg.gx) {
+      sx = -sx * pg.ix;
+      sy = -sy * pg.iy;
+      sz = -sz * pg.iz;
+    }
     S[c][0] = b[0] * sx + b[3] * sy + b[6] * sz;
     S[c][1] = b[1] * sx + b[4] * sy + b[7] * sz;
     S[c][2] = b[2] * sx + b[5] * sy + b[8] * sz;
@@ -532,6 +559,24 @@ __device__ __forceinline__ int morton3(int x, int y, int z) {
   return (int)(spread3((unsigned)x) | (spread3((unsigned)y) << 1) | (spread3((unsigned)z) << 2));
 }
 
+// Packed position of atom i from fractional s - img (see the entry format).
+__device__ __forceinline__ double3 packed_pos(const double *b, PackGeo pg, const int *cell, int i, double fx,
+                                              double fy, double fz) {
+  if (pg.gx) {
+    const unsigned k = (unsigned)cell[i];
+    fx -= ((double)compact3(k) + 0.5) * pg.ix;
+    fy -= ((double)compact3(k >> 1) + 0.5) * pg.iy;
+    fz -= ((double)compact3(k >> 2) + 0.5) * pg.iz;
+  }
+  return make_double3(b[0] * fx + b[3] * fy + b[6] * fz, b[1] * fx + b[4] * fy + b[7] * fz,
+                      b[2] * fx + b[5] * fy + b[8] * fz);
+}
+__device__ __forceinline__ void store_packed(double4 *pos4, float4 *p4f, int i, double3 r, double w) {
+  pos4[i] = make_double4(r.x, r.y, r.z, w);
+  if (p4f) p4f[i] = make_float4((float)r.x, (float)r.y, (float)r.z, (float)w);
+}
+
+
 // fine-grid cell of each atom (fractional, folded into [0,1))
 __global__ void k_fast_cells(const double *x0, const double *y0, const double *z0, int n, int gx,
                              int gy, int gz, int *cell) {
@@ -561,8 +606,8 @@ __device__ void check_cell_width(const double *b, int gx, int gy, int gz, double
 // from global memory (any grid; rounding images in dimensions with < 3 cells).
 __device__ void list_one_atom(int i, const double4 *__restrict__ pos4, const double *atoms, const int *img,
                               const int *cell, const int *cstart, int n, int gx, int gy, int gz,
-                              const double (*S)[3], double rl2, int pbc, int maxn, int *nbr, int *numn,
-                              float *sref, int *flags) {
+                              const double (*S)[3], double rl2, int pbc, bool rel, int maxn, int *nbr,
+                              int *numn, float *sref, int *flags) {
   const unsigned c = (unsigned)cell[i];
   const int cx = (int)compact3(c), cy = (int)compact3(c >> 1), cz = (int)compact3(c >> 2);
   const int sxr = gx >= 3 ? 1 : 0, syr = gy >= 3 ? 1 : 0, szr = gz >= 3 ? 1 : 0;
@@ -589,7 +634,8 @@ __device__ void list_one_atom(int i, const double4 *__restrict__ pos4, const dou
           if (!(pbc & 1)) tx = 0;
           if (!(pbc & 2)) ty = 0;
           if (!(pbc & 4)) tz = 0;
-          const int code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+          // cell-relative packing: the cell offset (>= 3 cells per dimension)
+          const int code = rel ? (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1) : (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
           const double4 pj = pos4[j];
           const double rx = pi.x - pj.x + S[code][0];
           const double ry = pi.y - pj.y + S[code][1];
@@ -625,47 +671,51 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
-  shift_table(b, S);
+  const PackGeo absolute = {0, 0, 0, 0.0, 0.0, 0.0};
+  shift_table(b, absolute, S);
   __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) check_cell_width(b, gx, gy, gz, rl2, flags);
   int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, pbc, maxn, nbr, numn, sref, flags);
+  list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, pbc, false, maxn, nbr, numn, sref,
+                flags);
 }
 
 // Full neighbour list by 2x2x2-cell blocks (8 consecutive Morton keys, so one
-// contiguous range of atoms), one wave per block.  The wave stages the atoms of
-// the 4x4x4 cells around the block in LDS -- positions relative to the block's
-// first atom, shifted by the periodic image through which the block sees them,
-// in FP32 (the 1e-6 A rounding is immaterial against the skin), with their
-// list entries (index | image code).  Every lane then tests the same staged
+// contiguous range of atoms), one wave per block, for cell-relative packing.
+// The wave stages the atoms of the 4x4x4 cells around the block in LDS --
+// positions relative to the centre of the block's first cell, in FP32 (the
+// 1e-6 A rounding is immaterial against the skin), with their index and staged
+// cell in the fourth component.  Every lane then tests the same staged
 // candidates at the same time (broadcast LDS reads, four per iteration) instead
-// of gathering them from global memory.  Fully periodic boxes with >= 4 cells
+// of gathering them from global memory; a partner's code is the offset of its
+// staged cell from the lane atom's cell.  Fully periodic boxes with >= 4 cells
 // per dimension (no staged cell occurs twice); a block whose neighbourhood
 // exceeds LIST_CAP atoms uses the global-gather loop.  blocks[] lists the
 // Morton key / 8 of every block that holds grid cells.
 #define LIST_CAP 512
+#define LIST_IDX26 0x03FFFFFF  // staged entry: atom index | staged cell << 26
 __global__ void __launch_bounds__(64) k_full_list_blk(
     const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
-    const int *__restrict__ cstart, const int *blocks, int n, int gx, int gy, int gz, const BoxState *bs,
+    const int *__restrict__ cstart, const int *blocks, int n, PackGeo pg, const BoxState *bs,
     double rl2, int maxn, int *nbr, int *numn, float *sref, int *flags) {
   __shared__ double S[32][3];
   __shared__ float4 cpos[LIST_CAP + 3];  // +3: the unrolled loop reads up to 3 slots past the end
-  __shared__ int cent[LIST_CAP + 3];
-  const int lane = threadIdx.x;
-  shift_table(bs->b0, S);
-  if (blockIdx.x == 0 && lane == 0) check_cell_width(bs->b0, gx, gy, gz, rl2, flags);
+  const int lane = threadIdx.x, gx = pg.gx, gy = pg.gy, gz = pg.gz;
+  const double *b = bs->b0;
+  shift_table(b, pg, S);
+  if (blockIdx.x == 0 && lane == 0) check_cell_width(b, gx, gy, gz, rl2, flags);
   const unsigned base = (unsigned)blocks[xcd_tile()] << 3;
   const int x0 = 2 * (int)compact3(base >> 3), y0 = 2 * (int)compact3(base >> 4), z0 = 2 * (int)compact3(base >> 5);
   const int a0 = cstart[base], a1 = cstart[base + 8];
   if (a1 <= a0) return;  // uniform: no atoms in this block
   // staged cell h = hx + 4 hy + 16 hz at (x0-1+hx, y0-1+hy, z0-1+hz): one per lane
-  int x = x0 - 1 + (lane & 3), y = y0 - 1 + ((lane >> 2) & 3), z = z0 - 1 + (lane >> 4);
-  int wx = 0, wy = 0, wz = 0;
-  if (x < 0) { x += gx; wx = 1; } else if (x >= gx) { x -= gx; wx = -1; }
-  if (y < 0) { y += gy; wy = 1; } else if (y >= gy) { y -= gy; wy = -1; }
-  if (z < 0) { z += gz; wz = 1; } else if (z >= gz) { z -= gz; wz = -1; }
-  const int key = morton3(x, y, z), code = (wx + 1) + 3 * (wy + 1) + 9 * (wz + 1);
+  const int hx = lane & 3, hy = (lane >> 2) & 3, hz = lane >> 4;
+  int x = x0 - 1 + hx, y = y0 - 1 + hy, z = z0 - 1 + hz;
+  if (x < 0) x += gx; else if (x >= gx) x -= gx;
+  if (y < 0) y += gy; else if (y >= gy) y -= gy;
+  if (z < 0) z += gz; else if (z >= gz) z -= gz;
+  const int key = morton3(x, y, z);
   const int cs = cstart[key], cnt = cstart[key + 1] - cs;
   // staged in ascending cell key, i.e. ascending atom index (atoms are sorted
   // by cell key): every list comes out sorted by partner index, so the lanes
@@ -680,40 +730,51 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
   __syncthreads();  // shift table
   if (total > LIST_CAP) {
     for (int i = a0 + lane; i < a1; i += 64)
-      list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, 7, maxn, nbr, numn, sref, flags);
+      list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, 7, true, maxn, nbr, numn, sref,
+                    flags);
     return;
   }
-  const double4 ref = pos4[a0];
+  // centre of the staged cell from the centre of cell (x0,y0,z0): B0 (h - 1) / g
+  const double fx = (hx - 1) * pg.ix, fy = (hy - 1) * pg.iy, fz = (hz - 1) * pg.iz;
+  const double vx = b[0] * fx + b[3] * fy + b[6] * fz, vy = b[1] * fx + b[4] * fy + b[7] * fz,
+               vz = b[2] * fx + b[5] * fy + b[8] * fz;
   for (int k = 0; k < cnt; k++) {
     const double4 p = pos4[cs + k];
-    cpos[off + k] = make_float4((float)(p.x - S[code][0] - ref.x), (float)(p.y - S[code][1] - ref.y),
-                                (float)(p.z - S[code][2] - ref.z), 0.f);
-    cent[off + k] = (cs + k) | (code << NBR_CODE_SHIFT);
+    cpos[off + k] = make_float4((float)(p.x + vx), (float)(p.y + vy), (float)(p.z + vz),
+                                __int_as_float((cs + k) | (lane << 26)));
   }
-  if (lane < 3) {  // far-away fillers for the unrolled tail (never within the list radius)
-    cpos[total + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, 0.f);
-    cent[total + lane] = NBR_NULL_CODE << NBR_CODE_SHIFT;
-  }
+  if (lane < 3)  // far-away fillers for the unrolled tail (never within the list radius)
+    cpos[total + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, __int_as_float(0));
   __syncthreads();
   const float rl2f = (float)rl2;
   for (int i = a0 + lane; i < a1; i += 64) {
+    const unsigned ck = (unsigned)cell[i];
+    const int ax = (int)compact3(ck) - x0, ay = (int)compact3(ck >> 1) - y0, az = (int)compact3(ck >> 2) - z0;
     const double4 pd = pos4[i];
-    const float px = (float)(pd.x - ref.x), py = (float)(pd.y - ref.y), pz = (float)(pd.z - ref.z);
+    const double ex = ax * pg.ix, ey = ay * pg.iy, ez = az * pg.iz;
+    const float px = (float)(pd.x + (b[0] * ex + b[3] * ey + b[6] * ez)),
+                py = (float)(pd.y + (b[1] * ex + b[4] * ey + b[7] * ez)),
+                pz = (float)(pd.z + (b[2] * ex + b[5] * ey + b[8] * ez));
+    // code of staged cell h for this atom: (hx-ax) + 3 (hy-ay) + 9 (hz-az) when
+    // every component is in {0,1,2} (adjacent cells; farther ones are beyond
+    // the list radius, whatever the FP32 rounding says)
     int m = 0;
     for (int c = 0; c < total; c += 4) {
       float4 q[4];
-      int e[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        q[u] = cpos[c + u];
-        e[u] = cent[c + u];
-      }
+      for (int u = 0; u < 4; u++) q[u] = cpos[c + u];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const float rx = px - q[u].x, ry = py - q[u].y, rz = pz - q[u].z;
-        if (rx * rx + ry * ry + rz * rz < rl2f && (e[u] & NBR_IDX_MASK) != i) {
-          if (m < maxn) nbr[nbr_at(m, i, n)] = e[u];
-          m++;
+        const int e = __float_as_int(q[u].w), j = e & LIST_IDX26;
+        if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
+          const int h = (int)((unsigned)e >> 26);
+          const unsigned ox = (unsigned)((h & 3) - ax), oy = (unsigned)(((h >> 2) & 3) - ay),
+                         oz = (unsigned)((h >> 4) - az);
+          if (ox <= 2u && oy <= 2u && oz <= 2u) {
+            if (m < maxn) nbr[nbr_at(m, i, n)] = j | ((int)(ox + 3 * oy + 9 * oz) << NBR_CODE_SHIFT);
+            m++;
+          }
         }
       }
     }
@@ -756,8 +817,8 @@ __device__ void drift_box(const BoxState *bin, BoxState *bout, int ivol) {
 // velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2 (also into the
 // packed copy); block 0 copies the box state into the other buffer, advancing
 // b1 and b0 there (one field per thread).
-__global__ void k_drift(double *atoms, double4 *pos4, const int *img, int n, const BoxState *bin,
-                        BoxState *bout, int ivol) {
+__global__ void k_drift(double *atoms, double4 *pos4, float4 *p4f, const int *img, const int *cell,
+                        PackGeo pg, int n, const BoxState *bin, BoxState *bout, int ivol) {
 #pragma clang fp contract(off)
   int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -776,9 +837,8 @@ __global__ void k_drift(double *atoms, double4 *pos4, const int *img, int n, con
       const bool upd = ivol < 1 || (ivol == 4 && k == 8);
       b[k] = upd ? bin->b0[k] + (bin->b1[k] + bin->b2[k]) : bin->b0[k];
     }
-    const double sx = xn - img[3 * i], sy = yn - img[3 * i + 1], sz = zn - img[3 * i + 2];
-    pos4[i] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
-                           b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
+    const double3 r = packed_pos(b, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
+    store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
   }
   if (blockIdx.x == 0) drift_box(bin, bout, ivol);
 }
@@ -842,29 +902,41 @@ __device__ __forceinline__ T pair_sep(const double (*S)[3], const double4 &pi, c
 }
 
 
-// Cartesian image-fixed positions R' = B0 (s - img) and the current F', packed
-// for one 32-byte gather per neighbour.  with_img recomputes img = floor(s).
+// Packed positions R (absolute or cell-relative, see the entry format) and the
+// current F', for one 32-byte gather per neighbour (16 bytes for the FP32
+// copy p4f, when given).  with_img recomputes img = floor(s).
 __global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n, int with_img,
-                       double4 *pos4) {
+                       double4 *pos4, float4 *p4f, const int *cell, PackGeo pg) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double *b = bs->b0;
   const double x = atoms[i], y = atoms[(size_t)n + i], z = atoms[2 * (size_t)n + i];
   if (with_img) {
     img[3 * i] = (int)floor(x);
     img[3 * i + 1] = (int)floor(y);
     img[3 * i + 2] = (int)floor(z);
   }
-  const double sx = x - img[3 * i], sy = y - img[3 * i + 1], sz = z - img[3 * i + 2];
-  pos4[i] = make_double4(b[0] * sx + b[3] * sy + b[6] * sz, b[1] * sx + b[4] * sy + b[7] * sz,
-                         b[2] * sx + b[5] * sy + b[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
+  const double3 r = packed_pos(bs->b0, pg, cell, i, x - img[3 * i], y - img[3 * i + 1], z - img[3 * i + 2]);
+  store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
+}
+
+// Neighbour j's packed position and F': the FP64 record, or (P32) the FP32
+// copy of a cell-relative packing, widened exactly.
+template <bool P32>
+__device__ __forceinline__ double4 partner(const double4 *__restrict__ pos4, const float4 *__restrict__ p4f,
+                                           unsigned j) {
+  if constexpr (P32) {
+    const float4 q = p4f[j];
+    return make_double4(q.x, q.y, q.z, q.w);
+  } else {
+    return pos4[j];
+  }
 }
 
 // Per-atom end of the density pass: rho, and with do_embed the embedding
 // energy F(rho), F'(rho) (also into the packed positions for the force pass)
 // and the block's embedding-energy partial.
-__device__ __forceinline__ void density_finish(double *atoms, double4 *pos4, int n, int i, double acc,
-                                               const PotDev &P, int do_embed, double *part) {
+__device__ __forceinline__ void density_finish(double *atoms, double4 *pos4, float4 *p4f, int n, int i,
+                                               double acc, const PotDev &P, int do_embed, double *part) {
   double pe[1] = {0.0};
   if (i < n) {
     double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
@@ -880,6 +952,7 @@ __device__ __forceinline__ void density_finish(double *atoms, double4 *pos4, int
         pe[0] = f;
       }
       pos4[i].w = dfi;
+      if (p4f) p4f[i].w = (float)dfi;
     }
   }
   if (do_embed) block_store<1, 0>(pe, part);
@@ -901,14 +974,14 @@ __device__ __forceinline__ void quad_put(int *nbr, int i, int n, int e, int &m, 
 // (src/potential_m.F90:148-169): rho_i = sum_j phi(r_ij) over the full list
 // one lane per atom; F, F' per atom; per-block embedding energy.  F' is
 // also stored in pos4[i].w for the force pass.
-template <typename T, int POTM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, int n,
+template <typename T, int POTM, bool P32>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *pos4, float4 *p4f, int n,
                                                        const int *__restrict__ nbr,
                                                        const int *__restrict__ numn,
-                                                       const BoxState *bs, PotDev P,
+                                                       const BoxState *bs, PackGeo pg, PotDev P,
                                                        int do_embed, double *part, int *nact) {
   __shared__ double S[32][3];
-  shift_table(bs->b0, S);
+  shift_table(bs->b0, pg, S);
   __syncthreads();
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double acc = 0.0;  // per-atom density accumulated in FP64 in both forms
@@ -927,7 +1000,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       double4 pj[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
+      for (int u = 0; u < 4; u++) pj[u] = partner<P32>(pos4, p4f, e[u] & NBR_IDX_MASK);
       T mn = rc2;  // one min chain per quad keeps the register footprint of the loop
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -942,7 +1015,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
     }
     nact[i] = na;
   }
-  density_finish(atoms, pos4, n, i, acc, P, do_embed, part);
+  density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
 // Density pass of the first step on a new list: as k_density, and it also
@@ -953,14 +1026,14 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density(double *atoms, double4 *p
 // stay in place, which costs only efficiency).  Each completed quad is written
 // after its slots have been read (an atom's list belongs to its lane alone).
 #define DENS_OBUF 16
-template <typename T, int POTM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, double4 *pos4, int n, int *nbr,
-                                                            const int *__restrict__ numn,
-                                                            const BoxState *bs, PotDev P, int do_embed,
-                                                            double *part, int *nact) {
+template <typename T, int POTM, bool P32>
+__global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, double4 *pos4, float4 *p4f, int n,
+                                                            int *nbr, const int *__restrict__ numn,
+                                                            const BoxState *bs, PackGeo pg, PotDev P,
+                                                            int do_embed, double *part, int *nact) {
   __shared__ double S[32][3];
   __shared__ int obuf[DENS_OBUF][MDK_BLOCK];
-  shift_table(bs->b0, S);
+  shift_table(bs->b0, pg, S);
   __syncthreads();
   const int t = threadIdx.x, i = xcd_tile() * blockDim.x + t;
   double acc = 0.0;
@@ -975,7 +1048,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
       const int e[4] = {e4.x, e4.y, e4.z, e4.w};
       double4 pj[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) pj[u] = pos4[(unsigned)e[u] & NBR_IDX_MASK];
+      for (int u = 0; u < 4; u++) pj[u] = partner<P32>(pos4, p4f, (unsigned)e[u] & NBR_IDX_MASK);
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         T rx, ry, rz;
@@ -1004,7 +1077,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
     }
     nact[i] = (last + 3) & ~3;
   }
-  density_finish(atoms, pos4, n, i, acc, P, do_embed, part);
+  density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
 // Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
@@ -1013,20 +1086,18 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
 // W B0^-T in the finishing block, i.e. sum r_a s_b fp) and the pair energy as
 // 1/2 sum V.  One lane per atom; MASSCHK keeps the
 // reference's exclusion of pairs of immobile atoms from the virial.
-template <typename T, bool MASSCHK, int POTM>
-__global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double4 *__restrict__ pos4,
-                                                     const int *zat, int n,
-                                                     const int *__restrict__ nbr,
-                                                     const int *__restrict__ nvisit,
-                                                     const BoxState *bs, PotDev P,
-                                                     double *part) {
+template <typename T, bool MASSCHK, int POTM, bool P32>
+__global__ void __launch_bounds__(MDK_BLOCK)
+k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restrict__ p4f, const int *zat,
+        int n, const int *__restrict__ nbr, const int *__restrict__ nvisit, const BoxState *bs, PackGeo pg,
+        PotDev P, double *part) {
   __shared__ double S[32][3];
   __shared__ float vtab[8][8];  // interval cubics of the pair function (FP32 Fe-Fe form)
   constexpr bool CUBIC = sizeof(T) == 4 && POTM == POT_FE;
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
-  shift_table(b, S);
+  shift_table(b, pg, S);
   if (CUBIC && threadIdx.x < 64) vtab[threadIdx.x >> 3][threadIdx.x & 7] = P.vcub[threadIdx.x >> 3][threadIdx.x & 7];
   __syncthreads();
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
@@ -1047,7 +1118,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force(double *atoms, const double
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       double4 pj[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) pj[u] = pos4[e[u] & NBR_IDX_MASK];
+      for (int u = 0; u < 4; u++) pj[u] = partner<P32>(pos4, p4f, e[u] & NBR_IDX_MASK);
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const unsigned eu = e[u];
@@ -1305,7 +1376,8 @@ template <bool DRIFT>
 __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *zat, int n,
                                                     BoxState *bs, int ivol, int mode,
                                                     const float *sref, double *part, double4 *pos4,
-                                                    const int *img, BoxState *bout) {
+                                                    float4 *p4f, const int *img, const int *cell,
+                                                    PackGeo pg, BoxState *bout) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1389,9 +1461,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
         const bool upd = ivol < 1 || (ivol == 4 && k == 8);
         bn[k] = upd ? bs->b0[k] + (bs->b1[k] + bs->b2[k]) : bs->b0[k];
       }
-      const double sx = xn - img[3 * i], sy = yn - img[3 * i + 1], sz = zn - img[3 * i + 2];
-      pos4[i] = make_double4(bn[0] * sx + bn[3] * sy + bn[6] * sz, bn[1] * sx + bn[4] * sy + bn[7] * sz,
-                             bn[2] * sx + bn[5] * sy + bn[8] * sz, atoms[(size_t)F_DAFRHO * n + i]);
+      const double3 r = packed_pos(bn, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
+      store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
     }
   }
   if (DRIFT && blockIdx.x == 0) drift_box(bs, bout, ivol);
@@ -1640,23 +1711,26 @@ static int rebuild_full(mdk_ctx *c) {
   std::swap(c->atoms, c->atoms_alt);
   std::swap(c->zat, c->zat_alt);
   std::swap(c->orig, c->orig_alt);
-  // images fixed for this list's lifetime; Cartesian packed positions
-  k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4);
-  c->pos4_valid = true;
   // reference cells move with the atoms
   k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
   std::swap(c->ic, c->zat_alt);  // zat_alt is free scratch after the swap above
   k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
                                                        n, g[0], g[1], g[2], c->cell_of);
   const double rl2 = c->r_list * c->r_list;
-  // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per dimension
-  const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4;
+  // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per
+  // dimension (and 26-bit atom indices); they pack positions cell-relative
+  const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4 && n < (1 << 26);
+  c->rel = blk;
+  // images fixed for this list's lifetime; packed positions
+  k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4, p4f_of(c),
+                                                 c->cell_of, pack_geo(c));
+  c->pos4_valid = true;
   if (blk) RCK(plan_list_blocks(c, g));
   for (int attempt = 0; attempt < 4; attempt++) {
     HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
     if (blk) {
       k_full_list_blk<<<c->nlblocks, 64, 0, c->stream>>>(
-          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, c->lblocks, n, g[0], g[1], g[2], c->box + c->cur,
+          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, c->lblocks, n, pack_geo(c), c->box + c->cur,
           rl2, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
       RCK(check_launch("k_full_list_blk"));
     } else {
@@ -1695,24 +1769,43 @@ static int rebuild_full(mdk_ctx *c) {
   return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (max %d)", c->maxn);
 }
 
-template <typename T, int POTM>
-static void density_dispatch(mdk_ctx *c, int do_embed) {
+// The FP32 form gathers the 16-byte FP32 copy of cell-relative packed
+// positions (P32); absolute packing and the FP64 form gather the FP64 record.
+template <typename T, int POTM, bool P32>
+static void density_dispatch_p(mdk_ctx *c, int do_embed) {
   // embedding-energy partials are value NPART_FORCE of the force pass's array
   double *pe = c->part_f + (size_t)NPART_FORCE * nblocks(c->n);
   if (c->part_due)
-    k_density_part<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, pe, c->nact);
+    k_density_part<T, POTM, P32><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->pos4, p4f_of(c), c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
+        c->nact);
   else
-    k_density<T, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->pos4, c->n, c->nbr, c->numn, c->box + c->cur, c->pot, do_embed, pe, c->nact);
+    k_density<T, POTM, P32><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->pos4, p4f_of(c), c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
+        c->nact);
   c->part_due = false;
+}
+template <typename T, int POTM>
+static void density_dispatch(mdk_ctx *c, int do_embed) {
+  if constexpr (sizeof(T) == 4) {
+    if (c->rel) return density_dispatch_p<T, POTM, true>(c, do_embed);
+  }
+  density_dispatch_p<T, POTM, false>(c, do_embed);
+}
+template <typename T, bool MC, int POTM, bool P32>
+static void force_dispatch_p(mdk_ctx *c) {
+  // tabulated pair terms interpolate across the range edge: those use the whole list
+  k_force<T, MC, POTM, P32><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
+      c->atoms, c->pos4, p4f_of(c), c->zat, c->n, c->nbr, POTM == POT_TABLE ? c->numn : c->nact, c->box + c->cur,
+      pack_geo(c), c->pot, c->part_f);
 }
 template <typename T, bool MC, int POTM>
 static void force_dispatch(mdk_ctx *c, int fin_mode) {
-  // tabulated pair terms interpolate across the range edge: those use the whole list
-  k_force<T, MC, POTM><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, c->zat, c->n, c->nbr, POTM == POT_TABLE ? c->numn : c->nact, c->box + c->cur, c->pot,
-      c->part_f);
+  (void)fin_mode;
+  if constexpr (sizeof(T) == 4) {
+    if (c->rel) return force_dispatch_p<T, MC, POTM, true>(c);
+  }
+  force_dispatch_p<T, MC, POTM, false>(c);
 }
 // lookup tables, the Fe-Fe knot counts, or any knot count
 static int pot_mode(const mdk_ctx *c) {
@@ -1738,7 +1831,8 @@ static void force_pot(mdk_ctx *c, int fin_mode) {
 
 static int ensure_pos4(mdk_ctx *c) {
   if (c->pos4_valid) return MDK_OK;
-  k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, c->n, 0, c->pos4);
+  k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, c->n, 0, c->pos4,
+                                                     p4f_of(c), c->cell_of, pack_geo(c));
   c->pos4_valid = true;
   return check_launch("k_pack");
 }
@@ -1795,12 +1889,12 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   const int nxt = 1 - c->cur;
   if (fuse_drift && mode == 0)
     k_kick<true><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, c->pos4, c->img,
-        c->box + nxt);
+        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, c->pos4, p4f_of(c),
+        c->img, c->cell_of, pack_geo(c), c->box + nxt);
   else
     k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, nullptr, nullptr,
-        nullptr);
+        nullptr, nullptr, pack_geo(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
   if (mode != 2) {
@@ -1825,8 +1919,8 @@ static int launch_drift(mdk_ctx *c) {
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
-  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->img, c->n, c->box + c->cur,
-                                                c->box + nxt, c->p.ivol);
+  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c),
+                                                c->n, c->box + c->cur, c->box + nxt, c->p.ivol);
   c->pos4_valid = true;
   tend(c, te);
   c->cur = nxt;
@@ -1953,6 +2047,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->dflags, sizeof(int) * 4);
   ALLOC(c->box, sizeof(BoxState) * 2);
   ALLOC(c->pos4, sizeof(double4) * (size_t)n);
+  ALLOC(c->p4f, sizeof(float4) * (size_t)n);
   ALLOC(c->img, sizeof(int) * 3 * (size_t)n);
   ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * (NPART_FORCE + 1));
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
@@ -1997,7 +2092,7 @@ extern "C" int mdk_destroy(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return MDK_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->pos4, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
+  void *ptrs[] = {c->pos4, c->p4f, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks};
@@ -2387,7 +2482,8 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
   TimedEv te;
   tbegin(c, TC_KICK, te);
   k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr);
+      c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr,
+      nullptr, pack_geo(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
   k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
