@@ -182,8 +182,7 @@ template <int K, int KM, int BLOCK = MDK_BLOCK>
 __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
   __shared__ double sh[BLOCK / 64][K];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
+  wave_reduce_all<K, KM>(v);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) sh[w][k] = v[k];
@@ -195,37 +194,6 @@ __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
     for (int q = 1; q < BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
     part[(size_t)k * gridDim.x + blockIdx.x] = s;
   }
-}
-
-// Fixed-order reduction of nb block partials (value-major) inside one block.
-template <int K, int KM, int BLOCK = MDK_BLOCK>
-__device__ __forceinline__ void block_final(const double *part, int nb, double (&out)[K]) {
-  __shared__ double sh[BLOCK / 64][K];
-  double v[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      double x = part[(size_t)k * nb + b];
-      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
-    }
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < K; k++) sh[w][k] = v[k];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    double s = sh[0][k];
-    for (int q = 1; q < (int)(blockDim.x / 64); q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
-    out[k] = s;
-  }
-  __syncthreads();
 }
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
@@ -838,7 +806,7 @@ __global__ void k_drift(double *atoms, double4 *pos4, float4 *p4f, const int *im
       b[k] = upd ? bin->b0[k] + (bin->b1[k] + bin->b2[k]) : bin->b0[k];
     }
     const double3 r = packed_pos(b, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
-    store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
+    store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
   }
   if (blockIdx.x == 0) drift_box(bin, bout, ivol);
 }
@@ -1097,8 +1065,10 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  __shared__ double bi[9];  // B0^-1 for the fractional forces
   shift_table(b, pg, S);
   if (CUBIC && threadIdx.x < 64) vtab[threadIdx.x >> 3][threadIdx.x & 7] = P.vcub[threadIdx.x >> 3][threadIdx.x & 7];
+  if (threadIdx.x == 64) inv3(b, bi);
   __syncthreads();
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   const double *mass = atoms + (size_t)F_MASS * n;
@@ -1151,8 +1121,6 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
   }
   if (i < n) {
     // fractional force f = B0^-1 F (the reference's fx = sum s fp, Cartesian = B0 f)
-    double bi[9];
-    inv3(b, bi);
     atoms[(size_t)F_FX * n + i] = bi[0] * fxi + bi[3] * fyi + bi[6] * fzi;
     atoms[(size_t)F_FY * n + i] = bi[1] * fxi + bi[4] * fyi + bi[7] * fzi;
     atoms[(size_t)F_FZ * n + i] = bi[2] * fxi + bi[5] * fyi + bi[8] * fzi;
@@ -1224,8 +1192,7 @@ __device__ __forceinline__ void fold_columns(const double *part, int nb, int b0,
       v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
     }
   }
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? wave_sum(v[k]) : wave_max(v[k]);
+  wave_reduce_all<K, KM>(v);
 }
 
 template <int K, int KM>
@@ -1462,7 +1429,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
         bn[k] = upd ? bs->b0[k] + (bs->b1[k] + bs->b2[k]) : bs->b0[k];
       }
       const double3 r = packed_pos(bn, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
-      store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
+      store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
     }
   }
   if (DRIFT && blockIdx.x == 0) drift_box(bs, bout, ivol);
