@@ -361,9 +361,32 @@ __global__ void k_permute_int(int *zdst, const int *zsrc, int *odst, const int *
 // ---------------------------------------------------------------------------
 // cells: counting sort helpers
 // ---------------------------------------------------------------------------
+// Runs of equal keys within a wave (atoms arrive nearly sorted by cell): every
+// lane learns its run's head lane and length, and only heads touch the
+// counters, once per run.  Lanes past n form their own runs and are skipped.
+struct KeyRun {
+  int key, head, len;
+  bool is_head;
+};
+__device__ __forceinline__ KeyRun key_run(const int *key, int n, int i) {
+  const int lane = threadIdx.x & 63;
+  const int k = i < n ? key[i] : -1;
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = lane == 0 || prev != k;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long above = lane == 63 ? 0ull : heads >> (lane + 1);
+  const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
+  KeyRun r;
+  r.key = k;
+  r.is_head = head;
+  r.len = above ? __builtin_ctzll(above) + 1 : 64 - lane;
+  r.head = 63 - __builtin_clzll(upto);
+  return r;
+}
 __global__ void k_count(const int *key, int n, int *count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&count[key[i]], 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const KeyRun r = key_run(key, n, i);
+  if (r.is_head && r.key >= 0) atomicAdd(&count[r.key], r.len);
 }
 // exclusive scan of count[0..m) into start[0..m] (start[m] = total) in three
 // passes: per-block scans of SCAN_CHUNK elements, a scan of the block totals,
@@ -412,11 +435,12 @@ __global__ void k_scan_c(int *start, const int *btot, int m) {
   if (k < m) start[k] += btot[k / SCAN_CHUNK];
 }
 __global__ void k_fill(const int *key, int n, const int *start, int *fill, int *members) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int c = key[i];
-  int pos = start[c] + atomicAdd(&fill[c], 1);
-  members[pos] = i;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const KeyRun r = key_run(key, n, i);
+  int base = 0;
+  if (r.is_head && r.key >= 0) base = start[r.key] + atomicAdd(&fill[r.key], r.len);
+  base = __shfl(base, r.head, 64);  // the run's slots follow its head's reservation
+  if (i < n) members[base + ((int)(threadIdx.x & 63) - r.head)] = i;
 }
 // deterministic order inside each cell: ascending sort key (key[members] or members)
 __global__ void k_sort_cells(const int *start, int m, int *members, const int *key) {
@@ -872,16 +896,22 @@ __device__ __forceinline__ T pair_sep(const double (*S)[3], const double4 &pi, c
 
 // Packed positions R (absolute or cell-relative, see the entry format) and the
 // current F', for one 32-byte gather per neighbour (16 bytes for the FP32
-// copy p4f, when given).  with_img recomputes img = floor(s).
+// copy p4f, when given).  At a rebuild (with_img) it also fixes img = floor(s)
+// and the list cell of each atom on the (gx, gy, gz) grid, as k_fast_cells.
 __global__ void k_pack(const double *atoms, int *img, const BoxState *bs, int n, int with_img,
-                       double4 *pos4, float4 *p4f, const int *cell, PackGeo pg) {
+                       double4 *pos4, float4 *p4f, int *cell, int gx, int gy, int gz, PackGeo pg) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x = atoms[i], y = atoms[(size_t)n + i], z = atoms[2 * (size_t)n + i];
   if (with_img) {
-    img[3 * i] = (int)floor(x);
-    img[3 * i + 1] = (int)floor(y);
-    img[3 * i + 2] = (int)floor(z);
+    const double fx = floor(x), fy = floor(y), fz = floor(z);
+    img[3 * i] = (int)fx;
+    img[3 * i + 1] = (int)fy;
+    img[3 * i + 2] = (int)fz;
+    const int cx = min(max((int)((x - fx) * gx), 0), gx - 1);
+    const int cy = min(max((int)((y - fy) * gy), 0), gy - 1);
+    const int cz = min(max((int)((z - fz) * gz), 0), gz - 1);
+    cell[i] = morton3(cx, cy, cz);
   }
   const double3 r = packed_pos(bs->b0, pg, cell, i, x - img[3 * i], y - img[3 * i + 1], z - img[3 * i + 2]);
   store_packed(pos4, p4f, i, r, atoms[(size_t)F_DAFRHO * n + i]);
@@ -1681,16 +1711,14 @@ static int rebuild_full(mdk_ctx *c) {
   // reference cells move with the atoms
   k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
   std::swap(c->ic, c->zat_alt);  // zat_alt is free scratch after the swap above
-  k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
-                                                       n, g[0], g[1], g[2], c->cell_of);
   const double rl2 = c->r_list * c->r_list;
   // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per
   // dimension (and 26-bit atom indices); they pack positions cell-relative
   const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4 && n < (1 << 26);
   c->rel = blk;
-  // images fixed for this list's lifetime; packed positions
+  // images and list cells fixed for this list's lifetime; packed positions
   k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4, p4f_of(c),
-                                                 c->cell_of, pack_geo(c));
+                                                 c->cell_of, g[0], g[1], g[2], pack_geo(c));
   c->pos4_valid = true;
   if (blk) RCK(plan_list_blocks(c, g));
   for (int attempt = 0; attempt < 4; attempt++) {
@@ -1799,7 +1827,7 @@ static void force_pot(mdk_ctx *c, int fin_mode) {
 static int ensure_pos4(mdk_ctx *c) {
   if (c->pos4_valid) return MDK_OK;
   k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, c->n, 0, c->pos4,
-                                                     p4f_of(c), c->cell_of, pack_geo(c));
+                                                     p4f_of(c), c->cell_of, 0, 0, 0, pack_geo(c));
   c->pos4_valid = true;
   return check_launch("k_pack");
 }
