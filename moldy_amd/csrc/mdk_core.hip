@@ -677,22 +677,22 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
 // contiguous range of atoms), one wave per block, for cell-relative packing.
 // The wave stages the atoms of the 4x4x4 cells around the block in LDS --
 // positions relative to the centre of the block's first cell, in FP32 (the
-// 1e-6 A rounding is immaterial against the skin), with their index and staged
-// cell in the fourth component.  Every lane then tests the same staged
-// candidates at the same time (broadcast LDS reads, four per iteration) instead
-// of gathering them from global memory; a partner's code is the offset of its
-// staged cell from the lane atom's cell.  Fully periodic boxes with >= 4 cells
-// per dimension (no staged cell occurs twice); a block whose neighbourhood
-// exceeds LIST_CAP atoms uses the global-gather loop.  blocks[] lists the
-// Morton key / 8 of every block that holds grid cells.
+// 1e-6 A rounding is immaterial against the skin) -- instead of gathering them
+// from global memory.  Each lane then walks the 27 staged cells around its own
+// atom's cell (about 155 candidates instead of all 368 staged ones); a
+// partner's code is the offset of its cell, uniform over the walk of one cell.
+// Fully periodic boxes with >= 4 cells per dimension (no staged cell occurs
+// twice); a block whose neighbourhood exceeds LIST_CAP atoms uses the
+// global-gather loop.  blocks[] lists the Morton key / 8 of every block that
+// holds grid cells.
 #define LIST_CAP 512
-#define LIST_IDX26 0x03FFFFFF  // staged entry: atom index | staged cell << 26
 __global__ void __launch_bounds__(64) k_full_list_blk(
     const double4 *__restrict__ pos4, const double *atoms, const int *img, const int *cell,
     const int *__restrict__ cstart, const int *blocks, int n, PackGeo pg, const BoxState *bs,
     double rl2, int maxn, int *nbr, int *numn, float *sref, int *flags) {
   __shared__ double S[32][3];
-  __shared__ float4 cpos[LIST_CAP + 3];  // +3: the unrolled loop reads up to 3 slots past the end
+  __shared__ float4 cpos[LIST_CAP];
+  __shared__ int hoff[64], hcnt[64];  // staged cell h: candidates [hoff, hoff + hcnt)
   const int lane = threadIdx.x, gx = pg.gx, gy = pg.gy, gz = pg.gz;
   const double *b = bs->b0;
   shift_table(b, pg, S);
@@ -709,17 +709,17 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
   if (z < 0) z += gz; else if (z >= gz) z -= gz;
   const int key = morton3(x, y, z);
   const int cs = cstart[key], cnt = cstart[key + 1] - cs;
-  // staged in ascending cell key, i.e. ascending atom index (atoms are sorted
-  // by cell key): every list comes out sorted by partner index, so the lanes
-  // of a wave gather nearby partners at the same list slot.  The 64 cells are
-  // distinct (>= 4 cells per dimension); off = atoms in cells of smaller key.
-  int off = 0, total = 0;
-  for (int l = 0; l < 64; l++) {
-    const int kl = __shfl(key, l, 64), cl = __shfl(cnt, l, 64);
-    off += kl < key ? cl : 0;
-    total += cl;
+  int off = cnt;  // exclusive prefix sum over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(off, o, 64);
+    if (lane >= o) off += u;
   }
-  __syncthreads();  // shift table
+  const int total = __shfl(off, 63, 64);
+  off -= cnt;
+  hoff[lane] = off;
+  hcnt[lane] = cnt;
+  __syncthreads();  // shift table, cell ranges
   if (total > LIST_CAP) {
     for (int i = a0 + lane; i < a1; i += 64)
       list_one_atom(i, pos4, atoms, img, cell, cstart, n, gx, gy, gz, S, rl2, 7, true, maxn, nbr, numn, sref,
@@ -732,11 +732,8 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
                vz = b[2] * fx + b[5] * fy + b[8] * fz;
   for (int k = 0; k < cnt; k++) {
     const double4 p = pos4[cs + k];
-    cpos[off + k] = make_float4((float)(p.x + vx), (float)(p.y + vy), (float)(p.z + vz),
-                                __int_as_float((cs + k) | (lane << 26)));
+    cpos[off + k] = make_float4((float)(p.x + vx), (float)(p.y + vy), (float)(p.z + vz), __int_as_float(cs + k));
   }
-  if (lane < 3)  // far-away fillers for the unrolled tail (never within the list radius)
-    cpos[total + lane] = make_float4(1.0e6f, 1.0e6f, 1.0e6f, __int_as_float(0));
   __syncthreads();
   const float rl2f = (float)rl2;
   for (int i = a0 + lane; i < a1; i += 64) {
@@ -747,29 +744,24 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
     const float px = (float)(pd.x + (b[0] * ex + b[3] * ey + b[6] * ez)),
                 py = (float)(pd.y + (b[1] * ex + b[4] * ey + b[7] * ez)),
                 pz = (float)(pd.z + (b[2] * ex + b[5] * ey + b[8] * ez));
-    // code of staged cell h for this atom: (hx-ax) + 3 (hy-ay) + 9 (hz-az) when
-    // every component is in {0,1,2} (adjacent cells; farther ones are beyond
-    // the list radius, whatever the FP32 rounding says)
+    const int h0 = ax + 4 * ay + 16 * az;  // staged cell at offset (-1,-1,-1) from the atom's
     int m = 0;
-    for (int c = 0; c < total; c += 4) {
-      float4 q[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) q[u] = cpos[c + u];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const float rx = px - q[u].x, ry = py - q[u].y, rz = pz - q[u].z;
-        const int e = __float_as_int(q[u].w), j = e & LIST_IDX26;
-        if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
-          const int h = (int)((unsigned)e >> 26);
-          const unsigned ox = (unsigned)((h & 3) - ax), oy = (unsigned)(((h >> 2) & 3) - ay),
-                         oz = (unsigned)((h >> 4) - az);
-          if (ox <= 2u && oy <= 2u && oz <= 2u) {
-            if (m < maxn) nbr[nbr_at(m, i, n)] = j | ((int)(ox + 3 * oy + 9 * oz) << NBR_CODE_SHIFT);
-            m++;
+    for (int dz = 0; dz < 3; dz++)
+      for (int dy = 0; dy < 3; dy++)
+        for (int dx = 0; dx < 3; dx++) {
+          const int h = h0 + dx + 4 * dy + 16 * dz;
+          const int lo = hoff[h], hi = lo + hcnt[h];
+          const int tag = (dx + 3 * dy + 9 * dz) << NBR_CODE_SHIFT;  // the cell offset's code
+          for (int c = lo; c < hi; c++) {
+            const float4 q = cpos[c];
+            const float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
+            const int j = __float_as_int(q.w);
+            if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
+              if (m < maxn) nbr[nbr_at(m, i, n)] = j | tag;
+              m++;
+            }
           }
         }
-      }
-    }
     numn[i] = min(m, maxn);
     if (m > maxn) atomicMax(&flags[0], m);
     for (int k = m; k < ((m + 3) & ~3) && k < maxn; k++)  // pad the last quad with null entries
