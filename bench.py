@@ -235,7 +235,7 @@ def main():
     achieved = bytes_per_launch / (kf["us_avg"] * 1e-6) / 1e9 if kf["us_avg"] else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": measured_traffic(),
-                "kernel": "k_force<float, false, POT_FE> (FP32 pair arithmetic, Fe-Fe knot counts)", "us_per_launch": kf["us_avg"],
+                "kernel": "k_force<float, false, POT_FE, true> (FP32 pair arithmetic, Fe-Fe knot counts, 16-byte partner records)", "us_per_launch": kf["us_avg"],
                 "bytes_per_launch": bytes_per_launch}
 
     batch = None

@@ -18,6 +18,7 @@
 //  * box/thermo tensors live on the device (BoxState, double-buffered per step)
 //    so an MD step needs no host round trip.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stddef.h>
@@ -1545,9 +1546,24 @@ static void tbegin(mdk_ctx *c, int cls, TimedEv &te) {
   te.b = pooled_event(c);
   hipEventRecord(te.a, c->stream);
 }
+// Events filled in by the kernel dispatch itself (hipExtLaunchKernel): the
+// interval is the kernel's own execution, with no gap before or after it.
+static void tbegin_ext(mdk_ctx *c, int cls, TimedEv &te) {
+  te.cls = -1;
+  if (!((c->timing_mask >> cls) & 1)) return;
+  if (c->timing_every > 1 && (c->t_seen[cls]++ % c->timing_every) != 0) return;
+  te.cls = cls;
+  te.a = pooled_event(c);
+  te.b = pooled_event(c);
+}
+static void tpush(mdk_ctx *c, TimedEv &te);
 static void tend(mdk_ctx *c, TimedEv &te) {
   if (te.cls < 0) return;
   hipEventRecord(te.b, c->stream);
+  tpush(c, te);
+}
+static void tpush(mdk_ctx *c, TimedEv &te) {
+  if (te.cls < 0) return;
   c->evs.push_back(te);
   if (c->evs.size() > 4096) {  // resolve old ones to bound memory
     hipEventSynchronize(te.b);
@@ -1780,19 +1796,19 @@ static void density_dispatch(mdk_ctx *c, int do_embed) {
   density_dispatch_p<T, POTM, false>(c, do_embed);
 }
 template <typename T, bool MC, int POTM, bool P32>
-static void force_dispatch_p(mdk_ctx *c) {
+static void force_dispatch_p(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   // tabulated pair terms interpolate across the range edge: those use the whole list
-  k_force<T, MC, POTM, P32><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->pos4, p4f_of(c), c->zat, c->n, c->nbr, POTM == POT_TABLE ? c->numn : c->nact, c->box + c->cur,
-      pack_geo(c), c->pot, c->part_f);
+  hipExtLaunchKernelGGL(k_force<T, MC, POTM, P32>, dim3(nblocks(c->n)), dim3(MDK_BLOCK), 0, c->stream, ea, eb, 0,
+                        c->atoms, (const double4 *)c->pos4, (const float4 *)p4f_of(c), (const int *)c->zat, c->n,
+                        (const int *)c->nbr, (const int *)(POTM == POT_TABLE ? c->numn : c->nact),
+                        (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f);
 }
 template <typename T, bool MC, int POTM>
-static void force_dispatch(mdk_ctx *c, int fin_mode) {
-  (void)fin_mode;
+static void force_dispatch(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   if constexpr (sizeof(T) == 4) {
-    if (c->rel) return force_dispatch_p<T, MC, POTM, true>(c);
+    if (c->rel) return force_dispatch_p<T, MC, POTM, true>(c, ea, eb);
   }
-  force_dispatch_p<T, MC, POTM, false>(c);
+  force_dispatch_p<T, MC, POTM, false>(c, ea, eb);
 }
 // lookup tables, the Fe-Fe knot counts, or any knot count
 static int pot_mode(const mdk_ctx *c) {
@@ -1808,11 +1824,11 @@ static void density_pot(mdk_ctx *c, int do_embed) {
   }
 }
 template <typename T, bool MC>
-static void force_pot(mdk_ctx *c, int fin_mode) {
+static void force_pot(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   switch (pot_mode(c)) {
-    case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c, fin_mode); break;
-    case POT_FE: force_dispatch<T, MC, POT_FE>(c, fin_mode); break;
-    default: force_dispatch<T, MC, POT_KNOTS>(c, fin_mode); break;
+    case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c, ea, eb); break;
+    case POT_FE: force_dispatch<T, MC, POT_FE>(c, ea, eb); break;
+    default: force_dispatch<T, MC, POT_KNOTS>(c, ea, eb); break;
   }
 }
 
@@ -1838,14 +1854,15 @@ static int launch_density(mdk_ctx *c, int do_embed) {
 
 static int launch_force(mdk_ctx *c, int fin_mode) {
   TimedEv te;
-  tbegin(c, TC_FORCE, te);
+  tbegin_ext(c, TC_FORCE, te);  // the force kernel's own start and end
+  hipEvent_t ea = te.cls >= 0 ? te.a : nullptr, eb = te.cls >= 0 ? te.b : nullptr;
   const bool f32 = c->p.precision == MDK_PREC_FP32, mc = c->masschk;
   if (f32) {
-    if (mc) force_pot<float, true>(c, fin_mode); else force_pot<float, false>(c, fin_mode);
+    if (mc) force_pot<float, true>(c, ea, eb); else force_pot<float, false>(c, ea, eb);
   } else {
-    if (mc) force_pot<double, true>(c, fin_mode); else force_pot<double, false>(c, fin_mode);
+    if (mc) force_pot<double, true>(c, ea, eb); else force_pot<double, false>(c, ea, eb);
   }
-  tend(c, te);
+  tpush(c, te);
   RCK(check_launch("k_force"));
   const int nb = block_count(c);
   k_finalize_force<<<(nb + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_f, nb, c->stage_f, c->fin_cnt,
