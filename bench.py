@@ -178,6 +178,61 @@ def cpu_baseline(sd, steps, threads):
             "sample": f"{sd.n} atoms x {steps} MD steps, C restatement with OpenMP"}
 
 
+def c0_leg(steps=1000):
+    """NVE energy conservation on the 2000-atom cell (10^3 bcc cells, 300 K,
+    dt = 1 fs, reference rebuild cadence): total energy per atom over the run."""
+    from moldy_amd import engine, lattice
+    sd = lattice.make_system(nc=10, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=1234)
+    out = {"workload": f"bcc Fe 10^3 cells = {sd.n} atoms, 300 K, NVE, dt=1 fs, {steps} steps, "
+                       "nbrupdate 10 (reference cadence)"}
+    for prec, tag in ((engine.PREC_FP64, "fp64"), (engine.PREC_FP32, "fp32")):
+        e = engine.Engine(sd, ivol=1, precision=prec, ntcm=10, rebuild=0)
+        e.linkup()
+        e.md_init()
+        tr = e.md_steps(steps, trace=True)
+        e.close()
+        te = tr[:, 2] / sd.n
+        t_ps = np.arange(steps) * 1e-3
+        slope = float(np.polyfit(t_ps, te, 1)[0])
+        out[tag] = {"te_first_eV_per_atom": float(te[0]), "te_last_eV_per_atom": float(te[-1]),
+                    "max_abs_dev_eV_per_atom": float(np.max(np.abs(te - te[0]))),
+                    "drift_eV_per_atom_per_ps": slope}
+    return out
+
+
+def c2_leg(steps=1000):
+    """Hot liquid: 120^3 bcc cells (3,456,000 atoms) at 7.0 g/cm^3, 2500 K,
+    NVE, dt = 0.5 fs, skin 0.3 A, rebuilds on the displacement criterion only;
+    rebuild count and neighbour-build time reported apart from the step rate."""
+    from moldy_amd import engine, lattice
+    a = lattice.liquid_lattice_parameter(7.0)
+    sd = lattice.make_system(nc=120, a=a, sigma=0.05, temp=2500.0, deltat_fs=0.5, seed=2500)
+    out = {"workload": f"bcc Fe 120^3 cells = {sd.n} atoms, a={a:.5f} A (7.0 g/cm^3), 2500 K, NVE, "
+                       f"dt=0.5 fs, skin 0.3 A, {steps} steps, rebuild when max displacement > skin/2"}
+    for prec, tag in ((engine.PREC_FP32, "fp32"), (engine.PREC_FP64, "fp64")):
+        e = engine.Engine(sd, ivol=1, precision=prec, rebuild=1, skin=0.3)
+        e.linkup()
+        e.md_init()
+        e.md_steps(10, trace=False)
+        e.synchronize()
+        e.timing(True, classes=("rebuild",), every=1)
+        r0 = e.stats()[0]
+        t0 = time.perf_counter()
+        tr = e.md_steps(steps, trace=True)
+        e.synchronize()
+        secs = time.perf_counter() - t0
+        reb = e.stats()[0] - r0
+        kt = e.timing_read()
+        e.timing(False)
+        e.close()
+        rb = kt.get("rebuild", (0.0, 0))
+        out[tag] = {"ms_per_step": secs / steps * 1e3, "atom_steps_per_s": sd.n * steps / secs,
+                    "rebuilds": reb, "rebuild_us_avg": 1e3 * rb[0] / max(rb[1], 1),
+                    "rebuild_ms_total": rb[0],
+                    "te_drift_eV_per_atom": float((tr[-1, 2] - tr[0, 2]) / sd.n)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +243,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the NVE-drift (2000 atoms) and hot-liquid (3.46M atoms) legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the FETCH_SIZE/WRITE_SIZE calibration copies")
     args = ap.parse_args()
@@ -245,6 +302,17 @@ def main():
         log(f"batch: {batch['ms_per_evaluation']:.3f} ms/evaluation, "
             f"{batch['adjoint']['grads_per_s']:.1f} grads/s")
 
+    c0 = c2 = None
+    if not args.no_workloads:
+        log("nve drift, 2000 atoms ...")
+        c0 = c0_leg()
+        log(f"nve drift: fp64 {c0['fp64']['drift_eV_per_atom_per_ps']:.3e}, "
+            f"fp32 {c0['fp32']['drift_eV_per_atom_per_ps']:.3e} eV/atom/ps")
+        log("hot liquid, 3.46M atoms ...")
+        c2 = c2_leg()
+        log(f"hot liquid fp32: {c2['fp32']['ms_per_step']:.3f} ms/step, {c2['fp32']['rebuilds']} rebuilds, "
+            f"{c2['fp32']['rebuild_us_avg']:.0f} us each")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -276,8 +344,11 @@ def main():
         "fp32_force_error_vs_fp64_eV_per_A": force_err,
         "kernels": r32["kernels"],
         "kernels_note": "per-kernel averages from a separate 50-step pass with events around every "
-                        "kernel; the timed region brackets every 10th force-kernel launch only",
+                        "kernel class; in the timed region only every 10th force-kernel launch records "
+                        "its own dispatch events",
         "roofline": roofline,
+        "nve_drift_2000": c0,
+        "hot_liquid_3456000": c2,
         "batch": batch,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (r32["atom_steps_per_s"] / cpu["value"]) if cpu else None,
