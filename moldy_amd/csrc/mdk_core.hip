@@ -77,6 +77,7 @@ struct TimedEv {
 };
 
 #define MON_STRIDE 16
+#define MDK_NSPEC 128  // species (atomic numbers) covered by the mass/del2 table
 
 // list grid for cell-relative packing (gx == 0: absolute packing)
 struct PackGeo {
@@ -98,6 +99,14 @@ struct mdk_ctx {
   int *img;                // integer image floor(s) fixed at the last full-list build
   bool pos4_valid;
   bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
+  // mass and del2 as functions of the species when every atom of a species
+  // carries the same values: spec_tab[2 z], spec_tab[2 z + 1] (device), used
+  // by the kick and kinten instead of the per-atom fields; host copies of the
+  // uploads (reference order) decide it
+  double *spec_tab;
+  bool spec_ok;
+  double *h_mass, *h_del2;
+  int *h_z;
   // reference link cells (1-based cell per atom, sorted order)
   int *ic;
   // parity half list in reference layout: numn_ref[nm] and nlist_ref[nm*nnbrs] (orig order)
@@ -171,6 +180,8 @@ static inline PackGeo pack_geo(const mdk_ctx *c) {
   return pg;
 }
 static inline float4 *p4f_of(const mdk_ctx *c) { return c->p.precision == MDK_PREC_FP32 ? c->p4f : nullptr; }
+// the (mass, del2) species table when it holds, else the per-atom fields
+static inline const double *spec_of(const mdk_ctx *c) { return c->spec_ok ? c->spec_tab : nullptr; }
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -1081,7 +1092,7 @@ template <typename T, bool MASSCHK, int POTM, bool P32>
 __global__ void __launch_bounds__(MDK_BLOCK)
 k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restrict__ p4f, const int *zat,
         int n, const int *__restrict__ nbr, const int *__restrict__ nvisit, const BoxState *bs, PackGeo pg,
-        PotDev P, double *part) {
+        PotDev P, double *part, const double *spec) {
   __shared__ double S[32][3];
   __shared__ float vtab[8][8];  // interval cubics of the pair function (FP32 Fe-Fe form)
   constexpr bool CUBIC = sizeof(T) == 4 && POTM == POT_FE;
@@ -1147,8 +1158,9 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
     atoms[(size_t)F_FX * n + i] = bi[0] * fxi + bi[3] * fyi + bi[6] * fzi;
     atoms[(size_t)F_FY * n + i] = bi[1] * fxi + bi[4] * fyi + bi[7] * fzi;
     atoms[(size_t)F_FZ * n + i] = bi[2] * fxi + bi[5] * fyi + bi[8] * fzi;
-    if (zat[i] != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
-      const double mi = mass[i];
+    const int zi = zat[i];
+    if (zi != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
+      const double mi = spec ? spec[2 * zi] : mass[i];
       const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
                    a3 = atoms[(size_t)F_Z1 * n + i];
       const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
@@ -1367,7 +1379,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
                                                     BoxState *bs, int ivol, int mode,
                                                     const float *sref, double *part, double4 *pos4,
                                                     float4 *p4f, const int *img, const int *cell,
-                                                    PackGeo pg, BoxState *bout) {
+                                                    PackGeo pg, BoxState *bout, const double *spec) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1376,8 +1388,10 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *za
     double *x1p = atoms + (size_t)F_X1 * n, *y1p = atoms + (size_t)F_Y1 * n, *z1p = atoms + (size_t)F_Z1 * n;
     double *x2p = atoms + (size_t)F_X2 * n, *y2p = atoms + (size_t)F_Y2 * n, *z2p = atoms + (size_t)F_Z2 * n;
     const double *fxp = atoms + (size_t)F_FX * n, *fyp = atoms + (size_t)F_FY * n, *fzp = atoms + (size_t)F_FZ * n;
-    const double m = atoms[(size_t)F_MASS * n + i], d2 = atoms[(size_t)F_DEL2 * n + i];
     const int z = zat[i];
+    // mass and del2 from the species table when they are functions of the species
+    const double m = spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i];
+    const double d2 = spec ? spec[2 * z + 1] : atoms[(size_t)F_DEL2 * n + i];
     double x1 = x1p[i], y1 = y1p[i], z1 = z1p[i];
     double x2 = x2p[i], y2 = y2p[i], z2 = z2p[i];
     if (mode == 0) {
@@ -1801,7 +1815,7 @@ static void force_dispatch_p(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   hipExtLaunchKernelGGL(k_force<T, MC, POTM, P32>, dim3(nblocks(c->n)), dim3(MDK_BLOCK), 0, c->stream, ea, eb, 0,
                         c->atoms, (const double4 *)c->pos4, (const float4 *)p4f_of(c), (const int *)c->zat, c->n,
                         (const int *)c->nbr, (const int *)(POTM == POT_TABLE ? c->numn : c->nact),
-                        (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f);
+                        (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f, spec_of(c));
 }
 template <typename T, bool MC, int POTM>
 static void force_dispatch(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
@@ -1894,11 +1908,11 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   if (fuse_drift && mode == 0)
     k_kick<true><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, c->pos4, p4f_of(c),
-        c->img, c->cell_of, pack_geo(c), c->box + nxt);
+        c->img, c->cell_of, pack_geo(c), c->box + nxt, spec_of(c));
   else
     k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, nullptr, nullptr,
-        nullptr, nullptr, pack_geo(c), nullptr);
+        nullptr, nullptr, pack_geo(c), nullptr, spec_of(c));
   tend(c, te);
   RCK(check_launch("k_kick"));
   if (mode != 2) {
@@ -2058,6 +2072,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * (NPART_FORCE + 1));
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
   ALLOC(c->fin_cnt, sizeof(unsigned) * 2);
+  ALLOC(c->spec_tab, sizeof(double) * 2 * MDK_NSPEC);
   if (hipMemset(c->fin_cnt, 0, sizeof(unsigned) * 2) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "counter initialisation failed");
     goto fail;
@@ -2099,10 +2114,13 @@ extern "C" int mdk_destroy(void *h) {
   void *ptrs[] = {c->pos4, c->p4f, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
-                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks};
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
+  free(c->h_mass);
+  free(c->h_del2);
+  free(c->h_z);
   for (int k = 0; k < 4; k++)
     if (c->mon_ev[k]) hipEventDestroy(c->mon_ev[k]);
   for (auto &e : c->evs) {
@@ -2209,6 +2227,36 @@ extern "C" int mdk_eval_potential(void *h, int which, const double *x, double *y
 static const int field_map[MDK_NFIELDS] = {F_X0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2,
                                            F_FX, F_FY, F_FZ, F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2};
 
+// Species table of (mass, del2) when both are functions of the species.
+static int keep_host_copy(double **dst, const double *src, int n) {
+  if (!*dst) *dst = (double *)malloc(sizeof(double) * n);
+  if (!*dst) return set_err(MDK_ERR_HIP, "host copy failed");
+  memcpy(*dst, src, sizeof(double) * n);
+  return MDK_OK;
+}
+static int update_species_table(mdk_ctx *c) {
+  c->spec_ok = false;
+  if (!c->h_mass || !c->h_del2 || !c->h_z) return MDK_OK;
+  double tab[2 * MDK_NSPEC];
+  bool seen[MDK_NSPEC] = {};
+  for (int i = 0; i < c->n; i++) {
+    const int z = c->h_z[i];
+    if (z < 0 || z >= MDK_NSPEC) return MDK_OK;
+    if (!seen[z]) {
+      seen[z] = true;
+      tab[2 * z] = c->h_mass[i];
+      tab[2 * z + 1] = c->h_del2[i];
+    } else if (memcmp(&tab[2 * z], &c->h_mass[i], sizeof(double)) != 0 ||
+               memcmp(&tab[2 * z + 1], &c->h_del2[i], sizeof(double)) != 0) {
+      return MDK_OK;  // per-atom values differ within a species
+    }
+  }
+  for (int z = 0; z < MDK_NSPEC; z++)
+    if (!seen[z]) tab[2 * z] = tab[2 * z + 1] = 0.0;
+  HIPCK(hipMemcpy(c->spec_tab, tab, sizeof(tab), hipMemcpyHostToDevice));
+  c->spec_ok = true;
+  return MDK_OK;
+}
 extern "C" int mdk_upload(void *h, int field, const double *data) {
   mdk_ctx *c = H(h);
   if (!c || field < 0 || field >= MDK_NFIELDS || !data) return set_err(MDK_ERR_BAD_ARG, "bad upload");
@@ -2225,6 +2273,10 @@ extern "C" int mdk_upload(void *h, int field, const double *data) {
     bool any = false;
     for (int i = 0; i < n; i++) any = any || !(data[i] > 0.05);
     c->masschk = any;
+  }
+  if (field == MDK_MASS || field == MDK_DEL2) {
+    RCK(keep_host_copy(field == MDK_MASS ? &c->h_mass : &c->h_del2, data, n));
+    RCK(update_species_table(c));
   }
   return MDK_OK;
 }
@@ -2249,7 +2301,10 @@ extern "C" int mdk_upload_species(void *h, const int *zn) {
   k_scatter_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat, c->zat_alt, c->slot_of, n);
   RCK(check_launch("k_scatter_i"));
   HIPCK(hipStreamSynchronize(c->stream));
-  return MDK_OK;
+  if (!c->h_z) c->h_z = (int *)malloc(sizeof(int) * n);
+  if (!c->h_z) return set_err(MDK_ERR_HIP, "host copy failed");
+  memcpy(c->h_z, zn, sizeof(int) * n);
+  return update_species_table(c);
 }
 
 static double *box_field(BoxState &s, int which) {
@@ -2487,7 +2542,7 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
   tbegin(c, TC_KICK, te);
   k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
       c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr,
-      nullptr, pack_geo(c), nullptr);
+      nullptr, pack_geo(c), nullptr, spec_of(c));
   tend(c, te);
   RCK(check_launch("k_kick"));
   k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
