@@ -201,6 +201,63 @@ def test_gpu_fp32_form_force_error(oracle):
     eng.close()
 
 
+@pytest.mark.parametrize("nc", [3, 5])
+def test_gpu_small_box_legacy_list(oracle, nc):
+    """Boxes with fewer than four list cells per dimension (2 and 3 here) take
+    the legacy full list: absolute packed positions, image-shift codes, and
+    per-pair rounding below three cells.  FP64: a 20-step PR NPT trajectory
+    against the oracle; FP32 form: the first forces within 1e-5 eV/A."""
+    sd = lattice.make_system(nc=nc, sigma=0.08, temp=300.0, seed=40 + nc)
+    say("small box", nc)
+    S = oracle.OracleSystem(sd, oracle.Potential(), ivol=0, ntcm=10)
+    S.call("linkup")
+    S.call("md_init")
+    f_init = S.forces_cart()
+    e32 = engine.Engine(sd, ivol=0, ntcm=10, precision=engine.PREC_FP32)
+    e32.linkup()
+    e32.md_init()
+    assert np.max(np.abs(e32.forces_cart() - f_init)) <= 1e-5  # eV/A
+    e32.close()
+    eng = engine.Engine(sd, ivol=0, ntcm=10)
+    eng.linkup()
+    eng.md_init()
+    assert rel(eng.forces_cart(), f_init) < 1e-10
+    tr = eng.md_steps(20)
+    pe = []
+    for _ in range(20):
+        S.call("md_step")
+        pe.append(S.scalar("pe"))
+    assert np.max(np.abs(tr[:, 0] - np.array(pe)) / np.abs(np.array(pe))) < 1e-11
+    assert rel(eng.forces_cart(), S.forces_cart()) < 1e-9
+    assert np.max(np.abs(eng.array("x0") - S.array("x0"))) < 1e-12
+    eng.close()
+
+
+def test_gpu_per_atom_masses(oracle):
+    """Masses that differ within a species keep the kick on the per-atom mass
+    and del2 fields (the species table only holds when all atoms of a species
+    agree): an NVE trajectory against the oracle."""
+    sd = lattice.make_system(nc=6, sigma=0.05, temp=300.0, seed=77)
+    sd.mass[::7] *= 1.25
+    say("per-atom masses")
+    eng = engine.Engine(sd, ivol=1, ntcm=10)
+    eng.linkup()
+    eng.md_init()
+    tr = eng.md_steps(20)
+    S = oracle.OracleSystem(sd, oracle.Potential(), ivol=1, ntcm=10)
+    S.call("linkup")
+    S.call("md_init")
+    ke = []
+    for _ in range(20):
+        S.call("md_step")
+        ke.append(S.scalar("ke"))
+    assert np.max(np.abs(tr[:, 1] - np.array(ke)) / np.abs(np.array(ke))) < 1e-10
+    v = np.stack([eng.array(n) for n in ("x1", "y1", "z1")], axis=1)
+    vr = np.stack([S.array(n) for n in ("x1", "y1", "z1")], axis=1)
+    assert rel(v, vr) < 1e-10
+    eng.close()
+
+
 def test_gpu_neighbour_overflow_reported():
     sd = lattice.make_system(nc=6, sigma=0.05, seed=1)
     say("overflow")
