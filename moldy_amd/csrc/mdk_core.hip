@@ -813,8 +813,9 @@ __device__ void drift_box(const BoxState *bin, BoxState *bout, int ivol) {
 // velocityverlet drift (src/quench_m.F90:574-589): x0 += x1 + x2 (also into the
 // packed copy); block 0 copies the box state into the other buffer, advancing
 // b1 and b0 there (one field per thread).
-__global__ void k_drift(double *atoms, double4 *pos4, float4 *p4f, const int *img, const int *cell,
-                        PackGeo pg, int n, const BoxState *bin, BoxState *bout, int ivol) {
+__global__ void k_drift(double *__restrict__ atoms, double4 *__restrict__ pos4, float4 *__restrict__ p4f,
+                        const int *__restrict__ img, const int *__restrict__ cell, PackGeo pg, int n,
+                        const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol) {
 #pragma clang fp contract(off)
   int i = xcd_tile() * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -1367,6 +1368,8 @@ __global__ void k_therm_box(BoxState *bs, int ivol) {
   }
 }
 
+// Every pointer is restrict-qualified: the box state is only read, so its
+// uniform fields come through scalar loads instead of one vector load per lane.
 // Velocity half-kick (src/quench_m.F90:604-622) fused with the per-atom sums of
 // update_therm_avs (src/dynamics_m.F90:702-732) and the list-skin monitor.
 // mode 0: VV kick + sums; 1: sums only; 2: initial accelerations (moldin.F90:461-472)
@@ -1375,11 +1378,13 @@ __global__ void k_therm_box(BoxState *bs, int ivol) {
 // advanced box state into bout -- what k_drift would do at the start of the
 // next step.
 template <bool DRIFT>
-__global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *atoms, const int *zat, int n,
-                                                    BoxState *bs, int ivol, int mode,
-                                                    const float *sref, double *part, double4 *pos4,
-                                                    float4 *p4f, const int *img, const int *cell,
-                                                    PackGeo pg, BoxState *bout, const double *spec) {
+__global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, const int *__restrict__ zat, int n,
+                                                    const BoxState *__restrict__ bs, int ivol, int mode,
+                                                    const float *__restrict__ sref, double *__restrict__ part,
+                                                    double4 *__restrict__ pos4, float4 *__restrict__ p4f,
+                                                    const int *__restrict__ img, const int *__restrict__ cell,
+                                                    PackGeo pg, BoxState *__restrict__ bout,
+                                                    const double *__restrict__ spec) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
