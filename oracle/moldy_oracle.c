@@ -146,8 +146,139 @@ static double dexpjoin(double a1, double x1, double v1, double a2, double x2, do
   return ds * exp(sp);
 }
 
+/* ======================================================================
+ * Potential: src/iron_carbon.F90, the Fe-Fe pair and the Fe embedding.
+ * Each Fortran expression is kept in its order: sums left to right,
+ * xH0(d)*a*d**3 as (xH0*a)*(d*d*d), integer powers as a left-to-right product
+ * (x**4 = ((x*x)*x)*x), x**0.5d0 as |sqrt(x)| and x**(-0.5d0) as pow(); the
+ * forms were fixed against the reference build's own outputs.
+ * ====================================================================== */
+static const double fec_bierfe = 9734.159682000001; /* :49-53 */
+static const double fec_bier[4] = {0.1818, 0.5099, 0.2802, 0.02817};
+static const double fec_bierx[4] = {-28.616172142695042, -8.42656844064423,
+                                    -3.6029549238411973, -1.8028188449897875};
+static const double fec_c2 = 6.7314115586063e-4, fec_c4 = 7.6514905604792e-8; /* :606-609 */
+
+/* vee_src Fe-Fe, src/iron_carbon.F90:118-158; the join/knot boundary is the
+ * REAL(4) literal 2.05 */
+static double fec_vee(const orc_pot *p, double r) {
+  if (r < 1.0) {
+    return (fec_bierfe / r) * (fec_bier[0] * exp(fec_bierx[0] * r) + fec_bier[1] * exp(fec_bierx[1] * r) +
+                               fec_bier[2] * exp(fec_bierx[2] * r) + fec_bier[3] * exp(fec_bierx[3] * r));
+  } else if (r > (double)2.05f) {
+    double s = 0.0;
+    for (int k = 0; k < 13; k++) {
+      const double d = p->vcut[k] - r;
+      const double t = (xh0(d) * p->vpar[k]) * (d * d * d);
+      s = k == 0 ? t : s + t;
+    }
+    return s;
+  }
+  const double *j = p->join;
+  return exp(j[0] + j[1] * r + j[2] * r * r + j[3] * r * r * r);
+}
+/* dvee_src Fe-Fe, src/iron_carbon.F90:274-314 */
+static double fec_dvee(const orc_pot *p, double r) {
+  if (r < 1.0) {
+    return -(fec_bierfe / r) *
+           ((1.0 / r - fec_bierx[0]) * fec_bier[0] * exp(fec_bierx[0] * r) +
+            (1.0 / r - fec_bierx[1]) * fec_bier[1] * exp(fec_bierx[1] * r) +
+            (1.0 / r - fec_bierx[2]) * fec_bier[2] * exp(fec_bierx[2] * r) +
+            (1.0 / r - fec_bierx[3]) * fec_bier[3] * exp(fec_bierx[3] * r));
+  } else if (r > (double)2.05f) {
+    double s = 0.0;
+    for (int k = 0; k < 13; k++) {
+      const double d = p->vcut[k] - r;
+      const double t = ((3.0 * xh0(d)) * p->vpar[k]) * (d * d);
+      s = k == 0 ? t : s + t;
+    }
+    return -s;
+  }
+  const double *j = p->join;
+  return (j[1] + 2.0 * j[2] * r + 3.0 * j[3] * r * r) * exp(j[0] + j[1] * r + j[2] * r * r + j[3] * r * r * r);
+}
+/* phi_src Fe-Fe, src/iron_carbon.F90:431-449 (no flat cap) */
+static double fec_phi(const orc_pot *p, double r) {
+  double t[3];
+  for (int k = 0; k < 3; k++) {
+    const double d = p->pcut[k] - r;
+    t[k] = (xh0(d) * p->ppar[k]) * (d * d * d);
+  }
+  return t[0] - t[1] + t[2];
+}
+/* dphi_src Fe-Fe, src/iron_carbon.F90:515-532 */
+static double fec_dphi(const orc_pot *p, double r) {
+  double t[3];
+  for (int k = 0; k < 3; k++) {
+    const double d = p->pcut[k] - r;
+    t[k] = ((3.0 * xh0(d)) * p->ppar[k]) * (d * d);
+  }
+  return -(t[0] - t[1] + t[2]);
+}
+/* emb_src Fe, src/iron_carbon.F90:599-615 */
+static double fec_emb(const orc_pot *p, double rho) {
+  double e = -1.0 * fabs(sqrt(rho)) - fec_c2 * (rho * rho) + fec_c4 * (rho * rho * rho * rho);
+  if (rho > p->rcludge) {
+    const double d = rho - p->rcludge;
+    e = e + p->acludge * (d * d * d);
+  }
+  if (rho > p->rx) {
+    const double x = p->rx, d = p->rx - p->rcludge;
+    e = -1.0 * fabs(sqrt(x)) - fec_c2 * (x * x) + fec_c4 * (x * x * x * x) + p->acludge * (d * d * d);
+  }
+  return e;
+}
+/* demb_src Fe, src/iron_carbon.F90:651-668 */
+static double fec_demb(const orc_pot *p, double rho) {
+  if (rho <= 0.0 || rho > p->rx) return 0.0;
+  double d = 0.5 * -1.0 * pow(rho, -0.5) - 2.0 * fec_c2 * rho + 4.0 * fec_c4 * (rho * rho * rho);
+  if (rho > p->rcludge) {
+    const double x = rho - p->rcludge;
+    d = d + (double)3.0f * p->acludge * (x * x);
+  }
+  return d;
+}
+
+/* Coefficients of src/iron_carbon.F90:44-74 */
+void orc_pot_init_iron_carbon(orc_pot *p, const double *iron_in) {
+  static const double vcut[13] = {5.3, 4.7, 4.2, 3.7, 3.3, 3.0, 2.8, 2.7, 2.6, 2.5, 2.4, 2.3, 2.2};
+  static const double vpar[13] = {-0.0030458824556234, -0.058531821042271, 0.35018615891957,
+                                  -1.0260416933564,   2.657740612828,     -2.3194358924605,
+                                  0.80656414937789,   -0.77361294129713,  4.2099676494795,
+                                  -2.4989799053251,   2.2077118733936,    15.738054058489,
+                                  -27.444805994228};
+  static const double join[4] = {7.411957380630116, -0.6409989885339922, -2.6049191728604733,
+                                 0.6263733940718166};
+  static const double pcut[3] = {4.2, 3.2, 2.4};
+  static const double ppar[3] = {0.47193527075943, 0.01471074009883, 11.68685940797};
+  memset(p, 0, sizeof(*p));
+  p->material = ORC_MAT_IRON_CARBON;
+  if (iron_in) {
+    memcpy(p->vcut, iron_in, sizeof(p->vcut));
+    memcpy(p->vpar, iron_in + 13, sizeof(p->vpar));
+    memcpy(p->join, iron_in + 26, sizeof(p->join));
+    memcpy(p->pcut, iron_in + 30, sizeof(p->pcut));
+    memcpy(p->ppar, iron_in + 33, sizeof(p->ppar));
+    p->acludge = iron_in[36];
+    p->rcludge = iron_in[37];
+  } else {
+    memcpy(p->vcut, vcut, sizeof(vcut));
+    memcpy(p->vpar, vpar, sizeof(vpar));
+    memcpy(p->join, join, sizeof(join));
+    memcpy(p->pcut, pcut, sizeof(pcut));
+    memcpy(p->ppar, ppar, sizeof(ppar));
+    p->acludge = 0.0;
+    p->rcludge = 28.5;
+  }
+  p->rx = 257.0;
+}
+
+/* pot_rmax of the material (src/fetransmetal.F90:589-593, src/iron_carbon.F90:93) */
+double orc_pot_range(const orc_pot *p) { return p->material == ORC_MAT_IRON_CARBON ? 5.4 : 6.0; }
+
 /* vee_src, src/fetransmetal.F90:100-159.  r == bier_cut1 exactly gives 0. */
 double orc_vee_src(const orc_pot *p, double r) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_vee(p, r);
   if (r > p->bier_cut2) return knot_v(p, r);
   if (r > p->bier_cut1)
     return expjoin(p->bier_cut1, p->sc[0], p->sc[1], p->bier_cut2, p->sc[2], p->sc[3], r);
@@ -156,6 +287,7 @@ double orc_vee_src(const orc_pot *p, double r) {
 }
 /* dvee_src, src/fetransmetal.F90:167-219 */
 double orc_dvee_src(const orc_pot *p, double r) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_dvee(p, r);
   if (r > p->bier_cut2) return knot_dv(p, r);
   if (r > p->bier_cut1)
     return dexpjoin(p->bier_cut1, p->sc[0], p->sc[1], p->bier_cut2, p->sc[2], p->sc[3], r);
@@ -164,17 +296,23 @@ double orc_dvee_src(const orc_pot *p, double r) {
 }
 /* phi_src, src/fetransmetal.F90:227-285 (flat below phi_cut(1)) */
 double orc_phi_src(const orc_pot *p, double r) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_phi(p, r);
   if (r < p->phi_cut_r) return p->phi_cut_v;
   return knot_p(p, r);
 }
 /* dphi_src, src/fetransmetal.F90:293-366 */
 double orc_dphi_src(const orc_pot *p, double r) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_dphi(p, r);
   if (r < p->phi_cut_r) return 0.0;
   return knot_dp(p, r);
 }
 /* emb_src / demb_src, src/fetransmetal.F90:375-404 */
-static double emb_src(double rho) { return -1.0 * sqrt(rho > 0.0 ? rho : 0.0); }
-static double demb_src(double rho) {
+static double emb_src(const orc_pot *p, double rho) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_emb(p, rho);
+  return -1.0 * sqrt(rho > 0.0 ? rho : 0.0);
+}
+static double demb_src(const orc_pot *p, double rho) {
+  if (p->material == ORC_MAT_IRON_CARBON) return fec_demb(p, rho);
   double d = 0.0;
   if (rho > 0.0) d = -0.5 / sqrt(rho);
   return d;
@@ -222,7 +360,7 @@ void orc_pot_init(orc_pot *p, int nv, const double *av, const double *rv, int np
 int orc_pot_build_lookup(orc_pot *p, int nlookup) {
   p->nlookup = nlookup;
   p->r_min = 0.0;
-  p->r_max = 6.0;
+  p->r_max = orc_pot_range(p);
   p->vp_range = p->r_max - p->r_min;
   p->vp_reciprange = 1.0 / p->vp_range;
   p->vp_recipnlkp = 1.0 / (double)nlookup;
@@ -244,8 +382,8 @@ int orc_pot_build_lookup(orc_pot *p, int nlookup) {
     p->tab[2][i] = orc_phi_src(p, r);
     p->tab[3][i] = orc_dphi_src(p, r);
     double rh = p->rho_min + (double)i * p->e_range * p->e_recipnlkp;
-    p->tab[4][i] = emb_src(rh);
-    p->tab[5][i] = demb_src(rh);
+    p->tab[4][i] = emb_src(p, rh);
+    p->tab[5][i] = demb_src(p, rh);
   }
   p->use_lookup = 1;
   return 0;
@@ -277,8 +415,8 @@ double orc_vee(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->
 double orc_dvee(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[1], r) : orc_dvee_src(p, r); }
 double orc_phi(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[2], r) : orc_phi_src(p, r); }
 double orc_dphi(const orc_pot *p, double r) { return p->use_lookup ? lk_vp(p, p->tab[3], r) : orc_dphi_src(p, r); }
-double orc_emb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[4], rho) : emb_src(rho); }
-double orc_demb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[5], rho) : demb_src(rho); }
+double orc_emb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[4], rho) : emb_src(p, rho); }
+double orc_demb(const orc_pot *p, double rho) { return p->use_lookup ? lk_e(p, p->tab[5], rho) : demb_src(p, rho); }
 
 /* ======================================================================
  * System state (src/particles_m.F90:52-66, src/parrinellorahman_m.F90:102-115)
@@ -343,7 +481,8 @@ orc_sys *orc_create(int n, const double *b0, const double *x0, const double *y0,
   s->bmass = (double)n * boxmass;
   s->bdel2 = s->deltat * s->deltat / (2.0 * s->bmass);
   s->rpad = rpad;
-  s->rcut = rcut > 6.0 ? rcut : 6.0; /* moldin.F90:226 with range 6 (fetransmetal.F90:589-593) */
+  /* moldin.F90:226 with the material's range (fetransmetal.F90:589-593, iron_carbon.F90:93) */
+  s->rcut = rcut > orc_pot_range(pot) ? rcut : orc_pot_range(pot);
   s->rnear = s->rcut + s->rpad;      /* moldin.F90:248 */
   memcpy(s->b0, b0, sizeof(s->b0));
   /* moldin.F90:252-257: link cells from the diagonal, at least 3, never recomputed */
@@ -880,6 +1019,126 @@ int orc_md_step(orc_sys *s) {
   s->th = s->h + s->boxke;
   if (s->ntc == 0) orc_relink(s);
   return 0;
+}
+
+/* energy_calc, src/dynamics_m.F90:410-470 (serial order; vee_src even in lookup mode) */
+void orc_energy_calc(orc_sys *s, double *en) {
+  for (int i = 0; i < s->n; i++) en[i] = s->afrho[i];
+  for (int i = 0; i < s->n; i++) {
+    for (int k = 0; k < s->numn[i]; k++) {
+      int j = s->nlist[(size_t)i * s->nnbrs + k] - 1;
+      double dx = s->x0[i] - s->x0[j], dy = s->y0[i] - s->y0[j], dz = s->z0[i] - s->z0[j];
+      double r = realsep(s->b0, s->ivol, &dx, &dy, &dz);
+      double pp = orc_vee_src(&s->pot, r);
+      en[i] = en[i] + pp / 2.0;
+      en[j] = en[j] + pp / 2.0;
+    }
+  }
+}
+
+/* ======================================================================
+ * Initial state as moldin builds it (random_m, read_system, setvel)
+ * ====================================================================== */
+
+/* rstart, src/random_m.F90:110-143 (the 24-bit constants are exact in REAL(4)) */
+void orc_rstart(orc_rng *g, int iw, int ix, int iy, int iz) {
+  int i = iw, j = ix, k = iy, l = iz;
+  for (int ii = 0; ii < 97; ii++) {
+    double sum = 0.0, t = 0.5;
+    for (int jj = 0; jj < 24; jj++) {
+      int m = ((i * j) % 179) * k % 179;
+      i = j;
+      j = k;
+      k = m;
+      l = (53 * l + 1) % 169;
+      if ((l * m) % 64 >= 32) sum = sum + t;
+      t = 0.5 * t;
+    }
+    g->u[ii] = sum;
+  }
+  g->c = (double)(362436.0f / 16777216.0f);
+  g->cd = (double)(7654321.0f / 16777216.0f);
+  g->cm = (double)(16777213.0f / 16777216.0f);
+  g->iu = 97;
+  g->ju = 33;
+}
+
+/* rand1, src/random_m.F90:73-95 */
+double orc_rand1(orc_rng *g) {
+  double uni = g->u[g->iu - 1] - g->u[g->ju - 1];
+  if (uni < 0.0) uni = uni + 1.0;
+  g->u[g->iu - 1] = uni;
+  if (--g->iu == 0) g->iu = 97;
+  if (--g->ju == 0) g->ju = 97;
+  g->c = g->c - g->cd;
+  if (g->c < 0.0) g->c = g->c + g->cm;
+  uni = uni - g->c;
+  if (uni < 0.0) uni = uni + 1.0;
+  return uni;
+}
+
+/* read_system's replication loop, src/io_m.F90:216-245 */
+void orc_replicate(orc_rng *g, int nbasis, const double *xb, const double *yb, const double *zb,
+                   const int *nt, const double *b0, double dsp, double *x0, double *y0, double *z0) {
+  const double rt[3] = {1.0 / nt[0], 1.0 / nt[1], 1.0 / nt[2]};
+  long l = 0;
+  for (int i = 0; i < nt[0]; i++)
+    for (int j = 0; j < nt[1]; j++)
+      for (int k = 0; k < nt[2]; k++)
+        for (int ib = 0; ib < nbasis; ib++, l++) {
+          x0[l] = (orc_rand1(g) - 0.5) * dsp / T(b0, 1, 1) + rt[0] * ((double)i + xb[ib]);
+          y0[l] = (orc_rand1(g) - 0.5) * dsp / T(b0, 2, 2) + rt[1] * ((double)j + yb[ib]);
+          z0[l] = (orc_rand1(g) - 0.5) * dsp / T(b0, 3, 3) + rt[2] * ((double)k + zb[ib]);
+        }
+}
+
+/* bk = boltzmann/electron, src/constants_m.F90:93 */
+double orc_bk(void) { return k_boltzmann / k_electron; }
+
+/* RQKE of init_thermostat_m, src/thermostat_m.F90:97 */
+double orc_rqke(double temprq, int nm) { return 1.5 * orc_bk() * temprq * (double)(nm - 1); }
+
+/* setvel, src/thermostat_m.F90:191-255 (the atom part; the routine returns
+ * before the box part for ivol < 1 and ivol == 4) */
+double orc_setvel(orc_rng *g, int n, const int *z, const double *mass, const double *b0,
+                  double deltat, double rqke, double *x1, double *y1, double *z1) {
+  int nmove = 0;
+  for (int i = 0; i < n; i++) {
+    if (z[i] == 0 || mass[i] < (double)0.1f) continue;
+    x1[i] = orc_rand1(g) - 0.5;
+    y1[i] = orc_rand1(g) - 0.5;
+    z1[i] = orc_rand1(g) - 0.5;
+    nmove++;
+  }
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int i = 0; i < n; i++) {
+    sx = sx + x1[i] * mass[i];
+    sy = sy + y1[i] * mass[i];
+    sz = sz + z1[i] * mass[i];
+  }
+  sx = sx / nmove;
+  sy = sy / nmove;
+  sz = sz / nmove;
+  double s3 = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (z[i] == 0 || mass[i] < (double)0.1f) continue;
+    x1[i] = x1[i] - sx / mass[i];
+    y1[i] = y1[i] - sy / mass[i];
+    z1[i] = z1[i] - sz / mass[i];
+    double rx = T(b0, 1, 1) * x1[i] + T(b0, 1, 2) * y1[i] + T(b0, 1, 3) * z1[i];
+    double ry = T(b0, 2, 1) * x1[i] + T(b0, 2, 2) * y1[i] + T(b0, 2, 3) * z1[i];
+    double rz = T(b0, 3, 1) * x1[i] + T(b0, 3, 2) * y1[i] + T(b0, 3, 3) * z1[i];
+    s3 = s3 + mass[i] * (rx * rx + ry * ry + rz * rz);
+  }
+  double ke = s3 / (deltat * deltat) / 2.0;
+  /* float(nmove)/float(nm) is a REAL(4) quotient and sqrt of it a REAL(4) sqrt */
+  double scale = sqrt(rqke / ke) * (double)sqrtf((float)nmove / (float)n);
+  for (int i = 0; i < n; i++) {
+    x1[i] = x1[i] * scale;
+    y1[i] = y1[i] * scale;
+    z1[i] = z1[i] * scale;
+  }
+  return rqke;
 }
 
 /* ---------------- accessors ---------------- */

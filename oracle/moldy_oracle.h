@@ -28,9 +28,17 @@
 extern "C" {
 #endif
 
-/* ---------------- potential (Fe-Fe, fetransmetal) ---------------- */
+/* ---------------- potential (Fe-Fe) ---------------- */
+/* Material module, chosen at compile time in the reference (src/potential_m.F90:49-87) */
+#define ORC_MAT_FETRANSMETAL 0
+#define ORC_MAT_IRON_CARBON 1
+
 typedef struct orc_pot {
-  int nv, np;                 /* number of pair / density knots */
+  int material;               /* ORC_MAT_* */
+  /* iron_carbon Fe-Fe data (src/iron_carbon.F90:42-67): 13 pair knots, the
+   * cubic in the join's exponent, 3 density knots, embedding cludge */
+  double vcut[13], vpar[13], join[4], pcut[3], ppar[3], acludge, rcludge, rx;
+  int nv, np;                 /* number of pair / density knots (fetransmetal) */
   double av[8], rv[8];        /* pair amplitudes and knot radii */
   double ap[4], rp[4];        /* density amplitudes and knot radii */
   double bier_cut1, bier_cut2;
@@ -46,6 +54,13 @@ typedef struct orc_pot {
 /* Sets coefficients and runs the set_up_m sequence (spline_coeff, phi_turn). */
 void orc_pot_init(orc_pot *p, int nv, const double *av, const double *rv,
                   int np, const double *ap, const double *rp);
+/* iron_carbon material (src/iron_carbon.F90).  iron_in == NULL keeps the
+ * module's built-in coefficients; otherwise 38 values in the order
+ * check_supported_atomic_numbers reads "iron.in" (:788-795): vcut1..13,
+ * vpar1..13, join1..4, pcut1..3, ppar1..3, acludge, rcludge. */
+void orc_pot_init_iron_carbon(orc_pot *p, const double *iron_in);
+/* get_supported_potential_range's rmax: 6 (fetransmetal), 5.4 (iron_carbon) */
+double orc_pot_range(const orc_pot *p);
 /* potential_set_lookup with the default 5000-point tables (call after orc_pot_init). */
 int  orc_pot_build_lookup(orc_pot *p, int nlookup);
 void orc_pot_free(orc_pot *p);
@@ -61,6 +76,30 @@ double orc_vee_src(const orc_pot *p, double r);
 double orc_dvee_src(const orc_pot *p, double r);
 double orc_phi_src(const orc_pot *p, double r);
 double orc_dphi_src(const orc_pot *p, double r);
+
+/* ---------------- initial state (test inputs, as moldin makes them) ---------------- */
+/* RANMAR generator of src/random_m.F90:73-143 (rand1 / rstart) */
+typedef struct orc_rng {
+  double u[97], c, cd, cm;
+  int iu, ju;
+} orc_rng;
+void orc_rstart(orc_rng *g, int iw, int ix, int iy, int iz);
+double orc_rand1(orc_rng *g);
+/* read_system's replication loop (src/io_m.F90:216-245): nbasis basis atoms
+ * (fractional in the unit volume) repeated nt[0] x nt[1] x nt[2] times, each
+ * coordinate displaced by (rand1()-0.5)*dsp/b0(k,k) with b0 the whole box;
+ * writes nbasis*nt[0]*nt[1]*nt[2] positions in the reference's atom order. */
+void orc_replicate(orc_rng *g, int nbasis, const double *xb, const double *yb, const double *zb,
+                   const int *nt, const double *b0, double dsp, double *x0, double *y0, double *z0);
+/* setvel (src/thermostat_m.F90:191-306) for ivol < 1 or ivol == 4 (atom
+ * velocities only; x1 = v dt / L).  deltat in model units; rqke =
+ * 1.5 bk temprq (nm - 1) as init_thermostat_m sets it.  Returns ke. */
+double orc_setvel(orc_rng *g, int n, const int *z, const double *mass, const double *b0,
+                  double deltat, double rqke, double *x1, double *y1, double *z1);
+/* 1.5 * bk * temprq * (nm - 1) with bk of src/constants_m.F90 */
+double orc_rqke(double temprq, int nm);
+/* Boltzmann constant in eV/K (src/constants_m.F90) */
+double orc_bk(void);
 
 /* ---------------- system ---------------- */
 typedef struct orc_sys orc_sys;
@@ -85,6 +124,9 @@ void orc_embed(orc_sys *s);
 int  orc_force(orc_sys *s);
 /* force() pieces without list maintenance (list must exist) */
 void orc_force_pairs(orc_sys *s);
+/* energy_calc (src/dynamics_m.F90:420-470): en_atom = F(rho_i) + half of each
+ * pair energy, over the current half list; needs rho/afrho from force() */
+void orc_energy_calc(orc_sys *s, double *en_atom);
 void orc_kinten(orc_sys *s);
 /* moldin.F90:446-477: initial force, tgid and accelerations */
 int  orc_md_init(orc_sys *s);

@@ -22,6 +22,10 @@ _lib = None
 
 class OrcPot(C.Structure):
     _fields_ = [
+        ("material", C.c_int),
+        ("vcut", C.c_double * 13), ("vpar", C.c_double * 13), ("join", C.c_double * 4),
+        ("pcut", C.c_double * 3), ("ppar", C.c_double * 3),
+        ("acludge", C.c_double), ("rcludge", C.c_double), ("rx", C.c_double),
         ("nv", C.c_int), ("np", C.c_int),
         ("av", C.c_double * 8), ("rv", C.c_double * 8),
         ("ap", C.c_double * 4), ("rp", C.c_double * 4),
@@ -46,6 +50,9 @@ def lib():
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
         L.orc_pot_init.argtypes = [C.POINTER(OrcPot), C.c_int, dp, dp, C.c_int, dp, dp]
+        L.orc_pot_init_iron_carbon.argtypes = [C.POINTER(OrcPot), dp]
+        L.orc_pot_range.argtypes = [C.POINTER(OrcPot)]
+        L.orc_pot_range.restype = C.c_double
         L.orc_pot_build_lookup.argtypes = [C.POINTER(OrcPot), C.c_int]
         L.orc_pot_free.argtypes = [C.POINTER(OrcPot)]
         for nm in ("orc_vee", "orc_dvee", "orc_phi", "orc_dphi", "orc_emb", "orc_demb",
@@ -82,6 +89,18 @@ def lib():
         L.orc_bruteforce_pairs.restype = C.c_long
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_eval_cell.argtypes = [C.POINTER(OrcPot), C.c_int, dp, dp, C.c_double, dp, dp, dp, dp]
+        L.orc_energy_calc.argtypes = [C.c_void_p, dp]
+        L.orc_rstart.argtypes = [C.POINTER(OrcRng), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_rand1.argtypes = [C.POINTER(OrcRng)]
+        L.orc_rand1.restype = C.c_double
+        L.orc_replicate.argtypes = [C.POINTER(OrcRng), C.c_int, dp, dp, dp, ip, dp, C.c_double,
+                                    dp, dp, dp]
+        L.orc_setvel.argtypes = [C.POINTER(OrcRng), C.c_int, ip, dp, dp, C.c_double, C.c_double,
+                                 dp, dp, dp]
+        L.orc_setvel.restype = C.c_double
+        L.orc_rqke.argtypes = [C.c_double, C.c_int]
+        L.orc_rqke.restype = C.c_double
+        L.orc_bk.restype = C.c_double
         _lib = L
     return _lib
 
@@ -94,15 +113,30 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
-class Potential:
-    """Fe-Fe potential of the oracle (set_up_m constants computed at init)."""
+class OrcRng(C.Structure):
+    _fields_ = [("u", C.c_double * 97), ("c", C.c_double), ("cd", C.c_double), ("cm", C.c_double),
+                ("iu", C.c_int), ("ju", C.c_int)]
 
-    def __init__(self, coeffs=None, lookup: bool = False, nlookup: int = 5000):
-        c = coeffs if coeffs is not None else FE_POTENTIAL
-        self._keep = [np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp")]
-        av, rv, ap, rp = self._keep
+
+class Potential:
+    """Fe-Fe potential of the oracle.  material "fetransmetal" (set_up_m
+    constants computed at init, knot coefficients from coeffs) or
+    "iron_carbon" (the module's coefficients, or the 38 iron.in values)."""
+
+    def __init__(self, coeffs=None, lookup: bool = False, nlookup: int = 5000,
+                 material: str = "fetransmetal", iron_in=None):
         self.s = OrcPot()
-        lib().orc_pot_init(C.byref(self.s), len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp))
+        self.material = material
+        if material == "iron_carbon":
+            self._keep = [np.ascontiguousarray(iron_in, dtype=np.float64)] if iron_in is not None else []
+            lib().orc_pot_init_iron_carbon(C.byref(self.s), _dp(self._keep[0]) if self._keep else None)
+        elif material == "fetransmetal":
+            c = coeffs if coeffs is not None else FE_POTENTIAL
+            self._keep = [np.ascontiguousarray(c[k], dtype=np.float64) for k in ("av", "rv", "ap", "rp")]
+            av, rv, ap, rp = self._keep
+            lib().orc_pot_init(C.byref(self.s), len(av), _dp(av), _dp(rv), len(ap), _dp(ap), _dp(rp))
+        else:
+            raise ValueError(material)
         if lookup:
             lib().orc_pot_build_lookup(C.byref(self.s), nlookup)
 
@@ -206,9 +240,56 @@ class OracleSystem:
         lib().orc_get_ic(self.h, _ip(out))
         return out
 
+    def energy_calc(self) -> np.ndarray:
+        """en_atom of energy_calc (needs afrho of a previous force())."""
+        out = np.empty(self.n)
+        lib().orc_energy_calc(self.h, _dp(out))
+        return out
+
     def forces_cart(self) -> np.ndarray:
         f = np.stack([self.array("fx"), self.array("fy"), self.array("fz")], axis=1)
         return f @ self.tensor("b0").T
+
+
+class Ranmar:
+    """The reference's RANMAR stream (rstart / rand1)."""
+
+    def __init__(self, i=97, j=33, k=21, l=82):
+        self.g = OrcRng()
+        lib().orc_rstart(C.byref(self.g), i, j, k, l)
+
+    def rand1(self, count: int = 1) -> np.ndarray:
+        return np.array([lib().orc_rand1(C.byref(self.g)) for _ in range(count)])
+
+
+def bk() -> float:
+    return lib().orc_bk()
+
+
+def moldin_new_run(basis_frac, basis_z, basis_mass, nt, cell_b0, dsp: float, temprq: float,
+                   deltat_fs: float):
+    """Initial state of a new run as moldin makes it: rstart(97,33,21,82),
+    read_system's replication (with its 3 draws per atom) and setvel.
+    Returns (b0 of the whole box, s (n,3), x1 (n,3), z, mass)."""
+    g = Ranmar()
+    nb = len(basis_z)
+    nt = np.ascontiguousarray(nt, dtype=np.int32)
+    b0 = np.asarray(cell_b0, dtype=np.float64) * nt[None, :]   # b0(:,i) *= nt(i)
+    bf = np.asfortranarray(b0).ravel(order="F").copy()
+    xb = np.ascontiguousarray(np.asarray(basis_frac, dtype=np.float64).T)
+    n = nb * int(np.prod(nt))
+    x0, y0, z0 = np.empty(n), np.empty(n), np.empty(n)
+    lib().orc_replicate(C.byref(g.g), nb, _dp(xb[0]), _dp(xb[1]), _dp(xb[2]), _ip(nt), _dp(bf), dsp,
+                        _dp(x0), _dp(y0), _dp(z0))
+    z = np.ascontiguousarray(np.tile(np.asarray(basis_z, dtype=np.int32), n // nb))
+    mass = np.ascontiguousarray(np.tile(np.asarray(basis_mass, dtype=np.float64), n // nb))
+    x1, y1, z1 = np.zeros(n), np.zeros(n), np.zeros(n)
+    # deltat in model units: fs * fs_to_time as read_params converts it
+    from moldy_amd import lattice
+    dt = deltat_fs * lattice.FS_TO_TIME
+    lib().orc_setvel(C.byref(g.g), n, _ip(z), _dp(mass), _dp(bf), dt, lib().orc_rqke(temprq, n),
+                     _dp(x1), _dp(y1), _dp(z1))
+    return b0, np.stack([x0, y0, z0], axis=1), np.stack([x1, y1, z1], axis=1), z, mass
 
 
 def bruteforce_pairs(b0, s, rcut: float, ivol: int = 1) -> np.ndarray:

@@ -9,13 +9,22 @@
 ! accelerations) and :752-791 (MD step with iverlet=1).  set_up is called
 ! BEFORE potential_set_lookup so that lookup mode is usable with the
 ! fetransmetal potential (moldin calls them the other way round).
+! Built with -DFETRANSMETAL (fetransmetal module) or -DIRON_CARBON
+! (iron_carbon module: one species, Fe, as read_system's generic branch sets it).
 !
 ! usage: ref_driver <input.bin> <output.bin>     (stream-unformatted, little endian)
 ! input : int32 n, ivol, ntcm, nsteps, uselookup, nnbrs, flags
-!         real64 deltat_fs, rpad, press_gpa, boxmass, b0(3,3)
+!         real64 deltat_fs, rpad, press_gpa, boxmass, temprq, b0(3,3)
 !         real64 x0(n),y0(n),z0(n),x1(n),y1(n),z1(n),mass(n); int32 z(n)
 !         int32 ngrid; real64 rgrid(ngrid); int32 nrho; real64 rhogrid(nrho)
 ! flags : bit 0 = timing only (no dumps, prints wall-clock seconds of the MD loop)
+!         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
+!                 (moldin.F90:175), the 3 draws per atom of read_system
+!                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 2 = extra record: per step pe, ke, vol, box kinetic energy
+!                 (moldin.F90:768-774), b0(3,3); after the run en_atom (energy_calc)
+! Compiled against the drop-in modules (-DMDK_DROPIN), bits 1 and 2 are not
+! available: random_m, setvel and scalev are outside the replaced hot path.
 !
 program ref_driver
   use constants_m
@@ -30,14 +39,17 @@ program ref_driver
   use thermostat_m
   use dynamics_m
   use quench_m
+#ifndef MDK_DROPIN
+  use random_m
+#endif
   implicit none
 
   character(len=1024) :: inpath, outpath
   integer :: n, ivol, ntcm, nsteps, uselk, nnbrs_in, flags
   integer :: ngrid, nrho, i, k, istep, ierror, c0, c1, crate
   real(kind_wp) :: dt_fs, rpad_in, press_gpa, boxmass, b0in(3,3)
-  real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3
-  real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:)
+  real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke
+  real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:), xtrace(:,:)
   type(simparameters) :: sp
   logical :: timing
 
@@ -46,7 +58,7 @@ program ref_driver
 
   open(10, file=trim(inpath), access='stream', form='unformatted', status='old')
   read(10) n, ivol, ntcm, nsteps, uselk, nnbrs_in, flags
-  read(10) dt_fs, rpad_in, press_gpa, boxmass
+  read(10) dt_fs, rpad_in, press_gpa, boxmass, temprq_in
   read(10) b0in
   timing = iand(flags, 1) /= 0
 
@@ -61,9 +73,13 @@ program ref_driver
   sp%rpad = rpad_in
   sp%press = press_gpa*gpa_to_press
   sp%rcut = 0.0d0
-  sp%temprq = 300.0d0
+  sp%temprq = temprq_in
   sp%ntc = 0
+#ifdef IRON_CARBON
+  sp%nspec = 1
+#else
   sp%nspec = 24
+#endif
   sp%uselookup = (uselk /= 0)
   sp%bmass = boxmass
   sp%bmass = sp%nm*sp%bmass
@@ -87,6 +103,17 @@ program ref_driver
   close(10)
   x2 = 0.0d0; y2 = 0.0d0; z2 = 0.0d0
 
+#ifdef IRON_CARBON
+  ! species bookkeeping of read_system's generic branch (io_m.F90:119-125): Fe only
+  ispec(1) = 26
+  do i = 1, n
+     if (atomic_number(i) == 26) then
+        atomic_index(i) = 1
+     else
+        atomic_index(i) = 0
+     end if
+  end do
+#else
   ! species bookkeeping of read_system's FETRANSMETAL branch (io_m.F90:133-163)
   ispec(1:24) = (/ 22,23,24,25,26,27,28,29, 40,41,42,43,44,45,46,47, 72,73,74,75,76,77,78,79 /)
   do i = 1, n
@@ -96,12 +123,15 @@ program ref_driver
         atomic_index(i) = 0
      end if
   end do
+#endif
 
   call get_potential(sp%nspec, ispec(1:sp%nspec), rmaxpot, ierror)
   sp = get_params()
   if (rmaxpot > sp%rcut) sp%rcut = rmaxpot
   call set_params(sp)
+#ifndef IRON_CARBON
   call set_up
+#endif
   if (sp%uselookup) then
      call potential_set_lookup
   else
@@ -127,6 +157,17 @@ program ref_driver
         del2(i) = 0.0
      end if
   end do
+
+#ifndef MDK_DROPIN
+  if (iand(flags, 2) /= 0) then
+     call rstart(97, 33, 21, 82)
+     do i = 1, 3*n
+        rdum = rand1()
+     end do
+     call setvel
+     call scalev
+  end if
+#endif
 
   call linkup
   call force
@@ -163,7 +204,7 @@ program ref_driver
      write(20) (demb(rhogrid(k),26), k = 1, nrho)
   end if
 
-  allocate(trace(6, max(nsteps,1)))
+  allocate(trace(6, max(nsteps,1)), xtrace(13, max(nsteps,1)))
   call system_clock(c0, crate)
   do istep = 1, nsteps
      call velocityverlet(sp%deltat)
@@ -176,6 +217,17 @@ program ref_driver
      trace(5, istep) = vol
      sp = get_params()
      trace(6, istep) = real(sp%ntc, kind_wp)
+     bke = 0.0d0
+     do i = 1, 3
+        do k = 1, 3
+           bke = bke + (b1(i,k)**2)/(sp%bdel2)
+        end do
+     end do
+     xtrace(1, istep) = pe
+     xtrace(2, istep) = ke
+     xtrace(3, istep) = vol
+     xtrace(4, istep) = bke
+     xtrace(5:13, istep) = reshape(b0, (/ 9 /))
      if (sp%ntc == 0) call relink
   end do
   call system_clock(c1)
@@ -187,6 +239,14 @@ program ref_driver
      write(20) trace
      write(20) x0, y0, z0, x1, y1, z1, x2, y2, z2, fx, fy, fz, rho, dafrho
      write(20) b0, b1, b2, tp, fb, tgid, tk
+#ifndef MDK_DROPIN
+     if (iand(flags, 4) /= 0) then
+        call energy_calc
+        write(20) xtrace
+        write(20) en_atom
+        write(20) x1, y1, z1
+     end if
+#endif
      close(20)
   end if
 end program ref_driver

@@ -19,15 +19,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 DRIVER_OMP = os.path.join(HERE, "_ref", "ref_driver_omp")
+DRIVER_FEC = os.path.join(HERE, "_ref", "ref_driver_fec")  # iron_carbon material
 POT_FILE = os.path.join(ROOT, "moldy_amd", "data", "pot_026_026.in")
 
 
 def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press, boxmass,
-                rgrid=(), rhogrid=()):
+                rgrid=(), rhogrid=(), temprq=300.0):
     n = sysdef.n
     with open(path, "wb") as fh:
         fh.write(struct.pack("<7i", n, ivol, ntcm, nsteps, uselookup, nnbrs, flags))
-        fh.write(struct.pack("<4d", sysdef.deltat_fs, rpad, press, boxmass))
+        fh.write(struct.pack("<5d", sysdef.deltat_fs, rpad, press, boxmass, temprq))
         fh.write(np.asarray(sysdef.b0, dtype="<f8").ravel(order="F").tobytes())
         for a in (sysdef.s[:, 0], sysdef.s[:, 1], sysdef.s[:, 2], sysdef.x1[:, 0],
                   sysdef.x1[:, 1], sysdef.x1[:, 2], sysdef.mass):
@@ -41,13 +42,13 @@ def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad,
 
 def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3, press=0.0,
         boxmass=50.0, flags=0, rgrid=(), rhogrid=(), driver=DRIVER, pot_file=POT_FILE,
-        threads=None, timeout=None):
+        threads=None, timeout=None, temprq=300.0):
     """Runs the driver; returns (stdout, output bytes or None)."""
     with tempfile.TemporaryDirectory() as td:
         os.symlink(pot_file, os.path.join(td, "pot_026_026.in"))
         inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(inp, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press,
-                    boxmass, rgrid, rhogrid)
+                    boxmass, rgrid, rhogrid, temprq)
         env = dict(os.environ)
         if threads is not None:
             env["OMP_NUM_THREADS"] = str(threads)

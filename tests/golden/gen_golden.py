@@ -1,8 +1,11 @@
 """Generates the golden fixtures under tests/golden/ from the REFERENCE.
 
-Runs oracle/_ref/ref_driver (the reference's own Fortran modules, built from
+Runs oracle/_ref/ref_driver and ref_driver_fec (the reference's own Fortran
+modules with the fetransmetal or the iron_carbon material, built from
 /root/reference/src by oracle/build_ref.sh) on seeded synthetic inputs and
-stores inputs + outputs as small .npz fixtures (no pickles).  Run in the
+stores inputs + outputs as small .npz fixtures (no pickles).  "bcc_iron"
+extracts the numbers printed by the reference's own shipped test
+(tests/reference/bcc_iron/output66.txt, system.out) with their inputs.  Run in the
 build container only (it needs /root/reference for pot_026_026.in and the
 built driver):
 
@@ -23,11 +26,20 @@ from moldy_amd import lattice  # noqa: E402
 
 REF_POT = "/root/reference/src/potentials/pot_026_026.in"
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+DRIVER_FEC = os.path.join(ROOT, "oracle", "_ref", "ref_driver_fec")
 OUT = os.path.dirname(os.path.abspath(__file__))
+BCC_IRON = "/root/reference/tests/reference/bcc_iron"
 
 
-def r_grid() -> np.ndarray:
+def r_grid(material: str = "fetransmetal") -> np.ndarray:
     base = np.linspace(0.05, 6.5, 1291)
+    if material == "iron_carbon":
+        # Biersack/join boundary 1.0, join/knot boundary REAL(4) 2.05, knots, table end 5.4
+        special = [1.0, float(np.float32(2.05)), 2.05, 5.4, 5.399999]
+        knots = [5.3, 4.7, 4.2, 3.7, 3.3, 3.0, 2.8, 2.7, 2.6, 2.5, 2.4, 2.3, 2.2, 3.2]
+        for k in knots + [1.0, float(np.float32(2.05))]:
+            special += [np.nextafter(k, 0.0), np.nextafter(k, 10.0), k - 1e-9, k + 1e-9]
+        return np.concatenate([base, np.array(special)])
     c1, c2 = float(np.float32(0.9)), float(np.float32(1.9))
     knots = [3.38247, 3.296475, 3.09582, 2.837835, 2.665845, 2.4824606625, 3.72645, 3.4398]
     special = [c1, c2, 1.8202052203699783, 6.0, 5.999999]
@@ -36,8 +48,13 @@ def r_grid() -> np.ndarray:
     return np.concatenate([base, np.array(special)])
 
 
-def rho_grid() -> np.ndarray:
-    return np.concatenate([np.linspace(0.0, 60.0, 601), [1e-41, 1e-40, -1.0, 9999.0, 10000.0, 12000.0]])
+def rho_grid(material: str = "fetransmetal") -> np.ndarray:
+    g = np.concatenate([np.linspace(0.0, 60.0, 601), [1e-41, 1e-40, -1.0, 9999.0, 10000.0, 12000.0]])
+    if material == "iron_carbon":
+        # embedding cludge threshold 28.5 and the rho cap 257
+        g = np.concatenate([g, np.linspace(60.0, 400.0, 171), [28.5, np.nextafter(28.5, 99.0), 257.0,
+                                                               np.nextafter(257.0, 999.0)]])
+    return g
 
 
 class Reader:
@@ -57,9 +74,10 @@ class Reader:
 
 
 def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
-                  press=0.0, boxmass=50.0, flags=0, driver=DRIVER, pot_file=REF_POT):
+                  press=0.0, boxmass=50.0, flags=0, driver=DRIVER, pot_file=REF_POT,
+                  material="fetransmetal"):
     import refrun
-    rg, rhog = r_grid(), rho_grid()
+    rg, rhog = r_grid(material), rho_grid(material)
     n = sysdef.n
     _, buf = refrun.run(sysdef, ivol=ivol, ntcm=ntcm, nsteps=nsteps, uselookup=uselookup,
                         nnbrs=nnbrs, rpad=rpad, press=press, boxmass=boxmass, flags=flags,
@@ -115,6 +133,18 @@ CASES = {
     # hot dense liquid-like state: close approaches reach the Biersack/join region
     "hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=5),
                  dict(ivol=1, ntcm=5, nsteps=10, uselookup=0)),
+    # iron_carbon material (ref_driver_fec): PR NPT at 0.1 GPa on a sheared box, exact mode
+    "fec_npt_1024": (dict(nc=8, a=2.87, sigma=0.05, temp=600.0, deltat_fs=1.0, seed=31,
+                          strain=np.array([[0.008, 0.003, -0.002], [0.003, -0.004, 0.001],
+                                           [-0.002, 0.001, 0.002]])),
+                     dict(ivol=0, ntcm=10, nsteps=12, uselookup=0, press=0.1, boxmass=50.0,
+                          material="iron_carbon")),
+    # iron_carbon lookup tables (5000 points on [0, 5.4] A), NVE
+    "fec_lookup_2000": (dict(nc=10, a=2.87, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=41),
+                        dict(ivol=1, ntcm=10, nsteps=3, uselookup=1, material="iron_carbon")),
+    # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
+    "fec_hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=43),
+                     dict(ivol=1, ntcm=5, nsteps=10, uselookup=0, material="iron_carbon")),
 }
 
 KEEP_FIRST = ["rho0", "afrho0", "dafrho0", "fx0", "fy0", "fz0", "x20", "y20", "z20"]
@@ -123,14 +153,93 @@ KEEP_TENS = ["tp0", "tk0", "tc0", "fb0", "tgid0", "b20", "b0_f", "b1_f", "b2_f",
              "tgid_f", "tk_f"]
 
 
-def main():
+def _printed(tok: str):
+    """A number as printed and half a unit in its last printed digit."""
+    t = tok.upper().replace("D", "E")
+    mant, _, ex = t.partition("E")
+    dec = len(mant.split(".")[1]) if "." in mant else 0
+    return float(t), 0.5 * 10.0 ** (-dec + (int(ex) if ex else 0))
+
+
+def bcc_iron_fixture():
+    """The reference's own shipped test tests/reference/bcc_iron: its inputs
+    (params.in, 25^3.in) and what output66.txt and system.out print after 10
+    steps, each value with half a unit of its last printed digit.  system.out
+    is sampled every 25th atom (1250 atoms) with the full-precision box."""
+    import re
+    txt = open(os.path.join(BCC_IRON, "output66.txt")).read()
+    fx = {}
+
+    def grab(label, count, after=None):
+        pos = txt.index(after) if after else 0
+        m = re.search(re.escape(label) + r"((?:\s*[-+]?[0-9.]+(?:[EeDd][-+]?[0-9]+)?){%d})" % count,
+                      txt[pos:])
+        toks = m.group(1).split()
+        vals = [_printed(t) for t in toks]
+        return np.array([v for v, _ in vals]), np.array([h for _, h in vals])
+
+    block = txt[txt.index("RUNNING AVERAGES AT STEP NO."):]
+    for line in block.splitlines():
+        m = re.match(r"^\s*(PE|KE|TE|T|H|TH|VOL):\s*(\S+)\s+A\w*:\s*(\S+)\s+\+/-\s*(\S+)", line)
+        if not m or ("o66_" + m.group(1).lower()) in fx:
+            continue
+        key = m.group(1).lower()
+        for pre, tok in (("", m.group(2)), ("a", m.group(3)), ("d", m.group(4))):
+            v, h = _printed(tok)
+            fx["o66_" + pre + key], fx["o66_" + pre + key + "_tol"] = np.float64(v), np.float64(h)
+    st, sth = [], []
+    for i in (1, 2, 3):
+        v, h = grab("STRESS I= %d" % i, 3)
+        st.append(v); sth.append(h)
+    fx["o66_stress"], fx["o66_stress_tol"] = np.array(st), np.array(sth)
+    mean, inst = [], []
+    for i in (1, 2, 3):
+        m = re.search(r"I=\s+%d MEAN VALUE:\s+(\S+)\s+(\S+)\s+(\S+)\s+INSTANTANEOUS VALUE:\s+(\S+)\s+(\S+)\s+(\S+)" % i, txt)
+        mean.append([_printed(t) for t in m.groups()[:3]])
+        inst.append([_printed(t) for t in m.groups()[3:]])
+    # MEAN/INSTANTANEOUS rows print b0(:,i), column i of the box
+    fx["o66_b0_mean"] = np.array([[v for v, _ in r] for r in mean]).T
+    fx["o66_b0_mean_tol"] = np.array([[h for _, h in r] for r in mean]).T
+    fx["o66_b0"] = np.array([[v for v, _ in r] for r in inst]).T
+    fx["o66_b0_tol"] = np.array([[h for _, h in r] for r in inst]).T
+    lines = open(os.path.join(BCC_IRON, "system.out")).read().splitlines()
+    n = int(lines[0].split()[0])
+    fx["sys_b0"] = np.array([[float(t.replace("D", "E")) for t in lines[2 + i].split()] for i in range(3)]).T
+    idx = np.arange(0, n, 25)
+    rows = [lines[5 + i].split() for i in idx]
+    fx["sys_index"] = idx.astype(np.int32)
+    for k, nm in enumerate(("x0", "y0", "z0")):
+        vals = [_printed(r[k]) for r in rows]
+        fx["sys_" + nm] = np.array([v for v, _ in vals])
+        fx["sys_" + nm + "_tol"] = np.array([h for _, h in vals])
+    vals = [_printed(r[5]) for r in rows]
+    fx["sys_en"] = np.array([v for v, _ in vals])
+    fx["sys_en_tol"] = np.array([h for _, h in vals])
+    # inputs (params.in, 25^3.in)
+    fx.update(n=np.int64(n), nt=np.array([25, 25, 25], dtype=np.int32), cell=np.diag([2.870] * 3),
+              basis=np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]), basis_z=np.array([26, 26], dtype=np.int32),
+              basis_mass=np.array([55.847, 55.847]), dsp=np.float64(0.0), temprq=np.float64(600.0),
+              press_gpa=np.float64(0.1), boxmass=np.float64(50.0), deltat_fs=np.float64(1.0),
+              rpad=np.float64(1.0), nsteps=np.int64(10), ntcm=np.int64(2000), ivol=np.int64(0),
+              uselookup=np.int64(1), nnbrs=np.int64(150))
+    path = os.path.join(OUT, "bcc_iron_golden.npz")
+    np.savez_compressed(path, **fx)
+    print(f"bcc_iron: {len(idx)} sampled atoms -> {path} ({os.path.getsize(path) // 1024} KiB)")
+
+
+def main(names=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     if not os.path.exists(DRIVER):
         raise SystemExit("build the reference driver first: oracle/build_ref.sh")
+    if names and "bcc_iron" in names:
+        bcc_iron_fixture()
     for name, (skw, rkw) in CASES.items():
+        if names and name not in names:
+            continue
         sd = lattice.make_system(**skw)
-        d = run_reference(sd, **rkw)
+        drv = DRIVER_FEC if rkw.get("material") == "iron_carbon" else DRIVER
+        d = run_reference(sd, driver=drv, **rkw)
         fx = {"b0": sd.b0, "s": sd.s, "x1": sd.x1, "mass": sd.mass, "z": sd.z,
               "deltat_fs": np.float64(sd.deltat_fs)}
         for k, v in rkw.items():
@@ -142,7 +251,7 @@ def main():
         fx["numn0"] = d["numn0"].astype(np.int16)
         fx["npairs0"] = np.int64(len(d["pairs0"]))
         fx["pairs0_sha256"] = np.array(po.pairset_digest(d["pairs0"]))
-        if name in ("c0_nve_2000", "lookup_2000"):
+        if name in ("c0_nve_2000", "lookup_2000", "fec_npt_1024", "fec_lookup_2000"):
             for k in ("rgrid", "rhogrid", "vee", "dvee", "phi", "dphi", "emb", "demb"):
                 fx[k] = d[k]
         path = os.path.join(OUT, name + ".npz")
@@ -152,4 +261,5 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    # no arguments: every case; otherwise the named cases ("bcc_iron" for the shipped test)
+    main(sys.argv[1:] or None)

@@ -15,11 +15,16 @@ from conftest import golden_system, load_golden, run_kwargs
 from moldy_amd import lattice
 from moldy_amd.potential import FE_POTENTIAL
 
-CASES = ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024"]
+CASES = ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
+         "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024"]
+
+
+def _material(F):
+    return str(F["run_material"]) if "run_material" in F.files else "fetransmetal"
 
 
 def _oracle_for(oracle, F):
-    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])))
+    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])), material=_material(F))
     S = oracle.OracleSystem(golden_system(F), pot, **run_kwargs(F))
     return pot, S
 
@@ -55,10 +60,10 @@ def test_oracle_bit_exact_vs_reference(oracle, case):
         np.testing.assert_array_equal(S.tensor(nm), F[nm + "_f"], err_msg=nm)
 
 
-@pytest.mark.parametrize("case", ["c0_nve_2000", "lookup_2000"])
+@pytest.mark.parametrize("case", ["c0_nve_2000", "lookup_2000", "fec_npt_1024", "fec_lookup_2000"])
 def test_potential_functions_bit_exact_vs_reference(oracle, case):
     F = load_golden(case)
-    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])))
+    pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])), material=_material(F))
     for fn in ("vee", "dvee", "phi", "dphi"):
         np.testing.assert_array_equal(pot.f(fn, F["rgrid"]), F[fn], err_msg=fn)
     for fn in ("emb", "demb"):
@@ -326,3 +331,122 @@ def test_eval_cell_matches_force_path(oracle, kind):
     np.testing.assert_allclose(rho, S.array("rho"), rtol=1e-12)
     tp = S.tensor("tp")
     np.testing.assert_allclose(w @ np.linalg.inv(sd.b0).T, tp, rtol=0, atol=1e-10 * np.abs(tp).max())
+
+
+# ---------------------------------------------------------------------------
+# iron_carbon material (src/iron_carbon.F90)
+
+def test_fec_hot_fixture_reaches_join(oracle):
+    F = load_golden("fec_hot_1024")
+    pr = oracle.bruteforce_pairs(F["b0"], F["s"], float(np.float32(2.05)))
+    assert len(pr) > 0
+
+
+FEC_V = (np.array([5.3, 4.7, 4.2, 3.7, 3.3, 3.0, 2.8, 2.7, 2.6, 2.5, 2.4, 2.3, 2.2]),
+         np.array([-0.0030458824556234, -0.058531821042271, 0.35018615891957, -1.0260416933564,
+                   2.657740612828, -2.3194358924605, 0.80656414937789, -0.77361294129713,
+                   4.2099676494795, -2.4989799053251, 2.2077118733936, 15.738054058489,
+                   -27.444805994228]))
+FEC_P = (np.array([4.2, 3.2, 2.4]), np.array([0.47193527075943, -0.01471074009883, 11.68685940797]))
+FEC_JOIN = np.array([7.411957380630116, -0.6409989885339922, -2.6049191728604733, 0.6263733940718166])
+
+
+def test_fec_closed_forms(oracle):
+    """Knot sums, the exp-cubic join, the screened Coulomb core and the
+    embedding polynomial, evaluated independently with numpy."""
+    pot = oracle.Potential(material="iron_carbon")
+    r = np.linspace(2.06, 6.0, 400)
+    d = np.maximum(FEC_V[0][None, :] - r[:, None], 0.0)
+    np.testing.assert_allclose(pot.f("vee", r), (FEC_V[1] * d ** 3).sum(1), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(pot.f("dvee", r), -(3 * FEC_V[1] * d ** 2).sum(1), rtol=1e-12, atol=1e-13)
+    r = np.linspace(0.5, 6.0, 500)
+    d = np.maximum(FEC_P[0][None, :] - r[:, None], 0.0)
+    np.testing.assert_allclose(pot.f("phi", r), (FEC_P[1] * d ** 3).sum(1), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(pot.f("dphi", r), -(3 * FEC_P[1] * d ** 2).sum(1), rtol=1e-12, atol=1e-13)
+    assert np.all(pot.f("vee", np.linspace(5.3, 6.0, 50)) == 0.0)
+    assert np.all(pot.f("phi", np.linspace(4.2, 6.0, 50)) == 0.0)
+    r = np.linspace(1.0, 2.05, 100)
+    np.testing.assert_allclose(pot.f("vee", r), np.exp(np.polyval(FEC_JOIN[::-1], r)), rtol=1e-13)
+    # Biersack-Ziegler with Z = 26: 26^2 e^2/(4 pi eps0) = 9734.16 eV A, screening
+    # length 0.88534 a0 0.8854... folded into the exponents
+    r = np.linspace(0.2, 0.99, 50)
+    ex = np.array([-28.616172142695042, -8.42656844064423, -3.6029549238411973, -1.8028188449897875])
+    am = np.array([0.1818, 0.5099, 0.2802, 0.02817])
+    v = 9734.159682000001 / r * (am[None, :] * np.exp(ex[None, :] * r[:, None])).sum(1)
+    np.testing.assert_allclose(pot.f("vee", r), v, rtol=1e-13)
+    np.testing.assert_allclose(am.sum(), 1.0, rtol=1e-3)  # screening function phi(0) ~ 1
+    rho = np.linspace(0.01, 250.0, 300)
+    f = -np.sqrt(rho) - 6.7314115586063e-4 * rho ** 2 + 7.6514905604792e-8 * rho ** 4
+    np.testing.assert_allclose(pot.f("emb", rho), f, rtol=1e-13)
+    df = -0.5 / np.sqrt(rho) - 2 * 6.7314115586063e-4 * rho + 4 * 7.6514905604792e-8 * rho ** 3
+    np.testing.assert_allclose(pot.f("demb", rho), df, rtol=1e-12)
+    # rho cap: constant above 257, derivative zero there and at rho <= 0
+    assert np.all(pot.f("emb", [258.0, 300.0, 1e4]) == pot.f("emb", [257.0])[0] * 0 + pot.f("emb", [1e4])[0])
+    assert np.all(pot.f("demb", [0.0, -1.0, 257.5, 1e4]) == 0.0)
+
+
+def test_fec_derivatives_match_finite_differences(oracle):
+    pot = oracle.Potential(material="iron_carbon")
+    h = 1e-6
+    for lo, hi in ((0.3, 0.99), (1.01, 2.04), (2.06, 5.29)):
+        r = np.linspace(lo, hi, 120)
+        for f, df in (("vee", "dvee"), ("phi", "dphi")):
+            fd = (pot.f(f, r + h) - pot.f(f, r - h)) / (2 * h)
+            np.testing.assert_allclose(pot.f(df, r), fd, rtol=2e-6, atol=2e-6, err_msg=f)
+    rho = np.linspace(0.5, 250.0, 200)
+    fd = (pot.f("emb", rho + h) - pot.f("emb", rho - h)) / (2 * h)
+    np.testing.assert_allclose(pot.f("demb", rho), fd, rtol=1e-6, atol=1e-7)
+
+
+def test_fec_lookup_nodes_are_exact_values(oracle):
+    """Table nodes r_i = 5.4 i / 5000 hold the exact functions; between nodes
+    the lookup is the chord (src/lookup_m.F90:161-203, 307-346)."""
+    ex = oracle.Potential(material="iron_carbon")
+    lk = oracle.Potential(material="iron_carbon", lookup=True)
+    i = np.arange(1, 5000, 7)
+    r = 0.0 + i * 5.4 * (1.0 / 5000)
+    for fn in ("vee", "dvee", "phi", "dphi"):
+        # at a node the interpolation weight is 0 or 1 up to the rounding of r
+        np.testing.assert_allclose(lk.f(fn, r), ex.f(fn, r), rtol=1e-13, atol=1e-15, err_msg=fn)
+    rm = r[:-1] + 0.4 * (5.4 / 5000)
+    for fn in ("vee", "phi"):
+        a, b = ex.f(fn, r[:-1]), ex.f(fn, r[:-1] + 5.4 / 5000)
+        np.testing.assert_allclose(lk.f(fn, rm), a + 0.4 * (b - a), rtol=1e-9, atol=1e-12)
+
+
+def _bcc_iron_oracle(oracle, F):
+    import bcc_iron_case as bc
+    sd = bc.initial_state(F)
+    pot = oracle.Potential(material="iron_carbon", lookup=bool(int(F["uselookup"])))
+    S = oracle.OracleSystem(sd, pot, ivol=int(F["ivol"]), rpad=float(F["rpad"]), nnbrs=int(F["nnbrs"]),
+                            ntcm=int(F["ntcm"]), press_gpa=float(F["press_gpa"]),
+                            boxmass=float(F["boxmass"]))
+    S.call("linkup")
+    assert S.call("md_init") == 0
+    steps = []
+    for _ in range(int(F["nsteps"])):
+        assert S.call("md_step") == 0
+        steps.append((S.scalar("pe"), S.scalar("ke"), S.scalar("vol"), S.scalar("boxke"), S.tensor("b0")))
+    return sd, S, steps
+
+
+def test_bcc_iron_reference_test_reproduced_by_oracle(oracle):
+    """The reference's shipped golden test (tests/reference/bcc_iron): the
+    oracle, started from the state moldin builds, prints what output66.txt and
+    system.out print after 10 steps, to every printed digit."""
+    import bcc_iron_case as bc
+    F = bc.load()
+    sd, S, steps = _bcc_iron_oracle(oracle, F)
+    got = bc.summarize(steps, S.tensor("tp"), S.tensor("b0"), S.scalar("press"), sd.n, oracle.bk())
+    assert bc.check(F, got) == []
+    en = S.energy_calc()
+    assert bc.check_system_out(F, S.tensor("b0"), S.array("x0"), S.array("y0"), S.array("z0"), en) == []
+
+
+def test_ranmar_known_sequence(oracle):
+    """RANMAR (Marsaglia-Zaman-Tsang) with the seeds of its published test,
+    rstart(12,34,56,78): draws 20001..20006 times 2^24 are the known integers."""
+    g = oracle.Ranmar(12, 34, 56, 78)
+    g.rand1(20000)
+    got = g.rand1(6) * 4096.0 * 4096.0
+    np.testing.assert_array_equal(got, [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0])
