@@ -93,10 +93,23 @@ const char *mdk_last_error(void);
  * (Biersack join, phi flat cap) are derived on the host exactly as set_up_m. */
 int mdk_set_potential(void *h, int nv, const double *av, const double *rv, int np,
                       const double *ap, const double *rp);
-/* potential_set_lookup: nlookup-point linear tables on [0,6] A and rho in [0,1e4];
- * nlookup <= 0 selects exact evaluation (potential_set_exact). */
+/* Selects the iron_carbon material instead (src/iron_carbon.F90, built into
+ * the reference with -DIRON_CARBON, src/potential_m.F90:49-51): its Fe-Fe pair
+ * function (13-knot cubic sum above 2.05 A, exp-cubic join, screened Coulomb
+ * core below 1 A), density function (3 knots) and embedding
+ * -rho^1/2 - c2 rho^2 + c4 rho^4.  iron_in == NULL keeps the module's
+ * coefficients; otherwise 38 values in the order check_supported_atomic_numbers
+ * reads "iron.in" (src/iron_carbon.F90:785-795): vcut1..13, vpar1..13,
+ * join1..4, pcut1..3, ppar1..3, acludge, rcludge.  FP64 form only (MDK_ERR_BAD_ARG
+ * for a handle created with MDK_PREC_FP32); Fe atoms only (the carbon
+ * functions are not built).  Replaces get_potential/check_supported_atomic_numbers
+ * of that build (src/potential_m.F90:656-673, src/iron_carbon.F90:745-803). */
+int mdk_set_iron_carbon(void *h, const double *iron_in);
+/* potential_set_lookup: nlookup-point linear tables on [0, rmax] A (rmax 6 for
+ * fetransmetal, 5.4 for iron_carbon) and rho in [0,1e4]; nlookup <= 0 selects
+ * exact evaluation (potential_set_exact). */
 int mdk_set_lookup(void *h, int nlookup);
-/* out[0..7] = bier_cut1, bier_cut2, spline_coeff(1:4), phi_cut(1:2) */
+/* out[0..7] = bier_cut1, bier_cut2, spline_coeff(1:4), phi_cut(1:2) (fetransmetal only) */
 int mdk_potential_constants(void *h, double *out);
 /* Evaluate potential_m's vee,dvee,phi,dphi (which=0..3) at r[k], or emb,demb
  * (which=4,5) at rho[k], on the device, k < count. */
@@ -105,6 +118,7 @@ int mdk_eval_potential(void *h, int which, const double *x, double *y, int count
 /* State transfer (ownership stays with the caller; arrays in reference order). */
 int mdk_upload(void *h, int field, const double *data);
 int mdk_download(void *h, int field, double *data);
+/* atomic numbers: 26 (Fe) or 0 (vacancy); others give MDK_ERR_BAD_ARG */
 int mdk_upload_species(void *h, const int *atomic_number);
 int mdk_set_tensor(void *h, int which, const double *t9);
 int mdk_get_tensor(void *h, int which, double *t9);
@@ -135,6 +149,11 @@ int mdk_force(void *h);
 int mdk_rhoset(void *h);
 int mdk_embed(void *h, double *pe_embed);
 int mdk_kinten(void *h);
+/* energy_calc (src/dynamics_m.F90:410-470): en_atom[i] = F(rho_i) + half of the
+ * exact pair energy with each neighbour (the per-atom energies system.out
+ * prints), in reference atom order.  Needs the list and densities of a
+ * previous force evaluation (MDK_ERR_STATE otherwise). */
+int mdk_energy_calc(void *h, double *en_atom);
 /* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
 int mdk_md_init(void *h);
 /* velocityverlet(dt) (src/quench_m.F90:562-624), including force */

@@ -155,6 +155,9 @@ struct mdk_ctx {
   bool pot_set;
   double *tabs;
   hostpot::Knots knots;
+  int material;            // MDK_MAT_*: fetransmetal (knots) or iron_carbon (fec)
+  hostpot::FecCoef fec;
+  double r_int_exact;      // interaction range of the exact functions
   // reference list threshold: r < rnear  <=>  r^2 < rnear2_exact (see near_threshold)
   double rnear2_exact;
   double r_list, skin;
@@ -1525,17 +1528,56 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_kick(const double *p
   }
 }
 
+// energy_calc (src/dynamics_m.F90:410-470): en_i = F(rho_i) + 1/2 sum_j V(r_ij)
+// over atom i's full list (each pair once from each end, as the reference's
+// half list adds V/2 to both ends).  The pair function is the exact one even
+// when the force path uses tables: the reference calls vee_src there.
+template <int POTM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_energy_calc(const double *atoms, const double4 *__restrict__ pos4,
+                                                           int n, const int *__restrict__ nbr,
+                                                           const int *__restrict__ numn, const BoxState *bs,
+                                                           PackGeo pg, PotDev P, double *en) {
+  __shared__ double S[32][3];
+  shift_table(bs->b0, pg, S);
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = pos4[i];
+  double s = 0.0;
+  for (int k = 0; k < numn[i]; k++) {
+    const unsigned e = (unsigned)nbr[nbr_at(k, i, n)];
+    const double4 pj = pos4[e & NBR_IDX_MASK];
+    double dx, dy, dz, rx, ry, rz;
+    const double r = sqrt(pair_sep<double>(S, pi, pj, (int)(e >> NBR_CODE_SHIFT), dx, dy, dz, rx, ry, rz));
+    double v, dv;
+    mdk_pair_vdv<double, POTM>(P, r, v, dv);
+    s += v / 2.0;
+  }
+  en[i] = atoms[(size_t)F_AFRHO * n + i] + s;
+}
+
 // potential evaluation for mdk_eval_potential
 __global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int cnt) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
   double r = x[k], v = 0.0, dv = 0.0;
-  const bool lk = P.use_lookup != 0;
+  const bool lk = P.use_lookup != 0, fec = P.material == MDK_MAT_IRON_CARBON;
   switch (which) {
-    case 0: if (lk) mdk_pair_vdv<double, POT_TABLE>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = v; break;
-    case 1: if (lk) mdk_pair_vdv<double, POT_TABLE>(P, r, v, dv); else mdk_pair_vdv<double>(P, r, v, dv); y[k] = dv; break;
-    case 2: y[k] = lk ? mdk_pair_phi<double, POT_TABLE>(P, r) : mdk_pair_phi<double>(P, r); break;
-    case 3: y[k] = lk ? mdk_pair_dphi<double, POT_TABLE>(P, r) : mdk_pair_dphi<double>(P, r); break;
+    case 0:
+    case 1:
+      if (lk) mdk_pair_vdv<double, POT_TABLE>(P, r, v, dv);
+      else if (fec) mdk_pair_vdv<double, POT_FEC>(P, r, v, dv);
+      else mdk_pair_vdv<double>(P, r, v, dv);
+      y[k] = which == 0 ? v : dv;
+      break;
+    case 2:
+      y[k] = lk ? mdk_pair_phi<double, POT_TABLE>(P, r)
+                : (fec ? mdk_pair_phi<double, POT_FEC>(P, r) : mdk_pair_phi<double>(P, r));
+      break;
+    case 3:
+      y[k] = lk ? mdk_pair_dphi<double, POT_TABLE>(P, r)
+                : (fec ? mdk_pair_dphi<double, POT_FEC>(P, r) : mdk_pair_dphi<double>(P, r));
+      break;
     case 4: mdk_embed_f(P, r, v, dv); y[k] = v; break;
     case 5: mdk_embed_f(P, r, v, dv); y[k] = dv; break;
   }
@@ -1829,9 +1871,11 @@ static void force_dispatch(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   }
   force_dispatch_p<T, MC, POTM, false>(c, ea, eb);
 }
-// lookup tables, the Fe-Fe knot counts, or any knot count
+// lookup tables, iron_carbon (FP64 only: set-up refuses the FP32 form), the
+// fetransmetal Fe-Fe knot counts, or any knot count
 static int pot_mode(const mdk_ctx *c) {
   if (c->pot.use_lookup) return POT_TABLE;
+  if (c->material == MDK_MAT_IRON_CARBON) return POT_FEC;
   return (c->pot.nv == MDK_FE_NV && c->pot.np == MDK_FE_NP) ? POT_FE : POT_KNOTS;
 }
 template <typename T>
@@ -1839,6 +1883,9 @@ static void density_pot(mdk_ctx *c, int do_embed) {
   switch (pot_mode(c)) {
     case POT_TABLE: density_dispatch<T, POT_TABLE>(c, do_embed); break;
     case POT_FE: density_dispatch<T, POT_FE>(c, do_embed); break;
+    case POT_FEC:
+      if constexpr (sizeof(T) == 8) density_dispatch<T, POT_FEC>(c, do_embed);
+      break;
     default: density_dispatch<T, POT_KNOTS>(c, do_embed); break;
   }
 }
@@ -1847,6 +1894,9 @@ static void force_pot(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   switch (pot_mode(c)) {
     case POT_TABLE: force_dispatch<T, MC, POT_TABLE>(c, ea, eb); break;
     case POT_FE: force_dispatch<T, MC, POT_FE>(c, ea, eb); break;
+    case POT_FEC:
+      if constexpr (sizeof(T) == 8) force_dispatch<T, MC, POT_FEC>(c, ea, eb);
+      break;
     default: force_dispatch<T, MC, POT_KNOTS>(c, ea, eb); break;
   }
 }
@@ -2155,6 +2205,27 @@ extern "C" int mdk_set_potential(void *h, int nv, const double *av, const double
   PotDev &P = c->pot;
   const int keep_lookup = P.use_lookup, keep_n = P.nlookup;
   const double rint = hostpot::fill_potdev(P, k);
+  c->material = MDK_MAT_FETRANSMETAL;
+  c->r_int_exact = rint;
+  c->r_list = rint + c->skin;
+  c->pot_set = true;
+  c->list_valid = false;
+  if (keep_lookup) return mdk_set_lookup(h, keep_n > 0 ? keep_n : 5000);
+  return MDK_OK;
+}
+
+extern "C" int mdk_set_iron_carbon(void *h, const double *iron_in) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (c->p.precision == MDK_PREC_FP32)
+    return set_err(MDK_ERR_BAD_ARG, "the FP32 form is built for the fetransmetal potential");
+  PotDev &P = c->pot;
+  const int keep_lookup = P.use_lookup, keep_n = P.nlookup;
+  hostpot::fec_coef(c->fec, iron_in);
+  memset(&c->knots, 0, sizeof(c->knots));
+  c->material = MDK_MAT_IRON_CARBON;
+  const double rint = hostpot::fill_potdev_fec(P, c->fec);
+  c->r_int_exact = rint;
   c->r_list = rint + c->skin;
   c->pot_set = true;
   c->list_valid = false;
@@ -2168,12 +2239,19 @@ extern "C" int mdk_set_lookup(void *h, int nlookup) {
   PotDev &P = c->pot;
   if (nlookup <= 0) {
     P.use_lookup = 0;
+    P.r_int = c->r_int_exact;
+    c->r_list = P.r_int + c->skin;
+    c->list_valid = false;
     return MDK_OK;
   }
-  const hostpot::Knots &k = c->knots;
-  // src/lookup_m.F90:161-251 with the ranges of fetransmetal (:589-593) and potential_m (:185)
+  hostpot::Material mat;
+  mat.id = c->material;
+  mat.k = c->knots;
+  mat.f = c->fec;
+  // src/lookup_m.F90:161-251 with the material's range (src/fetransmetal.F90:589-593,
+  // src/iron_carbon.F90:93 through get_supported_potential_range) and potential_m's rho range (:185)
   P.nlookup = nlookup;
-  P.r_min = 0.0; P.r_max = 6.0;
+  P.r_min = 0.0; P.r_max = mat.range();
   P.vp_range = P.r_max - P.r_min;
   P.vp_reciprange = 1.0 / P.vp_range;
   P.vp_recipnlkp = 1.0 / (double)nlookup;
@@ -2185,16 +2263,23 @@ extern "C" int mdk_set_lookup(void *h, int nlookup) {
   P.e_recipstep = (double)nlookup / P.e_range;
   const size_t m = (size_t)nlookup + 1;
   std::vector<double> t(6 * m);
+  size_t last = 0;  // last node where a pair table is nonzero
   for (size_t i = 0; i < m; i++) {
     double r = P.r_min + (double)i * P.vp_range * P.vp_recipnlkp;
-    t[0 * m + i] = hostpot::vee(k, r);
-    t[1 * m + i] = hostpot::dvee(k, r);
-    t[2 * m + i] = hostpot::phi(k, r);
-    t[3 * m + i] = hostpot::dphi(k, r);
+    t[0 * m + i] = mat.vee(r);
+    t[1 * m + i] = mat.dvee(r);
+    t[2 * m + i] = mat.phi(r);
+    t[3 * m + i] = mat.dphi(r);
+    if (t[i] != 0.0 || t[m + i] != 0.0 || t[2 * m + i] != 0.0 || t[3 * m + i] != 0.0) last = i;
     double rh = P.rho_min + (double)i * P.e_range * P.e_recipnlkp;
-    t[4 * m + i] = hostpot::emb(rh);
-    t[5 * m + i] = hostpot::demb(rh);
+    t[4 * m + i] = mat.emb(rh);
+    t[5 * m + i] = mat.demb(rh);
   }
+  // interpolation reaches one node beyond the last nonzero one: the list must
+  // hold those pairs (two nodes of margin for the rounding of the index)
+  P.r_int = std::max(c->r_int_exact, std::min(P.r_max, P.r_min + (double)(last + 2) * P.vp_range * P.vp_recipnlkp));
+  c->r_list = P.r_int + c->skin;
+  c->list_valid = false;
   if (c->tabs) hipFree(c->tabs);
   HIPCK(hipMalloc(&c->tabs, sizeof(double) * 6 * m));
   HIPCK(hipMemcpy(c->tabs, t.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice));
@@ -2207,6 +2292,7 @@ extern "C" int mdk_set_lookup(void *h, int nlookup) {
 extern "C" int mdk_potential_constants(void *h, double *out) {
   mdk_ctx *c = H(h);
   if (!c || !c->pot_set || !out) return set_err(MDK_ERR_STATE, "potential not set");
+  if (c->material != MDK_MAT_FETRANSMETAL) return set_err(MDK_ERR_STATE, "set_up constants belong to fetransmetal");
   out[0] = c->knots.bc1; out[1] = c->knots.bc2;
   for (int i = 0; i < 4; i++) out[2 + i] = c->knots.sc[i];
   out[6] = c->knots.pcr; out[7] = c->knots.pcv;
@@ -2227,6 +2313,30 @@ extern "C" int mdk_eval_potential(void *h, int which, const double *x, double *y
   hipStreamSynchronize(c->stream);
   hipFree(d);
   return rc;
+}
+
+extern "C" int mdk_energy_calc(void *h, double *en_atom) {
+  mdk_ctx *c = H(h);
+  if (!c || !en_atom) return set_err(MDK_ERR_BAD_ARG, "bad energy_calc arguments");
+  if (!c->pot_set || !c->list_valid)
+    return set_err(MDK_ERR_STATE, "energy_calc needs the neighbour list and densities of a force evaluation");
+  const int n = c->n;
+  RCK(ensure_pos4(c));
+  double *en = fld(c->atoms_alt, 1, n), *out = fld(c->atoms_alt, 0, n);
+  const dim3 g(nblocks(n)), b(MDK_BLOCK);
+  const BoxState *bs = c->box + c->cur;
+  if (c->material == MDK_MAT_IRON_CARBON)
+    k_energy_calc<POT_FEC><<<g, b, 0, c->stream>>>(c->atoms, c->pos4, n, c->nbr, c->numn, bs, pack_geo(c), c->pot, en);
+  else if (c->pot.nv == MDK_FE_NV && c->pot.np == MDK_FE_NP)
+    k_energy_calc<POT_FE><<<g, b, 0, c->stream>>>(c->atoms, c->pos4, n, c->nbr, c->numn, bs, pack_geo(c), c->pot, en);
+  else
+    k_energy_calc<POT_KNOTS><<<g, b, 0, c->stream>>>(c->atoms, c->pos4, n, c->nbr, c->numn, bs, pack_geo(c), c->pot, en);
+  RCK(check_launch("k_energy_calc"));
+  k_gather_d<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(out, en, c->slot_of, n);
+  RCK(check_launch("k_gather_d"));
+  HIPCK(hipMemcpyAsync(en_atom, out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
 }
 
 static const int field_map[MDK_NFIELDS] = {F_X0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2,
@@ -2302,6 +2412,9 @@ extern "C" int mdk_upload_species(void *h, const int *zn) {
   mdk_ctx *c = H(h);
   if (!c || !zn) return set_err(MDK_ERR_BAD_ARG, "bad species upload");
   const int n = c->n;
+  // the pair kernels evaluate the Fe-Fe functions: Fe atoms and vacancies only
+  for (int i = 0; i < n; i++)
+    if (zn[i] != 26 && zn[i] != 0) return set_err(MDK_ERR_BAD_ARG, "only Fe (Z=26) and vacancies (Z=0) are supported");
   HIPCK(hipMemcpyAsync(c->zat_alt, zn, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
   k_scatter_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat, c->zat_alt, c->slot_of, n);
   RCK(check_launch("k_scatter_i"));

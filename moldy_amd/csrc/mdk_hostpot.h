@@ -194,5 +194,151 @@ static void grad_consts(const Knots &k, GradDev &D) {
     D.cpr[m] = 3.0 * k.ap[m] * d * d * H;
   }
 }
+
+// ---------------------------------------------------------------------------
+// iron_carbon material, Fe-Fe (src/iron_carbon.F90).  Host functions in the
+// reference's operation order (sums left to right, (xH0*a)*d**3, integer
+// powers as left-to-right products, x**0.5 as sqrt and x**(-0.5) as pow), so
+// that lookup tables built from them are the reference's tables bit for bit.
+struct FecCoef {
+  double vcut[MDK_FEC_NV], vpar[MDK_FEC_NV], join[4], pcut[MDK_FEC_NP], ppar[MDK_FEC_NP];
+  double acl, rcl, rx;
+};
+// the module's coefficients (src/iron_carbon.F90:44-74), or the 38 values of
+// an "iron.in" in the order check_supported_atomic_numbers reads them (:788-795)
+static void fec_coef(FecCoef &f, const double *iron_in) {
+  static const double vcut[MDK_FEC_NV] = {5.3, 4.7, 4.2, 3.7, 3.3, 3.0, 2.8, 2.7, 2.6, 2.5, 2.4, 2.3, 2.2};
+  static const double vpar[MDK_FEC_NV] = {-0.0030458824556234, -0.058531821042271, 0.35018615891957,
+                                          -1.0260416933564,   2.657740612828,     -2.3194358924605,
+                                          0.80656414937789,   -0.77361294129713,  4.2099676494795,
+                                          -2.4989799053251,   2.2077118733936,    15.738054058489,
+                                          -27.444805994228};
+  static const double join[4] = {7.411957380630116, -0.6409989885339922, -2.6049191728604733,
+                                 0.6263733940718166};
+  static const double pcut[MDK_FEC_NP] = {4.2, 3.2, 2.4};
+  static const double ppar[MDK_FEC_NP] = {0.47193527075943, 0.01471074009883, 11.68685940797};
+  const double *src = iron_in;
+  memcpy(f.vcut, src ? src : vcut, sizeof(f.vcut));
+  memcpy(f.vpar, src ? src + 13 : vpar, sizeof(f.vpar));
+  memcpy(f.join, src ? src + 26 : join, sizeof(f.join));
+  memcpy(f.pcut, src ? src + 30 : pcut, sizeof(f.pcut));
+  memcpy(f.ppar, src ? src + 33 : ppar, sizeof(f.ppar));
+  f.acl = src ? src[36] : 0.0;
+  f.rcl = src ? src[37] : 28.5;
+  f.rx = 257.0;
+}
+static const double fec_bfe = 9734.159682000001;
+static const double fec_b[4] = {0.1818, 0.5099, 0.2802, 0.02817};
+static const double fec_bx[4] = {-28.616172142695042, -8.42656844064423, -3.6029549238411973,
+                                 -1.8028188449897875};
+static double fec_vee(const FecCoef &f, double r) {
+  if (r < 1.0)
+    return (fec_bfe / r) * (fec_b[0] * exp(fec_bx[0] * r) + fec_b[1] * exp(fec_bx[1] * r) +
+                            fec_b[2] * exp(fec_bx[2] * r) + fec_b[3] * exp(fec_bx[3] * r));
+  if (r > (double)2.05f) {
+    double s = 0.0;
+    for (int k = 0; k < MDK_FEC_NV; k++) {
+      const double d = f.vcut[k] - r, t = (h_step(d) * f.vpar[k]) * (d * d * d);
+      s = k == 0 ? t : s + t;
+    }
+    return s;
+  }
+  return exp(f.join[0] + f.join[1] * r + f.join[2] * r * r + f.join[3] * r * r * r);
+}
+static double fec_dvee(const FecCoef &f, double r) {
+  if (r < 1.0)
+    return -(fec_bfe / r) * ((1.0 / r - fec_bx[0]) * fec_b[0] * exp(fec_bx[0] * r) +
+                             (1.0 / r - fec_bx[1]) * fec_b[1] * exp(fec_bx[1] * r) +
+                             (1.0 / r - fec_bx[2]) * fec_b[2] * exp(fec_bx[2] * r) +
+                             (1.0 / r - fec_bx[3]) * fec_b[3] * exp(fec_bx[3] * r));
+  if (r > (double)2.05f) {
+    double s = 0.0;
+    for (int k = 0; k < MDK_FEC_NV; k++) {
+      const double d = f.vcut[k] - r, t = ((3.0 * h_step(d)) * f.vpar[k]) * (d * d);
+      s = k == 0 ? t : s + t;
+    }
+    return -s;
+  }
+  const double *j = f.join;
+  return (j[1] + 2.0 * j[2] * r + 3.0 * j[3] * r * r) * exp(j[0] + j[1] * r + j[2] * r * r + j[3] * r * r * r);
+}
+static double fec_phi(const FecCoef &f, double r) {
+  double t[MDK_FEC_NP];
+  for (int k = 0; k < MDK_FEC_NP; k++) {
+    const double d = f.pcut[k] - r;
+    t[k] = (h_step(d) * f.ppar[k]) * (d * d * d);
+  }
+  return t[0] - t[1] + t[2];
+}
+static double fec_dphi(const FecCoef &f, double r) {
+  double t[MDK_FEC_NP];
+  for (int k = 0; k < MDK_FEC_NP; k++) {
+    const double d = f.pcut[k] - r;
+    t[k] = ((3.0 * h_step(d)) * f.ppar[k]) * (d * d);
+  }
+  return -(t[0] - t[1] + t[2]);
+}
+static double fec_emb(const FecCoef &f, double rho) {
+  double e = -1.0 * fabs(sqrt(rho)) - MDK_FEC_C2 * (rho * rho) + MDK_FEC_C4 * (rho * rho * rho * rho);
+  if (rho > f.rcl) {
+    const double d = rho - f.rcl;
+    e = e + f.acl * (d * d * d);
+  }
+  if (rho > f.rx) {
+    const double x = f.rx, d = f.rx - f.rcl;
+    e = -1.0 * fabs(sqrt(x)) - MDK_FEC_C2 * (x * x) + MDK_FEC_C4 * (x * x * x * x) + f.acl * (d * d * d);
+  }
+  return e;
+}
+static double fec_demb(const FecCoef &f, double rho) {
+  if (rho <= 0.0 || rho > f.rx) return 0.0;
+  double d = 0.5 * -1.0 * pow(rho, -0.5) - 2.0 * MDK_FEC_C2 * rho + 4.0 * MDK_FEC_C4 * (rho * rho * rho);
+  if (rho > f.rcl) {
+    const double x = rho - f.rcl;
+    d = d + (double)3.0f * f.acl * (x * x);
+  }
+  return d;
+}
+// device struct for iron_carbon; returns the interaction range (outermost knot)
+static double fill_potdev_fec(PotDev &P, const FecCoef &f) {
+  memset(&P, 0, sizeof(P));
+  P.material = MDK_MAT_IRON_CARBON;
+  double rint = 0.0;
+  for (int k = 0; k < MDK_FEC_NV; k++) {
+    P.fv_r[k] = f.vcut[k];
+    P.fv_a[k] = f.vpar[k];
+    P.fv_a3[k] = (3.0 * 1.0) * f.vpar[k];
+    rint = std::max(rint, f.vcut[k]);
+  }
+  for (int k = 0; k < 4; k++) P.fjoin[k] = f.join[k];
+  for (int k = 0; k < MDK_FEC_NP; k++) {
+    P.fp_r[k] = f.pcut[k];
+    P.fp_a[k] = f.ppar[k];
+    P.fp_a3[k] = (3.0 * 1.0) * f.ppar[k];
+    rint = std::max(rint, f.pcut[k]);
+  }
+  P.f_acl = f.acl;
+  P.f_rcl = f.rcl;
+  P.f_rx = f.rx;
+  P.f_cap = fec_emb(f, f.rx + 1.0);
+  P.r_int = rint;
+  P.r_intf = (float)rint;
+  return rint;
+}
+
+// The material set on a handle, for the host-side table construction.
+struct Material {
+  int id;  // MDK_MAT_*
+  Knots k;
+  FecCoef f;
+  double vee(double r) const { return id == MDK_MAT_IRON_CARBON ? fec_vee(f, r) : hostpot::vee(k, r); }
+  double dvee(double r) const { return id == MDK_MAT_IRON_CARBON ? fec_dvee(f, r) : hostpot::dvee(k, r); }
+  double phi(double r) const { return id == MDK_MAT_IRON_CARBON ? fec_phi(f, r) : hostpot::phi(k, r); }
+  double dphi(double r) const { return id == MDK_MAT_IRON_CARBON ? fec_dphi(f, r) : hostpot::dphi(k, r); }
+  double emb(double x) const { return id == MDK_MAT_IRON_CARBON ? fec_emb(f, x) : hostpot::emb(x); }
+  double demb(double x) const { return id == MDK_MAT_IRON_CARBON ? fec_demb(f, x) : hostpot::demb(x); }
+  // get_supported_potential_range's rmax (src/fetransmetal.F90:589-593, src/iron_carbon.F90:93)
+  double range() const { return id == MDK_MAT_IRON_CARBON ? 5.4 : 6.0; }
+};
 }  // namespace hostpot
 

@@ -1,4 +1,8 @@
-// mdk_pot.h -- Fe-Fe Finnis-Sinclair potential on the device (gfx950).
+// mdk_pot.h -- Fe-Fe Finnis-Sinclair potentials on the device (gfx950).
+//
+// Two material modules of the reference (chosen at compile time there,
+// src/potential_m.F90:49-87; a run-time choice here): fetransmetal and
+// iron_carbon (Fe-Fe functions only; see the iron_carbon section below).
 //
 // Functions of src/fetransmetal.F90 for the Fe-Fe pair (find_coeff factors are
 // exactly 1 for Fe-Fe, :831,937-940):
@@ -17,6 +21,10 @@
 
 #define MDK_MAXV 8
 #define MDK_MAXP 4
+#define MDK_MAT_FETRANSMETAL 0
+#define MDK_MAT_IRON_CARBON 1
+#define MDK_FEC_NV 13
+#define MDK_FEC_NP 3
 
 struct PotDev {
   int nv, np, use_lookup, nlookup;
@@ -41,6 +49,14 @@ struct PotDev {
   // beyond r as (u, c0, c1, c2 | c3, 2 c2, 3 c3, 0), u the upper end
   float vknf[8];
   float vcub[8][8];
+  // material (MDK_MAT_*); iron_carbon Fe-Fe data (src/iron_carbon.F90:44-74):
+  // pair knots, amplitudes and 3x amplitudes of the 13-knot sum, the cubic in
+  // the join's exponent, density knots with the formula's signs (+, -, +),
+  // embedding cludge amplitude/threshold, rho cap and F at the cap
+  int material;
+  double fv_r[MDK_FEC_NV], fv_a[MDK_FEC_NV], fv_a3[MDK_FEC_NV], fjoin[4];
+  double fp_r[MDK_FEC_NP], fp_a[MDK_FEC_NP], fp_a3[MDK_FEC_NP];
+  double f_acl, f_rcl, f_rx, f_cap;
 };
 
 // Knot-parameter sensitivities of the set_up constants (filled on the host):
@@ -135,7 +151,7 @@ __device__ __noinline__ VdV<T> mdk_join(T a1, T a2, T x1, T v1, T x2, T v2, T r)
 // and 2 density knots (POT_FE) and for the padded capacity (POT_KNOTS, any
 // count up to MDK_MAXV/MDK_MAXP: unused knots are zero, so they add nothing
 // for r > 0); POT_TABLE selects the lookup tables.
-enum { POT_TABLE = 0, POT_KNOTS = 1, POT_FE = 2 };
+enum { POT_TABLE = 0, POT_KNOTS = 1, POT_FE = 2, POT_FEC = 3 };
 #define MDK_FE_NV 6
 #define MDK_FE_NP 2
 template <int POTM> struct KnotCount {
@@ -246,6 +262,86 @@ __device__ __forceinline__ void mdk_vee_fe_cubic(const PotDev &P, const float (*
   }
 }
 
+// ---------------------------------------------------------------------------
+// iron_carbon Fe-Fe (POT_FEC, FP64), src/iron_carbon.F90:
+//   r > 2.05 A (REAL(4) literal)  V = sum_13 a_k (r_k - r)^3 H(r_k - r)     (:128-153)
+//   1 <= r <= 2.05 A              V = exp(j0 + j1 r + j2 r^2 + j3 r^3)     (:154-157)
+//   r < 1 A                       V = Biersack-Ziegler, Z = 26 (exponents folded) (:119-126)
+//   phi = A1 (R1-r)^3 H - A2 (R2-r)^3 H + A3 (R3-r)^3 H (no flat cap)      (:439-449)
+//   F(rho) = -rho^1/2 - c2 rho^2 + c4 rho^4 (+ cludge above rcl), constant
+//   above the cap rx; F' = 0 for rho <= 0 or rho > rx                      (:599-668)
+// Sums follow the reference's left-to-right order.
+#define MDK_FEC_C2 6.7314115586063e-4
+#define MDK_FEC_C4 7.6514905604792e-8
+
+// short range (r <= 2.05 A): scalars only, so the struct is never spilled
+template <typename T>
+__device__ __noinline__ VdV<T> mdk_fec_short(T j0, T j1, T j2, T j3, T r) {
+  if (r < T(1)) {
+    const T x1 = T(-28.616172142695042), x2 = T(-8.42656844064423), x3 = T(-3.6029549238411973),
+            x4 = T(-1.8028188449897875);
+    const T e1 = T(0.1818) * mdk_exp<T>(x1 * r), e2 = T(0.5099) * mdk_exp<T>(x2 * r),
+            e3 = T(0.2802) * mdk_exp<T>(x3 * r), e4 = T(0.02817) * mdk_exp<T>(x4 * r);
+    const T zr = T(9734.159682000001) / r, ir = T(1) / r;
+    const T v = zr * (e1 + e2 + e3 + e4);
+    const T dv = -zr * ((ir - x1) * e1 + (ir - x2) * e2 + (ir - x3) * e3 + (ir - x4) * e4);
+    return VdV<T>{v, dv};
+  }
+  const T e = mdk_exp<T>(j0 + j1 * r + j2 * r * r + j3 * r * r * r);
+  return VdV<T>{e, (j1 + T(2) * j2 * r + T(3) * j3 * r * r) * e};
+}
+
+template <typename T>
+__device__ __forceinline__ void mdk_fec_vee(const PotDev &P, T r, T &v, T &dv) {
+  T s = T(0), d1 = T(0);
+#pragma unroll
+  for (int k = 0; k < MDK_FEC_NV; k++) {
+    const T d = mdk_pos<T>((T)P.fv_r[k] - r);
+    const T d2 = d * d;
+    s = s + (T)P.fv_a[k] * (d2 * d);
+    d1 = d1 + (T)P.fv_a3[k] * d2;
+  }
+  v = s;
+  dv = -d1;
+  if (!(r > T((double)2.05f))) {
+    VdV<T> q = mdk_fec_short<T>((T)P.fjoin[0], (T)P.fjoin[1], (T)P.fjoin[2], (T)P.fjoin[3], r);
+    v = q.v;
+    dv = q.dv;
+  }
+}
+template <typename T>
+__device__ __forceinline__ T mdk_fec_phi(const PotDev &P, T r) {
+  T t[MDK_FEC_NP];
+#pragma unroll
+  for (int k = 0; k < MDK_FEC_NP; k++) {
+    const T d = mdk_pos<T>((T)P.fp_r[k] - r);
+    t[k] = (T)P.fp_a[k] * (d * d * d);
+  }
+  return t[0] - t[1] + t[2];
+}
+template <typename T>
+__device__ __forceinline__ T mdk_fec_dphi(const PotDev &P, T r) {
+  T t[MDK_FEC_NP];
+#pragma unroll
+  for (int k = 0; k < MDK_FEC_NP; k++) {
+    const T d = mdk_pos<T>((T)P.fp_r[k] - r);
+    t[k] = (T)P.fp_a3[k] * (d * d);
+  }
+  return -(t[0] - t[1] + t[2]);
+}
+__device__ __forceinline__ void mdk_fec_embed(const PotDev &P, double rho, double &f, double &df) {
+  f = -1.0 * fabs(sqrt(rho)) - MDK_FEC_C2 * (rho * rho) + MDK_FEC_C4 * (rho * rho * rho * rho);
+  df = 0.0;
+  if (rho > 0.0 && !(rho > P.f_rx))
+    df = 0.5 * -1.0 / sqrt(rho) - 2.0 * MDK_FEC_C2 * rho + 4.0 * MDK_FEC_C4 * (rho * rho * rho);
+  if (rho > P.f_rcl) {
+    const double d = rho - P.f_rcl;
+    f = f + P.f_acl * (d * d * d);
+    if (!(rho > P.f_rx)) df = df + 3.0 * P.f_acl * (d * d);
+  }
+  if (rho > P.f_rx) f = P.f_cap;
+}
+
 // Padded-capacity forms (any knot count up to MDK_MAXV / MDK_MAXP).
 template <typename T>
 __device__ __forceinline__ void mdk_vee(const PotDev &P, T r, T &v, T &dv) {
@@ -284,6 +380,8 @@ __device__ __forceinline__ void mdk_pair_vdv(const PotDev &P, T r, T &v, T &dv) 
   if (POTM == POT_TABLE) {
     v = (T)mdk_lk_vp(P, P.tab_v, (double)r);
     dv = (T)mdk_lk_vp(P, P.tab_dv, (double)r);
+  } else if (POTM == POT_FEC) {
+    mdk_fec_vee<T>(P, r, v, dv);
   } else {
     mdk_vee_n<T, KnotCount<POTM>::nv>(P, r, v, dv);
   }
@@ -291,11 +389,13 @@ __device__ __forceinline__ void mdk_pair_vdv(const PotDev &P, T r, T &v, T &dv) 
 template <typename T, int POTM = POT_KNOTS>
 __device__ __forceinline__ T mdk_pair_phi(const PotDev &P, T r) {
   if (POTM == POT_TABLE) return (T)mdk_lk_vp(P, P.tab_p, (double)r);
+  if (POTM == POT_FEC) return mdk_fec_phi<T>(P, r);
   return mdk_phi_n<T, KnotCount<POTM>::np>(P, r);
 }
 template <typename T, int POTM = POT_KNOTS>
 __device__ __forceinline__ T mdk_pair_dphi(const PotDev &P, T r) {
   if (POTM == POT_TABLE) return (T)mdk_lk_vp(P, P.tab_dp, (double)r);
+  if (POTM == POT_FEC) return mdk_fec_dphi<T>(P, r);
   return mdk_dphi_n<T, KnotCount<POTM>::np>(P, r);
 }
 // Embedding F and F' in FP64 (per atom, cheap).
@@ -303,6 +403,8 @@ __device__ __forceinline__ void mdk_embed_f(const PotDev &P, double rho, double 
   if (P.use_lookup) {
     f = mdk_lk_e(P, P.tab_e, rho);
     df = mdk_lk_e(P, P.tab_de, rho);
+  } else if (P.material == MDK_MAT_IRON_CARBON) {
+    mdk_fec_embed(P, rho, f, df);
   } else {
     f = -1.0 * sqrt(rho > 0.0 ? rho : 0.0);
     df = 0.0;

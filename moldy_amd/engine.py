@@ -41,7 +41,12 @@ EXPORTS = [
     "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
+    "mdk_set_iron_carbon", "mdk_energy_calc",
 ]
+
+# get_supported_potential_range's rmax of each material module
+# (src/fetransmetal.F90:589-593, src/iron_carbon.F90:93)
+MATERIAL_RANGE = {"fetransmetal": 6.0, "iron_carbon": 5.4}
 
 
 class MdkParams(C.Structure):
@@ -75,6 +80,8 @@ def load_library(path: str = LIB_PATH):
         "mdk_destroy": [vp],
         "mdk_set_potential": [vp, C.c_int, dp, dp, C.c_int, dp, dp],
         "mdk_set_lookup": [vp, C.c_int],
+        "mdk_set_iron_carbon": [vp, dp],
+        "mdk_energy_calc": [vp, dp],
         "mdk_potential_constants": [vp, dp],
         "mdk_eval_potential": [vp, C.c_int, dp, dp, C.c_int],
         "mdk_upload": [vp, C.c_int, dp],
@@ -154,11 +161,17 @@ class Engine:
                  rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10, press_gpa: float = 0.0,
                  boxmass: float = 50.0, precision: int = PREC_FP64, rebuild: int = 0,
                  skin: float | None = None, max_full: int = 0, coeffs: dict | None = None,
-                 lookup: int = 0):
+                 lookup: int = 0, material: str = "fetransmetal", iron_in=None):
+        """material "fetransmetal" (knot coefficients coeffs, default the Fe-Fe
+        file) or "iron_carbon" (the module's coefficients, or the 38 values of
+        an iron.in given as iron_in)."""
         L = load_library()
         self.n = sysdef.n
         n = self.n
-        rcut = max(rcut, 6.0)                       # moldin.F90:226 with fetransmetal range 6
+        if material not in MATERIAL_RANGE:
+            raise ValueError(f"unknown material {material!r}")
+        self.material = material
+        rcut = max(rcut, MATERIAL_RANGE[material])  # moldin.F90:226 with the material's range
         self.rnear = rcut + rpad
         dt = sysdef.deltat_fs * FS_TO_TIME
         bmass = n * boxmass
@@ -175,8 +188,14 @@ class Engine:
         _check(L.mdk_create(C.byref(h), C.byref(p)))
         self.h = h
         self.L = L
-        c = coeffs if coeffs is not None else FE_POTENTIAL
-        self.set_potential(c)
+        if material == "iron_carbon":
+            self._iron_in = None if iron_in is None else np.ascontiguousarray(iron_in, dtype=np.float64)
+            if self._iron_in is not None and self._iron_in.shape != (38,):
+                raise ValueError("iron_in holds 38 values")
+            _check(L.mdk_set_iron_carbon(h, _dp(self._iron_in) if self._iron_in is not None else None))
+        else:
+            c = coeffs if coeffs is not None else FE_POTENTIAL
+            self.set_potential(c)
         if lookup:
             _check(L.mdk_set_lookup(h, int(lookup)))
         s = np.ascontiguousarray(sysdef.s)
@@ -280,6 +299,12 @@ class Engine:
 
     def md_init(self):
         self.call("md_init")
+
+    def energy_calc(self) -> np.ndarray:
+        """Per-atom energies of energy_calc (reference atom order)."""
+        out = np.empty(self.n)
+        _check(self.L.mdk_energy_calc(self.h, _dp(out)))
+        return out
 
     def velocityverlet(self):
         self.call("velocityverlet")
