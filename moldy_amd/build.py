@@ -4,7 +4,8 @@
 * oracle/liboracle.so           -- the CPU oracle (test infrastructure)
 * oracle/_ref/*                 -- the reference Fortran driver, only when the
                                    reference sources are present (build box)
-* moldy_amd/lib/moldy_fortran   -- Fortran drop-in modules + driver (amdflang)
+* moldy_amd/lib/moldy_fortran   -- Fortran drop-in modules + driver (amdflang);
+                                   lib/fec/ holds the -DIRON_CARBON build
 """
 from __future__ import annotations
 
@@ -66,6 +67,11 @@ def build_fortran() -> str | None:
     _run(["make", "-C", fdir, "FC=" + fc, "dropin",
           "CALLER=" + os.path.join(ROOT, "oracle", "ref_driver.F90"),
           "OUT=" + os.path.join(LIBDIR, "ref_driver_dropin")])
+    # the same with the iron_carbon material (-DIRON_CARBON, lib/fec/)
+    _run(["make", "-C", fdir, "FC=" + fc, "fec"])
+    _run(["make", "-C", fdir, "FC=" + fc, "dropin_fec",
+          "CALLER=" + os.path.join(ROOT, "oracle", "ref_driver.F90"),
+          "OUT=" + os.path.join(LIBDIR, "ref_driver_dropin_fec")])
     return os.path.join(LIBDIR, "libmoldy_fortran.a")
 
 

@@ -1,11 +1,12 @@
 !
 ! hotpath_m.F90 -- drop-in replacements of the reference's hot-path modules
 ! with the same names and argument lists, running on the GPU through the C ABI:
-!   potential_m      src/potential_m.F90:100-118   (fetransmetal Fe-Fe, src/fetransmetal.F90)
+!   potential_m      src/potential_m.F90:100-118   (fetransmetal Fe-Fe, src/fetransmetal.F90;
+!                    with -DIRON_CARBON the iron_carbon Fe-Fe functions, src/iron_carbon.F90)
 !   linkcell_m       src/linkcell_m.F90:52-54
 !   neighbourlist_m  src/neighbourlist_m.F90:52-58
 !   thermostat_m     src/thermostat_m.F90:94-177   (init, kinten)
-!   dynamics_m       src/dynamics_m.F90:69-83      (force, rhoset, update_therm_avs)
+!   dynamics_m       src/dynamics_m.F90:69-83      (force, rhoset, energy_calc, update_therm_avs)
 !   quench_m         src/quench_m.F90:59-63        (velocityverlet)
 !
 module potential_m
@@ -28,6 +29,59 @@ module potential_m
 
 contains
 
+#ifdef IRON_CARBON
+  !> check_supported_atomic_numbers of iron_carbon (src/iron_carbon.F90:745-803):
+  !> range pot_rmax = 5.4 A, or vcut1 when an "iron.in" supplies the 38
+  !> coefficients.  Only Fe (26) is supported here: the device kernels
+  !> evaluate the Fe-Fe functions (ierror = 1 for other atomic numbers).
+  subroutine get_potential(spnum, spna, range, ierror)
+    integer, intent(in) :: spnum
+    integer, intent(in) :: spna(spnum)
+    real(kind_wp), intent(out) :: range
+    integer, intent(out) :: ierror
+    integer :: u, ios, i
+    character(len=30) :: titlefe
+    real(kind_wp) :: c(38)
+    ierror = 0
+    range = 5.4_kind_wp
+    do i = 1, spnum
+       if (spna(i) /= 26) ierror = 1
+    end do
+    c = 0.0_kind_wp
+    open(newunit=u, file="iron.in", status='old', action='read', iostat=ios)
+    if (ios == 0) then
+       read(u, *, iostat=ios) titlefe
+       if (ios == 0) read(u, *, iostat=ios) c(1:7)
+       if (ios == 0) read(u, *, iostat=ios) c(8:13)
+       if (ios == 0) read(u, *, iostat=ios) c(14:20)
+       if (ios == 0) read(u, *, iostat=ios) c(21:26)
+       if (ios == 0) read(u, *, iostat=ios) c(27:30)
+       if (ios == 0) read(u, *, iostat=ios) c(31:33)
+       if (ios == 0) read(u, *, iostat=ios) c(34:36)
+       if (ios == 0) read(u, *, iostat=ios) c(37:38)
+       close(u)
+       if (ios /= 0) then
+          ierror = 2
+          return
+       end if
+       range = c(1)
+       call mdk_dev_set_iron_carbon(c, .true.)
+    else
+       call mdk_dev_set_iron_carbon(c, .false.)
+    end if
+    have_coeffs = .true.
+  end subroutine get_potential
+
+  subroutine get_available_potential_range(rmin, rmax)
+    real(kind_wp), intent(out) :: rmin, rmax
+    rmin = 0.
+    rmax = 5.4
+  end subroutine get_available_potential_range
+
+  !> iron_carbon has no set-up step (src/potential_m.F90:257-261)
+  subroutine set_up
+  end subroutine set_up
+#else
   !> Reads pot_026_026.in (AVTF layout, src/fetransmetal.F90:603-766) for the
   !> Fe-Fe pair; range is the potential's declared 6 A (:589-593).
   subroutine get_potential(spnum, spna, range, ierror)
@@ -68,6 +122,7 @@ contains
     if (.not. have_coeffs) stop 'potential_m: get_potential must run before set_up'
     call mdk_dev_set_potential(nv, av(1:nv), rv(1:nv), np, ap(1:np), rp(1:np))
   end subroutine set_up
+#endif
 
   subroutine potential_set_lookup(nlookup)
     integer, optional :: nlookup
@@ -310,7 +365,7 @@ module dynamics_m
   use mdk_device_m
   implicit none
   private
-  public :: init_dynamics_m, cleanup_dynamics_m, force, rhoset, update_therm_avs
+  public :: init_dynamics_m, cleanup_dynamics_m, force, rhoset, update_therm_avs, energy_calc
   public :: update_neighbourlist, write_energy_forces_stress, dynamics_pull_forces
   real(kind_wp), allocatable, public :: fx(:), fy(:), fz(:)
   real(kind_wp), allocatable, public :: x1_dt2(:), y1_dt2(:), z1_dt2(:)
@@ -356,6 +411,13 @@ contains
     call mdk_check(mdk_rhoset(mdk_dev_handle()), 'mdk_rhoset')
     call mdk_check(mdk_download(mdk_dev_handle(), MDK_RHO, rho_local), 'download rho')
   end subroutine rhoset
+
+  !> energy_calc (src/dynamics_m.F90:410-470): en_atom from the list and
+  !> densities of the last force, with the exact pair function
+  subroutine energy_calc
+    call mdk_dev_push()
+    call mdk_check(mdk_energy_calc(mdk_dev_handle(), en_atom), 'mdk_energy_calc')
+  end subroutine energy_calc
 
   !> update_therm_avs (src/dynamics_m.F90:649-734)
   subroutine update_therm_avs(s1, s2, s3)

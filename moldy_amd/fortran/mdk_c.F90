@@ -46,6 +46,15 @@ module mdk_c
        integer(c_int), value :: nv, np
        real(c_double), intent(in) :: av(*), rv(*), ap(*), rp(*)
      end function mdk_set_potential
+     integer(c_int) function mdk_set_iron_carbon(h, iron_in) bind(C, name="mdk_set_iron_carbon")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, iron_in
+     end function mdk_set_iron_carbon
+     integer(c_int) function mdk_energy_calc(h, en_atom) bind(C, name="mdk_energy_calc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: en_atom(*)
+     end function mdk_energy_calc
      integer(c_int) function mdk_set_lookup(h, nlookup) bind(C, name="mdk_set_lookup")
        import :: c_int, c_ptr
        type(c_ptr), value :: h

@@ -17,12 +17,16 @@ module mdk_device_m
   private
   public :: mdk_dev_create, mdk_dev_destroy, mdk_dev_handle, mdk_dev_ready
   public :: mdk_dev_set_potential, mdk_dev_set_lookup, mdk_dev_push, mdk_dev_pull
+  public :: mdk_dev_set_iron_carbon
   public :: mdk_dev_push_positions
 
   type(c_ptr), save :: handle = c_null_ptr
   logical, save :: pot_pending = .false.
   integer, save :: nv_p = 0, np_p = 0, nlookup_p = 0
   real(c_double), save :: av_p(8), rv_p(8), ap_p(4), rp_p(4)
+  ! iron_carbon material: coefficients from an iron.in, or the built-in ones
+  logical, save :: fec_p = .false., fec_file_p = .false.
+  real(c_double), target, save :: fec_c_p(38)
 
 contains
 
@@ -59,8 +63,16 @@ contains
   end subroutine mdk_dev_destroy
 
   subroutine apply_potential()
-    call mdk_check(mdk_set_potential(handle, int(nv_p, c_int), av_p, rv_p, int(np_p, c_int), ap_p, rp_p), &
-                   'mdk_set_potential')
+    if (fec_p) then
+       if (fec_file_p) then
+          call mdk_check(mdk_set_iron_carbon(handle, c_loc(fec_c_p)), 'mdk_set_iron_carbon')
+       else
+          call mdk_check(mdk_set_iron_carbon(handle, c_null_ptr), 'mdk_set_iron_carbon')
+       end if
+    else
+       call mdk_check(mdk_set_potential(handle, int(nv_p, c_int), av_p, rv_p, int(np_p, c_int), ap_p, rp_p), &
+                      'mdk_set_potential')
+    end if
     call mdk_check(mdk_set_lookup(handle, int(nlookup_p, c_int)), 'mdk_set_lookup')
   end subroutine apply_potential
 
@@ -72,6 +84,17 @@ contains
     pot_pending = .true.
     if (c_associated(handle)) call apply_potential()
   end subroutine mdk_dev_set_potential
+
+  !> iron_carbon material; c = the 38 values of an iron.in when from_file
+  subroutine mdk_dev_set_iron_carbon(c, from_file)
+    real(kind_wp), intent(in) :: c(38)
+    logical, intent(in) :: from_file
+    fec_p = .true.
+    fec_file_p = from_file
+    fec_c_p = c
+    pot_pending = .true.
+    if (c_associated(handle)) call apply_potential()
+  end subroutine mdk_dev_set_iron_carbon
 
   subroutine mdk_dev_set_lookup(nlookup)
     integer, intent(in) :: nlookup

@@ -23,8 +23,8 @@
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
 !         bit 2 = extra record: per step pe, ke, vol, box kinetic energy
 !                 (moldin.F90:768-774), b0(3,3); after the run en_atom (energy_calc)
-! Compiled against the drop-in modules (-DMDK_DROPIN), bits 1 and 2 are not
-! available: random_m, setvel and scalev are outside the replaced hot path.
+! Compiled against the drop-in modules (-DMDK_DROPIN), bit 1 is not available:
+! random_m, setvel and scalev are outside the replaced hot path.
 !
 program ref_driver
   use constants_m
@@ -239,14 +239,12 @@ program ref_driver
      write(20) trace
      write(20) x0, y0, z0, x1, y1, z1, x2, y2, z2, fx, fy, fz, rho, dafrho
      write(20) b0, b1, b2, tp, fb, tgid, tk
-#ifndef MDK_DROPIN
      if (iand(flags, 4) /= 0) then
         call energy_calc
         write(20) xtrace
         write(20) en_atom
         write(20) x1, y1, z1
      end if
-#endif
      close(20)
   end if
 end program ref_driver

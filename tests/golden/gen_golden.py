@@ -202,12 +202,18 @@ def bcc_iron_fixture():
     fx["o66_b0_mean_tol"] = np.array([[h for _, h in r] for r in mean]).T
     fx["o66_b0"] = np.array([[v for v, _ in r] for r in inst]).T
     fx["o66_b0_tol"] = np.array([[h for _, h in r] for r in inst]).T
+    # the printed lines themselves, for the writers: PE..VOL of the averages
+    # block and the six box lines
+    keep = [ln for ln in block.splitlines() if re.match(r"^\s*(PE|KE|TE|T|H|TH|VOL):", ln)][:7]
+    keep += [ln for ln in txt.splitlines() if "MEAN VALUE:" in ln or "INSTANTANEOUS VALUE:" in ln][:6]
+    fx["o66_lines"] = np.array(keep)
     lines = open(os.path.join(BCC_IRON, "system.out")).read().splitlines()
     n = int(lines[0].split()[0])
     fx["sys_b0"] = np.array([[float(t.replace("D", "E")) for t in lines[2 + i].split()] for i in range(3)]).T
     idx = np.arange(0, n, 25)
     rows = [lines[5 + i].split() for i in idx]
     fx["sys_index"] = idx.astype(np.int32)
+    fx["sys_pos_text"] = np.array([lines[5 + i][:42] for i in idx])
     for k, nm in enumerate(("x0", "y0", "z0")):
         vals = [_printed(r[k]) for r in rows]
         fx["sys_" + nm] = np.array([v for v, _ in vals])

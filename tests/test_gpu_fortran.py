@@ -1,7 +1,8 @@
 """The Fortran drop-in: the driver that produced the golden vectors
 (oracle/ref_driver.F90, a caller of the reference's module interfaces) is
 compiled UNCHANGED against moldy_amd/fortran (same module names and argument
-lists, kernels behind the C ABI) and must reproduce the reference's outputs.
+lists, kernels behind the C ABI) and must reproduce the reference's outputs;
+with -DIRON_CARBON on both sides for the iron_carbon fixtures.
 Tolerances as in test_gpu_parity: pair sets exact, forces/densities 1e-10
 relative (norm-wise), energies 1e-12 relative; lookup-mode potential values
 bit-exact."""
@@ -17,6 +18,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 pytestmark = pytest.mark.gpu
 
 DROPIN = os.path.join(ROOT, "moldy_amd", "lib", "ref_driver_dropin")
+DROPIN_FEC = os.path.join(ROOT, "moldy_amd", "lib", "ref_driver_dropin_fec")  # -DIRON_CARBON build
 
 
 def rel(a, b):
@@ -24,17 +26,19 @@ def rel(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / (den if den > 0 else 1.0))
 
 
-@pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024"])
+@pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
+                                  "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024"])
 def test_fortran_dropin_matches_reference(oracle, case):
     import gen_golden
     import refrun
-    if not os.path.exists(DROPIN):
-        pytest.fail("drop-in driver not built (moldy_amd.build.build_fortran)")
     F = load_golden(case)
+    kw = {k[4:]: (str(F[k]) if F[k].dtype.kind == "U" else int(F[k]) if F[k].dtype.kind == "i" else float(F[k]))
+          for k in F.files if k.startswith("run_")}
+    drv = DROPIN_FEC if kw.get("material") == "iron_carbon" else DROPIN
+    if not os.path.exists(drv):
+        pytest.fail("drop-in driver not built (moldy_amd.build.build_fortran)")
     print("[gpu-test] fortran drop-in", case, flush=True)
-    kw = {k[4:]: (int(F[k]) if F[k].dtype.kind == "i" else float(F[k])) for k in F.files
-          if k.startswith("run_")}
-    d = gen_golden.run_reference(golden_system(F), driver=DROPIN, pot_file=refrun.POT_FILE, **kw)
+    d = gen_golden.run_reference(golden_system(F), driver=drv, pot_file=refrun.POT_FILE, **kw)
     assert oracle.pairset_digest(d["pairs0"]) == str(F["pairs0_sha256"])
     for k in ("rho0", "dafrho0", "fx0", "fy0", "fz0", "x20"):
         assert rel(d[k], F[k]) < 1e-10, k
@@ -56,4 +60,5 @@ def test_fortran_dropin_matches_reference(oracle, case):
             if exact_cmp:
                 np.testing.assert_array_equal(d[k], F[k], err_msg=k)
             else:
-                assert rel(d[k][F["rgrid"] > 1.9], F[k][F["rgrid"] > 1.9]) < 1e-13, k
+                knot = F["rgrid"] > (2.05 if kw.get("material") == "iron_carbon" else 1.9)
+                assert rel(d[k][knot], F[k][knot]) < 1e-13, k

@@ -430,17 +430,66 @@ def _bcc_iron_oracle(oracle, F):
     return sd, S, steps
 
 
-def test_bcc_iron_reference_test_reproduced_by_oracle(oracle):
+@pytest.fixture(scope="module")
+def bcc_iron_run(oracle):
+    import bcc_iron_case as bc
+    F = bc.load()
+    sd, S, steps = _bcc_iron_oracle(oracle, F)
+    return F, sd, S, steps
+
+
+def test_bcc_iron_reference_test_reproduced_by_oracle(oracle, bcc_iron_run):
     """The reference's shipped golden test (tests/reference/bcc_iron): the
     oracle, started from the state moldin builds, prints what output66.txt and
     system.out print after 10 steps, to every printed digit."""
     import bcc_iron_case as bc
-    F = bc.load()
-    sd, S, steps = _bcc_iron_oracle(oracle, F)
+    F, sd, S, steps = bcc_iron_run
     got = bc.summarize(steps, S.tensor("tp"), S.tensor("b0"), S.scalar("press"), sd.n, oracle.bk())
     assert bc.check(F, got) == []
     en = S.energy_calc()
     assert bc.check_system_out(F, S.tensor("b0"), S.array("x0"), S.array("y0"), S.array("z0"), en) == []
+
+
+def test_writers_reproduce_printed_lines(oracle, bcc_iron_run, tmp_path):
+    """moldy_amd.io renders the averages block and system.out character for
+    character as the reference printed them in its shipped bcc_iron test
+    (the lines whose FORMAT is unchanged in the current source)."""
+    from moldy_amd import io as mio
+    F, sd, S, steps = bcc_iron_run
+    sums = mio.ThermoSums()
+    press = S.scalar("press")
+    for pe, ke, vol, bke, b0 in steps:
+        te = ke + pe
+        h = te + press * vol
+        sums.update(pe, ke, te, h, h + bke, vol, b0)
+    temp = steps[-1][1] / (1.5 * oracle.bk() * sd.n)
+    text = mio.runavs_text(10, sums, temp, sd.n, oracle.bk(), S.tensor("b0"), S.tensor("tp"), S.tensor("tk"))
+    lines = text.splitlines()
+    for want in F["o66_lines"]:
+        assert str(want) in lines, want
+    assert "TP+TK STRESS I= 1" in text
+    path = tmp_path / "system.out"
+    mio.write_system_out(str(path), S.tensor("b0"), S.array("x0"), S.array("y0"), S.array("z0"), sd.z, sd.mass,
+                         S.energy_calc())
+    out = path.read_text().splitlines()
+    assert int(out[0]) == sd.n and out[1].split() == ["1", "1", "1"]
+    np.testing.assert_allclose(np.array([[float(t) for t in out[2 + i].split()] for i in range(3)]).T,
+                               S.tensor("b0"), rtol=1e-16, atol=0)
+    for i, want in zip(F["sys_index"], F["sys_pos_text"]):
+        assert out[5 + int(i)][:42] == str(want)
+        assert len(out[5 + int(i)]) == 3 * 14 + 1 + 3 + 12 + 13
+
+
+def test_fortran_edit_descriptors():
+    from moldy_amd.io import fortran_e, fortran_f, fortran_g
+    assert fortran_e(-124968.466, 16, 8) == " -0.12496847E+06"
+    assert fortran_e(0.0, 13, 4) == "   0.0000E+00"
+    assert fortran_f(506.20477, 11, 4) == "   506.2048"
+    assert fortran_g(0.4995821, 14, 6) == "  0.499582    "
+    assert fortran_g(55.847, 12, 4) == "   55.85    "
+    assert fortran_g(123456.7, 14, 6) == "   123457.    "
+    assert fortran_g(1234567.0, 14, 6) == "  0.123457E+07"
+    assert fortran_g(-4.59586753e-05, 14, 6) == " -0.459587E-04"
 
 
 def test_ranmar_known_sequence(oracle):
