@@ -51,12 +51,13 @@ enum {
   MDK_FX, MDK_FY, MDK_FZ,       /* forces f (Cartesian force = B0 . f) */
   MDK_RHO, MDK_AFRHO, MDK_DAFRHO,
   MDK_MASS, MDK_DEL2,
+  MDK_X3, MDK_Y3, MDK_Z3,       /* third scaled derivative (predictor-corrector) */
   MDK_NFIELDS
 };
 /* ---- 3x3 tensors for mdk_get_tensor / mdk_set_tensor ---- */
 enum {
   MDK_B0 = 0, MDK_B1, MDK_B2, MDK_TK, MDK_TC, MDK_TG, MDK_TP, MDK_TGINV, MDK_TGDOT,
-  MDK_TGID, MDK_FB, MDK_NTENSORS
+  MDK_TGID, MDK_FB, MDK_B3, MDK_NTENSORS
 };
 /* ---- scalars for mdk_get_scalars (array of MDK_NSCALARS doubles) ---- */
 enum {
@@ -156,6 +157,17 @@ int mdk_kinten(void *h);
 int mdk_energy_calc(void *h, double *en_atom);
 /* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
 int mdk_md_init(void *h);
+/* Integrator of mdk_md_steps (moldin.F90:752-758): 1 velocity Verlet (default),
+ * 0 Gear predictor-corrector (PREDIC, FORCE, CORREC). */
+int mdk_set_integrator(void *h, int iverlet);
+/* predic (src/quench_m.F90:636-671): x0..x2 and the box from the scaled
+ * derivatives up to x3, b3.  Like the reference, it changes nothing until a
+ * correc or velocityverlet has run on the handle: predic reads quench_m's
+ * parameter copy, which only those refresh (:640, :569, :690). */
+int mdk_predic(void *h);
+/* correc (src/quench_m.F90:682-759): Gear corrector with the force of the
+ * last force evaluation, the current tgid and bdel2 fb for the box. */
+int mdk_correc(void *h);
 /* velocityverlet(dt) (src/quench_m.F90:562-624), including force */
 int mdk_velocityverlet(void *h);
 /* update_therm_avs(s1,s2,s3) (src/dynamics_m.F90:649-734) */

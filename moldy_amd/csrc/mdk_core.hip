@@ -68,7 +68,7 @@ extern "C" const char *mdk_last_error(void) { return g_err; }
 // ---------------------------------------------------------------------------
 enum {
   F_X0 = 0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2, F_FX, F_FY, F_FZ,
-  F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2, F_N
+  F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2, F_X3, F_Y3, F_Z3, F_N
 };
 
 struct TimedEv {
@@ -156,6 +156,8 @@ struct mdk_ctx {
   double *tabs;
   hostpot::Knots knots;
   int material;            // MDK_MAT_*: fetransmetal (knots) or iron_carbon (fec)
+  int iverlet;             // md_steps integrator: 1 velocity Verlet, 0 predictor-corrector
+  bool pc_primed;          // a correc or velocityverlet has run (predic is live)
   hostpot::FecCoef fec;
   double r_int_exact;      // interaction range of the exact functions
   // reference list threshold: r < rnear  <=>  r^2 < rnear2_exact (see near_threshold)
@@ -841,6 +843,96 @@ __global__ void k_drift(double *__restrict__ atoms, double4 *__restrict__ pos4, 
     store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
   }
   if (blockIdx.x == 0) drift_box(bin, bout, ivol);
+}
+
+// predic (src/quench_m.F90:636-671): x0 += x1 + x2 + x3, x1 += 2 x2 + 3 x3,
+// x2 += 3 x3, the box likewise for ivol 0 (all) or 4 (z only), packed
+// positions with the predicted box; block 0 writes the predicted box state
+// into bout.  primed == 0 leaves everything as it is (see mdk_predic).
+__device__ __forceinline__ void predict_box(const BoxState *bin, double *b0, double *b1, double *b2, int ivol,
+                                            int primed, int k) {
+#pragma clang fp contract(off)
+  const bool upd = primed && (ivol == 0 || (ivol == 4 && k == 8));
+  b0[k] = upd ? bin->b0[k] + bin->b1[k] + bin->b2[k] + bin->b3[k] : bin->b0[k];
+  b1[k] = upd ? bin->b1[k] + 2.0 * bin->b2[k] + 3.0 * bin->b3[k] : bin->b1[k];
+  b2[k] = upd ? bin->b2[k] + 3.0 * bin->b3[k] : bin->b2[k];
+}
+__global__ void k_predic(double *__restrict__ atoms, double4 *__restrict__ pos4, float4 *__restrict__ p4f,
+                         const int *__restrict__ img, const int *__restrict__ cell, PackGeo pg, int n,
+                         const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol, int primed) {
+#pragma clang fp contract(off)
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double *p = atoms;
+    double x[3];
+    for (int d = 0; d < 3; d++) {
+      double *q0 = p + (size_t)(F_X0 + d) * n, *q1 = p + (size_t)(F_X1 + d) * n, *q2 = p + (size_t)(F_X2 + d) * n;
+      const double *q3 = p + (size_t)(F_X3 + d) * n;
+      double v0 = q0[i];
+      if (primed) {
+        const double v1 = q1[i], v2 = q2[i], v3 = q3[i];
+        v0 = v0 + v1 + v2 + v3;
+        q0[i] = v0;
+        q1[i] = v1 + 2.0 * v2 + 3.0 * v3;
+        q2[i] = v2 + 3.0 * v3;
+      }
+      x[d] = v0;
+    }
+    double b[9], bd1[9], bd2[9];
+    for (int k = 0; k < 9; k++) predict_box(bin, b, bd1, bd2, ivol, primed, k);
+    const double3 r = packed_pos(b, pg, cell, i, x[0] - img[3 * i], x[1] - img[3 * i + 1], x[2] - img[3 * i + 2]);
+    store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
+  }
+  if (blockIdx.x == 0) {  // the whole box state into bout, one field per thread
+    const int nw = (int)(sizeof(BoxState) / sizeof(double));
+    const double *src = (const double *)bin;
+    double *dst = (double *)bout;
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = src[w];
+    __syncthreads();
+    if (threadIdx.x < 9) predict_box(bin, bout->b0, bout->b1, bout->b2, ivol, primed, threadIdx.x);
+  }
+}
+
+// correc (src/quench_m.F90:682-759).  The coefficients are the reference's
+// REAL(4) parameters (no d0 suffix): 1/6, 5/6, 1, 1/3 rounded to float.
+// cc = del2 f - tgid x1 / 2 - x2 per atom (vacancies skipped); the box (ivol
+// < 1: all, 4: z only) with cc = bdel2 fb - b2, in place by nine threads of
+// block 0 (no other thread reads the box's b fields).
+__global__ void k_correc(double *__restrict__ atoms, const int *__restrict__ zat, int n, BoxState *bs, int ivol,
+                         double bdel2, const double *__restrict__ spec) {
+#pragma clang fp contract(off)
+  const double ct0 = (double)0.166666666666666667f, ct1 = (double)0.833333333333333333f;
+  const double ct2 = (double)1.000000000000000000f, ct3 = (double)0.333333333333333333f;
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  if (i < n && zat[i] != 0) {
+    const double *g = bs->tgid;
+    const int z = zat[i];
+    const double d2 = spec ? spec[2 * z + 1] : atoms[(size_t)F_DEL2 * n + i];
+    const double x1 = atoms[(size_t)F_X1 * n + i], y1 = atoms[(size_t)F_Y1 * n + i], z1 = atoms[(size_t)F_Z1 * n + i];
+    const double s[3] = {TT(g, 1, 1) * x1 + TT(g, 1, 2) * y1 + TT(g, 1, 3) * z1,
+                         TT(g, 2, 1) * x1 + TT(g, 2, 2) * y1 + TT(g, 2, 3) * z1,
+                         TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1};
+    for (int d = 0; d < 3; d++) {
+      double *q0 = atoms + (size_t)(F_X0 + d) * n, *q1 = atoms + (size_t)(F_X1 + d) * n;
+      double *q2 = atoms + (size_t)(F_X2 + d) * n, *q3 = atoms + (size_t)(F_X3 + d) * n;
+      const double f = atoms[(size_t)(F_FX + d) * n + i];
+      const double cc = d2 * f - 0.5 * s[d] - q2[i];
+      q0[i] = q0[i] + cc * ct0;
+      q1[i] = q1[i] + cc * ct1;
+      q2[i] = q2[i] + cc * ct2;
+      q3[i] = q3[i] + cc * ct3;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 9) {
+    const int k = threadIdx.x;
+    if (ivol < 1 || (ivol == 4 && k == 8)) {
+      const double cc = bdel2 * bs->fb[k] - bs->b2[k];
+      bs->b0[k] = bs->b0[k] + ct0 * cc;
+      bs->b1[k] = bs->b1[k] + ct1 * cc;
+      bs->b2[k] = bs->b2[k] + ct2 * cc;
+      bs->b3[k] = bs->b3[k] + ct3 * cc;
+    }
+  }
 }
 
 // geometry of pair (i,j): fractional separation (minimum image) and Cartesian
@@ -2037,10 +2129,55 @@ static int lag_status(mdk_ctx *c) {
 // One moldin step.  allow_fuse lets the closing kick also perform the next
 // step's drift, except when relink follows this step (it must see the
 // positions of this step).
-static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool allow_fuse = false) {
+static int launch_predic(mdk_ctx *c) {
+  TimedEv te;
+  tbegin(c, TC_DRIFT, te);
+  const int nxt = 1 - c->cur;
+  k_predic<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c),
+                                                 c->n, c->box + c->cur, c->box + nxt, c->p.ivol,
+                                                 c->pc_primed ? 1 : 0);
+  tend(c, te);
+  c->pos4_valid = true;
+  c->cur = nxt;
+  return check_launch("k_predic");
+}
+
+static int launch_correc(mdk_ctx *c) {
+  TimedEv te;
+  tbegin(c, TC_KICK, te);
+  k_correc<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol,
+                                                 c->p.bdel2, spec_of(c));
+  tend(c, te);
+  c->pos4_valid = false;  // positions and box moved
+  c->pc_primed = true;
+  return check_launch("k_correc");
+}
+
+// One moldin step with iverlet = 0 (src/moldin.F90:754-758): PREDIC, FORCE,
+// CORREC, then update_therm_avs (box part, then the sums with the monitor).
+static int md_step_gear(mdk_ctx *c, double *trace_row_dev) {
   bool skin_due = skin_check(c);
   RCK(lag_status(c));
   long before = c->rebuilds;
+  RCK(launch_predic(c));
+  RCK(force_body(c, FIN_FORCE, skin_due));
+  if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
+  RCK(launch_correc(c));
+  k_therm_box<<<1, 64, 0, c->stream>>>(c->box + c->cur, c->p.ivol);
+  RCK(check_launch("k_therm_box"));
+  int slot = (int)(c->step_counter & 3);
+  RCK(launch_kick(c, 1, trace_row_dev, c->mon_dev + MON_STRIDE * slot));
+  HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
+  c->step_counter++;
+  return MDK_OK;
+}
+
+static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool allow_fuse = false) {
+  if (c->iverlet == 0) return md_step_gear(c, trace_row_dev);
+  bool skin_due = skin_check(c);
+  RCK(lag_status(c));
+  long before = c->rebuilds;
+  c->pc_primed = true;  // velocityverlet refreshes quench_m's parameter copy
   RCK(launch_drift(c));
   RCK(force_body(c, fused_therm ? FIN_VV_THERM : FIN_VV, skin_due));
   if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
@@ -2077,6 +2214,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   new (&c->evpool) std::vector<hipEvent_t>();
   c->p = *p;
   c->n = p->nm;
+  c->iverlet = 1;  // velocity Verlet unless mdk_set_integrator selects the predictor-corrector
   const int n = c->n;
   switch (p->ivol) {
     case 0: case 1: case 4: c->pbc = 7; break;
@@ -2339,8 +2477,9 @@ extern "C" int mdk_energy_calc(void *h, double *en_atom) {
   return MDK_OK;
 }
 
-static const int field_map[MDK_NFIELDS] = {F_X0, F_Y0, F_Z0, F_X1, F_Y1, F_Z1, F_X2, F_Y2, F_Z2,
-                                           F_FX, F_FY, F_FZ, F_RHO, F_AFRHO, F_DAFRHO, F_MASS, F_DEL2};
+static const int field_map[MDK_NFIELDS] = {F_X0,  F_Y0,  F_Z0,   F_X1,     F_Y1,      F_Z1,   F_X2,
+                                           F_Y2,  F_Z2,  F_FX,   F_FY,     F_FZ,      F_RHO,  F_AFRHO,
+                                           F_DAFRHO, F_MASS, F_DEL2, F_X3, F_Y3, F_Z3};
 
 // Species table of (mass, del2) when both are functions of the species.
 static int keep_host_copy(double **dst, const double *src, int n) {
@@ -2438,6 +2577,7 @@ static double *box_field(BoxState &s, int which) {
     case MDK_TGDOT: return s.tgdot;
     case MDK_TGID: return s.tgid;
     case MDK_FB: return s.fb;
+    case MDK_B3: return s.b3;
     default: return nullptr;
   }
 }
@@ -2638,6 +2778,7 @@ extern "C" int mdk_velocityverlet(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
   bool skin_due = skin_check(c);
+  c->pc_primed = true;
   RCK(launch_drift(c));
   RCK(force_body(c, FIN_VV, skin_due));
   // half kick without the thermodynamic sums' box part (tgid stays lagged)
@@ -2649,6 +2790,29 @@ extern "C" int mdk_velocityverlet(void *h) {
   poll_monitor(c);
   RCK(lag_status(c));
   return check_device_status(c);
+}
+
+extern "C" int mdk_set_integrator(void *h, int iverlet) {
+  mdk_ctx *c = H(h);
+  if (!c || (iverlet != 0 && iverlet != 1)) return set_err(MDK_ERR_BAD_ARG, "iverlet must be 0 or 1");
+  c->iverlet = iverlet;
+  return MDK_OK;
+}
+
+extern "C" int mdk_predic(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  RCK(launch_predic(c));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_correc(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  RCK(launch_correc(c));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
 }
 
 extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3) {

@@ -12,7 +12,7 @@
 // copies exist; a step reads copy `cur` in its drift kernel and writes the
 // advanced box into the other copy, which every later kernel of the step uses.
 struct BoxState {
-  double b0[9], b1[9], b2[9];
+  double b0[9], b1[9], b2[9], b3[9];  // b3: box jolt of the predictor-corrector
   double tk[9], tc[9], tg[9], tp[9], tginv[9], tgdot[9], tgid[9], fb[9];
   double tgid_lag[9];   // tgid seen by the velocity half-kick (quench_m.F90:595)
   double binv[9];       // cofactor(b0)/vol used by update_therm_avs (dynamics_m.F90:692-700)

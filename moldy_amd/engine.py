@@ -24,8 +24,8 @@ MDK_ERR_SINGULAR_BOX, MDK_ERR_STATE, MDK_ERR_SKIN, MDK_ERR_NO_DEVICE = 4, 5, 6, 
 PREC_FP64, PREC_FP32 = 0, 1
 
 FIELDS = ["x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2", "fx", "fy", "fz",
-          "rho", "afrho", "dafrho", "mass", "del2"]
-TENSORS = ["b0", "b1", "b2", "tk", "tc", "tg", "tp", "tginv", "tgdot", "tgid", "fb"]
+          "rho", "afrho", "dafrho", "mass", "del2", "x3", "y3", "z3"]
+TENSORS = ["b0", "b1", "b2", "tk", "tc", "tg", "tp", "tginv", "tgdot", "tgid", "fb", "b3"]
 SCALARS = ["pe", "ke", "vol", "s1", "s2", "s3", "te", "h", "th", "boxke", "maxdisp"]
 
 # Every symbol include/moldy_hip.h declares (checked by the CPU test suite).
@@ -41,7 +41,7 @@ EXPORTS = [
     "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
-    "mdk_set_iron_carbon", "mdk_energy_calc",
+    "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_set_integrator", "mdk_predic", "mdk_correc",
 ]
 
 # get_supported_potential_range's rmax of each material module
@@ -82,6 +82,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_set_lookup": [vp, C.c_int],
         "mdk_set_iron_carbon": [vp, dp],
         "mdk_energy_calc": [vp, dp],
+        "mdk_set_integrator": [vp, C.c_int], "mdk_predic": [vp], "mdk_correc": [vp],
         "mdk_potential_constants": [vp, dp],
         "mdk_eval_potential": [vp, C.c_int, dp, dp, C.c_int],
         "mdk_upload": [vp, C.c_int, dp],
@@ -161,10 +162,11 @@ class Engine:
                  rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10, press_gpa: float = 0.0,
                  boxmass: float = 50.0, precision: int = PREC_FP64, rebuild: int = 0,
                  skin: float | None = None, max_full: int = 0, coeffs: dict | None = None,
-                 lookup: int = 0, material: str = "fetransmetal", iron_in=None):
+                 lookup: int = 0, material: str = "fetransmetal", iron_in=None, iverlet: int = 1):
         """material "fetransmetal" (knot coefficients coeffs, default the Fe-Fe
         file) or "iron_carbon" (the module's coefficients, or the 38 values of
-        an iron.in given as iron_in)."""
+        an iron.in given as iron_in).  iverlet: 1 velocity Verlet, 0 Gear
+        predictor-corrector for md_steps."""
         L = load_library()
         self.n = sysdef.n
         n = self.n
@@ -214,6 +216,7 @@ class Engine:
         self.upload("del2", del2)
         _check(L.mdk_upload_species(h, _ip(np.ascontiguousarray(sysdef.z, dtype=np.int32))))
         self.set_tensor("b0", sysdef.b0)
+        _check(L.mdk_set_integrator(h, int(iverlet)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -299,6 +302,12 @@ class Engine:
 
     def md_init(self):
         self.call("md_init")
+
+    def predic(self):
+        self.call("predic")
+
+    def correc(self):
+        self.call("correc")
 
     def energy_calc(self) -> np.ndarray:
         """Per-atom energies of energy_calc (reference atom order)."""

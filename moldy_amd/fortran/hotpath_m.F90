@@ -7,7 +7,7 @@
 !   neighbourlist_m  src/neighbourlist_m.F90:52-58
 !   thermostat_m     src/thermostat_m.F90:94-177   (init, kinten)
 !   dynamics_m       src/dynamics_m.F90:69-83      (force, rhoset, energy_calc, update_therm_avs)
-!   quench_m         src/quench_m.F90:59-63        (velocityverlet)
+!   quench_m         src/quench_m.F90:59-63        (velocityverlet, predic, correc)
 !
 module potential_m
   use iso_c_binding
@@ -467,8 +467,23 @@ module quench_m
   use mdk_device_m
   implicit none
   private
-  public :: velocityverlet
+  public :: velocityverlet, predic, correc
 contains
+  !> predic (src/quench_m.F90:636-671); a no-op until correc or
+  !> velocityverlet has run, as the reference's (see mdk_predic)
+  subroutine predic
+    call mdk_dev_push()
+    call mdk_check(mdk_predic(mdk_dev_handle()), 'mdk_predic')
+    call mdk_dev_pull(.true.)
+  end subroutine predic
+
+  !> correc (src/quench_m.F90:682-759) with the forces of the last force call
+  subroutine correc
+    call mdk_dev_push()
+    call mdk_check(mdk_correc(mdk_dev_handle()), 'mdk_correc')
+    call mdk_dev_pull(.true.)
+  end subroutine correc
+
   !> velocityverlet(dt) (src/quench_m.F90:562-624), force included
   subroutine velocityverlet(dt)
     real(kind_wp) :: dt

@@ -15,9 +15,11 @@ module mdk_c
   integer(c_int), parameter :: MDK_X0 = 0, MDK_Y0 = 1, MDK_Z0 = 2, MDK_X1 = 3, MDK_Y1 = 4, MDK_Z1 = 5
   integer(c_int), parameter :: MDK_X2 = 6, MDK_Y2 = 7, MDK_Z2 = 8, MDK_FX = 9, MDK_FY = 10, MDK_FZ = 11
   integer(c_int), parameter :: MDK_RHO = 12, MDK_AFRHO = 13, MDK_DAFRHO = 14, MDK_MASS = 15, MDK_DEL2 = 16
+  integer(c_int), parameter :: MDK_X3 = 17, MDK_Y3 = 18, MDK_Z3 = 19
   ! tensors
   integer(c_int), parameter :: MDK_B0 = 0, MDK_B1 = 1, MDK_B2 = 2, MDK_TK = 3, MDK_TC = 4, MDK_TG = 5
   integer(c_int), parameter :: MDK_TP = 6, MDK_TGINV = 7, MDK_TGDOT = 8, MDK_TGID = 9, MDK_FB = 10
+  integer(c_int), parameter :: MDK_B3 = 11
   ! scalars
   integer, parameter :: MDK_S_PE = 1, MDK_S_KE = 2, MDK_S_VOL = 3, MDK_S_S1 = 4, MDK_S_S2 = 5
   integer, parameter :: MDK_S_S3 = 6, MDK_NSCALARS = 11
@@ -50,6 +52,14 @@ module mdk_c
        import :: c_int, c_ptr
        type(c_ptr), value :: h, iron_in
      end function mdk_set_iron_carbon
+     integer(c_int) function mdk_predic(h) bind(C, name="mdk_predic")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function mdk_predic
+     integer(c_int) function mdk_correc(h) bind(C, name="mdk_correc")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function mdk_correc
      integer(c_int) function mdk_energy_calc(h, en_atom) bind(C, name="mdk_energy_calc")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h

@@ -121,6 +121,9 @@ contains
     call mdk_check(mdk_upload(handle, MDK_X2, x2), 'upload x2')
     call mdk_check(mdk_upload(handle, MDK_Y2, y2), 'upload y2')
     call mdk_check(mdk_upload(handle, MDK_Z2, z2), 'upload z2')
+    call mdk_check(mdk_upload(handle, MDK_X3, x3), 'upload x3')
+    call mdk_check(mdk_upload(handle, MDK_Y3, y3), 'upload y3')
+    call mdk_check(mdk_upload(handle, MDK_Z3, z3), 'upload z3')
     call mdk_check(mdk_upload(handle, MDK_MASS, atomic_mass), 'upload mass')
     call mdk_check(mdk_upload(handle, MDK_DEL2, del2), 'upload del2')
     call mdk_check(mdk_upload(handle, MDK_AFRHO, afrho), 'upload afrho')
@@ -130,6 +133,7 @@ contains
     call mdk_check(mdk_upload_species(handle, z), 'upload species')
     call mdk_check(mdk_set_tensor(handle, MDK_B1, b1), 'set b1')
     call mdk_check(mdk_set_tensor(handle, MDK_B2, b2), 'set b2')
+    call mdk_check(mdk_set_tensor(handle, MDK_B3, b3), 'set b3')
     call mdk_check(mdk_set_tensor(handle, MDK_TGID, tgid), 'set tgid')
     sp = get_params()
     call mdk_check(mdk_set_ntc(handle, int(sp%ntc, c_int)), 'set ntc')
@@ -151,9 +155,13 @@ contains
        call mdk_check(mdk_download(handle, MDK_X2, x2), 'download x2')
        call mdk_check(mdk_download(handle, MDK_Y2, y2), 'download y2')
        call mdk_check(mdk_download(handle, MDK_Z2, z2), 'download z2')
+       call mdk_check(mdk_download(handle, MDK_X3, x3), 'download x3')
+       call mdk_check(mdk_download(handle, MDK_Y3, y3), 'download y3')
+       call mdk_check(mdk_download(handle, MDK_Z3, z3), 'download z3')
        call mdk_check(mdk_get_tensor(handle, MDK_B0, b0), 'get b0')
        call mdk_check(mdk_get_tensor(handle, MDK_B1, b1), 'get b1')
        call mdk_check(mdk_get_tensor(handle, MDK_B2, b2), 'get b2')
+       call mdk_check(mdk_get_tensor(handle, MDK_B3, b3), 'get b3')
     end if
     call mdk_check(mdk_download(handle, MDK_RHO, rho), 'download rho')
     call mdk_check(mdk_download(handle, MDK_AFRHO, afrho), 'download afrho')

@@ -425,12 +425,14 @@ struct orc_sys {
   int n, ivol, nnbrs, ntcm, ntc;
   int nlcx, nlcy, nlcz;
   double deltat, rpad, rcut, rnear, press, bmass, bdel2;
-  double *x0, *y0, *z0, *x1, *y1, *z1, *x2, *y2, *z2;
+  double *x0, *y0, *z0, *x1, *y1, *z1, *x2, *y2, *z2, *x3, *y3, *z3;
   double *x1dt2, *y1dt2, *z1dt2;
+  int iverlet; /* 1 velocity Verlet, 0 Gear predictor-corrector */
+  int pc_primed; /* quench_m's parameter copy is set (by correc or velocityverlet) */
   double *mass, *del2, *rho, *afrho, *dafrho, *fx, *fy, *fz;
   int *zat;
   int *l0, *link, *ic, *numn, *nlist;
-  double b0[9], b1[9], b2[9], tk[9], tc[9], tg[9], tp[9], tginv[9], tgdot[9], tgid[9], fb[9];
+  double b0[9], b1[9], b2[9], b3[9], tk[9], tc[9], tg[9], tp[9], tginv[9], tgdot[9], tgid[9], fb[9];
   double vol, pe, ke, s1, s2, s3, te, h, th, boxke;
   orc_pot pot;
 };
@@ -495,6 +497,8 @@ orc_sys *orc_create(int n, const double *b0, const double *x0, const double *y0,
   s->x0 = dalloc(n); s->y0 = dalloc(n); s->z0 = dalloc(n);
   s->x1 = dalloc(n); s->y1 = dalloc(n); s->z1 = dalloc(n);
   s->x2 = dalloc(n); s->y2 = dalloc(n); s->z2 = dalloc(n);
+  s->x3 = dalloc(n); s->y3 = dalloc(n); s->z3 = dalloc(n);
+  s->iverlet = 1;
   s->x1dt2 = dalloc(n); s->y1dt2 = dalloc(n); s->z1dt2 = dalloc(n);
   s->mass = dalloc(n); s->del2 = dalloc(n);
   s->rho = dalloc(n); s->afrho = dalloc(n); s->dafrho = dalloc(n);
@@ -522,7 +526,7 @@ orc_sys *orc_create(int n, const double *b0, const double *x0, const double *y0,
 
 void orc_destroy(orc_sys *s) {
   if (!s) return;
-  double *d[] = {s->x0, s->y0, s->z0, s->x1, s->y1, s->z1, s->x2, s->y2, s->z2,
+  double *d[] = {s->x0, s->y0, s->z0, s->x1, s->y1, s->z1, s->x2, s->y2, s->z2, s->x3, s->y3, s->z3,
                  s->x1dt2, s->y1dt2, s->z1dt2, s->mass, s->del2, s->rho, s->afrho,
                  s->dafrho, s->fx, s->fy, s->fz};
   for (size_t k = 0; k < sizeof(d) / sizeof(d[0]); k++) free(d[k]);
@@ -905,6 +909,7 @@ int orc_md_init(orc_sys *s) {
 /* velocityverlet, src/quench_m.F90:562-624 */
 static int velocity_verlet(orc_sys *s) {
   const int n = s->n;
+  s->pc_primed = 1; /* simparam=get_params(), :569 */
   for (int i = 0; i < n; i++) {
     s->x1dt2[i] = s->x1[i] + s->x2[i];
     s->y1dt2[i] = s->y1[i] + s->y2[i];
@@ -1004,9 +1009,87 @@ static void update_therm_avs(orc_sys *s) {
   s->s3 = s3;
 }
 
-/* one MD step, src/moldin.F90:752-791 (iverlet=1) */
+/* predic, src/quench_m.F90:636-671.  predic reads quench_m's module copy of
+ * the parameters, which only correc (:690) and velocityverlet (:569) refresh
+ * (its own get_params is commented out, :640) and which moldin never sets in an
+ * MD run (init_quench_m only for quenches, moldin.F90:566-575): until then it
+ * holds nm = 0 and ivol = -1 (src/params_m.F90:81,47), so predic changes
+ * nothing. */
+void orc_predic(orc_sys *s) {
+  if (!s->pc_primed) return;
+  for (int i = 0; i < s->n; i++) {
+    s->x0[i] = s->x0[i] + s->x1[i] + s->x2[i] + s->x3[i];
+    s->y0[i] = s->y0[i] + s->y1[i] + s->y2[i] + s->y3[i];
+    s->z0[i] = s->z0[i] + s->z1[i] + s->z2[i] + s->z3[i];
+    s->x1[i] = s->x1[i] + 2.0 * s->x2[i] + 3.0 * s->x3[i];
+    s->y1[i] = s->y1[i] + 2.0 * s->y2[i] + 3.0 * s->y3[i];
+    s->z1[i] = s->z1[i] + 2.0 * s->z2[i] + 3.0 * s->z3[i];
+    s->x2[i] = s->x2[i] + 3.0 * s->x3[i];
+    s->y2[i] = s->y2[i] + 3.0 * s->y3[i];
+    s->z2[i] = s->z2[i] + 3.0 * s->z3[i];
+  }
+  for (int k = 0; k < 9; k++) {
+    if (s->ivol == 0 || (s->ivol == 4 && k == 8)) {
+      s->b0[k] = s->b0[k] + s->b1[k] + s->b2[k] + s->b3[k];
+      s->b1[k] = s->b1[k] + 2.0 * s->b2[k] + 3.0 * s->b3[k];
+      s->b2[k] = s->b2[k] + 3.0 * s->b3[k];
+    }
+  }
+}
+
+/* correc, src/quench_m.F90:682-759.  The four coefficients are REAL(4)
+ * literals (the parameters carry no d0 suffix): 1/6, 5/6, 1, 1/3 in float. */
+void orc_correc(orc_sys *s) {
+  s->pc_primed = 1;
+  const double ct0 = (double)0.166666666666666667f, ct1 = (double)0.833333333333333333f;
+  const double ct2 = (double)1.000000000000000000f, ct3 = (double)0.333333333333333333f;
+  for (int k = 0; k < 9; k++) {
+    if (s->ivol < 1 || (s->ivol == 4 && k == 8)) {
+      const double cc = s->bdel2 * s->fb[k] - s->b2[k];
+      s->b0[k] = s->b0[k] + ct0 * cc;
+      s->b1[k] = s->b1[k] + ct1 * cc;
+      s->b2[k] = s->b2[k] + ct2 * cc;
+      s->b3[k] = s->b3[k] + ct3 * cc;
+    }
+  }
+  const double *g = s->tgid; /* get_tgid(): the value before this step's update_therm_avs */
+  for (int i = 0; i < s->n; i++) {
+    if (s->zat[i] == 0) continue;
+    const double sxi = T(g, 1, 1) * s->x1[i] + T(g, 1, 2) * s->y1[i] + T(g, 1, 3) * s->z1[i];
+    const double syi = T(g, 2, 1) * s->x1[i] + T(g, 2, 2) * s->y1[i] + T(g, 2, 3) * s->z1[i];
+    const double szi = T(g, 3, 1) * s->x1[i] + T(g, 3, 2) * s->y1[i] + T(g, 3, 3) * s->z1[i];
+    const double ccx = s->del2[i] * s->fx[i] - 0.5 * sxi - s->x2[i];
+    const double ccy = s->del2[i] * s->fy[i] - 0.5 * syi - s->y2[i];
+    const double ccz = s->del2[i] * s->fz[i] - 0.5 * szi - s->z2[i];
+    s->x0[i] = s->x0[i] + ccx * ct0;
+    s->y0[i] = s->y0[i] + ccy * ct0;
+    s->z0[i] = s->z0[i] + ccz * ct0;
+    s->x1[i] = s->x1[i] + ccx * ct1;
+    s->y1[i] = s->y1[i] + ccy * ct1;
+    s->z1[i] = s->z1[i] + ccz * ct1;
+    s->x2[i] = s->x2[i] + ccx * ct2;
+    s->y2[i] = s->y2[i] + ccy * ct2;
+    s->z2[i] = s->z2[i] + ccz * ct2;
+    s->x3[i] = s->x3[i] + ccx * ct3;
+    s->y3[i] = s->y3[i] + ccy * ct3;
+    s->z3[i] = s->z3[i] + ccz * ct3;
+  }
+}
+
+void orc_set_integrator(orc_sys *s, int iverlet) { s->iverlet = iverlet; }
+
+/* one MD step, src/moldin.F90:752-791: iverlet=1 velocityverlet, iverlet=0
+ * PREDIC, FORCE, CORREC; then update_therm_avs and the energies */
 int orc_md_step(orc_sys *s) {
-  int st = velocity_verlet(s);
+  int st;
+  if (s->iverlet == 0) {
+    orc_predic(s);
+    st = orc_force(s);
+    if (st) return st;
+    orc_correc(s);
+  } else {
+    st = velocity_verlet(s);
+  }
   if (st) return st;
   update_therm_avs(s);
   s->ke = s->s3 / (s->deltat * s->deltat) / 2.0; /* deltat**2 */
@@ -1144,9 +1227,11 @@ double orc_setvel(orc_rng *g, int n, const int *z, const double *mass, const dou
 /* ---------------- accessors ---------------- */
 static double *arr_by_name(const orc_sys *s, const char *nm) {
   static const char *names[] = {"x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2",
-                                "rho", "afrho", "dafrho", "fx", "fy", "fz", "del2", "mass"};
+                                "rho", "afrho", "dafrho", "fx", "fy", "fz", "del2", "mass",
+                                "x3", "y3", "z3"};
   double *ptrs[] = {s->x0, s->y0, s->z0, s->x1, s->y1, s->z1, s->x2, s->y2, s->z2,
-                    s->rho, s->afrho, s->dafrho, s->fx, s->fy, s->fz, s->del2, s->mass};
+                    s->rho, s->afrho, s->dafrho, s->fx, s->fy, s->fz, s->del2, s->mass,
+                    s->x3, s->y3, s->z3};
   for (size_t k = 0; k < sizeof(names) / sizeof(names[0]); k++)
     if (!strcmp(nm, names[k])) return ptrs[k];
   return NULL;
@@ -1168,6 +1253,7 @@ static double *ten_by_name(const orc_sys *s, const char *nm) {
   if (!strcmp(nm, "b0")) return m->b0;
   if (!strcmp(nm, "b1")) return m->b1;
   if (!strcmp(nm, "b2")) return m->b2;
+  if (!strcmp(nm, "b3")) return m->b3;
   if (!strcmp(nm, "tk")) return m->tk;
   if (!strcmp(nm, "tc")) return m->tc;
   if (!strcmp(nm, "tg")) return m->tg;

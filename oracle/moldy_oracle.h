@@ -16,6 +16,7 @@
  *   force + virial  src/dynamics_m.F90:135-401
  *   kinten          src/thermostat_m.F90:127-177
  *   velocity Verlet src/quench_m.F90:562-624
+ *   predictor-corrector src/quench_m.F90:636-759
  *   therm. averages src/dynamics_m.F90:649-734
  *   MD-step glue    src/moldin.F90:446-477, 752-791
  *
@@ -130,15 +131,21 @@ void orc_energy_calc(orc_sys *s, double *en_atom);
 void orc_kinten(orc_sys *s);
 /* moldin.F90:446-477: initial force, tgid and accelerations */
 int  orc_md_init(orc_sys *s);
-/* one MD step (moldin.F90:752-791, iverlet=1); ke returned via getters */
+/* one MD step (moldin.F90:752-791); ke returned via getters */
 int  orc_md_step(orc_sys *s);
+/* integrator of orc_md_step: 1 velocity Verlet (default), 0 Gear
+ * predictor-corrector (PREDIC, FORCE, CORREC; src/quench_m.F90:636-759); predic
+ * changes nothing until a correc or velocityverlet has run, as in the reference */
+void orc_set_integrator(orc_sys *s, int iverlet);
+void orc_predic(orc_sys *s);
+void orc_correc(orc_sys *s);
 
 /* getters / setters; name selects the array:
  * "x0","y0","z0","x1","y1","z1","x2","y2","z2","rho","afrho","dafrho",
- * "fx","fy","fz","del2","mass" (n doubles) */
+ * "fx","fy","fz","del2","mass","x3","y3","z3" (n doubles) */
 int  orc_get_array(const orc_sys *s, const char *name, double *out);
 int  orc_set_array(orc_sys *s, const char *name, const double *in);
-/* "b0","b1","b2","tk","tc","tg","tp","tginv","tgdot","tgid","fb" (9 doubles) */
+/* "b0","b1","b2","b3","tk","tc","tg","tp","tginv","tgdot","tgid","fb" (9 doubles) */
 int  orc_get_tensor(const orc_sys *s, const char *name, double *out);
 int  orc_set_tensor(orc_sys *s, const char *name, const double *in);
 /* scalars: 0 pe, 1 ke, 2 vol, 3 s1 (sum f^2), 4 s2, 5 s3, 6 te, 7 h, 8 th, 9 boxke,

@@ -69,7 +69,8 @@ def lib():
                    "orc_md_init", "orc_md_step"):
             getattr(L, nm).argtypes = [C.c_void_p]
             getattr(L, nm).restype = C.c_int
-        for nm in ("orc_rhoset", "orc_embed", "orc_force_pairs", "orc_kinten"):
+        L.orc_set_integrator.argtypes = [C.c_void_p, C.c_int]
+        for nm in ("orc_rhoset", "orc_embed", "orc_force_pairs", "orc_kinten", "orc_predic", "orc_correc"):
             getattr(L, nm).argtypes = [C.c_void_p]
             getattr(L, nm).restype = None
         L.orc_get_array.argtypes = [C.c_void_p, C.c_char_p, dp]
@@ -171,7 +172,7 @@ class OracleSystem:
 
     def __init__(self, sysdef, pot: Potential, ivol: int = 1, rpad: float = 0.3,
                  rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10,
-                 press_gpa: float = 0.0, boxmass: float = 50.0):
+                 press_gpa: float = 0.0, boxmass: float = 50.0, iverlet: int = 1):
         self.pot = pot
         n = sysdef.n
         b0 = np.asfortranarray(sysdef.b0).ravel(order="F").copy()
@@ -186,6 +187,7 @@ class OracleSystem:
                                   boxmass, C.byref(pot.s))
         if not self.h:
             raise MemoryError("orc_create failed")
+        lib().orc_set_integrator(self.h, int(iverlet))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -21,6 +21,9 @@
 !         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
 !                 (moldin.F90:175), the 3 draws per atom of read_system
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 3 = Gear predictor-corrector (iverlet=0): PREDIC, FORCE, CORREC
+!                 (moldin.F90:754-758) instead of velocityverlet; x3, y3, z3
+!                 and b3 follow the final record
 !         bit 2 = extra record: per step pe, ke, vol, box kinetic energy
 !                 (moldin.F90:768-774), b0(3,3); after the run en_atom (energy_calc)
 ! Compiled against the drop-in modules (-DMDK_DROPIN), bit 1 is not available:
@@ -67,6 +70,7 @@ program ref_driver
   sp%nm = n
   sp%ivol = ivol
   sp%iverlet = 1
+  if (iand(flags, 8) /= 0) sp%iverlet = 0
   sp%ntcm = ntcm
   sp%nnbrs = nnbrs_in
   sp%deltat = dt_fs*fs_to_time
@@ -207,7 +211,13 @@ program ref_driver
   allocate(trace(6, max(nsteps,1)), xtrace(13, max(nsteps,1)))
   call system_clock(c0, crate)
   do istep = 1, nsteps
-     call velocityverlet(sp%deltat)
+     if (iand(flags, 8) /= 0) then
+        call predic
+        call force
+        call correc
+     else
+        call velocityverlet(sp%deltat)
+     end if
      call update_therm_avs(s1, s2, s3)
      ke = s3/sp%deltat**2/2.d0
      trace(1, istep) = pe
@@ -239,6 +249,7 @@ program ref_driver
      write(20) trace
      write(20) x0, y0, z0, x1, y1, z1, x2, y2, z2, fx, fy, fz, rho, dafrho
      write(20) b0, b1, b2, tp, fb, tgid, tk
+     if (iand(flags, 8) /= 0) write(20) x3, y3, z3, b3
      if (iand(flags, 4) /= 0) then
         call energy_calc
         write(20) xtrace

@@ -75,8 +75,10 @@ class Reader:
 
 def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
                   press=0.0, boxmass=50.0, flags=0, driver=DRIVER, pot_file=REF_POT,
-                  material="fetransmetal"):
+                  material="fetransmetal", iverlet=1):
     import refrun
+    if iverlet == 0:
+        flags |= 8   # PREDIC, FORCE, CORREC
     rg, rhog = r_grid(material), rho_grid(material)
     n = sysdef.n
     _, buf = refrun.run(sysdef, ivol=ivol, ntcm=ntcm, nsteps=nsteps, uselookup=uselookup,
@@ -114,6 +116,11 @@ def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rp
     ten = R.f8(7 * 9).reshape(7, 9)
     for k, nm in enumerate(["b0", "b1", "b2", "tp", "fb", "tgid", "tk"]):
         d[nm + "_f"] = ten[k].reshape(3, 3, order="F")
+    if iverlet == 0:
+        x3 = R.f8(3 * n).reshape(3, n)
+        for k, nm in enumerate(["x3", "y3", "z3"]):
+            d[nm + "_f"] = x3[k]
+        d["b3_f"] = R.f8(9).reshape(3, 3, order="F")
     return d
 
 
@@ -142,6 +149,14 @@ CASES = {
     # iron_carbon lookup tables (5000 points on [0, 5.4] A), NVE
     "fec_lookup_2000": (dict(nc=10, a=2.87, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=41),
                         dict(ivol=1, ntcm=10, nsteps=3, uselookup=1, material="iron_carbon")),
+    # Gear predictor-corrector (iverlet=0): PR NPT on a sheared box, and the
+    # z-only box (ivol=4)
+    "gear_npt_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=61,
+                           strain=np.array([[0.01, 0.004, -0.003], [0.004, -0.006, 0.002],
+                                            [-0.003, 0.002, 0.003]])),
+                      dict(ivol=0, ntcm=10, nsteps=12, uselookup=0, press=0.5, boxmass=50.0, iverlet=0)),
+    "gear_z_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=62),
+                    dict(ivol=4, ntcm=10, nsteps=12, uselookup=0, press=0.5, boxmass=50.0, iverlet=0)),
     # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
     "fec_hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=43),
                      dict(ivol=1, ntcm=5, nsteps=10, uselookup=0, material="iron_carbon")),
@@ -149,6 +164,7 @@ CASES = {
 
 KEEP_FIRST = ["rho0", "afrho0", "dafrho0", "fx0", "fy0", "fz0", "x20", "y20", "z20"]
 KEEP_FINAL = ["x0_f", "y0_f", "z0_f", "x1_f", "y1_f", "z1_f", "fx_f", "fy_f", "fz_f"]
+KEEP_GEAR = ["x2_f", "y2_f", "z2_f", "x3_f", "y3_f", "z3_f", "b3_f"]
 KEEP_TENS = ["tp0", "tk0", "tc0", "fb0", "tgid0", "b20", "b0_f", "b1_f", "b2_f", "tp_f", "fb_f",
              "tgid_f", "tk_f"]
 
@@ -250,7 +266,7 @@ def main(names=None):
               "deltat_fs": np.float64(sd.deltat_fs)}
         for k, v in rkw.items():
             fx["run_" + k] = np.asarray(v)
-        for k in KEEP_FIRST + KEEP_FINAL + KEEP_TENS:
+        for k in KEEP_FIRST + KEEP_FINAL + KEEP_TENS + (KEEP_GEAR if rkw.get("iverlet") == 0 else []):
             fx[k] = d[k]
         for k in ("nlc", "rnear", "deltat", "bdel2", "press", "vol0", "pe0", "trace", "numn0"):
             fx[k] = np.asarray(d[k])

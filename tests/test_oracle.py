@@ -16,7 +16,7 @@ from moldy_amd import lattice
 from moldy_amd.potential import FE_POTENTIAL
 
 CASES = ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
-         "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024"]
+         "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024", "gear_npt_1024", "gear_z_1024"]
 
 
 def _material(F):
@@ -25,7 +25,8 @@ def _material(F):
 
 def _oracle_for(oracle, F):
     pot = oracle.Potential(lookup=bool(int(F["run_uselookup"])), material=_material(F))
-    S = oracle.OracleSystem(golden_system(F), pot, **run_kwargs(F))
+    iverlet = int(F["run_iverlet"]) if "run_iverlet" in F.files else 1
+    S = oracle.OracleSystem(golden_system(F), pot, iverlet=iverlet, **run_kwargs(F))
     return pot, S
 
 
@@ -58,6 +59,32 @@ def test_oracle_bit_exact_vs_reference(oracle, case):
         np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
     for nm in ("b0", "b1", "b2", "tp", "fb", "tgid", "tk"):
         np.testing.assert_array_equal(S.tensor(nm), F[nm + "_f"], err_msg=nm)
+    if "x3_f" in F.files:   # predictor-corrector state
+        for nm in ("x2", "y2", "z2", "x3", "y3", "z3"):
+            np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
+        np.testing.assert_array_equal(S.tensor("b3"), F["b3_f"])
+
+
+def test_gear_coefficients_are_the_textbook_ones():
+    """Gear's 4-value corrector for a second-order equation: 1/6, 5/6, 1, 1/3
+    (Allen & Tildesley, Computer Simulation of Liquids, table E.2), here in
+    REAL(4) as the reference declares them; the predictor is the Taylor
+    (Pascal-triangle) matrix.  A cubic trajectory with the corrector fed its
+    exact acceleration is followed exactly."""
+    c = np.array([np.float32(1 / 6), np.float32(5 / 6), np.float32(1.0), np.float32(1 / 3)], dtype=np.float64)
+    np.testing.assert_allclose(c, [1 / 6, 5 / 6, 1.0, 1 / 3], rtol=1e-7)
+    # scaled derivatives of x(t) = 1 + 2t + 3t^2 + 4t^3 at t=0, dt = 0.1
+    dt = 0.1
+    x = np.array([1.0, 2 * dt, 3 * dt ** 2, 4 * dt ** 3])
+    P = np.array([[1, 1, 1, 1], [0, 1, 2, 3], [0, 0, 1, 3], [0, 0, 0, 1]], dtype=np.float64)
+    xp = P @ x
+    t = dt
+    exact = np.array([1 + 2 * t + 3 * t ** 2 + 4 * t ** 3, (2 + 6 * t + 12 * t ** 2) * dt,
+                      (6 + 24 * t) * dt ** 2 / 2, 24 * dt ** 3 / 6])
+    np.testing.assert_allclose(xp, exact, rtol=1e-14)
+    # the corrector adds nothing when the predicted acceleration is exact
+    cc = exact[2] - xp[2]
+    np.testing.assert_allclose(xp + cc * c, exact, rtol=1e-14)
 
 
 @pytest.mark.parametrize("case", ["c0_nve_2000", "lookup_2000", "fec_npt_1024", "fec_lookup_2000"])
