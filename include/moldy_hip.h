@@ -157,6 +157,9 @@ int mdk_kinten(void *h);
 int mdk_energy_calc(void *h, double *en_atom);
 /* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
 int mdk_md_init(void *h);
+/* External pressure (model units) from now on: simparam%press as moldin
+ * changes it between strain loops (src/moldin.F90:434). */
+int mdk_set_pressure(void *h, double press);
 /* Integrator of mdk_md_steps (moldin.F90:752-758): 1 velocity Verlet (default),
  * 0 Gear predictor-corrector (PREDIC, FORCE, CORREC). */
 int mdk_set_integrator(void *h, int iverlet);

@@ -2792,6 +2792,13 @@ extern "C" int mdk_velocityverlet(void *h) {
   return check_device_status(c);
 }
 
+extern "C" int mdk_set_pressure(void *h, double press) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->p.press = press;
+  return MDK_OK;
+}
+
 extern "C" int mdk_set_integrator(void *h, int iverlet) {
   mdk_ctx *c = H(h);
   if (!c || (iverlet != 0 && iverlet != 1)) return set_err(MDK_ERR_BAD_ARG, "iverlet must be 0 or 1");

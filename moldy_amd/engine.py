@@ -42,6 +42,7 @@ EXPORTS = [
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
     "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_set_integrator", "mdk_predic", "mdk_correc",
+    "mdk_set_pressure",
 ]
 
 # get_supported_potential_range's rmax of each material module
@@ -83,6 +84,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_set_iron_carbon": [vp, dp],
         "mdk_energy_calc": [vp, dp],
         "mdk_set_integrator": [vp, C.c_int], "mdk_predic": [vp], "mdk_correc": [vp],
+        "mdk_set_pressure": [vp, C.c_double],
         "mdk_potential_constants": [vp, dp],
         "mdk_eval_potential": [vp, C.c_int, dp, dp, C.c_int],
         "mdk_upload": [vp, C.c_int, dp],
@@ -302,6 +304,11 @@ class Engine:
 
     def md_init(self):
         self.call("md_init")
+
+    def set_pressure(self, press_model: float):
+        """External pressure in model units (eV/A^3)."""
+        _check(self.L.mdk_set_pressure(self.h, float(press_model)))
+        self.params.press = float(press_model)
 
     def predic(self):
         self.call("predic")

@@ -1,0 +1,279 @@
+"""MOLDY's main program for molecular dynamics, running the hot path on the GPU.
+
+    python -m moldy_amd.moldin [--material fetransmetal|iron_carbon] [--dir DIR]
+
+Follows src/moldin.F90 for MD runs (iquen = 0), in its order:
+  rstart(97,33,21,82); read_params (params.in); read_system (file_system);
+  potential (lookup or exact) and link-cell sizes; restart 0 (setvel) or
+  -1/1 (read the checkpoint); strain loops: zero the sums, strain the box
+  (ivol != 0), adjust press/temprq, first force and accelerations; the MD loop
+  (velocity Verlet, or PREDIC/FORCE/CORREC for iverlet = 0; update_therm_avs;
+  relink), with the thermodynamic sums sampled nprint/10 times between
+  printouts and runavs every nprint steps; checkpoints every nchkpt steps and
+  at the end; energy_calc and system.out.
+The textout file receives the running-averages blocks and the final box.
+
+The material is a run-time choice here (the reference compiles one in):
+fetransmetal reads pot_026_026.in from the run directory (its Fe-Fe knots),
+iron_carbon reads iron.in when present.
+
+Checkpoint files keep write_checkpoint_file's record layout (Fortran
+sequential records with 4-byte length markers; src/moldin.F90:1053-1070):
+parameters, (x0,y0,z0,b0), (x1,y1,z1,b1), (x2,y2,z2,b2), (x3,y3,z3,b3),
+atomic numbers, masses, species indices, species list, thermodynamic sums.
+The reference's first record is a memory image of its simparameters type,
+which depends on the compiler; this program writes its own parameter record
+(MOLDYAMD tag, counters and scalars, little-endian) and, reading a
+checkpoint, uses the counters of that record when it is one of its own and
+those of params.in otherwise.  All other records are the reference's.
+
+Not supported (errors): quenching (iquen = 1), the Nose thermostat
+(nose > 0), box velocities from setvel (ivol 1, 2, 3, 5 with restart 0),
+time-step adaptation (iverlet = 2), more than one strain loop; metropolis,
+PKA and the dump files are ignored.  Steps run one md_steps call each, so
+that the per-step f^2 sums of the checkpoint are kept.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import struct
+import sys
+
+import numpy as np
+
+from . import engine, initial, io, lattice
+from .params import read_params
+
+_PTAG = b"MOLDYAMD"
+
+
+def _rec(fh, data: bytes) -> None:
+    fh.write(struct.pack("<i", len(data)))
+    fh.write(data)
+    fh.write(struct.pack("<i", len(data)))
+
+
+def write_checkpoint(path: str, counters: dict, scalars: dict, s, x1, x2, x3, b, z, mass, aidx, ispec,
+                     thm: np.ndarray, numsum: int) -> None:
+    """s, x1, x2, x3: (n,3); b: the four box matrices b0..b3; thm: the 24
+    doubles SPE..SVOLPE, SB0 of the thermodynamic sums."""
+    keys_i = ("nlcx", "nlcy", "nlcz", "prevsteps", "currentstep", "laststep", "lastprint", "lastchkpt", "ntc",
+              "iverlet", "ivol")
+    keys_d = ("deltat", "press", "temprq", "bmass", "bdel2")
+    head = _PTAG + struct.pack("<11i", *[int(counters[k]) for k in keys_i]) + \
+        struct.pack("<5d", *[float(scalars[k]) for k in keys_d])
+    with open(path, "wb") as fh:
+        _rec(fh, head)
+        for arr, bm in ((s, b[0]), (x1, b[1]), (x2, b[2]), (x3, b[3])):
+            a = np.asarray(arr, dtype="<f8")
+            _rec(fh, a[:, 0].tobytes() + a[:, 1].tobytes() + a[:, 2].tobytes() +
+                 np.asarray(bm, dtype="<f8").ravel(order="F").tobytes())
+        _rec(fh, np.asarray(z, dtype="<i4").tobytes())
+        _rec(fh, np.asarray(mass, dtype="<f8").tobytes())
+        _rec(fh, np.asarray(aidx, dtype="<i4").tobytes())
+        _rec(fh, np.asarray(ispec, dtype="<i4").tobytes())
+        _rec(fh, np.asarray(thm, dtype="<f8").tobytes() + struct.pack("<i", int(numsum)))
+
+
+def read_checkpoint(path: str, n: int) -> dict:
+    """Records of a checkpoint (this program's or the reference's)."""
+    raw = open(path, "rb").read()
+    recs, o = [], 0
+    while o < len(raw):
+        ln = struct.unpack_from("<i", raw, o)[0]
+        recs.append(raw[o + 4:o + 4 + ln])
+        o += ln + 8
+    if len(recs) < 10:
+        raise ValueError("checkpoint: expected 10 records, found %d" % len(recs))
+    out = {}
+    if recs[0][:8] == _PTAG:
+        vi = struct.unpack_from("<11i", recs[0], 8)
+        vd = struct.unpack_from("<5d", recs[0], 8 + 44)
+        out["counters"] = dict(zip(("nlcx", "nlcy", "nlcz", "prevsteps", "currentstep", "laststep", "lastprint",
+                                    "lastchkpt", "ntc", "iverlet", "ivol"), vi))
+        out["scalars"] = dict(zip(("deltat", "press", "temprq", "bmass", "bdel2"), vd))
+    for k, nm in enumerate(("0", "1", "2", "3")):
+        v = np.frombuffer(recs[1 + k], "<f8")
+        if v.size != 3 * n + 9:
+            raise ValueError("checkpoint does not hold %d atoms" % n)
+        out["x" + nm] = v[:3 * n].reshape(3, n).T.copy()
+        out["b" + nm] = v[3 * n:].reshape(3, 3, order="F").copy()
+    out["z"] = np.frombuffer(recs[5], "<i4").copy()
+    out["mass"] = np.frombuffer(recs[6], "<f8").copy()
+    out["aidx"] = np.frombuffer(recs[7], "<i4").copy()
+    out["ispec"] = np.frombuffer(recs[8], "<i4").copy()
+    out["thm"] = np.frombuffer(recs[9][:192], "<f8").copy()
+    out["numsum"] = struct.unpack_from("<i", recs[9], 192)[0]
+    return out
+
+
+class Sums(io.ThermoSums):
+    """Thermodynamic sums with the reference's per-step f^2 sums (moldin.F90:762-765)
+    and the volume-energy product (analysis_m.F90:155)."""
+
+    def __init__(self):
+        super().__init__()
+        self.sf2 = self.sf2sq = self.svolpe = 0.0
+
+    def update(self, pe, ke, te, h, th, vol, b0):
+        super().update(pe, ke, te, h, th, vol, b0)
+        self.svolpe = self.svolpe + vol * pe
+
+    def as_record(self) -> np.ndarray:
+        s, q = self.s, self.sq
+        return np.concatenate([[s["pe"], s["ke"], s["te"], s["h"], s["th"], self.sf2, s["vol"], q["pe"], q["ke"],
+                                q["te"], q["h"], q["th"], self.sf2sq, q["vol"], self.svolpe],
+                               np.asarray(self.sb0).ravel(order="F")])
+
+
+def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
+    """Runs the MD job described by workdir/params.in; returns a summary."""
+    p = read_params(os.path.join(workdir, "params.in"))
+    if p.iquen == 1:
+        raise NotImplementedError("quenching (iquen = 1) is not supported")
+    if p.nose > 0:
+        raise NotImplementedError("the Nose thermostat (nose > 0) is not supported")
+    if p.iverlet not in (0, 1):
+        raise NotImplementedError("iverlet must be 0 or 1")
+    out = open(os.path.join(workdir, p.file_textout), "w")
+    rng = initial.Ranmar(97, 33, 21, 82)                       # moldin.F90:175
+    rqke = initial.rqke_init(p.temprq, p.nm)                    # init_thermostat_m
+    text = open(os.path.join(workdir, p.file_system)).read()
+    b0, s, z, mass = initial.read_system(text, rng, p.dsp)
+    n = len(z)
+    if p.nm and p.nm != n:
+        raise ValueError(f"params.in nm = {p.nm} but {p.file_system} holds {n} atoms")
+    # species bookkeeping of read_system for the checkpoint (one index per distinct Z)
+    ispec = []
+    for zz in z:
+        if int(zz) not in ispec:
+            ispec.append(int(zz))
+    aidx = np.array([ispec.index(int(zz)) + 1 for zz in z], dtype=np.int32)
+    out.write(f"  STRUCTURE READ IN {len(initial.parse_system(text)[3]):12d}  BASIS ATOMS;"
+              f"{n // len(initial.parse_system(text)[3]):12d}  CELLS;{n:12d}   ATOMS.\n")
+    out.write(f" {p.title1}\n {p.title2}\n")
+    x1 = np.zeros((n, 3))
+    x2 = np.zeros((n, 3))
+    x3 = np.zeros((n, 3))
+    b = [np.asarray(b0, dtype=np.float64), np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))]
+    counters = dict(prevsteps=0, currentstep=0, laststep=p.nsteps, lastprint=0, lastchkpt=0, ntc=p.ntc)
+    if p.restart == 0:
+        if p.ivol not in (0, 4) and p.ivol >= 1:
+            raise NotImplementedError("setvel's box velocities (ivol 1, 2, 3, 5) are not supported")
+        x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+        out.write("\n          EQUILIBRATION STARTING FROM REGULAR LATTICE\n\n\n")
+        out.write(" " * 16 + "INITIAL TEMPERATURE".ljust(23) + io.fortran_f(rqke / (1.5 * initial.BK * n), 14, 8)
+                  + " K\n\n\n")
+    else:
+        ck = read_checkpoint(os.path.join(workdir, p.file_checkpointread), n)
+        s = ck["x0"]
+        b = [ck["b0"], ck["b1"], ck["b2"], ck["b3"]]
+        if p.restart == 1:
+            x1, x2, x3 = ck["x1"], ck["x2"], ck["x3"]
+        else:   # restart -1: positions only (moldin.F90:343-352)
+            b = [b[0], np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))]
+            x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+        if "counters" in ck:
+            c = ck["counters"]
+            counters.update(prevsteps=c["currentstep"], currentstep=c["currentstep"], lastprint=c["lastprint"],
+                            lastchkpt=c["lastchkpt"], ntc=c["ntc"])
+            counters["laststep"] = counters["prevsteps"] + p.nsteps
+    if p.strainloops != 1:
+        raise NotImplementedError("strainloops > 1 is not supported")
+    sd = lattice.System(b0=b[0], s=s, x1=x1, mass=mass, z=z, deltat_fs=p.deltat_fs)
+    eng = engine.Engine(sd, ivol=p.ivol, rpad=p.rpad, nnbrs=p.nnbrs or 150, ntcm=max(p.ntcm, 1),
+                        press_gpa=0.0, boxmass=p.bmass / max(n, 1), lookup=5000 if p.uselookup else 0,
+                        material=material, iverlet=p.iverlet, rebuild=0,
+                        iron_in=_iron_in(workdir) if material == "iron_carbon" else None,
+                        coeffs=_pot_file(workdir) if material == "fetransmetal" else None)
+    try:
+        for nm, arr in (("x2", x2), ("x3", x3)):
+            for k, ax in enumerate("xyz"):
+                eng.upload(ax + nm[1], arr[:, k])
+        for k, t in enumerate(("b1", "b2", "b3")):
+            eng.set_tensor(t, b[k + 1])
+        eng.ntc = counters["ntc"]
+        eng.linkup()
+        out.write("\n" + " " * 10 + " SET UP LINK CELLS\n\n\n")
+        # the strain loop (moldin.F90:418-477): sums zeroed, box strained for
+        # ivol != 0, pressure stepped, first force and accelerations
+        sums = Sums()
+        if p.ivol != 0:
+            eng.set_tensor("b0", np.asarray(p.strx) @ eng.tensor("b0"))
+        eng.set_pressure(p.press + p.pressstep)
+        eng.md_init()
+        every = p.nprint // 10 if (p.nprint >= 10 and p.nprint <= p.nsteps) else 0
+        summary = {}
+        for istep in range(counters["prevsteps"] + 1, counters["laststep"] + 1):
+            counters["currentstep"] = istep
+            counters["lastprint"] += 1
+            counters["lastchkpt"] += 1
+            pe, ke, te, h, th, vol = eng.md_steps(1)[0]
+            sc = eng.scalars()
+            sums.sf2 = sums.sf2 + sc["s1"]       # moldin.F90:762-765, every step
+            sums.sf2sq = sums.sf2sq + sc["s2"]
+            if every and counters["lastprint"] % every == 0:
+                sums.update(pe, ke, te, h, th, vol, eng.tensor("b0"))
+                if counters["lastprint"] % p.nprint == 0:
+                    out.write("\n" + io.runavs_text(istep - counters["prevsteps"], sums, ke / (1.5 * initial.BK * n),
+                                                    n, initial.BK, eng.tensor("b0"), eng.tensor("tp"),
+                                                    eng.tensor("tk")))
+            if p.nchkpt > 0 and counters["lastchkpt"] % p.nchkpt == 0:
+                _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
+                out.write("Checkpoint at step:%6d\n\n\n\n" % istep)
+            summary = dict(pe=pe, ke=ke, te=te, h=h, th=th, vol=vol, steps=istep)
+        out.write("\n" + " " * 20 + "QUIT MOLECULAR DYNAMICS LOOP\n")
+        en = eng.energy_calc()
+        if p.nchkpt > 0:
+            _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
+        bb = eng.tensor("b0")
+        out.write(f" VOLUME: {io.fortran_e(eng.scalar('vol'), 14, 7).replace('E', 'D')} MD-BOX VECTORS (Ang) :\n\n")
+        for i in range(3):
+            out.write(f"I={i + 1:3d}".ljust(6) + "FINAL VALUE:".ljust(20) +
+                      "".join(io.fortran_e(bb[j, i], 18, 10) for j in range(3)) + "\n\n")
+        io.write_system_out(os.path.join(workdir, "system.out"), bb, eng.array("x0"), eng.array("y0"),
+                            eng.array("z0"), z, mass, en)
+        summary["en_atom"] = en
+        return summary
+    finally:
+        out.close()
+        eng.close()
+
+
+def _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir):
+    path = os.path.join(workdir, p.file_checkpointwrite or "chkp_out.dat")
+    arr = {nm: np.stack([eng.array(a + nm[1:]) for a in "xyz"], axis=1) for nm in ("x0", "x1", "x2", "x3")}
+    write_checkpoint(path, dict(nlcx=eng.params.nlcx, nlcy=eng.params.nlcy, nlcz=eng.params.nlcz,
+                                iverlet=p.iverlet, ivol=p.ivol, **counters),
+                     dict(deltat=p.deltat, press=p.press, temprq=p.temprq, bmass=p.bmass, bdel2=p.bdel2),
+                     arr["x0"], arr["x1"], arr["x2"], arr["x3"],
+                     [eng.tensor(t) for t in ("b0", "b1", "b2", "b3")], z, mass, aidx,
+                     np.array(ispec, dtype=np.int32), sums.as_record(), sums.numsum)
+
+
+def _iron_in(workdir: str):
+    path = os.path.join(workdir, "iron.in")
+    if not os.path.exists(path):
+        return None
+    toks = open(path).read().split("\n", 1)[1].replace("d", "e").replace("D", "E").split()
+    return np.array([float(t) for t in toks[:38]])
+
+
+def _pot_file(workdir: str):
+    from .potential import read_avtf
+    path = os.path.join(workdir, "pot_026_026.in")
+    return read_avtf(path) if os.path.exists(path) else None
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--dir", default=".", help="run directory holding params.in")
+    ap.add_argument("--material", default="fetransmetal", choices=sorted(engine.MATERIAL_RANGE))
+    a = ap.parse_args(argv)
+    run(a.dir, a.material)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

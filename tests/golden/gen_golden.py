@@ -237,6 +237,28 @@ def bcc_iron_fixture():
     vals = [_printed(r[5]) for r in rows]
     fx["sys_en"] = np.array([v for v, _ in vals])
     fx["sys_en_tol"] = np.array([h for _, h in vals])
+    # chkp_out.dat: Fortran sequential records (4-byte markers) written by
+    # write_checkpoint_file (src/moldin.F90:1053-1070); read as raw little-endian
+    # numbers only.  Records: simparam, (x0,y0,z0,b0), (x1,y1,z1,b1),
+    # (x2,y2,z2,b2), (x3,y3,z3,b3), atomic_number, mass, index, ispec, thmsums.
+    raw = open(os.path.join(BCC_IRON, "chkp_out.dat"), "rb").read()
+    recs, o = [], 0
+    while o < len(raw):
+        ln = int(np.frombuffer(raw, "<i4", 1, o)[0])
+        recs.append(raw[o + 4:o + 4 + ln])
+        o += ln + 8
+    chk_s, chk_b = [], []
+    for r in recs[1:5]:
+        v = np.frombuffer(r, "<f8")
+        chk_s.append(v[:3 * n].reshape(3, n)[:, idx])
+        chk_b.append(v[3 * n:].reshape(3, 3, order="F"))
+    fx["chk_s"] = np.concatenate(chk_s)          # rows x0 y0 z0 x1 y1 z1 x2 y2 z2 x3 y3 z3
+    fx["chk_b"] = np.array(chk_b)                # b0, b1, b2, b3
+    th = recs[9]
+    fx["chk_thm"] = np.frombuffer(th[:192], "<f8").copy()   # SPE..SVOLPE, SB0(3,3)
+    fx["chk_numsum"] = np.int64(np.frombuffer(th[192:196], "<i4")[0])
+    fx["chk_z"] = np.frombuffer(recs[5], "<i4")[idx].copy()
+    fx["chk_mass"] = np.frombuffer(recs[6], "<f8")[idx].copy()
     # inputs (params.in, 25^3.in)
     fx.update(n=np.int64(n), nt=np.array([25, 25, 25], dtype=np.int32), cell=np.diag([2.870] * 3),
               basis=np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5]]), basis_z=np.array([26, 26], dtype=np.int32),
