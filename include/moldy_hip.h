@@ -157,6 +157,14 @@ int mdk_kinten(void *h);
 int mdk_energy_calc(void *h, double *en_atom);
 /* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
 int mdk_md_init(void *h);
+/* scalev (src/thermostat_m.F90:317-395): Nose-Hoover scaling of the atom
+ * velocities by 1 + snhv after snhv -= (ke - rqke)/(nose nm), with the mean
+ * of x1 removed first; nothing for nose <= 0.  moldin calls it about once per
+ * femtosecond of simulated time (src/moldin.F90:828-835). */
+int mdk_scalev(void *h, double nose, double rqke);
+/* get_snhv / set_snhv (src/thermostat_m.F90:53-85): the accumulated scaling */
+int mdk_get_snhv(void *h, double *snhv);
+int mdk_set_snhv(void *h, double snhv);
 /* External pressure (model units) from now on: simparam%press as moldin
  * changes it between strain loops (src/moldin.F90:434). */
 int mdk_set_pressure(void *h, double press);

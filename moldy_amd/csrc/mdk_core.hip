@@ -158,6 +158,8 @@ struct mdk_ctx {
   int material;            // MDK_MAT_*: fetransmetal (knots) or iron_carbon (fec)
   int iverlet;             // md_steps integrator: 1 velocity Verlet, 0 predictor-corrector
   bool pc_primed;          // a correc or velocityverlet has run (predic is live)
+  double snhv;             // Nose-Hoover velocity scaling accumulated by scalev
+  double *red;             // 4 doubles: results of the scalev folds
   hostpot::FecCoef fec;
   double r_int_exact;      // interaction range of the exact functions
   // reference list threshold: r < rnear  <=>  r^2 < rnear2_exact (see near_threshold)
@@ -1454,6 +1456,61 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *
   }
 }
 
+// scalev (src/thermostat_m.F90:317-395), Nose-Hoover velocity scaling, as
+// three passes with deterministic block partials: (1) sums of x1 over all
+// atoms; (2) moving atoms (Z != 0, mass >= 0.1) lose the mean, partial
+// kinetic sums m |B0 x1|^2; (3) moving atoms' x1 scaled by 1 + snhv (the
+// host forms ke, snhf and snhv between (2) and (3)).
+__global__ void __launch_bounds__(MDK_BLOCK) k_vsum(const double *__restrict__ atoms, int n, double *part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n)
+    for (int d = 0; d < 3; d++) v[d] = atoms[(size_t)(F_X1 + d) * n + i];
+  block_store<3, 0>(v, part);
+}
+__device__ __forceinline__ bool moving_atom(const double *atoms, const int *zat, const double *spec, int n, int i) {
+  const int z = zat[i];
+  const double m = spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i];
+  return !(z == 0 || m < (double)0.1f);
+}
+__global__ void __launch_bounds__(MDK_BLOCK) k_vsub_ke(double *__restrict__ atoms, const int *__restrict__ zat, int n,
+                                                       const double *__restrict__ mean, const BoxState *bs,
+                                                       const double *__restrict__ spec, double *part) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {0.0};
+  if (i < n && moving_atom(atoms, zat, spec, n, i)) {
+    double x[3];
+    for (int d = 0; d < 3; d++) {
+      double *q = atoms + (size_t)(F_X1 + d) * n + i;
+      x[d] = *q - mean[d];
+      *q = x[d];
+    }
+    const double *b = bs->b0;
+    const double rx = b[0] * x[0] + b[3] * x[1] + b[6] * x[2];
+    const double ry = b[1] * x[0] + b[4] * x[1] + b[7] * x[2];
+    const double rz = b[2] * x[0] + b[5] * x[1] + b[8] * x[2];
+    const double m = spec ? spec[2 * zat[i]] : atoms[(size_t)F_MASS * n + i];
+    v[0] = m * (rx * rx + ry * ry + rz * rz);
+  }
+  block_store<1, 0>(v, part);
+}
+__global__ void __launch_bounds__(MDK_BLOCK) k_vscale(double *__restrict__ atoms, const int *__restrict__ zat, int n,
+                                                      const double *__restrict__ spec, double f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && moving_atom(atoms, zat, spec, n, i))
+    for (int d = 0; d < 3; d++) atoms[(size_t)(F_X1 + d) * n + i] *= f;
+}
+// fixed-order fold of K per-block partials into out[0..K-1]
+template <int K>
+__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_fold(const double *part, int nb, double *stage, unsigned *cnt,
+                                                        double *out) {
+  double v[K];
+  if (!stage_and_finish<K, 0>(part, nb, stage, cnt, v)) return;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; k++) out[k] = v[k];
+}
+
 // update_therm_avs box part only (stand-alone entry point)
 __global__ void k_therm_box(BoxState *bs, int ivol) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -2265,6 +2322,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * (NPART_FORCE + 1));
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
   ALLOC(c->fin_cnt, sizeof(unsigned) * 2);
+  ALLOC(c->red, sizeof(double) * 4);
   ALLOC(c->spec_tab, sizeof(double) * 2 * MDK_NSPEC);
   if (hipMemset(c->fin_cnt, 0, sizeof(unsigned) * 2) != hipSuccess) {
     rc = set_err(MDK_ERR_HIP, "counter initialisation failed");
@@ -2307,7 +2365,7 @@ extern "C" int mdk_destroy(void *h) {
   void *ptrs[] = {c->pos4, c->p4f, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
-                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab};
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
@@ -2790,6 +2848,54 @@ extern "C" int mdk_velocityverlet(void *h) {
   poll_monitor(c);
   RCK(lag_status(c));
   return check_device_status(c);
+}
+
+extern "C" int mdk_scalev(void *h, double nose, double rqke) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (!(nose > 0.0)) return MDK_OK;  // only a positive nose acts (src/thermostat_m.F90:329)
+  const int n = c->n, nb = nblocks(n);
+  if (nb > c->nblk) return set_err(MDK_ERR_STATE, "partial buffers too small");
+  const unsigned fb = (unsigned)((nb + 63) / 64);
+  k_vsum<<<nb, MDK_BLOCK, 0, c->stream>>>(c->atoms, n, c->part_k);
+  RCK(check_launch("k_vsum"));
+  k_fold<3><<<fb, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_k, nb, c->stage_k, c->fin_cnt + 1, c->red);
+  RCK(check_launch("k_fold"));
+  // mean over all atoms: sx / float(nm), a REAL(4) conversion of nm
+  double s3[4];
+  HIPCK(hipMemcpyAsync(s3, c->red, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  const double fn = (double)(float)n;
+  const double mean[3] = {s3[0] / fn, s3[1] / fn, s3[2] / fn};
+  HIPCK(hipMemcpyAsync(c->red, mean, sizeof(mean), hipMemcpyHostToDevice, c->stream));
+  k_vsub_ke<<<nb, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, n, c->red, c->box + c->cur, spec_of(c), c->part_k);
+  RCK(check_launch("k_vsub_ke"));
+  k_fold<1><<<fb, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_k, nb, c->stage_k, c->fin_cnt + 1, c->red + 3);
+  RCK(check_launch("k_fold"));
+  HIPCK(hipMemcpyAsync(s3 + 3, c->red + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  const double ke = s3[3] / (c->p.deltat * c->p.deltat) / 2.0;
+  double snhf = 0.0;
+  if (ke > 0.0) snhf = (ke - rqke) / (nose * (double)n);
+  c->snhv = c->snhv - snhf;
+  k_vscale<<<nb, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, n, spec_of(c), 1.0 + c->snhv);
+  RCK(check_launch("k_vscale"));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+extern "C" int mdk_get_snhv(void *h, double *v) {
+  mdk_ctx *c = H(h);
+  if (!c || !v) return set_err(MDK_ERR_BAD_ARG, "bad snhv arguments");
+  *v = c->snhv;
+  return MDK_OK;
+}
+
+extern "C" int mdk_set_snhv(void *h, double v) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->snhv = v;
+  return MDK_OK;
 }
 
 extern "C" int mdk_set_pressure(void *h, double press) {

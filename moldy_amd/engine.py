@@ -42,7 +42,7 @@ EXPORTS = [
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
     "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_set_integrator", "mdk_predic", "mdk_correc",
-    "mdk_set_pressure",
+    "mdk_set_pressure", "mdk_scalev", "mdk_get_snhv", "mdk_set_snhv",
 ]
 
 # get_supported_potential_range's rmax of each material module
@@ -85,6 +85,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_energy_calc": [vp, dp],
         "mdk_set_integrator": [vp, C.c_int], "mdk_predic": [vp], "mdk_correc": [vp],
         "mdk_set_pressure": [vp, C.c_double],
+        "mdk_scalev": [vp, C.c_double, C.c_double], "mdk_get_snhv": [vp, dp], "mdk_set_snhv": [vp, C.c_double],
         "mdk_potential_constants": [vp, dp],
         "mdk_eval_potential": [vp, C.c_int, dp, dp, C.c_int],
         "mdk_upload": [vp, C.c_int, dp],
@@ -309,6 +310,16 @@ class Engine:
         """External pressure in model units (eV/A^3)."""
         _check(self.L.mdk_set_pressure(self.h, float(press_model)))
         self.params.press = float(press_model)
+
+    def scalev(self, nose: float, rqke: float):
+        """Nose-Hoover velocity scaling (scalev); rqke in eV."""
+        _check(self.L.mdk_scalev(self.h, float(nose), float(rqke)))
+
+    @property
+    def snhv(self) -> float:
+        v = C.c_double()
+        _check(self.L.mdk_get_snhv(self.h, C.byref(v)))
+        return v.value
 
     def predic(self):
         self.call("predic")

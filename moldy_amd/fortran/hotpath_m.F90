@@ -5,7 +5,7 @@
 !                    with -DIRON_CARBON the iron_carbon Fe-Fe functions, src/iron_carbon.F90)
 !   linkcell_m       src/linkcell_m.F90:52-54
 !   neighbourlist_m  src/neighbourlist_m.F90:52-58
-!   thermostat_m     src/thermostat_m.F90:94-177   (init, kinten)
+!   thermostat_m     src/thermostat_m.F90:94-395   (init, kinten, scalev, snhv)
 !   dynamics_m       src/dynamics_m.F90:69-83      (force, rhoset, energy_calc, update_therm_avs)
 !   quench_m         src/quench_m.F90:59-63        (velocityverlet, predic, correc)
 !
@@ -305,14 +305,37 @@ contains
 end module neighbourlist_m
 
 module thermostat_m
+  use iso_c_binding
+  use mdk_c
   use constants_m
   use params_m
   use particles_m
   use parrinellorahman_m
+  use mdk_device_m
   implicit none
   private
-  public :: init_thermostat_m, cleanup_thermostat_m, kinten
+  public :: init_thermostat_m, cleanup_thermostat_m, kinten, scalev, set_snhv, get_snhv
 contains
+  !> scalev (src/thermostat_m.F90:317-395): Nose-Hoover velocity scaling on
+  !> the device; snhv lives with the device handle
+  subroutine scalev
+    type(simparameters) :: sp
+    sp = get_params()
+    if (.not. (sp%nose > 0.0d0)) return
+    call mdk_dev_push()
+    call mdk_check(mdk_scalev(mdk_dev_handle(), real(sp%nose, c_double), real(sp%rqke, c_double)), 'mdk_scalev')
+    call mdk_dev_pull(.true.)
+  end subroutine scalev
+  subroutine set_snhv(v)
+    real(kind_wp), intent(in) :: v
+    call mdk_check(mdk_set_snhv(mdk_dev_handle(), real(v, c_double)), 'mdk_set_snhv')
+  end subroutine set_snhv
+  real(kind_wp) function get_snhv()
+    real(c_double) :: v
+    call mdk_check(mdk_get_snhv(mdk_dev_handle(), v), 'mdk_get_snhv')
+    get_snhv = v
+  end function get_snhv
+
   subroutine init_thermostat_m
     type(simparameters) :: sp
     sp = get_params()

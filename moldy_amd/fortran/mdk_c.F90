@@ -52,6 +52,21 @@ module mdk_c
        import :: c_int, c_ptr
        type(c_ptr), value :: h, iron_in
      end function mdk_set_iron_carbon
+     integer(c_int) function mdk_scalev(h, nose, rqke) bind(C, name="mdk_scalev")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: nose, rqke
+     end function mdk_scalev
+     integer(c_int) function mdk_get_snhv(h, v) bind(C, name="mdk_get_snhv")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: v
+     end function mdk_get_snhv
+     integer(c_int) function mdk_set_snhv(h, v) bind(C, name="mdk_set_snhv")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: v
+     end function mdk_set_snhv
      integer(c_int) function mdk_predic(h) bind(C, name="mdk_predic")
        import :: c_int, c_ptr
        type(c_ptr), value :: h

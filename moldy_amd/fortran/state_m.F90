@@ -36,6 +36,7 @@ module params_m
      real(kind_wp) :: dsp = 0._kind_wp, rpad = 0._kind_wp, rcut = 0._kind_wp, rnear = 0._kind_wp
      real(kind_wp) :: deltat = 0._kind_wp, temprq = 0._kind_wp, rqke = 0._kind_wp
      real(kind_wp) :: press = 0._kind_wp
+     real(kind_wp) :: nose = 0._kind_wp, therm_con = -1._kind_wp   ! Nose thermostat (src/params_m.F90:102-103)
      integer :: nsteps = 0, ntcm = 10, ntc = 0
      logical :: uselookup = .false., atomstress = .false.
      real(kind_wp) :: boxtem = 0._kind_wp, bdel2 = 0._kind_wp, bmass = 0._kind_wp

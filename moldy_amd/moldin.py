@@ -27,8 +27,11 @@ which depends on the compiler; this program writes its own parameter record
 checkpoint, uses the counters of that record when it is one of its own and
 those of params.in otherwise.  All other records are the reference's.
 
-Not supported (errors): quenching (iquen = 1), the Nose thermostat
-(nose > 0), box velocities from setvel (ivol 1, 2, 3, 5 with restart 0),
+The Nose thermostat (nose > 0, or therm_con > 0 converted to a Nose time)
+runs scalev after setvel and whenever the accumulated time reaches one fs.
+
+Not supported (errors): quenching (iquen = 1), box velocities from setvel
+(ivol 1, 2, 3, 5 with restart 0),
 time-step adaptation (iverlet = 2), more than one strain loop; metropolis,
 PKA and the dump files are ignored.  Steps run one md_steps call each, so
 that the per-step f^2 sums of the checkpoint are kept.
@@ -132,8 +135,6 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     p = read_params(os.path.join(workdir, "params.in"))
     if p.iquen == 1:
         raise NotImplementedError("quenching (iquen = 1) is not supported")
-    if p.nose > 0:
-        raise NotImplementedError("the Nose thermostat (nose > 0) is not supported")
     if p.iverlet not in (0, 1):
         raise NotImplementedError("iverlet must be 0 or 1")
     out = open(os.path.join(workdir, p.file_textout), "w")
@@ -181,6 +182,10 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
             counters["laststep"] = counters["prevsteps"] + p.nsteps
     if p.strainloops != 1:
         raise NotImplementedError("strainloops > 1 is not supported")
+    nose = p.nose
+    if nose != 0 and p.therm_con > 0:   # thermal conductivity -> Nose time (moldin.F90:206-215)
+        bd = b[0][0, 0] * 1e-10 + b[0][1, 1] * 1e-10 + b[0][2, 2] * 1e-10
+        nose = (3.0 * n * 1.3806504e-23) / (2.0 * p.therm_con * bd) / 1.0e-15
     sd = lattice.System(b0=b[0], s=s, x1=x1, mass=mass, z=z, deltat_fs=p.deltat_fs)
     eng = engine.Engine(sd, ivol=p.ivol, rpad=p.rpad, nnbrs=p.nnbrs or 150, ntcm=max(p.ntcm, 1),
                         press_gpa=0.0, boxmass=p.bmass / max(n, 1), lookup=5000 if p.uselookup else 0,
@@ -194,6 +199,8 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
         for k, t in enumerate(("b1", "b2", "b3")):
             eng.set_tensor(t, b[k + 1])
         eng.ntc = counters["ntc"]
+        if p.restart in (0, -1) and nose > 0:
+            eng.scalev(nose, rqke)                       # after setvel (moldin.F90:308, 339)
         eng.linkup()
         out.write("\n" + " " * 10 + " SET UP LINK CELLS\n\n\n")
         # the strain loop (moldin.F90:418-477): sums zeroed, box strained for
@@ -202,6 +209,9 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
         if p.ivol != 0:
             eng.set_tensor("b0", np.asarray(p.strx) @ eng.tensor("b0"))
         eng.set_pressure(p.press + p.pressstep)
+        rqke_md = 1.5 * initial.BK * (p.temprq + p.tempsp) * (n - 3)   # moldin.F90:437
+        fs_ref = float(np.float32(0.098227))    # one fs in model units, a REAL(4) literal (moldin.F90:132)
+        thermotimer = 0.0
         eng.md_init()
         every = p.nprint // 10 if (p.nprint >= 10 and p.nprint <= p.nsteps) else 0
         summary = {}
@@ -222,6 +232,11 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
             if p.nchkpt > 0 and counters["lastchkpt"] % p.nchkpt == 0:
                 _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
                 out.write("Checkpoint at step:%6d\n\n\n\n" % istep)
+            thermotimer = thermotimer + p.deltat                  # moldin.F90:828-835
+            if thermotimer >= fs_ref:
+                thermotimer = thermotimer - fs_ref
+                if nose != 0:
+                    eng.scalev(nose, rqke_md)
             summary = dict(pe=pe, ke=ke, te=te, h=h, th=th, vol=vol, steps=istep)
         out.write("\n" + " " * 20 + "QUIT MOLECULAR DYNAMICS LOOP\n")
         en = eng.energy_calc()

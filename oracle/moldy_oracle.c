@@ -429,6 +429,7 @@ struct orc_sys {
   double *x1dt2, *y1dt2, *z1dt2;
   int iverlet; /* 1 velocity Verlet, 0 Gear predictor-corrector */
   int pc_primed; /* quench_m's parameter copy is set (by correc or velocityverlet) */
+  double snhv;   /* Nose-Hoover velocity scaling, thermostat_m's module variable */
   double *mass, *del2, *rho, *afrho, *dafrho, *fx, *fy, *fz;
   int *zat;
   int *l0, *link, *ic, *numn, *nlist;
@@ -1077,6 +1078,45 @@ void orc_correc(orc_sys *s) {
 }
 
 void orc_set_integrator(orc_sys *s, int iverlet) { s->iverlet = iverlet; }
+
+/* scalev, src/thermostat_m.F90:317-395 (Nose-Hoover, Euler): remove the mean
+ * of x1 over all atoms (divided by float(nm), a REAL(4) conversion) from the
+ * moving atoms, KE of the moving atoms, snhv -= (ke - rqke)/(nose nm), moving
+ * atoms' x1 scaled by 1 + snhv.  snhf keeps its value when ke <= 0 (a local
+ * of the reference that is then undefined; 0 here). */
+void orc_scalev(orc_sys *s, double nose, double rqke) {
+  if (!(nose > 0.0)) return;
+  const int n = s->n;
+  double sx = 0.0, sy = 0.0, sz = 0.0, s3 = 0.0, snhf = 0.0;
+  for (int i = 0; i < n; i++) {
+    sx = sx + s->x1[i];
+    sy = sy + s->y1[i];
+    sz = sz + s->z1[i];
+  }
+  sx = sx / (double)(float)n;
+  sy = sy / (double)(float)n;
+  sz = sz / (double)(float)n;
+  const double *b = s->b0;
+  for (int i = 0; i < n; i++) {
+    if (s->zat[i] == 0 || s->mass[i] < (double)0.1f) continue;
+    s->x1[i] = s->x1[i] - sx;
+    s->y1[i] = s->y1[i] - sy;
+    s->z1[i] = s->z1[i] - sz;
+    const double rx = T(b, 1, 1) * s->x1[i] + T(b, 1, 2) * s->y1[i] + T(b, 1, 3) * s->z1[i];
+    const double ry = T(b, 2, 1) * s->x1[i] + T(b, 2, 2) * s->y1[i] + T(b, 2, 3) * s->z1[i];
+    const double rz = T(b, 3, 1) * s->x1[i] + T(b, 3, 2) * s->y1[i] + T(b, 3, 3) * s->z1[i];
+    s3 = s3 + s->mass[i] * (rx * rx + ry * ry + rz * rz);
+  }
+  const double ke = s3 / (s->deltat * s->deltat) / 2.0;
+  if (ke > 0.0) snhf = (ke - rqke) / (nose * (double)n);
+  s->snhv = s->snhv - snhf;
+  for (int i = 0; i < n; i++) {
+    if (s->zat[i] == 0 || s->mass[i] < (double)0.1f) continue;
+    s->x1[i] = s->x1[i] * (1.0 + s->snhv);
+    s->y1[i] = s->y1[i] * (1.0 + s->snhv);
+    s->z1[i] = s->z1[i] * (1.0 + s->snhv);
+  }
+}
 
 /* one MD step, src/moldin.F90:752-791: iverlet=1 velocityverlet, iverlet=0
  * PREDIC, FORCE, CORREC; then update_therm_avs and the energies */

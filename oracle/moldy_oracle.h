@@ -16,6 +16,7 @@
  *   force + virial  src/dynamics_m.F90:135-401
  *   kinten          src/thermostat_m.F90:127-177
  *   velocity Verlet src/quench_m.F90:562-624
+ *   Nose scalev     src/thermostat_m.F90:317-395
  *   predictor-corrector src/quench_m.F90:636-759
  *   therm. averages src/dynamics_m.F90:649-734
  *   MD-step glue    src/moldin.F90:446-477, 752-791
@@ -138,6 +139,9 @@ int  orc_md_step(orc_sys *s);
  * changes nothing until a correc or velocityverlet has run, as in the reference */
 void orc_set_integrator(orc_sys *s, int iverlet);
 void orc_predic(orc_sys *s);
+/* scalev (src/thermostat_m.F90:317-395) with the Nose parameter and required
+ * kinetic energy; the snhv accumulator persists in the system */
+void orc_scalev(orc_sys *s, double nose, double rqke);
 void orc_correc(orc_sys *s);
 
 /* getters / setters; name selects the array:

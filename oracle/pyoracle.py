@@ -70,6 +70,7 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p]
             getattr(L, nm).restype = C.c_int
         L.orc_set_integrator.argtypes = [C.c_void_p, C.c_int]
+        L.orc_scalev.argtypes = [C.c_void_p, C.c_double, C.c_double]
         for nm in ("orc_rhoset", "orc_embed", "orc_force_pairs", "orc_kinten", "orc_predic", "orc_correc"):
             getattr(L, nm).argtypes = [C.c_void_p]
             getattr(L, nm).restype = None
@@ -241,6 +242,9 @@ class OracleSystem:
         out = np.empty(self.n, dtype=np.int32)
         lib().orc_get_ic(self.h, _ip(out))
         return out
+
+    def scalev(self, nose: float, rqke: float) -> None:
+        lib().orc_scalev(self.h, float(nose), float(rqke))
 
     def energy_calc(self) -> np.ndarray:
         """en_atom of energy_calc (needs afrho of a previous force())."""

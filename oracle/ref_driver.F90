@@ -14,13 +14,16 @@
 !
 ! usage: ref_driver <input.bin> <output.bin>     (stream-unformatted, little endian)
 ! input : int32 n, ivol, ntcm, nsteps, uselookup, nnbrs, flags
-!         real64 deltat_fs, rpad, press_gpa, boxmass, temprq, b0(3,3)
+!         real64 deltat_fs, rpad, press_gpa, boxmass, temprq, nose, b0(3,3)
 !         real64 x0(n),y0(n),z0(n),x1(n),y1(n),z1(n),mass(n); int32 z(n)
 !         int32 ngrid; real64 rgrid(ngrid); int32 nrho; real64 rhogrid(nrho)
 ! flags : bit 0 = timing only (no dumps, prints wall-clock seconds of the MD loop)
 !         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
 !                 (moldin.F90:175), the 3 draws per atom of read_system
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 4 = Nose thermostat as moldin drives it (moldin.F90:828-835): scalev
+!                 whenever the accumulated time reaches fs = 0.098227 (REAL(4)),
+!                 with RQKE of the strain loop (moldin.F90:437)
 !         bit 3 = Gear predictor-corrector (iverlet=0): PREDIC, FORCE, CORREC
 !                 (moldin.F90:754-758) instead of velocityverlet; x3, y3, z3
 !                 and b3 follow the final record
@@ -51,7 +54,8 @@ program ref_driver
   integer :: n, ivol, ntcm, nsteps, uselk, nnbrs_in, flags
   integer :: ngrid, nrho, i, k, istep, ierror, c0, c1, crate
   real(kind_wp) :: dt_fs, rpad_in, press_gpa, boxmass, b0in(3,3)
-  real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke
+  real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke, nose_in, thermotimer
+  real(kind_wp) :: fs_ref = 0.098227
   real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:), xtrace(:,:)
   type(simparameters) :: sp
   logical :: timing
@@ -61,7 +65,7 @@ program ref_driver
 
   open(10, file=trim(inpath), access='stream', form='unformatted', status='old')
   read(10) n, ivol, ntcm, nsteps, uselk, nnbrs_in, flags
-  read(10) dt_fs, rpad_in, press_gpa, boxmass, temprq_in
+  read(10) dt_fs, rpad_in, press_gpa, boxmass, temprq_in, nose_in
   read(10) b0in
   timing = iand(flags, 1) /= 0
 
@@ -78,6 +82,7 @@ program ref_driver
   sp%press = press_gpa*gpa_to_press
   sp%rcut = 0.0d0
   sp%temprq = temprq_in
+  sp%nose = nose_in
   sp%ntc = 0
 #ifdef IRON_CARBON
   sp%nspec = 1
@@ -209,6 +214,10 @@ program ref_driver
   end if
 
   allocate(trace(6, max(nsteps,1)), xtrace(13, max(nsteps,1)))
+  sp = get_params()
+  sp%rqke = 1.5d0*bk*sp%temprq*(sp%nm-3)
+  call set_params(sp)
+  thermotimer = 0.0d0
   call system_clock(c0, crate)
   do istep = 1, nsteps
      if (iand(flags, 8) /= 0) then
@@ -239,6 +248,13 @@ program ref_driver
      xtrace(4, istep) = bke
      xtrace(5:13, istep) = reshape(b0, (/ 9 /))
      if (sp%ntc == 0) call relink
+     if (iand(flags, 16) /= 0) then
+        thermotimer = thermotimer + sp%deltat
+        if (thermotimer >= fs_ref) then
+           thermotimer = thermotimer - fs_ref
+           if (sp%nose /= 0) call scalev
+        end if
+     end if
   end do
   call system_clock(c1)
 

@@ -24,11 +24,11 @@ POT_FILE = os.path.join(ROOT, "moldy_amd", "data", "pot_026_026.in")
 
 
 def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press, boxmass,
-                rgrid=(), rhogrid=(), temprq=300.0):
+                rgrid=(), rhogrid=(), temprq=300.0, nose=0.0):
     n = sysdef.n
     with open(path, "wb") as fh:
         fh.write(struct.pack("<7i", n, ivol, ntcm, nsteps, uselookup, nnbrs, flags))
-        fh.write(struct.pack("<5d", sysdef.deltat_fs, rpad, press, boxmass, temprq))
+        fh.write(struct.pack("<6d", sysdef.deltat_fs, rpad, press, boxmass, temprq, nose))
         fh.write(np.asarray(sysdef.b0, dtype="<f8").ravel(order="F").tobytes())
         for a in (sysdef.s[:, 0], sysdef.s[:, 1], sysdef.s[:, 2], sysdef.x1[:, 0],
                   sysdef.x1[:, 1], sysdef.x1[:, 2], sysdef.mass):
@@ -42,13 +42,13 @@ def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad,
 
 def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3, press=0.0,
         boxmass=50.0, flags=0, rgrid=(), rhogrid=(), driver=DRIVER, pot_file=POT_FILE,
-        threads=None, timeout=None, temprq=300.0):
+        threads=None, timeout=None, temprq=300.0, nose=0.0):
     """Runs the driver; returns (stdout, output bytes or None)."""
     with tempfile.TemporaryDirectory() as td:
         os.symlink(pot_file, os.path.join(td, "pot_026_026.in"))
         inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(inp, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad, press,
-                    boxmass, rgrid, rhogrid, temprq)
+                    boxmass, rgrid, rhogrid, temprq, nose)
         env = dict(os.environ)
         if threads is not None:
             env["OMP_NUM_THREADS"] = str(threads)

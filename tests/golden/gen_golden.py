@@ -75,7 +75,7 @@ class Reader:
 
 def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
                   press=0.0, boxmass=50.0, flags=0, driver=DRIVER, pot_file=REF_POT,
-                  material="fetransmetal", iverlet=1):
+                  material="fetransmetal", iverlet=1, nose=0.0, temprq=300.0):
     import refrun
     if iverlet == 0:
         flags |= 8   # PREDIC, FORCE, CORREC
@@ -83,7 +83,7 @@ def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rp
     n = sysdef.n
     _, buf = refrun.run(sysdef, ivol=ivol, ntcm=ntcm, nsteps=nsteps, uselookup=uselookup,
                         nnbrs=nnbrs, rpad=rpad, press=press, boxmass=boxmass, flags=flags,
-                        rgrid=rg, rhogrid=rhog, driver=driver, pot_file=pot_file)
+                        rgrid=rg, rhogrid=rhog, driver=driver, pot_file=pot_file, nose=nose, temprq=temprq)
     R = Reader(buf)
     d = {}
     d["nlc"] = R.i4(3)
@@ -157,6 +157,9 @@ CASES = {
                       dict(ivol=0, ntcm=10, nsteps=12, uselookup=0, press=0.5, boxmass=50.0, iverlet=0)),
     "gear_z_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=62),
                     dict(ivol=4, ntcm=10, nsteps=12, uselookup=0, press=0.5, boxmass=50.0, iverlet=0)),
+    # Nose thermostat as moldin drives it (flag 16): scalev about once per fs
+    "nose_nvt_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=71),
+                      dict(ivol=1, ntcm=10, nsteps=15, uselookup=0, flags=16, nose=50.0, temprq=600.0)),
     # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
     "fec_hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=43),
                      dict(ivol=1, ntcm=5, nsteps=10, uselookup=0, material="iron_carbon")),

@@ -28,7 +28,7 @@ def rel(a, b):
 
 @pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
                                   "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024",
-                                  "gear_npt_1024", "gear_z_1024"])
+                                  "gear_npt_1024", "gear_z_1024", "nose_nvt_1024"])
 def test_fortran_dropin_matches_reference(oracle, case):
     import gen_golden
     import refrun

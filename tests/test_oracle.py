@@ -65,6 +65,30 @@ def test_oracle_bit_exact_vs_reference(oracle, case):
         np.testing.assert_array_equal(S.tensor("b3"), F["b3_f"])
 
 
+def test_nose_thermostat_bit_exact_vs_reference(oracle):
+    """scalev as moldin drives it (once per fs = 0.098227, RQKE of the strain
+    loop): the reference build's run with the Nose thermostat, bit for bit."""
+    F = load_golden("nose_nvt_1024")
+    S = oracle.OracleSystem(golden_system(F), oracle.Potential(), **run_kwargs(F))
+    S.call("linkup")
+    assert S.call("md_init") == 0
+    rqke = 1.5 * oracle.bk() * float(F["run_temprq"]) * (S.n - 3)
+    fs, tt, dt = float(np.float32(0.098227)), 0.0, S.scalar("deltat")
+    tr, calls = [], 0
+    for _ in range(int(F["run_nsteps"])):
+        assert S.call("md_step") == 0
+        tr.append((S.scalar("pe"), S.scalar("ke"), S.scalar("s1"), S.scalar("s3"), S.scalar("vol"), S.int("ntc")))
+        tt = tt + dt
+        if tt >= fs:
+            tt = tt - fs
+            S.scalev(float(F["run_nose"]), rqke)
+            calls += 1
+    assert calls == int(F["run_nsteps"]) - 1      # dt = 1 fs is a hair below fs: every step but the first
+    np.testing.assert_array_equal(np.array(tr), F["trace"])
+    for nm in ("x0", "y0", "z0", "x1", "y1", "z1", "fx", "fy", "fz"):
+        np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
+
+
 def test_gear_coefficients_are_the_textbook_ones():
     """Gear's 4-value corrector for a second-order equation: 1/6, 5/6, 1, 1/3
     (Allen & Tildesley, Computer Simulation of Liquids, table E.2), here in
