@@ -30,8 +30,11 @@ those of params.in otherwise.  All other records are the reference's.
 The Nose thermostat (nose > 0, or therm_con > 0 converted to a Nose time)
 runs scalev after setvel and whenever the accumulated time reaches one fs.
 
-Not supported (errors): quenching (iquen = 1), box velocities from setvel
-(ivol 1, 2, 3, 5 with restart 0),
+iquen = 1 relaxes the structure with moldy_amd.quench (conjugate gradients
+on the enthalpy) and writes the FINAL U line and system.out.
+
+Not supported (errors): box velocities from setvel (ivol 1, 2, 3, 5 with
+restart 0),
 time-step adaptation (iverlet = 2), more than one strain loop; metropolis,
 PKA and the dump files are ignored.  Steps run one md_steps call each, so
 that the per-step f^2 sums of the checkpoint are kept.
@@ -133,8 +136,6 @@ class Sums(io.ThermoSums):
 def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     """Runs the MD job described by workdir/params.in; returns a summary."""
     p = read_params(os.path.join(workdir, "params.in"))
-    if p.iquen == 1:
-        raise NotImplementedError("quenching (iquen = 1) is not supported")
     if p.iverlet not in (0, 1):
         raise NotImplementedError("iverlet must be 0 or 1")
     out = open(os.path.join(workdir, p.file_textout), "w")
@@ -209,6 +210,20 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
         if p.ivol != 0:
             eng.set_tensor("b0", np.asarray(p.strx) @ eng.tensor("b0"))
         eng.set_pressure(p.press + p.pressstep)
+        if p.iquen == 1:   # static relaxation instead of MD (moldin.F90:566-572)
+            eng.md_init()
+            from . import quench as q
+            r = q.quench(eng, p.ivol, p.press + p.pressstep)
+            out.write(" QUENCH, FINDING MINIMUM ENERGY CONFIGURATION OF%5d ATOMS\n" % n)
+            out.write("FINAL U= %s V=%s P=%s H=%s\n" % (
+                io.fortran_e(r["pe"] / n, 18, 10).replace("E", "D"), io.fortran_f(r["vol"] / n, 10, 6),
+                io.fortran_e((p.press + p.pressstep) / initial_gpa(), 12, 4).replace("E", "D"),
+                io.fortran_e(r["h"] / n, 13, 7).replace("E", "D")))
+            en = eng.energy_calc()
+            io.write_system_out(os.path.join(workdir, "system.out"), eng.tensor("b0"), eng.array("x0"),
+                                eng.array("y0"), eng.array("z0"), z, mass, en)
+            r["en_atom"] = en
+            return r
         rqke_md = 1.5 * initial.BK * (p.temprq + p.tempsp) * (n - 3)   # moldin.F90:437
         fs_ref = float(np.float32(0.098227))    # one fs in model units, a REAL(4) literal (moldin.F90:132)
         thermotimer = 0.0
@@ -254,6 +269,11 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     finally:
         out.close()
         eng.close()
+
+
+def initial_gpa() -> float:
+    from .params import GPA_TO_PRESS
+    return GPA_TO_PRESS
 
 
 def _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir):

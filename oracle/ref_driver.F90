@@ -21,6 +21,9 @@
 !         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
 !                 (moldin.F90:175), the 3 draws per atom of read_system
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 5 = quench after the first force (moldin.F90:566-572: init_quench_m,
+!                 quench, cleanup_quench_m) instead of MD; pe and vol follow
+!                 the final record
 !         bit 4 = Nose thermostat as moldin drives it (moldin.F90:828-835): scalev
 !                 whenever the accumulated time reaches fs = 0.098227 (REAL(4)),
 !                 with RQKE of the strain loop (moldin.F90:437)
@@ -213,6 +216,13 @@ program ref_driver
      write(20) (demb(rhogrid(k),26), k = 1, nrho)
   end if
 
+#ifndef MDK_DROPIN
+  if (iand(flags, 32) /= 0) then
+     call init_quench_m
+     call quench
+     call cleanup_quench_m
+  end if
+#endif
   allocate(trace(6, max(nsteps,1)), xtrace(13, max(nsteps,1)))
   sp = get_params()
   sp%rqke = 1.5d0*bk*sp%temprq*(sp%nm-3)
@@ -266,6 +276,7 @@ program ref_driver
      write(20) x0, y0, z0, x1, y1, z1, x2, y2, z2, fx, fy, fz, rho, dafrho
      write(20) b0, b1, b2, tp, fb, tgid, tk
      if (iand(flags, 8) /= 0) write(20) x3, y3, z3, b3
+     if (iand(flags, 32) /= 0) write(20) pe, vol
      if (iand(flags, 4) /= 0) then
         call energy_calc
         write(20) xtrace

@@ -116,6 +116,8 @@ def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rp
     ten = R.f8(7 * 9).reshape(7, 9)
     for k, nm in enumerate(["b0", "b1", "b2", "tp", "fb", "tgid", "tk"]):
         d[nm + "_f"] = ten[k].reshape(3, 3, order="F")
+    if flags & 32:
+        d["pe_q"], d["vol_q"] = R.f8(2)
     if iverlet == 0:
         x3 = R.f8(3 * n).reshape(3, n)
         for k, nm in enumerate(["x3", "y3", "z3"]):
@@ -160,6 +162,10 @@ CASES = {
     # Nose thermostat as moldin drives it (flag 16): scalev about once per fs
     "nose_nvt_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=71),
                       dict(ivol=1, ntcm=10, nsteps=15, uselookup=0, flags=16, nose=50.0, temprq=600.0)),
+    # quench (flag 32, the reference's VA14CD conjugate gradients): a 5^3 bcc
+    # cell without its first atom, displaced by 0.1 A, box free at 0 GPa
+    "quench_vac_249": (dict(nc=5, a=2.8665, sigma=0.1, temp=0.0, deltat_fs=1.0, seed=81, vacancy=0),
+                       dict(ivol=0, ntcm=10, nsteps=0, uselookup=0, flags=32, press=0.0, boxmass=50.0)),
     # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
     "fec_hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=43),
                      dict(ivol=1, ntcm=5, nsteps=10, uselookup=0, material="iron_carbon")),
@@ -284,7 +290,13 @@ def main(names=None):
     for name, (skw, rkw) in CASES.items():
         if names and name not in names:
             continue
+        skw = dict(skw)
+        vac = skw.pop("vacancy", None)
         sd = lattice.make_system(**skw)
+        if vac is not None:       # remove one atom
+            keep = np.arange(sd.n) != vac
+            sd = lattice.System(b0=sd.b0, s=sd.s[keep], x1=sd.x1[keep], mass=sd.mass[keep], z=sd.z[keep],
+                                deltat_fs=sd.deltat_fs)
         drv = DRIVER_FEC if rkw.get("material") == "iron_carbon" else DRIVER
         d = run_reference(sd, driver=drv, **rkw)
         fx = {"b0": sd.b0, "s": sd.s, "x1": sd.x1, "mass": sd.mass, "z": sd.z,
@@ -293,6 +305,8 @@ def main(names=None):
             fx["run_" + k] = np.asarray(v)
         for k in KEEP_FIRST + KEEP_FINAL + KEEP_TENS + (KEEP_GEAR if rkw.get("iverlet") == 0 else []):
             fx[k] = d[k]
+        if "pe_q" in d:
+            fx["pe_q"], fx["vol_q"] = np.float64(d["pe_q"]), np.float64(d["vol_q"])
         for k in ("nlc", "rnear", "deltat", "bdel2", "press", "vol0", "pe0", "trace", "numn0"):
             fx[k] = np.asarray(d[k])
         fx["numn0"] = d["numn0"].astype(np.int16)
