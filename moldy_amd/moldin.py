@@ -31,7 +31,9 @@ The Nose thermostat (nose > 0, or therm_con > 0 converted to a Nose time)
 runs scalev after setvel and whenever the accumulated time reaches one fs.
 
 iquen = 1 relaxes the structure with moldy_amd.quench (conjugate gradients
-on the enthalpy) and writes the FINAL U line and system.out.
+on the enthalpy) and writes the FINAL U line and system.out; iquen = 2 runs
+MD with the box quench (box velocities that oppose their acceleration zeroed
+after every step).
 
 Not supported (errors): box velocities from setvel (ivol 1, 2, 3, 5 with
 restart 0),
@@ -252,6 +254,8 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
                 thermotimer = thermotimer - fs_ref
                 if nose != 0:
                     eng.scalev(nose, rqke_md)
+            if p.iquen == 2:
+                _box_quench(eng)
             summary = dict(pe=pe, ke=ke, te=te, h=h, th=th, vol=vol, steps=istep)
         out.write("\n" + " " * 20 + "QUIT MOLECULAR DYNAMICS LOOP\n")
         en = eng.energy_calc()
@@ -269,6 +273,15 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     finally:
         out.close()
         eng.close()
+
+
+def _box_quench(eng) -> None:
+    """iquen = 2 (src/moldin.F90:845-853): after each MD step every box
+    velocity element b1(i,j) that is not moving along its acceleration
+    b2(i,j) (b1 b2 <= 0) is set to zero, so the box relaxes while the atoms
+    keep their dynamics."""
+    b1, b2 = eng.tensor("b1"), eng.tensor("b2")
+    eng.set_tensor("b1", np.where(b1 * b2 > 0.0, b1, 0.0))
 
 
 def initial_gpa() -> float:

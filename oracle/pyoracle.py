@@ -246,6 +246,16 @@ class OracleSystem:
     def scalev(self, nose: float, rqke: float) -> None:
         lib().orc_scalev(self.h, float(nose), float(rqke))
 
+    def box_quench(self) -> None:
+        """moldin's iquen = 2 step (src/moldin.F90:845-853): each b1(i,j) is
+        kept only when b1(i,j) * b2(i,j) > 0, else set to zero."""
+        b1, b2 = self.tensor("b1"), self.tensor("b2")
+        for i in range(3):
+            for j in range(3):
+                if not b1[i, j] * b2[i, j] > 0.0:
+                    b1[i, j] = 0.0
+        self.set_tensor("b1", b1)
+
     def energy_calc(self) -> np.ndarray:
         """en_atom of energy_calc (needs afrho of a previous force())."""
         out = np.empty(self.n)

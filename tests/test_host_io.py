@@ -114,3 +114,31 @@ def test_checkpoint_round_trip(tmp_path):
     first = int(np.frombuffer(raw, "<i4", 1, 0)[0])
     second = int(np.frombuffer(raw, "<i4", 1, first + 8)[0])
     assert second == 8 * (3 * n + 9)
+
+
+class _BoxOnly:
+    """Holds b1 and b2 the way the engine and the oracle expose them."""
+
+    def __init__(self, b1, b2):
+        self.t = {"b1": np.array(b1, dtype=np.float64), "b2": np.array(b2, dtype=np.float64)}
+
+    def tensor(self, name):
+        return self.t[name].copy()
+
+    def set_tensor(self, name, v):
+        self.t[name] = np.array(v, dtype=np.float64)
+
+
+def test_box_quench_known_answer(oracle):
+    """iquen = 2 (src/moldin.F90:845-853): b1(i,j) survives only when it has
+    the sign of b2(i,j); opposite signs and a zero b2 zero it."""
+    b1 = [[1e-3, -2e-4, 3e-5], [-4e-6, 0.0, 5e-4], [7e-5, -8e-5, -9e-4]]
+    b2 = [[2e-9, 1e-9, -3e-9], [-1e-9, 4e-9, 0.0], [7e-10, -6e-10, -1e-12]]
+    want = [[1e-3, 0.0, 0.0], [-4e-6, 0.0, 0.0], [7e-5, -8e-5, -9e-4]]
+    box = _BoxOnly(b1, b2)
+    moldin._box_quench(box)
+    np.testing.assert_array_equal(box.t["b1"], want)
+    np.testing.assert_array_equal(box.t["b2"], b2)
+    box = _BoxOnly(b1, b2)
+    oracle.OracleSystem.box_quench(box)
+    np.testing.assert_array_equal(box.t["b1"], want)
