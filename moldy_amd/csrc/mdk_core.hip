@@ -642,9 +642,13 @@ __device__ void list_one_atom(int i, const double4 *__restrict__ pos4, const dou
           if (!sxr && (pbc & 1)) tx = -(int)round(sxi - (atoms[j] - img[3 * j]));
           if (!syr && (pbc & 2)) ty = -(int)round(syi - (atoms[(size_t)n + j] - img[3 * j + 1]));
           if (!szr && (pbc & 4)) tz = -(int)round(szi - (atoms[2 * (size_t)n + j] - img[3 * j + 2]));
-          if (!(pbc & 1)) tx = 0;
-          if (!(pbc & 2)) ty = 0;
-          if (!(pbc & 4)) tz = 0;
+          // open axes take the raw fractional difference: positions are folded
+          // by img, so the shift restores img_i - img_j, which is -1, 0 or 1
+          // for any pair closer than half the box
+          if (!(pbc & 1)) tx = img[3 * i] - img[3 * j];
+          if (!(pbc & 2)) ty = img[3 * i + 1] - img[3 * j + 1];
+          if (!(pbc & 4)) tz = img[3 * i + 2] - img[3 * j + 2];
+          if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1) continue;
           // cell-relative packing: the cell offset (>= 3 cells per dimension)
           const int code = rel ? (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1) : (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
           const double4 pj = pos4[j];

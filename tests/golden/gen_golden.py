@@ -169,7 +169,38 @@ CASES = {
     # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
     "fec_hot_1024": (dict(nc=8, a=2.98118, sigma=0.30, temp=2500.0, deltat_fs=0.5, seed=43),
                      dict(ivol=1, ntcm=5, nsteps=10, uselookup=0, material="iron_carbon")),
+    # open boundaries (pr_get_realsep_from_dx without an image along the open
+    # axes): each open axis is centred on fractional 0, so the atoms straddle
+    # the fold at 0, and widened 1.6x with the Cartesian positions kept.
+    # ivol=2: free surfaces normal to z, the lowest eighth of the slab immobile (mass 0)
+    "slab_ivol2_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=91,
+                             open_axes=(2,), immobile=0.125),
+                        dict(ivol=2, ntcm=10, nsteps=12, uselookup=0)),
+    # ivol=5: a pillar, periodic along z only
+    "pillar_ivol5_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=92, open_axes=(0, 1)),
+                          dict(ivol=5, ntcm=10, nsteps=12, uselookup=0)),
+    # ivol=3: a free cluster, no periodic axis
+    "cluster_ivol3_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=93,
+                                open_axes=(0, 1, 2)),
+                           dict(ivol=3, ntcm=10, nsteps=12, uselookup=0)),
 }
+
+
+def open_box(sd, axes, immobile=0.0, widen=1.6):
+    """Open boundaries along axes: positions there centred on fractional 0 and
+    the box widened by widen (Cartesian positions and velocities kept); with
+    immobile > 0 the atoms of the lowest fraction along z get mass 0 and no
+    velocity (the reference moves atoms with mass < 0.1 only by x1)."""
+    b0, s, x1, mass = sd.b0.copy(), sd.s.copy(), sd.x1.copy(), sd.mass.copy()
+    for ax in axes:
+        s[:, ax] = (s[:, ax] - 0.5) / widen
+        x1[:, ax] = x1[:, ax] / widen
+        b0[:, ax] = b0[:, ax] * widen
+    if immobile > 0.0:
+        low = np.argsort(s[:, 2], kind="stable")[:int(round(immobile * sd.n))]
+        mass[low] = 0.0
+        x1[low] = 0.0
+    return lattice.System(b0=b0, s=s, x1=x1, mass=mass, z=sd.z, deltat_fs=sd.deltat_fs)
 
 KEEP_FIRST = ["rho0", "afrho0", "dafrho0", "fx0", "fy0", "fz0", "x20", "y20", "z20"]
 KEEP_FINAL = ["x0_f", "y0_f", "z0_f", "x1_f", "y1_f", "z1_f", "fx_f", "fy_f", "fz_f"]
@@ -292,7 +323,11 @@ def main(names=None):
             continue
         skw = dict(skw)
         vac = skw.pop("vacancy", None)
+        axes = skw.pop("open_axes", ())
+        immobile = skw.pop("immobile", 0.0)
         sd = lattice.make_system(**skw)
+        if axes:
+            sd = open_box(sd, axes, immobile)
         if vac is not None:       # remove one atom
             keep = np.arange(sd.n) != vac
             sd = lattice.System(b0=sd.b0, s=sd.s[keep], x1=sd.x1[keep], mass=sd.mass[keep], z=sd.z[keep],

@@ -18,7 +18,9 @@ from moldy_amd import engine, lattice
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["c0_nve_2000", "npt_tric_1024", "hot_1024"]
+# open boundaries (ivol 2, 5, 3): the slab has an immobile layer (mass 0)
+OPEN_CASES = ["slab_ivol2_1024", "pillar_ivol5_1024", "cluster_ivol3_1024"]
+CASES = ["c0_nve_2000", "npt_tric_1024", "hot_1024"] + OPEN_CASES
 
 
 def say(*a):
@@ -198,6 +200,22 @@ def test_gpu_fp32_form_force_error(oracle):
     dsig = np.max(np.abs(eng.tensor("tp") - S.tensor("tp"))) / S.scalar("vol") * ev_a3_to_gpa
     say(f"fp32 virial error {dsig:.3e} GPa")
     assert dsig < 1e-4
+    eng.close()
+
+
+@pytest.mark.parametrize("case", OPEN_CASES)
+def test_gpu_fp32_form_open_boundaries(oracle, case):
+    """FP32 form with open axes (atoms straddling the fold at fractional 0):
+    first forces within 1e-5 eV/A of the oracle."""
+    F = load_golden(case)
+    say("fp32 form, open boundaries", case)
+    eng = make_engine(F, precision=engine.PREC_FP32)
+    eng.linkup()
+    eng.md_init()
+    S = oracle.OracleSystem(golden_system(F), oracle.Potential(), **run_kwargs(F))
+    S.call("linkup")
+    S.call("md_init")
+    assert np.max(np.abs(eng.forces_cart() - S.forces_cart())) <= 1e-5
     eng.close()
 
 

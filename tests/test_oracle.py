@@ -16,7 +16,8 @@ from moldy_amd import lattice
 from moldy_amd.potential import FE_POTENTIAL
 
 CASES = ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
-         "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024", "gear_npt_1024", "gear_z_1024"]
+         "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024", "gear_npt_1024", "gear_z_1024",
+         "slab_ivol2_1024", "pillar_ivol5_1024", "cluster_ivol3_1024"]
 
 
 def _material(F):
@@ -236,6 +237,28 @@ def test_cell_list_equals_bruteforce(oracle, case):
         near = {tuple(p) for p in oracle.canonical_pairs(
             oracle.bruteforce_pairs(F["b0"], F["s"], 3.72645 + 0.3)).tolist()}
         assert near <= a
+
+
+@pytest.mark.parametrize("case,periodic", [("slab_ivol2_1024", (1, 1, 0)), ("pillar_ivol5_1024", (0, 0, 1)),
+                                           ("cluster_ivol3_1024", (0, 0, 0))])
+def test_open_boundaries_pairs_are_raw_separations(oracle, case, periodic):
+    """Open axes (src/parrinellorahman_m.F90:157-166): the reference's pair set
+    at rnear is exactly the brute-force set with the minimum image taken along
+    the periodic axes only and the raw fractional difference along the open
+    ones; the atoms straddle fractional 0 along every open axis."""
+    F = load_golden(case)
+    s, b0 = np.asarray(F["s"]), np.asarray(F["b0"])
+    for ax in range(3):
+        if not periodic[ax]:
+            assert s[:, ax].min() < 0.0 < s[:, ax].max()
+    d = s[:, None, :] - s[None, :, :]
+    for ax in range(3):
+        if periodic[ax]:
+            d[:, :, ax] -= np.rint(d[:, :, ax])
+    r2 = np.einsum("ijk,ijk->ij", d @ b0.T, d @ b0.T)
+    i, j = np.nonzero(np.triu(r2 < float(F["rnear"]) ** 2, 1))
+    assert len(i) == int(F["npairs0"])
+    assert oracle.pairset_digest(np.stack([i, j], axis=1)) == str(F["pairs0_sha256"])
 
 
 def test_excess_neighbours_is_reported(oracle):
