@@ -131,6 +131,30 @@ def setvel(rng: Ranmar, z, mass, b0, deltat: float, rqke: float) -> np.ndarray:
     return x1 * scale
 
 
+def setvel_box(rng: Ranmar, ivol: int, nm: int, temprq: float, bdel2: float) -> np.ndarray:
+    """b1 of setvel's box part (src/thermostat_m.F90:256-299), drawn after the
+    atom part from the same stream.  Only ivol 1, 2, 3 and 5 reach it (the box
+    of those runs does not move, so b1 enters the box kinetic energy and the
+    checkpoint); otherwise b1 is zero and nothing is drawn.  Rows are drawn in
+    order, each row loses its sum / nm, and the whole is scaled to the box
+    temperature temprq, boxtem = (sum b1^2 / bdel2) / (4.5 bk)."""
+    b1 = np.zeros((3, 3))
+    if ivol < 1 or ivol == 4:
+        return b1
+    s3 = 0.0
+    for i in range(3):
+        s = 0.0
+        for j in range(3):
+            b1[i, j] = rng.rand1() - 0.5
+            s = s + b1[i, j]
+        s = s / nm
+        for j in range(3):
+            b1[i, j] = b1[i, j] - s
+            s3 = s3 + b1[i, j] * b1[i, j]
+    boxtem = (s3 / bdel2) / (4.5 * BK)
+    return b1 * np.sqrt(temprq / boxtem)
+
+
 def rqke_init(temprq: float, nm: int) -> float:
     """RQKE of init_thermostat_m (src/thermostat_m.F90:97)."""
     return 1.5 * BK * temprq * (nm - 1)

@@ -4,9 +4,10 @@
 
 Follows src/moldin.F90 for MD runs (iquen = 0), in its order:
   rstart(97,33,21,82); read_params (params.in); read_system (file_system);
-  potential (lookup or exact) and link-cell sizes; restart 0 (setvel) or
-  -1/1 (read the checkpoint); strain loops: zero the sums, strain the box
-  (ivol != 0), adjust press/temprq, first force and accelerations; the MD loop
+  potential (lookup or exact) and link-cell sizes; restart 0 (setvel and its
+  box part, scalev) or -1/1 (read the checkpoint); strain loops: zero the
+  sums, strain the box (ivol != 0), adjust press/temprq, first force and
+  accelerations; the MD loop
   (velocity Verlet, or PREDIC/FORCE/CORREC for iverlet = 0; update_therm_avs;
   relink), with the thermodynamic sums sampled nprint/10 times between
   printouts and runavs every nprint steps; checkpoints every nchkpt steps and
@@ -35,11 +36,14 @@ on the enthalpy) and writes the FINAL U line and system.out; iquen = 2 runs
 MD with the box quench (box velocities that oppose their acceleration zeroed
 after every step).
 
-Not supported (errors): box velocities from setvel (ivol 1, 2, 3, 5 with
-restart 0),
-time-step adaptation (iverlet = 2), more than one strain loop; metropolis,
-PKA and the dump files are ignored.  Steps run one md_steps call each, so
-that the per-step f^2 sums of the checkpoint are kept.
+New runs draw setvel's box velocities too (ivol 1, 2, 3, 5, whose box does
+not move: they enter the box kinetic energy and the checkpoint).  Each of the
+strainloops loops strains the box by strx (ivol != 0), steps press and temprq
+by pressstep and tempsp, and runs nsteps steps (or one quench).
+
+Not supported (errors): time-step adaptation (iverlet = 2); metropolis, PKA
+and the dump files are ignored.  Steps run one md_steps call each, so that
+the per-step f^2 sums of the checkpoint are kept.
 """
 from __future__ import annotations
 
@@ -163,12 +167,10 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     b = [np.asarray(b0, dtype=np.float64), np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))]
     counters = dict(prevsteps=0, currentstep=0, laststep=p.nsteps, lastprint=0, lastchkpt=0, ntc=p.ntc)
     if p.restart == 0:
-        if p.ivol not in (0, 4) and p.ivol >= 1:
-            raise NotImplementedError("setvel's box velocities (ivol 1, 2, 3, 5) are not supported")
-        x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+        if p.iquen != 1:                                        # moldin.F90:307
+            x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+            b[1] = initial.setvel_box(rng, p.ivol, n, p.temprq, p.bdel2)
         out.write("\n          EQUILIBRATION STARTING FROM REGULAR LATTICE\n\n\n")
-        out.write(" " * 16 + "INITIAL TEMPERATURE".ljust(23) + io.fortran_f(rqke / (1.5 * initial.BK * n), 14, 8)
-                  + " K\n\n\n")
     else:
         ck = read_checkpoint(os.path.join(workdir, p.file_checkpointread), n)
         s = ck["x0"]
@@ -177,14 +179,14 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
             x1, x2, x3 = ck["x1"], ck["x2"], ck["x3"]
         else:   # restart -1: positions only (moldin.F90:343-352)
             b = [b[0], np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))]
-            x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+            if p.iquen != 1:                                    # moldin.F90:338
+                x1 = initial.setvel(rng, z, mass, b[0], p.deltat, rqke)
+                b[1] = initial.setvel_box(rng, p.ivol, n, p.temprq, p.bdel2)
         if "counters" in ck:
             c = ck["counters"]
             counters.update(prevsteps=c["currentstep"], currentstep=c["currentstep"], lastprint=c["lastprint"],
                             lastchkpt=c["lastchkpt"], ntc=c["ntc"])
             counters["laststep"] = counters["prevsteps"] + p.nsteps
-    if p.strainloops != 1:
-        raise NotImplementedError("strainloops > 1 is not supported")
     nose = p.nose
     if nose != 0 and p.therm_con > 0:   # thermal conductivity -> Nose time (moldin.F90:206-215)
         bd = b[0][0, 0] * 1e-10 + b[0][1, 1] * 1e-10 + b[0][2, 2] * 1e-10
@@ -202,61 +204,84 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
         for k, t in enumerate(("b1", "b2", "b3")):
             eng.set_tensor(t, b[k + 1])
         eng.ntc = counters["ntc"]
-        if p.restart in (0, -1) and nose > 0:
+        if p.restart in (0, -1) and nose > 0 and p.iquen != 1:
             eng.scalev(nose, rqke)                       # after setvel (moldin.F90:308, 339)
+        if p.restart == 0:
+            # ke as setvel leaves it (RQKE), or as scalev recomputes it before
+            # scaling (moldin.F90:309-310)
+            ke0 = _scalev_ke(x1, b[0], mass, z, p.deltat) if (nose > 0 and p.iquen != 1) else rqke
+            out.write(" " * 16 + "INITIAL TEMPERATURE".ljust(23) + io.fortran_f(ke0 / (1.5 * initial.BK * n), 14, 8)
+                      + " K\n\n\n")
         eng.linkup()
         out.write("\n" + " " * 10 + " SET UP LINK CELLS\n\n\n")
-        # the strain loop (moldin.F90:418-477): sums zeroed, box strained for
-        # ivol != 0, pressure stepped, first force and accelerations
-        sums = Sums()
-        if p.ivol != 0:
-            eng.set_tensor("b0", np.asarray(p.strx) @ eng.tensor("b0"))
-        eng.set_pressure(p.press + p.pressstep)
-        if p.iquen == 1:   # static relaxation instead of MD (moldin.F90:566-572)
+        fs_ref = float(np.float32(0.098227))    # one fs in model units, a REAL(4) literal (moldin.F90:132)
+        thermotimer = 0.0
+        every = p.nprint // 10 if (p.nprint >= 10 and p.nprint <= p.nsteps) else 0
+        summary = {}
+        press, temprq = p.press, p.temprq
+        for _loop in range(max(p.strainloops, 1)):
+            # one strain loop (moldin.F90:418-477): sums zeroed, box strained
+            # for ivol != 0, pressure and temperature stepped, first force and
+            # accelerations, then nsteps MD steps
+            out.write(f" Strainloop: T= {io.fortran_f(temprq, 12, 4)} dT= {io.fortran_f(p.tempsp, 12, 4)}\n")
+            out.write(f" Strainloop: P= {io.fortran_e(press, 14, 6)} dP= {io.fortran_e(p.pressstep, 14, 6)}\n")
+            sums = Sums()
+            if p.ivol != 0:
+                eng.set_tensor("b0", np.asarray(p.strx) @ eng.tensor("b0"))
+                bb = eng.tensor("b0")
+                out.write(" " * 10 + "TEMPRQ =".ljust(10) + io.fortran_f(temprq, 10, 2) + " " * 10 +
+                          "b0*strx:".ljust(10) + "\n")
+                for i in range(3):
+                    out.write("".join(io.fortran_e(bb[k, i], 17, 8).replace("E", "D") for k in range(3)) + "\n")
+                out.write("\n")
+            press = press + p.pressstep
+            temprq = temprq + p.tempsp
+            rqke_md = 1.5 * initial.BK * temprq * (n - 3)       # moldin.F90:437
+            eng.set_pressure(press)
             eng.md_init()
-            from . import quench as q
-            r = q.quench(eng, p.ivol, p.press + p.pressstep)
-            out.write(" QUENCH, FINDING MINIMUM ENERGY CONFIGURATION OF%5d ATOMS\n" % n)
-            out.write("FINAL U= %s V=%s P=%s H=%s\n" % (
-                io.fortran_e(r["pe"] / n, 18, 10).replace("E", "D"), io.fortran_f(r["vol"] / n, 10, 6),
-                io.fortran_e((p.press + p.pressstep) / initial_gpa(), 12, 4).replace("E", "D"),
-                io.fortran_e(r["h"] / n, 13, 7).replace("E", "D")))
+            if p.iquen == 1:   # static relaxation instead of MD (moldin.F90:566-572)
+                from . import quench as q
+                r = q.quench(eng, p.ivol, press)
+                out.write(" QUENCH, FINDING MINIMUM ENERGY CONFIGURATION OF%5d ATOMS\n" % n)
+                out.write("FINAL U= %s V=%s P=%s H=%s\n" % (
+                    io.fortran_e(r["pe"] / n, 18, 10).replace("E", "D"), io.fortran_f(r["vol"] / n, 10, 6),
+                    io.fortran_e(press / initial_gpa(), 12, 4).replace("E", "D"),
+                    io.fortran_e(r["h"] / n, 13, 7).replace("E", "D")))
+                summary = r
+                continue
+            for istep in range(counters["prevsteps"] + 1, counters["laststep"] + 1):
+                counters["currentstep"] = istep
+                counters["lastprint"] += 1
+                counters["lastchkpt"] += 1
+                pe, ke, te, h, th, vol = eng.md_steps(1)[0]
+                sc = eng.scalars()
+                sums.sf2 = sums.sf2 + sc["s1"]       # moldin.F90:762-765, every step
+                sums.sf2sq = sums.sf2sq + sc["s2"]
+                if every and counters["lastprint"] % every == 0:
+                    sums.update(pe, ke, te, h, th, vol, eng.tensor("b0"))
+                    if counters["lastprint"] % p.nprint == 0:
+                        out.write("\n" + io.runavs_text(istep - counters["prevsteps"], sums,
+                                                        ke / (1.5 * initial.BK * n), n, initial.BK,
+                                                        eng.tensor("b0"), eng.tensor("tp"), eng.tensor("tk")))
+                if p.nchkpt > 0 and counters["lastchkpt"] % p.nchkpt == 0:
+                    _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
+                    out.write("Checkpoint at step:%6d\n\n\n\n" % istep)
+                thermotimer = thermotimer + p.deltat                  # moldin.F90:828-835
+                if thermotimer >= fs_ref:
+                    thermotimer = thermotimer - fs_ref
+                    if nose != 0:
+                        eng.scalev(nose, rqke_md)
+                if p.iquen == 2:
+                    _box_quench(eng)
+                summary = dict(pe=pe, ke=ke, te=te, h=h, th=th, vol=vol, steps=istep)
+            counters["prevsteps"] = counters["currentstep"]             # moldin.F90:862, 902
+            counters["laststep"] = counters["currentstep"] + p.nsteps
+        if p.iquen == 1:
             en = eng.energy_calc()
             io.write_system_out(os.path.join(workdir, "system.out"), eng.tensor("b0"), eng.array("x0"),
                                 eng.array("y0"), eng.array("z0"), z, mass, en)
-            r["en_atom"] = en
-            return r
-        rqke_md = 1.5 * initial.BK * (p.temprq + p.tempsp) * (n - 3)   # moldin.F90:437
-        fs_ref = float(np.float32(0.098227))    # one fs in model units, a REAL(4) literal (moldin.F90:132)
-        thermotimer = 0.0
-        eng.md_init()
-        every = p.nprint // 10 if (p.nprint >= 10 and p.nprint <= p.nsteps) else 0
-        summary = {}
-        for istep in range(counters["prevsteps"] + 1, counters["laststep"] + 1):
-            counters["currentstep"] = istep
-            counters["lastprint"] += 1
-            counters["lastchkpt"] += 1
-            pe, ke, te, h, th, vol = eng.md_steps(1)[0]
-            sc = eng.scalars()
-            sums.sf2 = sums.sf2 + sc["s1"]       # moldin.F90:762-765, every step
-            sums.sf2sq = sums.sf2sq + sc["s2"]
-            if every and counters["lastprint"] % every == 0:
-                sums.update(pe, ke, te, h, th, vol, eng.tensor("b0"))
-                if counters["lastprint"] % p.nprint == 0:
-                    out.write("\n" + io.runavs_text(istep - counters["prevsteps"], sums, ke / (1.5 * initial.BK * n),
-                                                    n, initial.BK, eng.tensor("b0"), eng.tensor("tp"),
-                                                    eng.tensor("tk")))
-            if p.nchkpt > 0 and counters["lastchkpt"] % p.nchkpt == 0:
-                _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
-                out.write("Checkpoint at step:%6d\n\n\n\n" % istep)
-            thermotimer = thermotimer + p.deltat                  # moldin.F90:828-835
-            if thermotimer >= fs_ref:
-                thermotimer = thermotimer - fs_ref
-                if nose != 0:
-                    eng.scalev(nose, rqke_md)
-            if p.iquen == 2:
-                _box_quench(eng)
-            summary = dict(pe=pe, ke=ke, te=te, h=h, th=th, vol=vol, steps=istep)
+            summary["en_atom"] = en
+            return summary
         out.write("\n" + " " * 20 + "QUIT MOLECULAR DYNAMICS LOOP\n")
         en = eng.energy_calc()
         if p.nchkpt > 0:
@@ -273,6 +298,14 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
     finally:
         out.close()
         eng.close()
+
+
+def _scalev_ke(x1, b0, mass, z, deltat) -> float:
+    """Kinetic energy scalev forms before it scales (src/thermostat_m.F90:336-363):
+    the mean of x1 over all atoms removed from the moving ones."""
+    moving = (np.asarray(z) != 0) & (np.asarray(mass) >= float(np.float32(0.1)))
+    v = (np.asarray(x1)[moving] - np.asarray(x1).sum(axis=0) / float(len(z))) @ np.asarray(b0).T
+    return float((np.asarray(mass)[moving] * (v * v).sum(axis=1)).sum()) / deltat ** 2 / 2.0
 
 
 def _box_quench(eng) -> None:

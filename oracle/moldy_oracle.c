@@ -1264,6 +1264,32 @@ double orc_setvel(orc_rng *g, int n, const int *z, const double *mass, const dou
   return rqke;
 }
 
+/* setvel's box part (src/thermostat_m.F90:256-299), reached for ivol 1, 2, 3
+   and 5 only (ivol < 1 and ivol 4 return before it): b1(i,j) = rand1 - 0.5
+   drawn row by row, each row less its sum / nm, then every element scaled by
+   sqrt(temprq / boxtem) with boxtem = (sum b1(i,j)^2 / bdel2) / (4.5 bk).
+   b1 is column-major; it is left untouched when the box part is skipped. */
+void orc_setvel_box(orc_rng *g, int ivol, int nm, double temprq, double bdel2, double *b1) {
+  if (ivol < 1 || ivol == 4) return;
+  double s3 = 0.0;
+  for (int i = 1; i <= 3; i++) {
+    double s = 0.0;
+    for (int j = 1; j <= 3; j++) {
+      T(b1, i, j) = orc_rand1(g) - 0.5;
+      s = s + T(b1, i, j);
+    }
+    s = s / nm;
+    for (int j = 1; j <= 3; j++) {
+      T(b1, i, j) = T(b1, i, j) - s;
+      s3 = s3 + T(b1, i, j) * T(b1, i, j);
+    }
+  }
+  const double boxtem = (s3 / bdel2) / (4.5 * orc_bk());
+  const double scale = sqrt(temprq / boxtem);
+  for (int i = 1; i <= 3; i++)
+    for (int j = 1; j <= 3; j++) T(b1, i, j) = T(b1, i, j) * scale;
+}
+
 /* ---------------- accessors ---------------- */
 static double *arr_by_name(const orc_sys *s, const char *nm) {
   static const char *names[] = {"x0", "y0", "z0", "x1", "y1", "z1", "x2", "y2", "z2",

@@ -98,6 +98,7 @@ void orc_replicate(orc_rng *g, int nbasis, const double *xb, const double *yb, c
  * 1.5 bk temprq (nm - 1) as init_thermostat_m sets it.  Returns ke. */
 double orc_setvel(orc_rng *g, int n, const int *z, const double *mass, const double *b0,
                   double deltat, double rqke, double *x1, double *y1, double *z1);
+void orc_setvel_box(orc_rng *g, int ivol, int nm, double temprq, double bdel2, double *b1);
 /* 1.5 * bk * temprq * (nm - 1) with bk of src/constants_m.F90 */
 double orc_rqke(double temprq, int nm);
 /* Boltzmann constant in eV/K (src/constants_m.F90) */

@@ -100,6 +100,8 @@ def lib():
         L.orc_setvel.argtypes = [C.POINTER(OrcRng), C.c_int, ip, dp, dp, C.c_double, C.c_double,
                                  dp, dp, dp]
         L.orc_setvel.restype = C.c_double
+        L.orc_setvel_box.argtypes = [C.POINTER(OrcRng), C.c_int, C.c_int, C.c_double, C.c_double, dp]
+        L.orc_setvel_box.restype = None
         L.orc_rqke.argtypes = [C.c_double, C.c_int]
         L.orc_rqke.restype = C.c_double
         L.orc_bk.restype = C.c_double
@@ -283,10 +285,11 @@ def bk() -> float:
 
 
 def moldin_new_run(basis_frac, basis_z, basis_mass, nt, cell_b0, dsp: float, temprq: float,
-                   deltat_fs: float):
+                   deltat_fs: float, ivol=None, boxmass: float = 0.0):
     """Initial state of a new run as moldin makes it: rstart(97,33,21,82),
     read_system's replication (with its 3 draws per atom) and setvel.
-    Returns (b0 of the whole box, s (n,3), x1 (n,3), z, mass)."""
+    Returns (b0 of the whole box, s (n,3), x1 (n,3), z, mass); with ivol given,
+    also setvel's box velocities b1 (3,3) for that ivol and box mass per atom."""
     g = Ranmar()
     nb = len(basis_z)
     nt = np.ascontiguousarray(nt, dtype=np.int32)
@@ -305,7 +308,36 @@ def moldin_new_run(basis_frac, basis_z, basis_mass, nt, cell_b0, dsp: float, tem
     dt = deltat_fs * lattice.FS_TO_TIME
     lib().orc_setvel(C.byref(g.g), n, _ip(z), _dp(mass), _dp(bf), dt, lib().orc_rqke(temprq, n),
                      _dp(x1), _dp(y1), _dp(z1))
-    return b0, np.stack([x0, y0, z0], axis=1), np.stack([x1, y1, z1], axis=1), z, mass
+    out = (b0, np.stack([x0, y0, z0], axis=1), np.stack([x1, y1, z1], axis=1), z, mass)
+    if ivol is None:
+        return out
+    b1 = np.zeros(9)
+    lib().orc_setvel_box(C.byref(g.g), int(ivol), n, float(temprq), dt * dt / (2.0 * n * boxmass), _dp(b1))
+    return out + (b1.reshape(3, 3, order="F"),)
+
+
+def new_run_velocities(sysdef, temprq: float, ivol: int, boxmass: float):
+    """Velocities of a new run for a system given atom by atom, as moldin
+    makes them (and the reference driver with its flag 2): rstart(97,33,21,82),
+    the 3 draws per atom of read_system, setvel's atom part with
+    RQKE = 1.5 bk temprq (nm - 1), then its box part (ivol 1, 2, 3, 5).
+    Returns (x1 (n,3), b1 (3,3)); atoms that setvel skips keep sysdef.x1."""
+    g = Ranmar()
+    n = sysdef.n
+    for _ in range(3 * n):
+        lib().orc_rand1(C.byref(g.g))
+    z = np.ascontiguousarray(sysdef.z, dtype=np.int32)
+    mass = np.ascontiguousarray(sysdef.mass, dtype=np.float64)
+    bf = np.asfortranarray(sysdef.b0).ravel(order="F").copy()
+    x1 = np.ascontiguousarray(np.asarray(sysdef.x1, dtype=np.float64).T).copy()
+    from moldy_amd import lattice
+    dt = sysdef.deltat_fs * lattice.FS_TO_TIME
+    lib().orc_setvel(C.byref(g.g), n, _ip(z), _dp(mass), _dp(bf), dt, lib().orc_rqke(temprq, n),
+                     _dp(x1[0]), _dp(x1[1]), _dp(x1[2]))
+    b1 = np.zeros(9)
+    bdel2 = dt * dt / (2.0 * n * boxmass)
+    lib().orc_setvel_box(C.byref(g.g), int(ivol), n, float(temprq), bdel2, _dp(b1))
+    return x1.T.copy(), b1.reshape(3, 3, order="F")
 
 
 def bruteforce_pairs(b0, s, rcut: float, ivol: int = 1) -> np.ndarray:

@@ -29,6 +29,7 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 DRIVER_FEC = os.path.join(ROOT, "oracle", "_ref", "ref_driver_fec")
 OUT = os.path.dirname(os.path.abspath(__file__))
 BCC_IRON = "/root/reference/tests/reference/bcc_iron"
+REF_TESTS = "/root/reference/tests/reference"
 
 
 def r_grid(material: str = "fetransmetal") -> np.ndarray:
@@ -176,6 +177,13 @@ CASES = {
     "slab_ivol2_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=91,
                              open_axes=(2,), immobile=0.125),
                         dict(ivol=2, ntcm=10, nsteps=12, uselookup=0)),
+    # the slab with a new run's velocities as moldin makes them (flag 2: the
+    # reference's setvel, whose box part draws b1 for ivol=2, and scalev) and
+    # the Nose thermostat during the run (flag 16)
+    "setvel_slab_ivol2_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=0.0, deltat_fs=1.0, seed=94,
+                                    open_axes=(2,), immobile=0.125),
+                               dict(ivol=2, ntcm=10, nsteps=12, uselookup=0, flags=18, nose=50.0,
+                                    temprq=300.0, boxmass=50.0)),
     # ivol=5: a pillar, periodic along z only
     "pillar_ivol5_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=92, open_axes=(0, 1)),
                           dict(ivol=5, ntcm=10, nsteps=12, uselookup=0)),
@@ -311,6 +319,16 @@ def bcc_iron_fixture():
     print(f"bcc_iron: {len(idx)} sampled atoms -> {path} ({os.path.getsize(path) // 1024} KiB)")
 
 
+def shear_inputs_fixture():
+    """The system file of the reference's shipped test tests/reference/shear
+    (3480 atoms given one by one, 348 of them immobile), kept as text for the
+    main program's runs on its inputs."""
+    text = open(os.path.join(REF_TESTS, "shear", "system.in")).read()
+    path = os.path.join(OUT, "shear_inputs.npz")
+    np.savez_compressed(path, system_in=np.array(text))
+    print(f"shear_inputs -> {path} ({os.path.getsize(path) // 1024} KiB)")
+
+
 def main(names=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -318,6 +336,8 @@ def main(names=None):
         raise SystemExit("build the reference driver first: oracle/build_ref.sh")
     if names and "bcc_iron" in names:
         bcc_iron_fixture()
+    if names and "shear_inputs" in names:
+        shear_inputs_fixture()
     for name, (skw, rkw) in CASES.items():
         if names and name not in names:
             continue

@@ -167,3 +167,111 @@ def test_gpu_main_program_box_quench_matches_oracle(tmp_path, oracle):
     # without the quench the same run ends measurably elsewhere
     U = oracle_run(False)
     assert abs(U.scalar("pe") - S.scalar("pe")) > 1e-6 * abs(S.scalar("pe"))
+
+
+def _shear_params(nsteps, loops):
+    """params.in of the reference's shipped test tests/reference/shear (less
+    the prevsteps/lastprint/lastchkpt lines, keys today's read_params stops
+    on), with nsteps and strainloops cut down."""
+    return ('title1 = "shear"\ntitle2 = "fe_bcc_101010"\nfile_system = system.in\n'
+            "file_textout = output66.txt\nfile_checkpointwrite = chkp_out.dat\n"
+            "boxmass = 50.\ndeltat = 1.0d0\nrpad = 1.0d0\ntemprq = 300.d0\npress = 0.0\nivol = 2\n"
+            "iquen = 0\nnose = 100\ndsp = 0.0d0\n"
+            "straintensor = 1.0 0.0 0.0 0.0 1.0 0.0 0.0 0.0 1.0\n"
+            "straintensor_row1 = 1.000 0.0 0.001\nstraintensor_row2 = 0.0 1.000 0.0\n"
+            "straintensor_row3 = -0.001 0.0 1.000\nnm = 3480\nnspec = 1\n"
+            f"nsteps = {nsteps}\nnbrupdate = 2000\nstrainloops = {loops}\nrestart = 0\nnprint = 10\n"
+            "nchkpt = 1000\nnnbrs = 150\niverlet = 1\nuselookup = .true.\n")
+
+
+def test_gpu_main_program_strain_loops_open_slab_matches_oracle(tmp_path, oracle):
+    """Strain loops, setvel's box velocities (ivol=2), immobile atoms, the
+    Nose thermostat and lookup tables through the main program, on the inputs
+    of tests/reference/shear (3 loops of 20 steps), against the oracle driven
+    loop by loop as moldin.F90:418-477 drives the reference.  Positions 1e-11
+    (fractional), velocities 1e-8 relative, PE 1e-10, box to 1e-13 and the
+    box velocities exactly (they do not move for ivol=2)."""
+    from moldy_amd import initial, lattice, moldin
+    from conftest import load_golden
+    nsteps, loops = 20, 3
+    text = str(load_golden("shear_inputs")["system_in"])
+    d = str(tmp_path)
+    with open(os.path.join(d, "params.in"), "w") as fh:
+        fh.write(_shear_params(nsteps, loops))
+    with open(os.path.join(d, "system.in"), "w") as fh:
+        fh.write(text)
+    print("[gpu-test] main program, strain loops on the shear inputs", flush=True)
+    got = moldin.run(d, material="iron_carbon")
+    ck = moldin.read_checkpoint(os.path.join(d, "chkp_out.dat"), 3480)
+    out = open(os.path.join(d, "output66.txt")).read()
+    assert out.count("Strainloop: T=") == loops
+    assert out.count("RUNNING AVERAGES AT STEP NO.    20") == loops
+
+    nbasis, nt, cell, basis = initial.parse_system(text)
+    bas = np.array([b[:3] for b in basis])
+    b0, s, x1, z, mass, b1 = oracle.moldin_new_run(bas, np.array([b[3] for b in basis]),
+                                                   np.array([b[4] for b in basis]), np.array(nt), cell, 0.0,
+                                                   300.0, 1.0, ivol=2, boxmass=50.0)
+    assert int((mass < 0.1).sum()) == 348
+    S = oracle.OracleSystem(lattice.System(b0=b0, s=s, x1=x1, mass=mass, z=z, deltat_fs=1.0),
+                            oracle.Potential(material="iron_carbon", lookup=True), ivol=2, rpad=1.0,
+                            nnbrs=150, ntcm=2000, press_gpa=0.0, boxmass=50.0)
+    S.set_tensor("b1", b1)
+    S.scalev(100.0, oracle.lib().orc_rqke(300.0, 3480))
+    S.call("linkup")
+    strx = np.array([[1.0, 0.0, -0.001], [0.0, 1.0, 0.0], [0.001, 0.0, 1.0]])   # strx(:,j) = row j as read
+    fs, tt, dt = float(np.float32(0.098227)), 0.0, S.scalar("deltat")
+    rq = 1.5 * oracle.bk() * 300.0 * (3480 - 3)
+    for _ in range(loops):
+        S.set_tensor("b0", strx @ S.tensor("b0"))
+        assert S.call("md_init") == 0
+        for _ in range(nsteps):
+            assert S.call("md_step") == 0
+            tt = tt + dt
+            if tt >= fs:
+                tt = tt - fs
+                S.scalev(100.0, rq)
+    bw = S.tensor("b0")
+    assert np.max(np.abs(ck["b0"] - bw)) <= 1e-13 * np.max(np.abs(bw))
+    np.testing.assert_array_equal(ck["b1"], S.tensor("b1"))
+    np.testing.assert_array_equal(ck["b1"], b1)
+    for k, nm in enumerate(("x0", "y0", "z0")):
+        assert np.max(np.abs(ck["x0"][:, k] - S.array(nm))) < 1e-11, nm
+    v = np.stack([S.array(nm) for nm in ("x1", "y1", "z1")], axis=1)
+    assert np.max(np.abs(ck["x1"] - v)) <= 1e-8 * np.max(np.abs(v))
+    assert abs(got["pe"] - S.scalar("pe")) <= 1e-10 * abs(S.scalar("pe"))
+
+
+def test_gpu_main_program_quench_matches_reference(tmp_path):
+    """iquen = 1 through the main program: the 249-atom vacancy cell of the
+    fixture quench_vac_249 written as a system file (atoms one by one), relaxed
+    by the conjugate gradients on the device force, against the reference's
+    own quench (VA14CD) of the same cell: the printed FINAL U within the
+    reference's stopping tolerance 5e-7 eV per atom (twice: both runs stop
+    within it of the minimum)."""
+    from conftest import load_golden
+    from moldy_amd import moldin
+    F = load_golden("quench_vac_249")
+    n = int(len(F["z"]))
+    d = str(tmp_path)
+    with open(os.path.join(d, "params.in"), "w") as fh:
+        fh.write('title1 = "vacancy"\ntitle2 = "quench"\nfile_system = system.in\n'
+                 "file_textout = output66.txt\nboxmass = 50.\ndeltat = 1.0d0\nrpad = 0.3d0\n"
+                 f"temprq = 0.d0\npress = 0.0\nivol = {int(F['run_ivol'])}\niquen = 1\nnm = {n}\n"
+                 "nspec = 1\nnsteps = 0\nnbrupdate = 10\nrestart = 0\nnprint = 100\nnchkpt = -1\n"
+                 "iverlet = 1\nuselookup = .false.\n")
+    b0, s = np.asarray(F["b0"]), np.asarray(F["s"])
+    with open(os.path.join(d, "system.in"), "w") as fh:
+        fh.write(f"{n}\n1 1 1\n")
+        for i in range(3):
+            fh.write(" ".join(repr(float(v)) for v in b0[:, i]) + "\n")
+        for k in range(n):
+            fh.write(" ".join(repr(float(v)) for v in s[k]) + f" {int(F['z'][k])} {float(F['mass'][k])!r}\n")
+    print("[gpu-test] main program, quench", flush=True)
+    r = moldin.run(d)
+    line = [ln for ln in open(os.path.join(d, "output66.txt")).read().splitlines() if ln.startswith("FINAL U=")]
+    assert len(line) == 1
+    u = float(line[0].split()[2].replace("D", "E"))
+    assert abs(u - float(F["pe_q"]) / n) <= 1e-6, (u, float(F["pe_q"]) / n)
+    assert abs(r["pe"] - float(F["pe_q"])) <= 1e-6 * n
+    assert os.path.exists(os.path.join(d, "system.out"))

@@ -90,6 +90,56 @@ def test_nose_thermostat_bit_exact_vs_reference(oracle):
         np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
 
 
+def _thermostat_run(S, F, oracle):
+    rqke = 1.5 * oracle.bk() * float(F["run_temprq"]) * (S.n - 3)
+    fs, tt, dt = float(np.float32(0.098227)), 0.0, S.scalar("deltat")
+    tr = []
+    for _ in range(int(F["run_nsteps"])):
+        assert S.call("md_step") == 0
+        tr.append((S.scalar("pe"), S.scalar("ke"), S.scalar("s1"), S.scalar("s3"), S.scalar("vol"), S.int("ntc")))
+        tt = tt + dt
+        if tt >= fs:
+            tt = tt - fs
+            S.scalev(float(F["run_nose"]), rqke)
+    return np.array(tr)
+
+
+def test_new_run_velocities_bit_exact_vs_reference(oracle):
+    """A new run's velocities for an open slab with immobile atoms, as moldin
+    makes them (the reference driver's flag 2: RANMAR, read_system's draws,
+    setvel with its box part for ivol=2, scalev), then the Nose thermostat
+    during the run.  The box velocities b1 (constant for ivol=2) and the
+    trajectory bit for bit; the product's setvel and setvel_box draw the same."""
+    from moldy_amd import initial
+    F = load_golden("setvel_slab_ivol2_1024")
+    sd = golden_system(F)
+    temprq, ivol, boxmass = float(F["run_temprq"]), int(F["run_ivol"]), float(F["run_boxmass"])
+    x1, b1 = oracle.new_run_velocities(sd, temprq, ivol, boxmass)
+    np.testing.assert_array_equal(b1, F["b1_f"])
+    moving = sd.mass >= float(np.float32(0.1))
+    assert 0 < int((~moving).sum()) < sd.n
+    rng = initial.Ranmar(97, 33, 21, 82)
+    for _ in range(3 * sd.n):
+        rng.rand1()
+    dt = sd.deltat_fs * lattice.FS_TO_TIME
+    x1p = initial.setvel(rng, sd.z, sd.mass, sd.b0, dt, initial.rqke_init(temprq, sd.n))
+    b1p = initial.setvel_box(rng, ivol, sd.n, temprq, dt * dt / (2.0 * sd.n * boxmass))
+    np.testing.assert_array_equal(x1p[moving], x1[moving])
+    np.testing.assert_array_equal(b1p, b1)
+    assert not x1p[~moving].any() and not x1[~moving].any()
+    S = oracle.OracleSystem(lattice.System(b0=sd.b0, s=sd.s, x1=x1, mass=sd.mass, z=sd.z,
+                                           deltat_fs=sd.deltat_fs), oracle.Potential(), **run_kwargs(F))
+    S.set_tensor("b1", b1)
+    S.scalev(float(F["run_nose"]), oracle.lib().orc_rqke(temprq, sd.n))
+    S.call("linkup")
+    assert S.call("md_init") == 0
+    np.testing.assert_array_equal(_thermostat_run(S, F, oracle), F["trace"])
+    for nm in ("x0", "y0", "z0", "x1", "y1", "z1", "fx", "fy", "fz"):
+        np.testing.assert_array_equal(S.array(nm), F[nm + "_f"], err_msg=nm)
+    for nm in ("b0", "b1"):
+        np.testing.assert_array_equal(S.tensor(nm), F[nm + "_f"], err_msg=nm)
+
+
 def test_gear_coefficients_are_the_textbook_ones():
     """Gear's 4-value corrector for a second-order equation: 1/6, 5/6, 1, 1/3
     (Allen & Tildesley, Computer Simulation of Liquids, table E.2), here in
