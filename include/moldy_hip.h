@@ -113,7 +113,10 @@ int mdk_set_lookup(void *h, int nlookup);
 /* out[0..7] = bier_cut1, bier_cut2, spline_coeff(1:4), phi_cut(1:2) (fetransmetal only) */
 int mdk_potential_constants(void *h, double *out);
 /* Evaluate potential_m's vee,dvee,phi,dphi (which=0..3) at r[k], or emb,demb
- * (which=4,5) at rho[k], on the device, k < count. */
+ * (which=4,5) at rho[k], on the device, k < count.  which=6..8: V, V' and phi'
+ * as the FP32 force pass evaluates them (its interval table; exact-mode
+ * fetransmetal with 6 pair and 2 density knots only, MDK_ERR_BAD_ARG
+ * otherwise). */
 int mdk_eval_potential(void *h, int which, const double *x, double *y, int count);
 
 /* State transfer (ownership stays with the caller; arrays in reference order). */

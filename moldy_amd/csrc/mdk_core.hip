@@ -154,6 +154,8 @@ struct mdk_ctx {
   PotDev pot;
   bool pot_set;
   double *tabs;
+  float *fetab;   // interval table of the FP32 Fe-Fe pair terms (MDK_FE_TAB floats)
+  bool fe_ok;     // fetab holds the current knots' table
   hostpot::Knots knots;
   int material;            // MDK_MAT_*: fetransmetal (knots) or iron_carbon (fec)
   int iverlet;             // md_steps integrator: 1 velocity Verlet, 0 predictor-corrector
@@ -1194,16 +1196,23 @@ template <typename T, bool MASSCHK, int POTM, bool P32>
 __global__ void __launch_bounds__(MDK_BLOCK)
 k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restrict__ p4f, const int *zat,
         int n, const int *__restrict__ nbr, const int *__restrict__ nvisit, const BoxState *bs, PackGeo pg,
-        PotDev P, double *part, const double *spec) {
+        PotDev P, double *part, const double *spec, const float *fetab) {
   __shared__ double S[32][3];
-  __shared__ float vtab[8][8];  // interval cubics of the pair function (FP32 Fe-Fe form)
+  // interval table of the FP32 Fe-Fe pair terms: bin records, then intervals
+  __shared__ float4 fbin[MDK_FE_NB];
+  __shared__ __align__(16) float fiv[8 * MDK_FE_NI];
   constexpr bool CUBIC = sizeof(T) == 4 && POTM == POT_FE;
   double b[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
   __shared__ double bi[9];  // B0^-1 for the fractional forces
   shift_table(b, pg, S);
-  if (CUBIC && threadIdx.x < 64) vtab[threadIdx.x >> 3][threadIdx.x & 7] = P.vcub[threadIdx.x >> 3][threadIdx.x & 7];
+  if (CUBIC) {
+    const float4 *src = (const float4 *)fetab;
+    if (threadIdx.x < MDK_FE_NB) fbin[threadIdx.x] = src[threadIdx.x];
+    else if (threadIdx.x < MDK_FE_NB + 2 * MDK_FE_NI)
+      ((float4 *)fiv)[threadIdx.x - MDK_FE_NB] = src[threadIdx.x];
+  }
   if (threadIdx.x == 64) inv3(b, bi);
   __syncthreads();
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
@@ -1234,10 +1243,13 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
         const T r2 = pair_sep<T>(S, pi, pj[u], (int)(eu >> NBR_CODE_SHIFT), dx, dy, dz, rx, ry, rz);
         T r, rr;
         r_and_inv(r2, r, rr);
-        T vv, dv;
-        if constexpr (CUBIC) mdk_vee_fe_cubic(P, vtab, r, vv, dv);
-        else mdk_pair_vdv<T, POTM>(P, r, vv, dv);
-        const T dph = mdk_pair_dphi<T, POTM>(P, r);
+        T vv, dv, dph;
+        if constexpr (CUBIC) {
+          mdk_fe_terms(P, fbin, fiv, r, vv, dv, dph);
+        } else {
+          mdk_pair_vdv<T, POTM>(P, r, vv, dv);
+          dph = mdk_pair_dphi<T, POTM>(P, r);
+        }
         const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj[u].w) * rr);
         const double fpd = (double)fp;
         const double gx = dx * fpd, gy = dy * fpd, gz = dz * fpd;
@@ -1710,9 +1722,15 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_energy_calc(const double *atoms, 
 }
 
 // potential evaluation for mdk_eval_potential
-__global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int cnt) {
+__global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int cnt, const float *fetab) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
+  if (which >= 6) {  // the FP32 force pass's V, V', phi' (interval table)
+    float v32, dv32, dph32;
+    mdk_fe_terms(P, (const float4 *)fetab, fetab + 4 * MDK_FE_NB, (float)x[k], v32, dv32, dph32);
+    y[k] = which == 6 ? v32 : (which == 7 ? dv32 : dph32);
+    return;
+  }
   double r = x[k], v = 0.0, dv = 0.0;
   const bool lk = P.use_lookup != 0, fec = P.material == MDK_MAT_IRON_CARBON;
   switch (which) {
@@ -2015,7 +2033,8 @@ static void force_dispatch_p(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   hipExtLaunchKernelGGL(k_force<T, MC, POTM, P32>, dim3(nblocks(c->n)), dim3(MDK_BLOCK), 0, c->stream, ea, eb, 0,
                         c->atoms, (const double4 *)c->pos4, (const float4 *)p4f_of(c), (const int *)c->zat, c->n,
                         (const int *)c->nbr, (const int *)(POTM == POT_TABLE ? c->numn : c->nact),
-                        (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f, spec_of(c));
+                        (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f, spec_of(c),
+                        (const float *)c->fetab);
 }
 template <typename T, bool MC, int POTM>
 static void force_dispatch(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
@@ -2029,7 +2048,8 @@ static void force_dispatch(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
 static int pot_mode(const mdk_ctx *c) {
   if (c->pot.use_lookup) return POT_TABLE;
   if (c->material == MDK_MAT_IRON_CARBON) return POT_FEC;
-  return (c->pot.nv == MDK_FE_NV && c->pot.np == MDK_FE_NP) ? POT_FE : POT_KNOTS;
+  if (c->pot.nv != MDK_FE_NV || c->pot.np != MDK_FE_NP) return POT_KNOTS;
+  return (c->p.precision == MDK_PREC_FP32 && !c->fe_ok) ? POT_KNOTS : POT_FE;
 }
 template <typename T>
 static void density_pot(mdk_ctx *c, int do_embed) {
@@ -2369,7 +2389,7 @@ extern "C" int mdk_destroy(void *h) {
   void *ptrs[] = {c->pos4, c->p4f, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
-                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red};
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red, c->fetab};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
@@ -2405,6 +2425,12 @@ extern "C" int mdk_set_potential(void *h, int nv, const double *av, const double
   PotDev &P = c->pot;
   const int keep_lookup = P.use_lookup, keep_n = P.nlookup;
   const double rint = hostpot::fill_potdev(P, k);
+  float fetab[MDK_FE_TAB];
+  c->fe_ok = hostpot::build_fe_table(k, fetab, P.fe_binh, P.fe_binc);
+  if (c->fe_ok) {
+    if (!c->fetab) HIPCK(hipMalloc(&c->fetab, sizeof(fetab)));
+    HIPCK(hipMemcpy(c->fetab, fetab, sizeof(fetab), hipMemcpyHostToDevice));
+  }
   c->material = MDK_MAT_FETRANSMETAL;
   c->r_int_exact = rint;
   c->r_list = rint + c->skin;
@@ -2502,12 +2528,14 @@ extern "C" int mdk_potential_constants(void *h, double *out) {
 extern "C" int mdk_eval_potential(void *h, int which, const double *x, double *y, int count) {
   mdk_ctx *c = H(h);
   if (!c || !c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
-  if (which < 0 || which > 5 || count < 0 || (count > 0 && (!x || !y))) return set_err(MDK_ERR_BAD_ARG, "bad eval args");
+  if (which < 0 || which > 8 || count < 0 || (count > 0 && (!x || !y))) return set_err(MDK_ERR_BAD_ARG, "bad eval args");
+  if (which >= 6 && (!c->fe_ok || c->material != MDK_MAT_FETRANSMETAL))
+    return set_err(MDK_ERR_BAD_ARG, "no FP32 interval table for this potential");
   if (count == 0) return MDK_OK;
   double *d = nullptr;
   HIPCK(hipMalloc(&d, sizeof(double) * 2 * (size_t)count));
   HIPCK(hipMemcpyAsync(d, x, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
-  k_eval_pot<<<nblocks(count), MDK_BLOCK, 0, c->stream>>>(c->pot, which, d, d + count, count);
+  k_eval_pot<<<nblocks(count), MDK_BLOCK, 0, c->stream>>>(c->pot, which, d, d + count, count, c->fetab);
   int rc = check_launch("k_eval_pot");
   hipMemcpyAsync(y, d + count, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream);
   hipStreamSynchronize(c->stream);

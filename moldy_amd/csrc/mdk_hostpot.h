@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "mdk_pot.h"
 
@@ -147,25 +148,101 @@ static double fill_potdev(PotDev &P, const Knots &k) {
   P.phi_cut_vf = (float)P.phi_cut_v; P.r_intf = (float)rint;
   P.jx1f = (float)P.jx1; P.jv1f = (float)P.jv1; P.jx2f = (float)P.jx2; P.jv2f = (float)P.jv2;
   P.bier_af = (float)P.bier_a; P.bier_zedf = (float)P.bier_zed;
-  // interval cubics of the pair function (used by the FP32 form of 6-knot sets)
-  int ord[MDK_MAXV];
-  for (int i = 0; i < k.nv; i++) ord[i] = i;
-  std::sort(ord, ord + k.nv, [&](int a, int b) { return k.rv[a] > k.rv[b]; });
-  for (int i = 0; i < 8; i++) P.vknf[i] = i < k.nv ? (float)k.rv[ord[i]] : -1.0f;
-  for (int j = 0; j <= k.nv && j < 8; j++) {
-    const double u = j > 0 ? k.rv[ord[j - 1]] : (k.nv > 0 ? k.rv[ord[0]] : 0.0);
-    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-    for (int i = 0; i < j; i++) {
-      const double a = k.av[ord[i]], d = k.rv[ord[i]] - u;
-      c0 += a * d * d * d;
-      c1 += 3.0 * a * d * d;
-      c2 += 3.0 * a * d;
-      c3 += a;
-    }
-    const double row[8] = {u, c0, c1, c2, c3, 2.0 * c2, 3.0 * c3, 0.0};
-    for (int m = 0; m < 8; m++) P.vcub[j][m] = (float)row[m];
-  }
+  P.fe_binh = 0.0f;
+  P.fe_binc = 0.0f;
   return rint;
+}
+
+// Interval table of the FP32 Fe-Fe pair terms (mdk_fe_terms): breakpoints are
+// the pair knots, the density knots and the density maximum r*, in descending
+// order B[0] > B[1] > ...; interval m is B[m] <= r < B[m-1] (m = 0: r >= B[0],
+// where every term vanishes; the last: r < B[last]).  Record of interval m,
+// with u its expansion point and d = knot - u over the knots at or above the
+// interval's upper end:
+//   u, c0, c1, c2, c3, q0, q1, q2
+// c0..c3 = sum a d^3, 3 sum a d^2, 3 sum a d, sum a (pair knots); q0..q2 =
+// -3 sum A d^2, -6 sum A d, -3 sum A (density knots, zero when u <= r*).  The
+// bin grid: h = 2^-k (k = 4..8), the first with every breakpoint spacing
+// above h, every breakpoint at least 1e-6 A from a bin boundary, and all
+// breakpoints inside MDK_FE_NB bins from r_lo = floor(B_min / h) h.  Returns
+// false when no such grid exists (then the FP32 form uses the knot sums).
+// tab: MDK_FE_NB bin records (split, m below, m at or above, 0; m as the
+// float offset 8 m, stored as int bits) then the MDK_FE_NI interval records.
+inline bool build_fe_table(const Knots &k, float *tab, float &binh, float &binc) {
+  if (k.nv != MDK_FE_NV || k.np != MDK_FE_NP) return false;
+  std::vector<double> B;
+  for (int i = 0; i < k.nv; i++) B.push_back(k.rv[i]);
+  for (int i = 0; i < k.np; i++) B.push_back(k.rp[i]);
+  if (k.pcr > 0.0) B.push_back(k.pcr);
+  std::sort(B.begin(), B.end(), [](double a, double b) { return a > b; });
+  const int nb = (int)B.size();
+  if (nb + 1 > MDK_FE_NI) return false;
+  for (int i = 1; i < nb; i++)
+    if (!(B[i - 1] - B[i] > 0.0)) return false;
+  memset(tab, 0, sizeof(float) * MDK_FE_TAB);
+  float *iv = tab + 4 * MDK_FE_NB;
+  for (int m = 0; m < MDK_FE_NI; m++) {
+    // expansion point: the interval's midpoint (its upper end for the two
+    // unbounded ones), which halves |x| and the rounding of the Horner steps
+    const int mm = std::min(m, nb);
+    const double u = mm == 0 ? B[0] : (mm < nb ? 0.5 * (B[mm - 1] + B[mm]) : B[mm - 1]);
+    double c[4] = {0, 0, 0, 0}, q[3] = {0, 0, 0};
+    if (m > 0) {
+      const double top = B[mm - 1];
+      for (int i = 0; i < k.nv; i++) {
+        if (k.rv[i] < top) continue;
+        const double a = k.av[i], d = k.rv[i] - u;
+        c[0] += a * d * d * d; c[1] += 3.0 * a * d * d; c[2] += 3.0 * a * d; c[3] += a;
+      }
+      if (top > k.pcr) {
+        for (int i = 0; i < k.np; i++) {
+          if (k.rp[i] < top) continue;
+          const double a = k.ap[i], d = k.rp[i] - u;
+          q[0] += -3.0 * a * d * d; q[1] += -6.0 * a * d; q[2] += -3.0 * a;
+        }
+      }
+    }
+    const double row[8] = {u, c[0], c[1], c[2], c[3], q[0], q[1], q[2]};
+    for (int t = 0; t < 8; t++) iv[8 * m + t] = (float)row[t];
+  }
+  for (int e = 4; e <= 8; e++) {
+    const double h = ldexp(1.0, -e);
+    bool ok = true;
+    for (int i = 1; i < nb && ok; i++) ok = B[i - 1] - B[i] > h;
+    const double lo = floor(B[nb - 1] / h) * h;
+    ok = ok && (B[0] - lo) / h < MDK_FE_NB - 1;
+    for (int i = 0; i < nb && ok; i++) {
+      const double f = (B[i] - lo) / h;
+      ok = std::min(f - floor(f), ceil(f) - f) * h > 1e-6;
+    }
+    if (!ok) continue;
+    for (int b = 0; b < MDK_FE_NB; b++) {
+      const double blo = lo + b * h, bhi = blo + h;
+      int above = 0, split = -1;
+      for (int i = 0; i < nb; i++) {
+        if (B[i] >= bhi) above++;
+        else if (B[i] >= blo) split = i;
+      }
+      float *r = tab + 4 * b;
+      int mlo, mge;
+      if (split >= 0) {
+        r[0] = (float)B[split];
+        mge = split;        // breakpoints strictly above the split
+        mlo = split + 1;
+      } else {
+        r[0] = INFINITY;
+        mge = mlo = above;
+      }
+      const int olo = 8 * mlo, oge = 8 * mge;
+      memcpy(&r[1], &olo, sizeof(int));
+      memcpy(&r[2], &oge, sizeof(int));
+      r[3] = 0.0f;
+    }
+    binh = (float)(1.0 / h);
+    binc = (float)(-lo / h);
+    return true;
+  }
+  return false;
 }
 
 // Sensitivities of the set_up-derived constants to the knot parameters, for

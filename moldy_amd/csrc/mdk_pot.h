@@ -44,11 +44,9 @@ struct PotDev {
   // pre-derived join constants: log(V) and V'/V at both ends
   double jx1, jv1, jx2, jv2;
   float jx1f, jv1f, jx2f, jv2f, bier_af, bier_zedf;
-  // FP32 pair function of a 6-knot set as one cubic per knot interval: vknf
-  // holds the knots in descending order, vcub[j] the interval with j knots
-  // beyond r as (u, c0, c1, c2 | c3, 2 c2, 3 c3, 0), u the upper end
-  float vknf[8];
-  float vcub[8][8];
+  // FP32 Fe-Fe pair terms (the force pass of the FP32 form): bin of r in the
+  // interval table is (int)(r * fe_binh + fe_binc), fe_binh = 1/h
+  float fe_binh, fe_binc;
   // material (MDK_MAT_*); iron_carbon Fe-Fe data (src/iron_carbon.F90:44-74):
   // pair knots, amplitudes and 3x amplitudes of the 13-knot sum, the cubic in
   // the join's exponent, density knots with the formula's signs (+, -, +),
@@ -230,31 +228,44 @@ __device__ __forceinline__ T mdk_dphi_n(const PotDev &P, T r) {
   return r < Q.pcr() ? T(0) : s;
 }
 
-// FP32 pair function V, V' of a 6-knot set (POT_FE) from the interval cubics:
-// j = number of knots beyond r picks the interval, x = u_j - r >= 0, and
-// V = c0 + x (c1 + x (c2 + x c3)) = sum_k a_k (r_k - r)^3 over the knots
-// beyond r (c0..c3 = sum a d^3, 3 sum a d^2, 3 sum a d, sum a with d = r_k - u_j,
-// formed in FP64 on the host), V' = -(c1 + x (2 c2 + 3 c3 x)).  Horner in the
-// local variable is about twice as accurate as the FP32 knot sum and costs
-// 19 VALU operations instead of 36.  tab is the coefficient table in LDS.
-__device__ __forceinline__ void mdk_vee_fe_cubic(const PotDev &P, const float (*tab)[8], float r, float &v,
-                                                 float &dv) {
-  int j = 0;
-#pragma unroll
-  for (int k = 0; k < MDK_FE_NV; k++) j += r < P.vknf[k] ? 1 : 0;
-  const float4 a = *(const float4 *)tab[j], b = *(const float4 *)(tab[j] + 4);
+// FP32 pair terms V, V' and phi' of a 6+2-knot set (POT_FE) for the force
+// pass.  The breakpoints -- the 6 pair knots, the 2 density knots and the
+// density maximum r* (phi' = 0 below it) -- split r into MDK_FE_NI intervals;
+// interval m holds, in the local variable x = u - r (u its midpoint),
+//   V = c0 + x (c1 + x (c2 + x c3)) = sum a_k (r_k - r)^3 over the knots beyond r,
+//   phi' = q0 + x (q1 + x q2) = -3 sum A_k (R_k - r)^2,
+// coefficients formed in FP64 on the host; V' = -dV/dx comes from the same
+// Horner steps (p2 = c2 + x c3, p1 = c1 + x p2, dV/dx = p1 + x (p2 + x c3)).  The interval comes from a uniform
+// grid of MDK_FE_NB bins of width h (a power of 2 below the smallest
+// breakpoint spacing, so a bin holds at most one breakpoint): a bin record is
+// (split, m below it, m at or above it), m as the float offset of the
+// interval's 8-float record (u, c0, c1, c2 | c3, q0, q1, q2).  One bin read and one compare replace a compare
+// per breakpoint; phi' costs two fused multiply-adds instead of a knot sum.
+// The short-range forms (r <= 1.9 A, rare) are a divergent call, as in the
+// knot form.
+#define MDK_FE_NB 64
+#define MDK_FE_NI 10
+#define MDK_FE_TAB (4 * MDK_FE_NB + 8 * MDK_FE_NI)
+__device__ __forceinline__ void mdk_fe_terms(const PotDev &P, const float4 *bins, const float *iv, float r, float &v,
+                                             float &dv, float &dph) {
+  const int b = min(max((int)fmaf(r, P.fe_binh, P.fe_binc), 0), MDK_FE_NB - 1);
+  const float4 rec = bins[b];
+  const float *q = iv + __float_as_int(r >= rec.x ? rec.z : rec.y);
+  const float4 a = *(const float4 *)q, c = *(const float4 *)(q + 4);
   const float x = a.x - r;
-  v = a.y + x * (a.z + x * (a.w + x * b.x));
-  dv = -(a.z + x * (b.y + x * b.z));
+  const float p2 = a.w + x * c.x, p1 = a.z + x * p2;
+  v = a.y + x * p1;
+  dv = -(p1 + x * (p2 + x * c.x));
+  dph = c.y + x * (c.z + x * c.w);
   if (!(r > P.bc2f)) {
     if (r > P.bc1f) {
-      VdV<float> q = mdk_join<float>(P.bc1f, P.bc2f, P.jx1f, P.jv1f, P.jx2f, P.jv2f, r);
-      v = q.v;
-      dv = q.dv;
+      VdV<float> s = mdk_join<float>(P.bc1f, P.bc2f, P.jx1f, P.jv1f, P.jx2f, P.jv2f, r);
+      v = s.v;
+      dv = s.dv;
     } else if (r < P.bc1f) {
-      VdV<float> q = mdk_biersack<float>(P.bier_af, P.bier_zedf, r);
-      v = q.v;
-      dv = q.dv;
+      VdV<float> s = mdk_biersack<float>(P.bier_af, P.bier_zedf, r);
+      v = s.v;
+      dv = s.dv;
     } else {
       v = 0.f;
       dv = 0.f;

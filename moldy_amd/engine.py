@@ -243,7 +243,8 @@ class Engine:
         return out
 
     def eval_potential(self, which: str, x) -> np.ndarray:
-        k = ["vee", "dvee", "phi", "dphi", "emb", "demb"].index(which)
+        # vee32, dvee32, dphi32: the FP32 force pass's pair terms (interval table)
+        k = ["vee", "dvee", "phi", "dphi", "emb", "demb", "vee32", "dvee32", "dphi32"].index(which)
         x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
         y = np.empty_like(x)
         _check(self.L.mdk_eval_potential(self.h, k, _dp(x), _dp(y), len(x)))

@@ -351,3 +351,29 @@ def test_gpu_large_random_properties():
     f2 = np.stack([eng2.array(n) for n in ("fx", "fy", "fz")], axis=1)
     assert rel(f2, f) < 1e-10
     eng2.close()
+
+
+def test_gpu_fp32_interval_table(oracle):
+    """The FP32 force pass's V, V' and phi' (one polynomial per interval
+    between the pair knots, the density knots and r*, found through a uniform
+    bin grid) against the exact FP64 functions: within 1.5e-6 (V), 4.5e-6 (V')
+    and 1.5e-6 eV/A (phi') absolute over 1.91-4.1 A, where |V'| reaches 7.3
+    eV/A; exactly zero beyond the outermost knots, and phi' exactly zero below
+    the density maximum r*."""
+    F = load_golden("c0_nve_2000")
+    say("fp32 interval table")
+    eng = make_engine(F, precision=engine.PREC_FP32)
+    pot = oracle.Potential()
+    r = np.linspace(1.91, 4.1, 20001)
+    for nm, ref, tol in (("vee32", "vee", 1.5e-6), ("dvee32", "dvee", 4.5e-6), ("dphi32", "dphi", 1.5e-6)):
+        got = eng.eval_potential(nm, r)
+        err = np.max(np.abs(got - pot.f(ref, r)))
+        say(f"{nm}: max abs error {err:.3e}")
+        assert err <= tol, (nm, err)
+    far = np.array([3.3825, 3.5, 3.72646, 4.0])
+    assert not eng.eval_potential("vee32", far).any() and not eng.eval_potential("dvee32", far).any()
+    assert not eng.eval_potential("dphi32", np.array([3.72646, 3.9, 5.0])).any()
+    rstar = pot.phi_cut[0]
+    assert not eng.eval_potential("dphi32", np.array([1.7, 1.8, rstar - 1e-4])).any()
+    assert eng.eval_potential("dphi32", np.array([rstar + 1e-3]))[0] != 0.0
+    eng.close()
