@@ -36,7 +36,9 @@ extern "C" {
 #define MDK_ERR_BAD_ARG 3       /* invalid argument / size */
 #define MDK_ERR_SINGULAR_BOX 4  /* matrix_invert on a zero determinant (src/matrix_m.F90:110-113) */
 #define MDK_ERR_STATE 5         /* call out of order (e.g. force before set_potential) */
-#define MDK_ERR_SKIN 6          /* an atom moved farther than half the list skin between rebuilds */
+#define MDK_ERR_SKIN 6          /* a step used a device list that may miss pairs: an atom moved d and
+                                   the box strained e since the list was built with 2d + e*r_list >=
+                                   skin (reported by mdk_md_steps / mdk_velocityverlet) */
 #define MDK_ERR_NO_DEVICE 7     /* no GPU visible */
 
 /* ---- precision forms ---- */
@@ -75,7 +77,8 @@ typedef struct mdk_params {
   int ntcm;        /* neighbour-list update interval (nbrupdate) */
   int nlcx, nlcy, nlcz; /* reference link-cell grid (moldin.F90:252-257) */
   int precision;   /* MDK_PREC_FP64 or MDK_PREC_FP32 */
-  int rebuild;     /* 0: reference interval ntcm only; 1: also on displacement > skin/2 */
+  int rebuild;     /* device list rebuilds: 0 on the reference's ntcm interval and whenever the
+                      skin bound could be reached; 1 only when the skin bound could be reached */
   int max_full;    /* capacity of the device full list per atom (0 = default 64) */
   double deltat;   /* model time units */
   double rnear;    /* rcut + rpad: reference list radius */

@@ -122,6 +122,11 @@ struct mdk_ctx {
   int ncell_cap;
   int grid[3];
   int *dflags;             // [0] full-list overflow, [1] half-list overflow, [2] grid too coarse
+  // list-kernel flags copied to host-mapped memory after a deferred build,
+  // and the event that marks them complete
+  int *flags_host, *flags_map;
+  hipEvent_t list_ev;
+  bool list_pending;       // a deferred build whose flags the host has not checked yet
   // 2x2x2-cell blocks of the list build (Morton key / 8), for grid lblock_g
   int *lblocks, nlblocks, lblocks_cap, lblock_g[3];
   long rebuilds;
@@ -144,7 +149,6 @@ struct mdk_ctx {
   hipEvent_t mon_ev[4];
   double b0_lag[9];        // B0 of a completed step (from the monitor ring)
   long b0_lag_step;        // its step index, -1 when none
-  int lag_error;           // MDK_ERR_* seen in a monitor sample
   bool drifted;            // the next step's drift was fused into the last kick
   long step_counter;
   // trace buffer on device
@@ -1039,28 +1043,29 @@ __device__ __forceinline__ double4 partner(const double4 *__restrict__ pos4, con
 }
 
 // Per-atom end of the density pass: rho, and with do_embed the embedding
-// energy F(rho), F'(rho) (also into the packed positions for the force pass)
-// and the block's embedding-energy partial.
+// energy F(rho), F'(rho) (also into the packed positions for the force pass);
+// returns atom i's embedding energy (0 without do_embed).
+__device__ __forceinline__ double density_atom(double *atoms, double4 *pos4, float4 *p4f, int n, int i,
+                                               double acc, const PotDev &P, int do_embed) {
+  double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n, *daf = atoms + (size_t)F_DAFRHO * n;
+  rho[i] = acc;
+  if (!do_embed) return 0.0;
+  double dfi = daf[i], pe = 0.0;
+  if (acc > 1e-40) {  // src/potential_m.F90:161 leaves stale values below
+    double f;
+    mdk_embed_f(P, acc, f, dfi);
+    af[i] = f;
+    daf[i] = dfi;
+    pe = f;
+  }
+  pos4[i].w = dfi;
+  if (p4f) p4f[i].w = (float)dfi;
+  return pe;
+}
+// ... and the block's embedding-energy partial
 __device__ __forceinline__ void density_finish(double *atoms, double4 *pos4, float4 *p4f, int n, int i,
                                                double acc, const PotDev &P, int do_embed, double *part) {
-  double pe[1] = {0.0};
-  if (i < n) {
-    double *rho = atoms + (size_t)F_RHO * n, *af = atoms + (size_t)F_AFRHO * n,
-           *daf = atoms + (size_t)F_DAFRHO * n;
-    rho[i] = acc;
-    if (do_embed) {
-      double dfi = daf[i];
-      if (acc > 1e-40) {  // src/potential_m.F90:161 leaves stale values below
-        double f;
-        mdk_embed_f(P, acc, f, dfi);
-        af[i] = f;
-        daf[i] = dfi;
-        pe[0] = f;
-      }
-      pos4[i].w = dfi;
-      if (p4f) p4f[i].w = (float)dfi;
-    }
-  }
+  double pe[1] = {i < n ? density_atom(atoms, pos4, p4f, n, i, acc, P, do_embed) : 0.0};
   if (do_embed) block_store<1, 0>(pe, part);
 }
 
@@ -1141,9 +1146,9 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
   __shared__ int obuf[DENS_OBUF][MDK_BLOCK];
   shift_table(bs->b0, pg, S);
   __syncthreads();
+  const T rc = (T)P.r_int + (T)1e-4, rc2 = rc * rc;
   const int t = threadIdx.x, i = xcd_tile() * blockDim.x + t;
   double acc = 0.0;
-  const T rc = (T)P.r_int + (T)1e-4, rc2 = rc * rc;
   if (i < n) {
     const double4 pi = pos4[i];
     const int nn = numn[i];
@@ -1280,6 +1285,162 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
     const int zi = zat[i];
     if (zi != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
       const double mi = spec ? spec[2 * zi] : mass[i];
+      const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
+                   a3 = atoms[(size_t)F_Z1 * n + i];
+      const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
+      const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
+      const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
+      v[10] = a1 * mvx; v[11] = a1 * mvy; v[12] = a1 * mvz;
+      v[13] = a2 * mvx; v[14] = a2 * mvy; v[15] = a2 * mvz;
+      v[16] = a3 * mvx; v[17] = a3 * mvy; v[18] = a3 * mvz;
+    }
+  }
+  block_store<NPART_FORCE, 0>(v, part);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 Fe-Fe pair passes on cell-relative block lists (the FP32 form's main
+// path).  Both atoms of a pair come from the 16-byte FP32 packed records and
+// the separation (R_i - R_j) + D[code] is formed in FP32; r and 1/r come from
+// the hardware reciprocal square root with one Newton step.  One lane per atom.
+// ---------------------------------------------------------------------------
+// D[code] as an unevaluated FP32 sum hi + lo: a rounded D would bias every
+// separation of one cell offset the same way (pairs (i,j) and (j,i) alike),
+// and that bias adds up in the energy; ((R_i - R_j) + hi) + lo does not.
+__device__ __forceinline__ void shift_table_f32(const double (*S)[3], float4 *Sf, float4 *Sl) {
+  for (int c = threadIdx.x; c < 32; c += blockDim.x) {
+    const float hx = (float)S[c][0], hy = (float)S[c][1], hz = (float)S[c][2];
+    Sf[c] = make_float4(hx, hy, hz, 0.f);
+    Sl[c] = make_float4((float)(S[c][0] - hx), (float)(S[c][1] - hy), (float)(S[c][2] - hz), 0.f);
+  }
+}
+__device__ __forceinline__ float pair_r2_f32(const float4 &pi, const float4 &pj, const float4 *Sf,
+                                             const float4 *Sl, unsigned e, float &rx, float &ry, float &rz) {
+  const float4 d = Sf[e >> NBR_CODE_SHIFT], dl = Sl[e >> NBR_CODE_SHIFT];
+  rx = ((pi.x - pj.x) + d.x) + dl.x;
+  ry = ((pi.y - pj.y) + d.y) + dl.y;
+  rz = ((pi.z - pj.z) + d.z) + dl.z;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// Density pass (rhoset + embed, src/dynamics_m.F90:497-579,
+// src/potential_m.F90:148-169): phi in FP32, each atom's density summed in
+// FP64; nact[i] = list slots up to the end of the last quad holding a partner
+// within r_int + 1e-4 A (the force pass's visit length).
+__global__ void __launch_bounds__(MDK_BLOCK) k_density_f32(double *atoms, double4 *pos4, float4 *p4f, int n,
+                                                           const int *__restrict__ nbr,
+                                                           const int *__restrict__ numn, const BoxState *bs,
+                                                           PackGeo pg, PotDev P, int do_embed, double *part,
+                                                           int *nact) {
+  __shared__ double S[32][3];
+  __shared__ float4 Sf[32], Sl[32];
+  shift_table(bs->b0, pg, S);
+  __syncthreads();
+  shift_table_f32(S, Sf, Sl);
+  __syncthreads();
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  const float rc = (float)P.r_int + 1e-4f, rc2 = rc * rc;
+  double acc = 0.0;
+  if (i < n) {
+    const float4 pi = p4f[i];
+    const int nn = numn[i];
+    int na = 0;
+    for (int q = 0; 4 * q < nn; q++) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
+      float4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = p4f[e[u] & NBR_IDX_MASK];
+      float mn = rc2;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        float rx, ry, rz;
+        const float r2 = pair_r2_f32(pi, pj[u], Sf, Sl, e[u], rx, ry, rz);
+        float r, ri;
+        r_and_inv(r2, r, ri);
+        acc += (double)mdk_pair_phi<float, POT_FE>(P, r);
+        mn = fminf(mn, r2);
+      }
+      na = mn < rc2 ? 4 * q + 4 : na;
+    }
+    nact[i] = na;
+  }
+  density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
+}
+
+// Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177):
+// V, V' and phi' from the FP32 interval table, fp = -(V' + phi'(F'_i +
+// F'_j))/r, F_i = sum r fp.  Each atom's force and the lane's pair energy and
+// virial W = 1/2 sum r r^T fp are FP32 sums over the atom's few tens of terms;
+// the sums over lanes, waves and workgroups are FP64.
+__global__ void __launch_bounds__(MDK_BLOCK) k_force_f32(double *atoms, const float4 *__restrict__ p4f,
+                                                         const int *zat, int n, const int *__restrict__ nbr,
+                                                         const int *__restrict__ nvisit, const BoxState *bs,
+                                                         PackGeo pg, PotDev P, double *part, const double *spec,
+                                                         const float *fetab) {
+  __shared__ double S[32][3];
+  __shared__ float4 Sf[32], Sl[32];
+  __shared__ float4 fbin[MDK_FE_NB];
+  __shared__ __align__(16) float fiv[8 * MDK_FE_NI];
+  __shared__ double bi[9];
+  double b[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+  shift_table(b, pg, S);
+  {
+    const float4 *src = (const float4 *)fetab;
+    if (threadIdx.x < MDK_FE_NB) fbin[threadIdx.x] = src[threadIdx.x];
+    else if (threadIdx.x < MDK_FE_NB + 2 * MDK_FE_NI) ((float4 *)fiv)[threadIdx.x - MDK_FE_NB] = src[threadIdx.x];
+  }
+  if (threadIdx.x == 128) inv3(b, bi);
+  __syncthreads();
+  shift_table_f32(S, Sf, Sl);
+  __syncthreads();
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  double v[NPART_FORCE];
+#pragma unroll
+  for (int k = 0; k < NPART_FORCE; k++) v[k] = 0.0;
+  if (i < n) {
+    const float4 pi = p4f[i];
+    const int nn = nvisit[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    // sums of V and of r r^T fp (the factor 1/2 of both is applied once below)
+    float pe2 = 0.f, wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    for (int q = 0; 4 * q < nn; q++) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
+      float4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = p4f[e[u] & NBR_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        float rx, ry, rz;
+        const float r2 = pair_r2_f32(pi, pj[u], Sf, Sl, e[u], rx, ry, rz);
+        float r, ri;
+        r_and_inv(r2, r, ri);
+        float vv, dv, dph;
+        mdk_fe_terms(P, fbin, fiv, r, vv, dv, dph);
+        const float fp = -(dv + dph * (pi.w + pj[u].w)) * ri;
+        const float gx = rx * fp, gy = ry * fp, gz = rz * fp;
+        fx += gx;
+        fy += gy;
+        fz += gz;
+        pe2 += vv;
+        wxx += rx * gx; wyy += ry * gy; wzz += rz * gz;
+        wxy += rx * gy; wxz += rx * gz; wyz += ry * gz;
+      }
+    }
+    v[0] = 0.5 * (double)pe2;
+    v[1] = 0.5 * (double)wxx; v[2] = 0.5 * (double)wyy; v[3] = 0.5 * (double)wzz;
+    v[4] = 0.5 * (double)wxy; v[5] = 0.5 * (double)wxz; v[6] = 0.5 * (double)wyz;
+    // fractional force f = B0^-1 F (the reference's fx = sum s fp, Cartesian = B0 f)
+    const double Fx = fx, Fy = fy, Fz = fz;
+    atoms[(size_t)F_FX * n + i] = bi[0] * Fx + bi[3] * Fy + bi[6] * Fz;
+    atoms[(size_t)F_FY * n + i] = bi[1] * Fx + bi[4] * Fy + bi[7] * Fz;
+    atoms[(size_t)F_FZ * n + i] = bi[2] * Fx + bi[5] * Fy + bi[8] * Fz;
+    const int zi = zat[i];
+    if (zi != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
+      const double mi = spec ? spec[2 * zi] : atoms[(size_t)F_MASS * n + i];
       const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
                    a3 = atoms[(size_t)F_Z1 * n + i];
       const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
@@ -1532,6 +1693,15 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_fold(const double *part, int 
     for (int k = 0; k < K; k++) out[k] = v[k];
 }
 
+// the box of a new neighbour list (one thread)
+__global__ void k_set_bref(BoxState *bs) {
+  if (threadIdx.x < 9) bs->bref[threadIdx.x] = bs->b0[threadIdx.x];
+}
+// the list kernel's flags into host-mapped memory
+__global__ void k_copy_flags(const int *dflags, int *out) {
+  if (threadIdx.x < 4) out[threadIdx.x] = dflags[threadIdx.x];
+}
+
 // update_therm_avs box part only (stand-alone entry point)
 __global__ void k_therm_box(BoxState *bs, int ivol) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1657,8 +1827,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
 __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_kick(const double *pk, int nb, double *stage,
                                                                  unsigned *cnt, BoxState *bs, double dt,
                                                                  double bdel2, double press,
-                                                                 double *trace_row, double *mon,
-                                                                 const int *dflags) {
+                                                                 double *trace_row, double *mon) {
 #pragma clang fp contract(off)
   double v[NPART_KICK];
   if (!stage_and_finish<NPART_KICK, 2>(pk, nb, stage, cnt, v)) return;
@@ -1690,11 +1859,31 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_kick(const double *p
     trace_row[3] = hh; trace_row[4] = th; trace_row[5] = vol;
   }
   if (mon) {
+    // box strain since the list was built, Frobenius norm of B0 Bref^-1 - I,
+    // and the strain of one step, that of (b1 + b2) Bref^-1
+    double bri[9], b2[9];
+    for (int k = 0; k < 9; k++) {
+      bri[k] = bs->bref[k];
+      b2[k] = bs->b2[k];
+    }
+    double inv[9];
+    inv3(bri, inv);
+    double e2 = 0.0, d2 = 0.0;
+    for (int i = 1; i <= 3; i++)
+      for (int j = 1; j <= 3; j++) {
+        double x = (i == j) ? -1.0 : 0.0, y = 0.0;
+        for (int k = 1; k <= 3; k++) {
+          x += TT(b0, i, k) * TT(inv, k, j);
+          y += (TT(b1, i, k) + TT(b2, i, k)) * TT(inv, k, j);
+        }
+        e2 += x * x;
+        d2 += y * y;
+      }
     mon[0] = v[3];
     mon[1] = v[4];
     for (int k = 0; k < 9; k++) mon[2 + k] = b0[k];
-    mon[11] = (double)dflags[0];
-    mon[12] = (double)dflags[2];
+    mon[11] = sqrt(e2);
+    mon[12] = sqrt(d2);
   }
 }
 
@@ -1919,90 +2108,115 @@ static int plan_list_blocks(mdk_ctx *c, const int g[3]) {
   return MDK_OK;
 }
 
-static int rebuild_full(mdk_ctx *c) {
+// defer: one attempt whose flags the caller checks after queueing the next
+// kernels (confirm_list); otherwise every attempt is checked here.
+static int rebuild_full(mdk_ctx *c, bool defer = false) {
   const int n = c->n;
   c->list_valid = false;
   TimedEv te;
   tbegin(c, TC_REBUILD, te);
   // Grid from the box.  Inside the MD loop the box of a completed step comes
-  // from the monitor ring (no host round trip; cells planned 1 % wider than
-  // the list radius cover the box change of a few steps, and the list kernel
-  // flags any cell that is still too narrow); otherwise read synchronously.
-  const bool lagged = c->rebuilds > 0 && c->b0_lag_step >= 0 && c->step_counter - c->b0_lag_step <= 4;
-  double b0[9];
-  if (lagged) {
-    memcpy(b0, c->b0_lag, sizeof(b0));
-  } else {
-    BoxState bs;
-    RCK(read_box(c, &bs));
-    memcpy(b0, bs.b0, sizeof(b0));
-  }
-  int g[3];
-  plan_grid(c, b0, g, lagged ? 1.01 : 1.0);
-  int p2 = 1;
-  while (p2 < std::max(g[0], std::max(g[1], g[2]))) p2 <<= 1;
-  const int m = p2 * p2 * p2;  // Morton key space
-  c->grid[0] = g[0]; c->grid[1] = g[1]; c->grid[2] = g[2];
-  k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
-                                                       n, g[0], g[1], g[2], c->cell_of);
-  RCK(bin_atoms(c, c->cell_of, m, nullptr));
-  // physical re-sort of every per-atom array into cell order
-  k_permute<<<nblocks((long)n * F_N), MDK_BLOCK, 0, c->stream>>>(c->atoms_alt, c->atoms, c->members, n, F_N);
-  k_permute_int<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->zat, c->orig_alt, c->orig,
-                                                        c->slot_of, c->members, n);
-  std::swap(c->atoms, c->atoms_alt);
-  std::swap(c->zat, c->zat_alt);
-  std::swap(c->orig, c->orig_alt);
-  // reference cells move with the atoms
-  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
-  std::swap(c->ic, c->zat_alt);  // zat_alt is free scratch after the swap above
-  const double rl2 = c->r_list * c->r_list;
-  // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per
-  // dimension (and 26-bit atom indices); they pack positions cell-relative
-  const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4 && n < (1 << 26);
-  c->rel = blk;
-  // images and list cells fixed for this list's lifetime; packed positions
-  k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4, p4f_of(c),
-                                                 c->cell_of, g[0], g[1], g[2], pack_geo(c));
-  c->pos4_valid = true;
-  if (blk) RCK(plan_list_blocks(c, g));
-  for (int attempt = 0; attempt < 4; attempt++) {
-    HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
-    if (blk) {
-      k_full_list_blk<<<c->nlblocks, 64, 0, c->stream>>>(
-          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, c->lblocks, n, pack_geo(c), c->box + c->cur,
-          rl2, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
-      RCK(check_launch("k_full_list_blk"));
-    } else {
-      k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
-          c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
-          c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
-      RCK(check_launch("k_full_list"));
-    }
+  // from the monitor ring (no host round trip before the build; cells planned
+  // 1 % wider than the list radius cover the box change of a few steps);
+  // otherwise it is read synchronously.  The host reads the list kernel's
+  // flags before any force uses the list (here, or in confirm_list after the
+  // density pass is queued): a list that overflowed grows its capacity and is
+  // built again, and a grid found too narrow for the current box is planned
+  // again from the exact box -- so no flagged list is ever used.
+  bool lagged = c->rebuilds > 0 && c->b0_lag_step >= 0 && c->step_counter - c->b0_lag_step <= 4;
+  for (int plan = 0; plan < 2; plan++) {
+    double b0[9];
     if (lagged) {
-      // overflow and cell-width flags reach the host through the monitor ring
-      c->list_valid = true;
-      c->part_due = true;
-      c->rebuilds++;
-      tend(c, te);
-      return MDK_OK;
+      memcpy(b0, c->b0_lag, sizeof(b0));
+    } else {
+      BoxState bs;
+      RCK(read_box(c, &bs));
+      memcpy(b0, bs.b0, sizeof(b0));
     }
-    int fl[4];
-    HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(hipStreamSynchronize(c->stream));
-    if (fl[2] != 0) return set_err(MDK_ERR_STATE, "list cells narrower than the list radius");
-    if (fl[0] == 0) {
-      c->list_valid = true;
-      c->part_due = true;
-      c->rebuilds++;
-      tend(c, te);
-      return MDK_OK;
+    int g[3];
+    plan_grid(c, b0, g, lagged ? 1.01 : 1.0);
+    int p2 = 1;
+    while (p2 < std::max(g[0], std::max(g[1], g[2]))) p2 <<= 1;
+    const int m = p2 * p2 * p2;  // Morton key space
+    c->grid[0] = g[0]; c->grid[1] = g[1]; c->grid[2] = g[2];
+    k_fast_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n,
+                                                         n, g[0], g[1], g[2], c->cell_of);
+    RCK(bin_atoms(c, c->cell_of, m, nullptr));
+    // physical re-sort of every per-atom array into cell order
+    k_permute<<<nblocks((long)n * F_N), MDK_BLOCK, 0, c->stream>>>(c->atoms_alt, c->atoms, c->members, n, F_N);
+    k_permute_int<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->zat, c->orig_alt, c->orig,
+                                                          c->slot_of, c->members, n);
+    std::swap(c->atoms, c->atoms_alt);
+    std::swap(c->zat, c->zat_alt);
+    std::swap(c->orig, c->orig_alt);
+    // reference cells move with the atoms
+    k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->ic, c->members, n);
+    std::swap(c->ic, c->zat_alt);  // zat_alt is free scratch after the swap above
+    const double rl2 = c->r_list * c->r_list;
+    // staged 2x2x2-cell blocks need a fully periodic box with >= 4 cells per
+    // dimension (and 26-bit atom indices); they pack positions cell-relative
+    const bool blk = c->pbc == 7 && g[0] >= 4 && g[1] >= 4 && g[2] >= 4 && n < (1 << 26);
+    c->rel = blk;
+    // images and list cells fixed for this list's lifetime; packed positions
+    k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4, p4f_of(c),
+                                                   c->cell_of, g[0], g[1], g[2], pack_geo(c));
+    c->pos4_valid = true;
+    if (blk) RCK(plan_list_blocks(c, g));
+    bool replan = false;
+    for (int attempt = 0; attempt < 4; attempt++) {
+      HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
+      if (blk) {
+        k_full_list_blk<<<c->nlblocks, 64, 0, c->stream>>>(
+            c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, c->lblocks, n, pack_geo(c), c->box + c->cur,
+            rl2, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+        RCK(check_launch("k_full_list_blk"));
+      } else {
+        k_full_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
+            c->pos4, c->atoms, c->img, c->cell_of, c->cell_start, n, g[0], g[1], g[2], c->box + c->cur, rl2,
+            c->pbc, c->maxn, c->nbr, c->numn, c->sref, c->dflags);
+        RCK(check_launch("k_full_list"));
+      }
+      if (defer) {
+        k_copy_flags<<<1, 64, 0, c->stream>>>(c->dflags, c->flags_map);
+        HIPCK(hipEventRecord(c->list_ev, c->stream));
+        k_set_bref<<<1, 64, 0, c->stream>>>(c->box + c->cur);
+        RCK(check_launch("k_set_bref"));
+        c->list_valid = true;
+        c->list_pending = true;
+        c->part_due = true;
+        c->rebuilds++;
+        tend(c, te);
+        return MDK_OK;
+      }
+      int fl[4];
+      HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
+      HIPCK(hipStreamSynchronize(c->stream));
+      if (fl[2] != 0) {
+        if (!lagged) {
+          tend(c, te);
+          return set_err(MDK_ERR_STATE, "list cells narrower than the list radius");
+        }
+        replan = true;  // planned from a stale box: plan again from the exact one
+        break;
+      }
+      if (fl[0] == 0) {
+        k_set_bref<<<1, 64, 0, c->stream>>>(c->box + c->cur);
+        RCK(check_launch("k_set_bref"));
+        c->list_valid = true;
+        c->part_due = true;
+        c->rebuilds++;
+        tend(c, te);
+        return MDK_OK;
+      }
+      // grow capacity to the observed maximum (+25%) and retry
+      const int newmax = ((int)(fl[0] * 1.25) + 8 + 3) & ~3;
+      HIPCK(hipFree(c->nbr));
+      c->nbr = nullptr;
+      HIPCK(hipMalloc(&c->nbr, sizeof(int) * (size_t)newmax * n));
+      c->maxn = newmax;
     }
-    // grow capacity to the observed maximum (+25%) and retry
-    int newmax = ((int)(fl[0] * 1.25) + 8 + 3) & ~3;
-    HIPCK(hipFree(c->nbr));
-    HIPCK(hipMalloc(&c->nbr, sizeof(int) * (size_t)newmax * n));
-    c->maxn = newmax;
+    if (!replan) break;
+    lagged = false;
   }
   tend(c, te);
   HIPCK(hipMemsetAsync(c->numn, 0, sizeof(int) * n, c->stream));
@@ -2011,6 +2225,34 @@ static int rebuild_full(mdk_ctx *c) {
 
 // The FP32 form gathers the 16-byte FP32 copy of cell-relative packed
 // positions (P32); absolute packing and the FP64 form gather the FP64 record.
+static int pot_mode(const mdk_ctx *c);
+// The FP32 form's lean kernels: Fe-Fe interval table, cell-relative lists, no
+// immobile atoms (the per-pair mass test).
+static bool lean_path(const mdk_ctx *c) {
+  return c->rel && c->p.precision == MDK_PREC_FP32 && pot_mode(c) == POT_FE && !c->masschk;
+}
+// workgroups of the pair passes (and per-block partials per value)
+static int block_count(mdk_ctx *c) { return nblocks(c->n); }
+
+static void lean_density(mdk_ctx *c, int do_embed) {
+  double *pe = c->part_f + (size_t)NPART_FORCE * block_count(c);
+  if (c->part_due)
+    k_density_part<float, POT_FE, true><<<block_count(c), MDK_BLOCK, 0, c->stream>>>(
+        c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
+        c->nact);
+  else
+    k_density_f32<<<block_count(c), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn,
+                                                               c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
+                                                               c->nact);
+  c->part_due = false;
+}
+static void lean_force(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
+  hipExtLaunchKernelGGL(k_force_f32, dim3(block_count(c)), dim3(MDK_BLOCK), 0, c->stream, ea, eb, 0, c->atoms,
+                        (const float4 *)c->p4f, (const int *)c->zat, c->n, (const int *)c->nbr,
+                        (const int *)c->nact, (const BoxState *)(c->box + c->cur), pack_geo(c), c->pot, c->part_f,
+                        spec_of(c), (const float *)c->fetab);
+}
+
 template <typename T, int POTM, bool P32>
 static void density_dispatch_p(mdk_ctx *c, int do_embed) {
   // embedding-energy partials are value NPART_FORCE of the force pass's array
@@ -2087,14 +2329,17 @@ static int ensure_pos4(mdk_ctx *c) {
   return check_launch("k_pack");
 }
 
-static int block_count(mdk_ctx *c) { return nblocks(c->n); }
-
 static int launch_density(mdk_ctx *c, int do_embed) {
   RCK(ensure_pos4(c));
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
-  if (c->p.precision == MDK_PREC_FP32) density_pot<float>(c, do_embed);
-  else density_pot<double>(c, do_embed);
+  if (lean_path(c)) {
+    lean_density(c, do_embed);
+  } else if (c->p.precision == MDK_PREC_FP32) {
+    density_pot<float>(c, do_embed);
+  } else {
+    density_pot<double>(c, do_embed);
+  }
   tend(c, te);
   return check_launch("k_density");
 }
@@ -2104,7 +2349,9 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   tbegin_ext(c, TC_FORCE, te);  // the force kernel's own start and end
   hipEvent_t ea = te.cls >= 0 ? te.a : nullptr, eb = te.cls >= 0 ? te.b : nullptr;
   const bool f32 = c->p.precision == MDK_PREC_FP32, mc = c->masschk;
-  if (f32) {
+  if (lean_path(c)) {
+    lean_force(c, ea, eb);
+  } else if (f32) {
     if (mc) force_pot<float, true>(c, ea, eb); else force_pot<float, false>(c, ea, eb);
   } else {
     if (mc) force_pot<double, true>(c, ea, eb); else force_pot<double, false>(c, ea, eb);
@@ -2118,6 +2365,24 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   return check_launch("k_finalize_force");
 }
 
+// Checks the flags of a deferred build (the density pass on it is queued and
+// may run meanwhile; a flagged list yields wrong values but valid indices).
+// A flagged build is discarded: the list is built again with every attempt
+// checked (grown capacity, grid from the exact box) and the density pass
+// repeated, before any force uses it.
+static int confirm_list(mdk_ctx *c) {
+  if (!c->list_pending) return MDK_OK;
+  c->list_pending = false;
+  HIPCK(hipEventSynchronize(c->list_ev));
+  const volatile int *fl = c->flags_host;
+  if (fl[0] == 0 && fl[2] == 0) return MDK_OK;
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->rebuilds--;
+  c->b0_lag_step = -1;
+  RCK(rebuild_full(c, false));
+  return launch_density(c, 1);
+}
+
 // force() bookkeeping of src/dynamics_m.F90:186-198 plus the skin criterion
 static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
   if (!c->pot_set) return set_err(MDK_ERR_STATE, "potential not set");
@@ -2125,8 +2390,9 @@ static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
   bool ref_due = (c->ntc == 1);
   if (c->ntc == c->p.ntcm) c->ntc = 0;
   bool due = !c->list_valid || skin_due || (ref_due && c->p.rebuild == 0);
-  if (due) RCK(rebuild_full(c));
+  if (due) RCK(rebuild_full(c, true));
   RCK(launch_density(c, 1));
+  RCK(confirm_list(c));
   return launch_force(c, fin_mode);
 }
 
@@ -2151,7 +2417,7 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   if (mode != 2) {
     k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
         c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
-        trace_row_dev, mon, c->dflags);
+        trace_row_dev, mon);
     RCK(check_launch("k_finalize_kick"));
   }
   if (fuse_drift && mode == 0) {
@@ -2179,7 +2445,7 @@ static int launch_drift(mdk_ctx *c) {
 }
 
 // Reads the monitor sample of step s-2 (its kernels have completed or the
-// host waits for them): box for the next grid plan, list/grid error flags.
+// host waits for them): the box for the next grid plan and the skin bound.
 static const double *poll_monitor(mdk_ctx *c) {
   const long s = c->step_counter;
   if (s < 2) return nullptr;
@@ -2189,26 +2455,36 @@ static const double *poll_monitor(mdk_ctx *c) {
   const double *m = c->mon_host + MON_STRIDE * slot;
   for (int k = 0; k < 9; k++) c->b0_lag[k] = m[2 + k];
   c->b0_lag_step = probe;
-  if (m[11] > 0.0) c->lag_error = MDK_ERR_NBR_OVERFLOW;
-  if (m[12] > 0.0) c->lag_error = MDK_ERR_STATE;
   return m;
 }
 
-// Decide from a completed monitor sample whether the next force evaluation
-// could see an atom displaced by more than half the skin since the last build.
-static bool skin_check(mdk_ctx *c) {
-  const double *m = poll_monitor(c);
-  if (!c->list_valid) return true;
-  if (c->step_counter - c->last_rebuild_step < 3 || !m) return false;
-  double disp = sqrt(fmax(m[0], 0.0));
-  double stp = sqrt(fmax(m[1], 0.0));
-  return disp + 3.0 * stp >= 0.5 * c->skin;
+// The list a step uses holds every pair within the potential's range while
+// each atom has moved less than d and the box has strained by less than e
+// (monitor sample: d^2 = m[0], e = m[11]) since the list was built, with
+// 2 d + e r_list < skin: no pair separation has then changed by the skin.
+static double skin_used(const mdk_ctx *c, const double *m) {
+  return sqrt(fmax(m[0], 0.0)) + 0.5 * m[11] * c->r_list;  // against skin / 2
+}
+static int skin_violation(const mdk_ctx *c, const double *m, long step) {
+  return set_err(MDK_ERR_SKIN,
+                 "step %ld: an atom moved %.4g A and the box strained %.3g since the list was built "
+                 "(bound: skin/2 = %.4g A); pairs may have been missed",
+                 step, sqrt(fmax(m[0], 0.0)), m[11], 0.5 * c->skin);
 }
 
-static int lag_status(mdk_ctx *c) {
-  if (c->lag_error == MDK_ERR_NBR_OVERFLOW)
-    return set_err(MDK_ERR_NBR_OVERFLOW, "full neighbour list overflow (capacity %d)", c->maxn);
-  if (c->lag_error) return set_err(c->lag_error, "list cells narrower than the list radius");
+// Checks the completed sample of step s-2 against the skin bound (an error if
+// it was exceeded), then decides whether the next force evaluation could see
+// it exceeded: that sample's displacement and strain plus three times its
+// step's (m[1]: squared displacement of the next drift, m[12]: strain of one
+// box step).
+static int skin_check(mdk_ctx *c, bool *due) {
+  const double *m = poll_monitor(c);
+  *due = !c->list_valid;
+  if (!m) return MDK_OK;
+  if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, c->step_counter - 2);
+  if (c->step_counter - c->last_rebuild_step < 3) return MDK_OK;  // the sample predates the list
+  const double stp = sqrt(fmax(m[1], 0.0)) + 0.5 * m[12] * c->r_list;
+  *due = *due || skin_used(c, m) + 3.0 * stp >= 0.5 * c->skin;
   return MDK_OK;
 }
 
@@ -2242,8 +2518,8 @@ static int launch_correc(mdk_ctx *c) {
 // One moldin step with iverlet = 0 (src/moldin.F90:754-758): PREDIC, FORCE,
 // CORREC, then update_therm_avs (box part, then the sums with the monitor).
 static int md_step_gear(mdk_ctx *c, double *trace_row_dev) {
-  bool skin_due = skin_check(c);
-  RCK(lag_status(c));
+  bool skin_due;
+  RCK(skin_check(c, &skin_due));
   long before = c->rebuilds;
   RCK(launch_predic(c));
   RCK(force_body(c, FIN_FORCE, skin_due));
@@ -2260,8 +2536,8 @@ static int md_step_gear(mdk_ctx *c, double *trace_row_dev) {
 
 static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool allow_fuse = false) {
   if (c->iverlet == 0) return md_step_gear(c, trace_row_dev);
-  bool skin_due = skin_check(c);
-  RCK(lag_status(c));
+  bool skin_due;
+  RCK(skin_check(c, &skin_due));
   long before = c->rebuilds;
   c->pc_primed = true;  // velocityverlet refreshes quench_m's parameter copy
   RCK(launch_drift(c));
@@ -2310,6 +2586,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   }
   hipGetDevice(&c->device);
   c->maxn = ((p->max_full > 0 ? p->max_full : 64) + 3) & ~3;
+
   c->skin = p->rskin > 0.0 ? p->rskin : 0.3;
   c->rnear2_exact = near_threshold(p->rnear);
   c->nblk = nblocks(n);
@@ -2363,6 +2640,12 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
     goto fail;
   }
   for (int k = 0; k < 4; k++) hipEventCreateWithFlags(&c->mon_ev[k], hipEventDisableTiming);
+  if (hipHostMalloc((void **)&c->flags_host, sizeof(int) * 4, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->flags_map, c->flags_host, 0) != hipSuccess ||
+      hipEventCreateWithFlags(&c->list_ev, hipEventDisableTiming) != hipSuccess) {
+    rc = set_err(MDK_ERR_HIP, "mapped flag buffer failed");
+    goto fail;
+  }
   hipMemsetAsync(c->atoms, 0, sizeof(double) * (size_t)F_N * n, c->stream);
   hipMemsetAsync(c->zat, 0, sizeof(int) * n, c->stream);
   hipMemsetAsync(c->box, 0, sizeof(BoxState) * 2, c->stream);
@@ -2398,6 +2681,8 @@ extern "C" int mdk_destroy(void *h) {
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
+  if (c->flags_host) hipHostFree(c->flags_host);
+  if (c->list_ev) hipEventDestroy(c->list_ev);
   free(c->h_mass);
   free(c->h_del2);
   free(c->h_z);
@@ -2615,7 +2900,10 @@ extern "C" int mdk_upload(void *h, int field, const double *data) {
   k_scatter_d<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(fld(c->atoms, field_map[field], n), tmp, c->slot_of, n);
   RCK(check_launch("k_scatter_d"));
   HIPCK(hipStreamSynchronize(c->stream));
-  if (field <= MDK_Z0) c->list_valid = false;  // positions replaced: rebuild before the next force
+  if (field <= MDK_Z0) {  // positions replaced: rebuild (from the exact box) before the next force
+    c->list_valid = false;
+    c->b0_lag_step = -1;
+  }
   if (field <= MDK_Z0 || field == MDK_DAFRHO) c->pos4_valid = false;
   if (field == MDK_MASS) {
     // the per-pair test m_i + m_j > 0.1 (REAL(4) literal) can only fail with a mass <= 0.05
@@ -2690,6 +2978,7 @@ extern "C" int mdk_set_tensor(void *h, int which, const double *t9) {
             TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
     c->list_valid = false;
     c->pos4_valid = false;
+    c->b0_lag_step = -1;  // the next list is planned from this box, not a lagged one
   }
   HIPCK(hipMemcpyAsync(c->box + c->cur, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
@@ -2872,7 +3161,8 @@ extern "C" int mdk_md_init(void *h) {
 extern "C" int mdk_velocityverlet(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
-  bool skin_due = skin_check(c);
+  bool skin_due;
+  RCK(skin_check(c, &skin_due));
   c->pc_primed = true;
   RCK(launch_drift(c));
   RCK(force_body(c, FIN_VV, skin_due));
@@ -2882,8 +3172,8 @@ extern "C" int mdk_velocityverlet(void *h) {
   HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
   c->step_counter++;
   HIPCK(hipStreamSynchronize(c->stream));
-  poll_monitor(c);
-  RCK(lag_status(c));
+  const double *m = c->mon_host + MON_STRIDE * slot;  // this step's sample
+  if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, c->step_counter - 1);
   return check_device_status(c);
 }
 
@@ -2979,7 +3269,7 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
   RCK(check_launch("k_kick"));
   k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
       c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
-      nullptr, nullptr, c->dflags);
+      nullptr, nullptr);
   RCK(check_launch("k_finalize_kick"));
   BoxState s;
   RCK(read_box(c, &s));
@@ -3006,12 +3296,11 @@ extern "C" int mdk_md_steps(void *h, int nsteps, double *trace) {
   if (trace && nsteps > 0)
     HIPCK(hipMemcpyAsync(trace, c->trace_dev, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
-  if (c->step_counter > 0) {  // flags of the last steps, not yet polled
-    const double *m = c->mon_host + MON_STRIDE * (int)((c->step_counter - 1) & 3);
-    if (m[11] > 0.0) c->lag_error = MDK_ERR_NBR_OVERFLOW;
-    if (m[12] > 0.0) c->lag_error = MDK_ERR_STATE;
+  // the samples of the last two steps, not yet polled
+  for (long st = std::max(c->step_counter - 2, c->step_counter - nsteps); st < c->step_counter; st++) {
+    const double *m = c->mon_host + MON_STRIDE * (int)(st & 3);
+    if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, st);
   }
-  RCK(lag_status(c));
   return check_device_status(c);
 }
 

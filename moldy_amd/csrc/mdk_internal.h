@@ -16,6 +16,7 @@ struct BoxState {
   double tk[9], tc[9], tg[9], tp[9], tginv[9], tgdot[9], tgid[9], fb[9];
   double tgid_lag[9];   // tgid seen by the velocity half-kick (quench_m.F90:595)
   double binv[9];       // cofactor(b0)/vol used by update_therm_avs (dynamics_m.F90:692-700)
+  double bref[9];       // b0 when the current neighbour list was built (skin bound)
   double vol, pe, pe_emb, ke, s1, s2, s3, te, h, th, boxke;
   double maxdisp2;      // max |B0 (s - s_ref)|^2 since the last full-list build
   double maxstep2;      // max |B0 (x1 + x2)|^2 of the last step (next drift)
