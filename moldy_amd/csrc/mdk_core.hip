@@ -97,7 +97,8 @@ struct mdk_ctx {
   float *sref;             // fractional positions at the last full-list build
   double4 *pos4;           // packed positions (absolute or cell-relative) and F' for the pair kernels
   int *img;                // integer image floor(s) fixed at the last full-list build
-  bool pos4_valid;
+  bool pos4_valid;         // pos4 (and p4f) match the positions and box
+  bool p4f_valid;          // p4f matches them (the FP32 lean passes read only p4f)
   bool masschk;            // some atom has mass <= 0.05: keep the per-pair mass test
   // mass and del2 as functions of the species when every atom of a species
   // carries the same values: spec_tab[2 z], spec_tab[2 z + 1] (device), used
@@ -136,8 +137,8 @@ struct mdk_ctx {
   BoxState *box;
   int cur;
   // partials
-  // per-block partials, value-major: force pass (NPART_FORCE values) plus the
-  // density pass's embedding energy as value NPART_FORCE; half kick
+  // per-block partials, value-major: force() (NPART_F values: force pass,
+  // kinten, embedding energy); half kick
   double *part_f, *part_k;
   double *stage_f, *stage_k;  // first-stage reductions of the partials
   unsigned *fin_cnt;          // arrival counters of the finishing kernels (force, kick), zeroed
@@ -150,6 +151,7 @@ struct mdk_ctx {
   double b0_lag[9];        // B0 of a completed step (from the monitor ring)
   long b0_lag_step;        // its step index, -1 when none
   bool drifted;            // the next step's drift was fused into the last kick
+  bool kin_fresh;          // kinten's partial sums match the velocities and box of the next force
   long step_counter;
   // trace buffer on device
   double *trace_dev;
@@ -219,6 +221,20 @@ __device__ __forceinline__ void block_store(double (&v)[K], double *part) {
     for (int q = 1; q < BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
     part[(size_t)k * gridDim.x + blockIdx.x] = s;
   }
+}
+
+// kinten's terms of one atom (src/thermostat_m.F90:127-177): tk(a,b) += m
+// (B0 x1)_a x1_b, column-major in v[0..8], with B0 the box of the force
+// evaluation the velocities x1 belong to
+__device__ __forceinline__ void kin_terms(const double *b, double a1, double a2, double a3, double mi,
+                                          double (&v)[9]) {
+#pragma clang fp contract(off)  // the same rounding in every kernel that calls it
+  const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
+  const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
+  const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
+  v[0] = a1 * mvx; v[1] = a1 * mvy; v[2] = a1 * mvz;
+  v[3] = a2 * mvx; v[4] = a2 * mvy; v[5] = a2 * mvz;
+  v[6] = a3 * mvx; v[7] = a3 * mvy; v[8] = a3 * mvz;
 }
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
@@ -579,6 +595,7 @@ __device__ __forceinline__ int morton3(int x, int y, int z) {
 // Packed position of atom i from fractional s - img (see the entry format).
 __device__ __forceinline__ double3 packed_pos(const double *b, PackGeo pg, const int *cell, int i, double fx,
                                               double fy, double fz) {
+#pragma clang fp contract(off)  // the same packed positions from every kernel that writes them
   if (pg.gx) {
     const unsigned k = (unsigned)cell[i];
     fx -= ((double)compact3(k) + 0.5) * pg.ix;
@@ -589,7 +606,7 @@ __device__ __forceinline__ double3 packed_pos(const double *b, PackGeo pg, const
                       b[2] * fx + b[5] * fy + b[8] * fz);
 }
 __device__ __forceinline__ void store_packed(double4 *pos4, float4 *p4f, int i, double3 r, double w) {
-  pos4[i] = make_double4(r.x, r.y, r.z, w);
+  if (pos4) pos4[i] = make_double4(r.x, r.y, r.z, w);
   if (p4f) p4f[i] = make_float4((float)r.x, (float)r.y, (float)r.z, (float)w);
 }
 
@@ -832,9 +849,11 @@ __device__ void drift_box(const BoxState *bin, BoxState *bout, int ivol) {
 // b1 and b0 there (one field per thread).
 __global__ void k_drift(double *__restrict__ atoms, double4 *__restrict__ pos4, float4 *__restrict__ p4f,
                         const int *__restrict__ img, const int *__restrict__ cell, PackGeo pg, int n,
-                        const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol) {
+                        const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol,
+                        const int *__restrict__ zat, const double *__restrict__ spec, double *__restrict__ kin) {
 #pragma clang fp contract(off)
   int i = xcd_tile() * blockDim.x + threadIdx.x;
+  double kv[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
     const double *x1 = atoms + 3 * (size_t)n, *y1 = atoms + 4 * (size_t)n, *z1 = atoms + 5 * (size_t)n;
@@ -853,8 +872,11 @@ __global__ void k_drift(double *__restrict__ atoms, double4 *__restrict__ pos4, 
     }
     const double3 r = packed_pos(b, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
     store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
+    const int z = zat[i];
+    if (z != 0) kin_terms(b, x1[i], y1[i], z1[i], spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i], kv);
   }
   if (blockIdx.x == 0) drift_box(bin, bout, ivol);
+  block_store<9, 0>(kv, kin);
 }
 
 // predic (src/quench_m.F90:636-671): x0 += x1 + x2 + x3, x1 += 2 x2 + 3 x3,
@@ -871,12 +893,14 @@ __device__ __forceinline__ void predict_box(const BoxState *bin, double *b0, dou
 }
 __global__ void k_predic(double *__restrict__ atoms, double4 *__restrict__ pos4, float4 *__restrict__ p4f,
                          const int *__restrict__ img, const int *__restrict__ cell, PackGeo pg, int n,
-                         const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol, int primed) {
+                         const BoxState *__restrict__ bin, BoxState *__restrict__ bout, int ivol, int primed,
+                         const int *__restrict__ zat, const double *__restrict__ spec, double *__restrict__ kin) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  double kv[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     double *p = atoms;
-    double x[3];
+    double x[3], v[3];
     for (int d = 0; d < 3; d++) {
       double *q0 = p + (size_t)(F_X0 + d) * n, *q1 = p + (size_t)(F_X1 + d) * n, *q2 = p + (size_t)(F_X2 + d) * n;
       const double *q3 = p + (size_t)(F_X3 + d) * n;
@@ -889,11 +913,14 @@ __global__ void k_predic(double *__restrict__ atoms, double4 *__restrict__ pos4,
         q2[i] = v2 + 3.0 * v3;
       }
       x[d] = v0;
+      v[d] = q1[i];
     }
     double b[9], bd1[9], bd2[9];
     for (int k = 0; k < 9; k++) predict_box(bin, b, bd1, bd2, ivol, primed, k);
     const double3 r = packed_pos(b, pg, cell, i, x[0] - img[3 * i], x[1] - img[3 * i + 1], x[2] - img[3 * i + 2]);
     store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
+    const int z = zat[i];
+    if (z != 0) kin_terms(b, v[0], v[1], v[2], spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i], kv);
   }
   if (blockIdx.x == 0) {  // the whole box state into bout, one field per thread
     const int nw = (int)(sizeof(BoxState) / sizeof(double));
@@ -903,6 +930,25 @@ __global__ void k_predic(double *__restrict__ atoms, double4 *__restrict__ pos4,
     __syncthreads();
     if (threadIdx.x < 9) predict_box(bin, bout->b0, bout->b1, bout->b2, ivol, primed, threadIdx.x);
   }
+  block_store<9, 0>(kv, kin);
+}
+
+// kinten's partial sums for the velocities and box as they are (a force
+// evaluation that no drift, predictor or fused half kick has prepared)
+__global__ void __launch_bounds__(MDK_BLOCK) k_kin(const double *__restrict__ atoms, const int *__restrict__ zat,
+                                                   int n, const BoxState *__restrict__ bs,
+                                                   const double *__restrict__ spec, double *__restrict__ kin) {
+  const int i = xcd_tile() * blockDim.x + threadIdx.x;
+  double kv[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const int z = zat[i];
+    double b[9];
+    for (int k = 0; k < 9; k++) b[k] = bs->b0[k];
+    if (z != 0)
+      kin_terms(b, atoms[(size_t)F_X1 * n + i], atoms[(size_t)F_Y1 * n + i], atoms[(size_t)F_Z1 * n + i],
+                spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i], kv);
+  }
+  block_store<9, 0>(kv, kin);
 }
 
 // correc (src/quench_m.F90:682-759).  The coefficients are the reference's
@@ -1191,7 +1237,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
   density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
-// Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177).
+// Force pass (src/dynamics_m.F90:235-346); kinten's partial sums come from
+// the kernel that last set the velocities (kin_terms).
 // fp = -(V' + phi'(F'_i + F'_j))/r;  F_i = sum_j r_ij fp (Cartesian) and
 // f_i = B0^-1 F_i;  the virial is accumulated as W = 1/2 sum r r^T fp (tp =
 // W B0^-T in the finishing block, i.e. sum r_a s_b fp) and the pair energy as
@@ -1282,18 +1329,6 @@ k_force(double *atoms, const double4 *__restrict__ pos4, const float4 *__restric
     atoms[(size_t)F_FX * n + i] = bi[0] * fxi + bi[3] * fyi + bi[6] * fzi;
     atoms[(size_t)F_FY * n + i] = bi[1] * fxi + bi[4] * fyi + bi[7] * fzi;
     atoms[(size_t)F_FZ * n + i] = bi[2] * fxi + bi[5] * fyi + bi[8] * fzi;
-    const int zi = zat[i];
-    if (zi != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
-      const double mi = spec ? spec[2 * zi] : mass[i];
-      const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
-                   a3 = atoms[(size_t)F_Z1 * n + i];
-      const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
-      const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
-      const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
-      v[10] = a1 * mvx; v[11] = a1 * mvy; v[12] = a1 * mvz;
-      v[13] = a2 * mvx; v[14] = a2 * mvy; v[15] = a2 * mvz;
-      v[16] = a3 * mvx; v[17] = a3 * mvy; v[18] = a3 * mvz;
-    }
   }
   block_store<NPART_FORCE, 0>(v, part);
 }
@@ -1356,9 +1391,11 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_f32(double *atoms, double
       for (int u = 0; u < 4; u++) {
         float rx, ry, rz;
         const float r2 = pair_r2_f32(pi, pj[u], Sf, Sl, e[u], rx, ry, rz);
-        float r, ri;
-        r_and_inv(r2, r, ri);
-        acc += (double)mdk_pair_phi<float, POT_FE>(P, r);
+        if (r2 < rc2) {  // phi is zero beyond r_int; the list's tail is out of range
+          float r, ri;
+          r_and_inv(r2, r, ri);
+          acc += (double)mdk_pair_phi<float, POT_FE>(P, r);
+        }
         mn = fminf(mn, r2);
       }
       na = mn < rc2 ? 4 * q + 4 : na;
@@ -1368,8 +1405,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_f32(double *atoms, double
   density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
-// Force pass (src/dynamics_m.F90:235-346) + kinten (src/thermostat_m.F90:127-177):
-// V, V' and phi' from the FP32 interval table, fp = -(V' + phi'(F'_i +
+// Force pass (src/dynamics_m.F90:235-346): V, V' and phi' from the FP32 interval table, fp = -(V' + phi'(F'_i +
 // F'_j))/r, F_i = sum r fp.  Each atom's force and the lane's pair energy and
 // virial W = 1/2 sum r r^T fp are FP32 sums over the atom's few tens of terms;
 // the sums over lanes, waves and workgroups are FP64.
@@ -1438,18 +1474,6 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force_f32(double *atoms, const fl
     atoms[(size_t)F_FX * n + i] = bi[0] * Fx + bi[3] * Fy + bi[6] * Fz;
     atoms[(size_t)F_FY * n + i] = bi[1] * Fx + bi[4] * Fy + bi[7] * Fz;
     atoms[(size_t)F_FZ * n + i] = bi[2] * Fx + bi[5] * Fy + bi[8] * Fz;
-    const int zi = zat[i];
-    if (zi != 0) {  // kinten: tk(a,b) += x1_b * mv_a, column-major from v[10]
-      const double mi = spec ? spec[2 * zi] : atoms[(size_t)F_MASS * n + i];
-      const double a1 = atoms[(size_t)F_X1 * n + i], a2 = atoms[(size_t)F_Y1 * n + i],
-                   a3 = atoms[(size_t)F_Z1 * n + i];
-      const double mvx = (b[0] * a1 + b[3] * a2 + b[6] * a3) * mi;
-      const double mvy = (b[1] * a1 + b[4] * a2 + b[7] * a3) * mi;
-      const double mvz = (b[2] * a1 + b[5] * a2 + b[8] * a3) * mi;
-      v[10] = a1 * mvx; v[11] = a1 * mvy; v[12] = a1 * mvz;
-      v[13] = a2 * mvx; v[14] = a2 * mvy; v[15] = a2 * mvz;
-      v[16] = a3 * mvx; v[17] = a3 * mvy; v[18] = a3 * mvz;
-    }
   }
   block_store<NPART_FORCE, 0>(v, part);
 }
@@ -1530,18 +1554,18 @@ __device__ bool stage_and_finish(const double *part, int nb, double *stage, unsi
 }
 
 // Finishing kernel of force().  The per-block partials are in one value-major
-// array: values 0..18 from the force pass (NPART_FORCE) and value 19, the
-// embedding energy, from the density pass (nb blocks); reduced by
-// stage_and_finish, then the last block completes force().
+// array (NPART_F values of nb blocks: force pass, kinten, embedding energy);
+// reduced by stage_and_finish, then the last block completes force().
 __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *pf, int nb, double *stage,
                                                                   unsigned *cnt, BoxState *bs, int mode,
                                                                   int ivol, double press, double bdel2,
                                                                   double dt) {
+#pragma clang fp contract(off)  // therm_box's rounding: the step API and the fused step agree bit for bit
   // small matrices live in LDS: the nine tensor threads index them with their
   // own (ii,jj), which private arrays would turn into scratch accesses.  The
   // box fields are fetched before the partial sums so the two memory round
   // trips overlap.
-  __shared__ double b0[9], b1[9], b2[9], tgid0[9], tg[9], tgd[9], tgi[9], sv[NPART_FORCE], sbi[9];
+  __shared__ double b0[9], b1[9], b2[9], tgid0[9], tg[9], tgd[9], tgi[9], sv[NPART_F], sbi[9];
   const int t = threadIdx.x;
   const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
   if (t < 9) {
@@ -1550,14 +1574,14 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *
     b2[t] = bs->b2[t];
     tgid0[t] = bs->tgid[t];
   }
-  double v[NPART_FORCE + 1];
-  if (!stage_and_finish<NPART_FORCE + 1, 0>(pf, nb, stage, cnt, v)) return;
+  double v[NPART_F];
+  if (!stage_and_finish<NPART_F, 0>(pf, nb, stage, cnt, v)) return;
   if (t == 0)
-    for (int k = 0; k < NPART_FORCE; k++) sv[k] = v[k];
+    for (int k = 0; k < NPART_F; k++) sv[k] = v[k];
   __syncthreads();
   if (t == 0) {
-    bs->pe_emb = v[NPART_FORCE];
-    bs->pe = v[NPART_FORCE] + v[0];
+    bs->pe_emb = v[PART_EMB];
+    bs->pe = v[PART_EMB] + v[0];
     bs->vol = dev_volume(b0);
     inv3(b0, sbi);
   }
@@ -1570,14 +1594,14 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *
       const int w = ii == k ? ii : (ii + k == 3 ? 4 : (ii + k == 4 ? 5 : 6));
       tpv += sv[w] * TT(sbi, jj, k);
     }
-    const double tkv = sv[10 + t] / (dt * dt);
+    const double tkv = sv[PART_KIN + t] / (dt * dt);
     // tc = cofactor matrix of b0 (dynamics_m.F90:365-373)
     const int i1 = ii % 3 + 1, i2 = (ii + 1) % 3 + 1, j1 = jj % 3 + 1, j2 = (jj + 1) % 3 + 1;
     const double tcv = TT(b0, i1, j1) * TT(b0, i2, j2) - TT(b0, i1, j2) * TT(b0, i2, j1);  // cofactor C(ii,jj)
     double tgv = 0.0, tgdv = 0.0;
     for (int k = 1; k <= 3; k++) {
       tgv += TT(b0, k, ii) * TT(b0, k, jj);
-      tgdv += TT(b0, k, ii) * TT(b1, k, jj) + TT(b1, k, ii) * TT(b0, k, jj);
+      tgdv = tgdv + TT(b0, k, ii) * TT(b1, k, jj) + TT(b1, k, ii) * TT(b0, k, jj);
     }
     const double fbv = tkv + tpv - press * tcv;
     bs->tp[t] = tpv;
@@ -1608,7 +1632,7 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *
     if (t < 9 && mode == FIN_VV_THERM) {
       double tgdv = 0.0;
       if (ivol != 1)
-        for (int k = 1; k <= 3; k++) tgdv += TT(b1, k, ii) * TT(b0, k, jj) + TT(b0, k, ii) * TT(b1, k, jj);
+        for (int k = 1; k <= 3; k++) tgdv = tgdv + TT(b1, k, ii) * TT(b0, k, jj) + TT(b0, k, ii) * TT(b1, k, jj);
       tgd[t] = tgdv;
       bs->tg[t] = tg[t];
       bs->tgdot[t] = tgdv;
@@ -1727,10 +1751,11 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
                                                     double4 *__restrict__ pos4, float4 *__restrict__ p4f,
                                                     const int *__restrict__ img, const int *__restrict__ cell,
                                                     PackGeo pg, BoxState *__restrict__ bout,
-                                                    const double *__restrict__ spec) {
+                                                    const double *__restrict__ spec, double *__restrict__ kin) {
 #pragma clang fp contract(off)
   const int i = xcd_tile() * blockDim.x + threadIdx.x;
   double v[NPART_KICK] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double kv[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < n) {
     double *x0p = atoms, *y0p = atoms + n, *z0p = atoms + 2 * (size_t)n;
     double *x1p = atoms + (size_t)F_X1 * n, *y1p = atoms + (size_t)F_Y1 * n, *z1p = atoms + (size_t)F_Z1 * n;
@@ -1815,10 +1840,12 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
       }
       const double3 r = packed_pos(bn, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
       store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
+      if (z != 0) kin_terms(bn, x1, y1, z1, m, kv);  // kinten of the next force evaluation
     }
   }
   if (DRIFT && blockIdx.x == 0) drift_box(bs, bout, ivol);
   block_store<NPART_KICK, 2>(v, part);
+  if (DRIFT) block_store<9, 0>(kv, kin);
 }
 
 // Finishing kernel of the half kick: s1,s2,s3, KE, box KE, TE, H, TH
@@ -2160,7 +2187,7 @@ static int rebuild_full(mdk_ctx *c, bool defer = false) {
     // images and list cells fixed for this list's lifetime; packed positions
     k_pack<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, n, 1, c->pos4, p4f_of(c),
                                                    c->cell_of, g[0], g[1], g[2], pack_geo(c));
-    c->pos4_valid = true;
+    c->pos4_valid = c->p4f_valid = true;
     if (blk) RCK(plan_list_blocks(c, g));
     bool replan = false;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -2185,6 +2212,7 @@ static int rebuild_full(mdk_ctx *c, bool defer = false) {
         c->list_pending = true;
         c->part_due = true;
         c->rebuilds++;
+        c->last_rebuild_step = (int)c->step_counter;  // skin samples before this step predate the list
         tend(c, te);
         return MDK_OK;
       }
@@ -2205,6 +2233,7 @@ static int rebuild_full(mdk_ctx *c, bool defer = false) {
         c->list_valid = true;
         c->part_due = true;
         c->rebuilds++;
+        c->last_rebuild_step = (int)c->step_counter;  // skin samples before this step predate the list
         tend(c, te);
         return MDK_OK;
       }
@@ -2233,9 +2262,11 @@ static bool lean_path(const mdk_ctx *c) {
 }
 // workgroups of the pair passes (and per-block partials per value)
 static int block_count(mdk_ctx *c) { return nblocks(c->n); }
+// kinten's column of force()'s partials (written with c->nblk == nblocks(n) blocks)
+static double *kin_part(mdk_ctx *c) { return c->part_f + (size_t)PART_KIN * c->nblk; }
 
 static void lean_density(mdk_ctx *c, int do_embed) {
-  double *pe = c->part_f + (size_t)NPART_FORCE * block_count(c);
+  double *pe = c->part_f + (size_t)PART_EMB * block_count(c);
   if (c->part_due)
     k_density_part<float, POT_FE, true><<<block_count(c), MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
@@ -2255,8 +2286,8 @@ static void lean_force(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
 
 template <typename T, int POTM, bool P32>
 static void density_dispatch_p(mdk_ctx *c, int do_embed) {
-  // embedding-energy partials are value NPART_FORCE of the force pass's array
-  double *pe = c->part_f + (size_t)NPART_FORCE * nblocks(c->n);
+  // embedding-energy partials are value PART_EMB of force()'s array
+  double *pe = c->part_f + (size_t)PART_EMB * nblocks(c->n);
   if (c->part_due)
     k_density_part<T, POTM, P32><<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->pos4, p4f_of(c), c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
@@ -2323,6 +2354,7 @@ static void force_pot(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
 
 static int ensure_pos4(mdk_ctx *c) {
   if (c->pos4_valid) return MDK_OK;
+  c->p4f_valid = true;
   k_pack<<<nblocks(c->n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->img, c->box + c->cur, c->n, 0, c->pos4,
                                                      p4f_of(c), c->cell_of, 0, 0, 0, pack_geo(c));
   c->pos4_valid = true;
@@ -2330,7 +2362,7 @@ static int ensure_pos4(mdk_ctx *c) {
 }
 
 static int launch_density(mdk_ctx *c, int do_embed) {
-  RCK(ensure_pos4(c));
+  if (!(lean_path(c) && !c->part_due && c->p4f_valid)) RCK(ensure_pos4(c));
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
   if (lean_path(c)) {
@@ -2345,6 +2377,12 @@ static int launch_density(mdk_ctx *c, int do_embed) {
 }
 
 static int launch_force(mdk_ctx *c, int fin_mode) {
+  if (!c->kin_fresh) {  // no drift, predictor or fused kick prepared kinten's sums
+    k_kin<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, spec_of(c),
+                                                kin_part(c));
+    RCK(check_launch("k_kin"));
+  }
+  c->kin_fresh = false;  // consumed by this force evaluation
   TimedEv te;
   tbegin_ext(c, TC_FORCE, te);  // the force kernel's own start and end
   hipEvent_t ea = te.cls >= 0 ? te.a : nullptr, eb = te.cls >= 0 ? te.b : nullptr;
@@ -2406,12 +2444,13 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   const int nxt = 1 - c->cur;
   if (fuse_drift && mode == 0)
     k_kick<true><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, c->pos4, p4f_of(c),
-        c->img, c->cell_of, pack_geo(c), c->box + nxt, spec_of(c));
+        c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k,
+        lean_path(c) ? nullptr : c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c), c->box + nxt, spec_of(c),
+        kin_part(c));
   else
     k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k, nullptr, nullptr,
-        nullptr, nullptr, pack_geo(c), nullptr, spec_of(c));
+        nullptr, nullptr, pack_geo(c), nullptr, spec_of(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
   if (mode != 2) {
@@ -2421,9 +2460,11 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
     RCK(check_launch("k_finalize_kick"));
   }
   if (fuse_drift && mode == 0) {
-    c->pos4_valid = true;
+    c->pos4_valid = !lean_path(c);
+    c->p4f_valid = true;
     c->cur = nxt;
     c->drifted = true;
+    c->kin_fresh = true;
   }
   return MDK_OK;
 }
@@ -2436,9 +2477,12 @@ static int launch_drift(mdk_ctx *c) {
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
-  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c),
-                                                c->n, c->box + c->cur, c->box + nxt, c->p.ivol);
-  c->pos4_valid = true;
+  k_drift<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, lean_path(c) ? nullptr : c->pos4, p4f_of(c), c->img,
+                                                c->cell_of, pack_geo(c), c->n, c->box + c->cur, c->box + nxt,
+                                                c->p.ivol, c->zat, spec_of(c), kin_part(c));
+  c->pos4_valid = !lean_path(c);
+  c->p4f_valid = true;
+  c->kin_fresh = true;
   tend(c, te);
   c->cur = nxt;
   return check_launch("k_drift");
@@ -2497,9 +2541,10 @@ static int launch_predic(mdk_ctx *c) {
   const int nxt = 1 - c->cur;
   k_predic<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c),
                                                  c->n, c->box + c->cur, c->box + nxt, c->p.ivol,
-                                                 c->pc_primed ? 1 : 0);
+                                                 c->pc_primed ? 1 : 0, c->zat, spec_of(c), kin_part(c));
   tend(c, te);
-  c->pos4_valid = true;
+  c->pos4_valid = c->p4f_valid = true;
+  c->kin_fresh = true;
   c->cur = nxt;
   return check_launch("k_predic");
 }
@@ -2510,7 +2555,7 @@ static int launch_correc(mdk_ctx *c) {
   k_correc<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol,
                                                  c->p.bdel2, spec_of(c));
   tend(c, te);
-  c->pos4_valid = false;  // positions and box moved
+  c->pos4_valid = c->p4f_valid = false;  // positions and box moved
   c->pc_primed = true;
   return check_launch("k_correc");
 }
@@ -2520,10 +2565,8 @@ static int launch_correc(mdk_ctx *c) {
 static int md_step_gear(mdk_ctx *c, double *trace_row_dev) {
   bool skin_due;
   RCK(skin_check(c, &skin_due));
-  long before = c->rebuilds;
   RCK(launch_predic(c));
   RCK(force_body(c, FIN_FORCE, skin_due));
-  if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
   RCK(launch_correc(c));
   k_therm_box<<<1, 64, 0, c->stream>>>(c->box + c->cur, c->p.ivol);
   RCK(check_launch("k_therm_box"));
@@ -2538,11 +2581,9 @@ static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool all
   if (c->iverlet == 0) return md_step_gear(c, trace_row_dev);
   bool skin_due;
   RCK(skin_check(c, &skin_due));
-  long before = c->rebuilds;
   c->pc_primed = true;  // velocityverlet refreshes quench_m's parameter copy
   RCK(launch_drift(c));
   RCK(force_body(c, fused_therm ? FIN_VV_THERM : FIN_VV, skin_due));
-  if (c->rebuilds != before) c->last_rebuild_step = (int)c->step_counter;
   int slot = (int)(c->step_counter & 3);
   RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + MON_STRIDE * slot, allow_fuse && c->ntc != 0));
   HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
@@ -2623,9 +2664,9 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->pos4, sizeof(double4) * (size_t)n);
   ALLOC(c->p4f, sizeof(float4) * (size_t)n);
   ALLOC(c->img, sizeof(int) * 3 * (size_t)n);
-  ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * (NPART_FORCE + 1));
+  ALLOC(c->part_f, sizeof(double) * (size_t)c->nblk * NPART_F);
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
-  ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * (NPART_FORCE + 1));
+  ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_F);
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
   ALLOC(c->fin_cnt, sizeof(unsigned) * 2);
   ALLOC(c->red, sizeof(double) * 4);
@@ -2894,6 +2935,7 @@ static int update_species_table(mdk_ctx *c) {
 extern "C" int mdk_upload(void *h, int field, const double *data) {
   mdk_ctx *c = H(h);
   if (!c || field < 0 || field >= MDK_NFIELDS || !data) return set_err(MDK_ERR_BAD_ARG, "bad upload");
+  c->kin_fresh = false;  // velocities, masses or box may change: kinten's sums are recomputed
   const int n = c->n;
   double *tmp = fld(c->atoms_alt, 0, n);  // staging in the alternate buffer
   HIPCK(hipMemcpyAsync(tmp, data, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -2904,7 +2946,7 @@ extern "C" int mdk_upload(void *h, int field, const double *data) {
     c->list_valid = false;
     c->b0_lag_step = -1;
   }
-  if (field <= MDK_Z0 || field == MDK_DAFRHO) c->pos4_valid = false;
+  if (field <= MDK_Z0 || field == MDK_DAFRHO) c->pos4_valid = c->p4f_valid = false;
   if (field == MDK_MASS) {
     // the per-pair test m_i + m_j > 0.1 (REAL(4) literal) can only fail with a mass <= 0.05
     bool any = false;
@@ -2933,6 +2975,7 @@ extern "C" int mdk_download(void *h, int field, double *data) {
 extern "C" int mdk_upload_species(void *h, const int *zn) {
   mdk_ctx *c = H(h);
   if (!c || !zn) return set_err(MDK_ERR_BAD_ARG, "bad species upload");
+  c->kin_fresh = false;  // velocities, masses or box may change: kinten's sums are recomputed
   const int n = c->n;
   // the pair kernels evaluate the Fe-Fe functions: Fe atoms and vacancies only
   for (int i = 0; i < n; i++)
@@ -2968,6 +3011,7 @@ static double *box_field(BoxState &s, int which) {
 extern "C" int mdk_set_tensor(void *h, int which, const double *t9) {
   mdk_ctx *c = H(h);
   if (!c || !t9 || which < 0 || which >= MDK_NTENSORS) return set_err(MDK_ERR_BAD_ARG, "bad tensor");
+  c->kin_fresh = false;  // velocities, masses or box may change: kinten's sums are recomputed
   BoxState s;
   RCK(read_box(c, &s));
   memcpy(box_field(s, which), t9, 9 * sizeof(double));
@@ -2977,7 +3021,7 @@ extern "C" int mdk_set_tensor(void *h, int which, const double *t9) {
             TT(b, 2, 1) * (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) +
             TT(b, 3, 1) * (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3));
     c->list_valid = false;
-    c->pos4_valid = false;
+    c->pos4_valid = c->p4f_valid = false;
     c->b0_lag_step = -1;  // the next list is planned from this box, not a lagged one
   }
   HIPCK(hipMemcpyAsync(c->box + c->cur, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
@@ -3135,7 +3179,7 @@ extern "C" int mdk_embed(void *h, double *pe_embed) {
   if (pe_embed) {
     const int nb = block_count(c);
     std::vector<double> part(nb);
-    HIPCK(hipMemcpyAsync(part.data(), c->part_f + (size_t)NPART_FORCE * nb, sizeof(double) * nb,
+    HIPCK(hipMemcpyAsync(part.data(), c->part_f + (size_t)PART_EMB * nb, sizeof(double) * nb,
                          hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     double s = 0.0;
@@ -3154,7 +3198,6 @@ extern "C" int mdk_md_init(void *h) {
   RCK(force_body(c, FIN_INIT, false));
   RCK(launch_kick(c, 2, nullptr, nullptr));
   HIPCK(hipStreamSynchronize(c->stream));
-  c->last_rebuild_step = (int)c->step_counter;
   return check_device_status(c);
 }
 
@@ -3180,6 +3223,7 @@ extern "C" int mdk_velocityverlet(void *h) {
 extern "C" int mdk_scalev(void *h, double nose, double rqke) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->kin_fresh = false;  // velocities, masses or box may change: kinten's sums are recomputed
   if (!(nose > 0.0)) return MDK_OK;  // only a positive nose acts (src/thermostat_m.F90:329)
   const int n = c->n, nb = nblocks(n);
   if (nb > c->nblk) return set_err(MDK_ERR_STATE, "partial buffers too small");
@@ -3250,6 +3294,7 @@ extern "C" int mdk_predic(void *h) {
 extern "C" int mdk_correc(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  c->kin_fresh = false;  // velocities, masses or box may change: kinten's sums are recomputed
   RCK(launch_correc(c));
   HIPCK(hipStreamSynchronize(c->stream));
   return MDK_OK;
@@ -3264,7 +3309,7 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
   tbegin(c, TC_KICK, te);
   k_kick<false><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
       c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, 1, nullptr, c->part_k, nullptr, nullptr, nullptr,
-      nullptr, pack_geo(c), nullptr, spec_of(c));
+      nullptr, pack_geo(c), nullptr, spec_of(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
   k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(

@@ -27,9 +27,15 @@ struct BoxState {
 // Kernel classes for the HIP-event timing accumulators.
 enum { TC_DRIFT = 0, TC_DENSITY, TC_FORCE, TC_KICK, TC_REBUILD, TC_NCLASS };
 
-// Number of per-block partial sums written by the force kernel:
-// [0] pair energy, [1..9] tp, [10..18] kinten tk
-#define NPART_FORCE 19
+// Per-block partial sums of force() (value-major, one column of nblocks per
+// value): [0] pair energy and [1..6] virial W (xx yy zz xy xz yz) from the
+// force pass; [7..15] kinten's tk (column-major) from the kernel that last
+// set the velocities (half kick, drift, predictor, or k_kin); [16] embedding
+// energy from the density pass
+#define NPART_FORCE 7
+#define PART_KIN 7
+#define PART_EMB 16
+#define NPART_F 17
 // Partials of the half-kick kernel: s1, s2, s3, maxdisp2, maxstep2
 #define NPART_KICK 5
 

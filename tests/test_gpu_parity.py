@@ -173,6 +173,8 @@ def test_gpu_step_api_matches_md_steps():
         if b.ntc == 0:
             b.relink()
         assert abs(b.scalar("ke") - tr[k, 1]) <= 1e-12 * tr[k, 1]
+    say("list builds", a.stats()[0], b.stats()[0])
+    assert a.stats()[0] == b.stats()[0]  # the same skin decisions on the same samples
     for nm in ("x0", "x1", "fx"):
         np.testing.assert_array_equal(a.array(nm), b.array(nm))
     a.close()
