@@ -257,6 +257,21 @@ int mdk_batch_vjp(void *h, const double *dE, const double *dF, double *grad);
 /* average ms of reps loss + gradient launches */
 int mdk_batch_time_loss_grad(void *h, int reps, double *ms_avg);
 
+/* mdk_batch_loss_grad that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight training loops, harnesses that forbid hipMalloc).
+ * loss (1 double) and grad (2 nv + 2 np doubles) are DEVICE pointers, written
+ * asynchronously on the batch's stream; mdk_batch_synchronize before reading
+ * them (it also reports a neighbour-capacity overflow).  The work-queue
+ * counter and the per-cell partials live in a static device array sized for
+ * MDK_NOALLOC_MAX_CELLS cells (1.6 MB on each GPU the library runs on, held
+ * until the process ends); a batch with more cells gives MDK_ERR_BAD_ARG.
+ * All calls on one GPU share that array: calls that overlap on one GPU (two
+ * streams or threads) race.  Results equal mdk_batch_loss_grad's bit for bit;
+ * the batch's graph is not used. */
+#define MDK_NOALLOC_MAX_CELLS 8192
+int mdk_batch_loss_grad_noalloc(void *h, double *loss, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -544,15 +545,50 @@ __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, Gra
 
 // Sums the per-cell partials over cells in a fixed order: block k reduces
 // slot k (k < nslot) of out_g, block nslot the per-cell losses.
+// Block nslot (launched only for the loss) writes *loss.
 __global__ void __launch_bounds__(256) k_grad_final(const double *out_g, const double *out_loss,
-                                                    int ncell, int nslot, double *res) {
+                                                    int ncell, int nslot, double *res, double *loss) {
   __shared__ double red[4];
   const int k = blockIdx.x;
   double s = 0.0;
   for (int c = threadIdx.x; c < ncell; c += 256)
     s += k < nslot ? out_g[(size_t)c * NGRAD_SLOTS + k] : out_loss[c];
   s = block_sum<256>(s, red);
-  if (threadIdx.x == 0) res[k] = s;
+  if (threadIdx.x == 0) {
+    if (k < nslot) res[k] = s;
+    else *loss = s;
+  }
+}
+}  // namespace
+
+// Scratch of mdk_batch_loss_grad_noalloc, one copy per GPU, held until the
+// process ends: the work-queue counter (first 256 bytes), then the per-cell
+// gradient partials (NGRAD_SLOTS per cell) and per-cell losses of at most
+// MDK_NOALLOC_MAX_CELLS cells -- 1.6 MB.  Host code uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_base), never
+// the array's name.
+#define NOALLOC_HEAD 32
+static __device__ __align__(256) double scratch[NOALLOC_HEAD + (NGRAD_SLOTS + 1) * (size_t)MDK_NOALLOC_MAX_CELLS];
+
+namespace {
+std::mutex scratch_mu;
+double *scratch_base[64];  // per device ordinal: that device's copy of scratch
+
+// Looks up the current device's copy of scratch, once per device.
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= 64) return berr(MDK_ERR_BAD_ARG, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_base[dev]) return MDK_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return berr(MDK_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+  scratch_base[dev] = (double *)p;
+  return MDK_OK;
+}
+double *scratch_of(int dev) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return (dev >= 0 && dev < 64) ? scratch_base[dev] : nullptr;
 }
 }  // namespace
 
@@ -658,7 +694,7 @@ int enqueue_grad(mdk_batch *b, bool vjp) {
     else launch_grad<float, false>(b->block, grid, b->stream, A, b->pot, b->gd);
   }
   k_grad_final<<<A.ngrad + (vjp ? 0 : 1), 256, 0, b->stream>>>(b->out_g, b->out_loss, b->ncell,
-                                                               A.ngrad, b->res);
+                                                               A.ngrad, b->res, b->res + A.ngrad);
   return launch_status();
 }
 
@@ -734,6 +770,13 @@ extern "C" int mdk_batch_create(void **handle, int ncell, const int *natoms, con
   b->wg_per_cu = 4;
   b->use_graph = true;
   hipGetDevice(&b->device);
+  {
+    const int rc = scratch_setup(b->device);  // mdk_batch_loss_grad_noalloc's scratch on this device
+    if (rc != MDK_OK) {
+      delete b;
+      return rc;
+    }
+  }
   hipDeviceProp_t prop;
   hipGetDeviceProperties(&prop, b->device);
   b->ncu = prop.multiProcessorCount;
@@ -1110,6 +1153,34 @@ extern "C" int mdk_batch_vjp(void *h, const double *dE, const double *dF, double
   int rc = enqueue_grad(b, true);
   if (rc != MDK_OK) return rc;
   return read_res(b, 2 * b->knots.nv + 2 * b->knots.np, grad);
+}
+
+// mdk_batch_loss_grad into caller device memory without allocating: the
+// work-queue counter and the per-cell partials are in this device's copy of
+// scratch; the kernels are those of mdk_batch_loss_grad (no graph), so the
+// results agree bit for bit.
+extern "C" int mdk_batch_loss_grad_noalloc(void *h, double *loss, double *grad) {
+  mdk_batch *b = (mdk_batch *)h;
+  if (!b || !loss || !grad) return berr(MDK_ERR_BAD_ARG, "bad arguments");
+  if (!b->pot_set || !b->targets_set) return berr(MDK_ERR_STATE, "batch potential or targets not set");
+  if (b->ncell > MDK_NOALLOC_MAX_CELLS)
+    return berr(MDK_ERR_BAD_ARG, "%d cells exceed the no-allocation limit of %d cells", b->ncell,
+                MDK_NOALLOC_MAX_CELLS);
+  double *base = scratch_of(b->device);
+  if (!base) return berr(MDK_ERR_STATE, "no scratch address for device %d", b->device);
+  BatchArgs A = args_of(b);
+  A.counter = (int *)base;
+  A.out_g = base + NOALLOC_HEAD;
+  A.out_loss = A.out_g + (size_t)NGRAD_SLOTS * b->ncell;
+  k_batch_reset<<<1, 64, 0, b->stream>>>(A.counter);  // a kernel, not a memset: capture-safe
+  const int grid = grid_of(b);
+  if (b->precision == MDK_PREC_FP64)
+    launch_grad<double, false>(b->block, grid, b->stream, A, b->pot, b->gd);
+  else
+    launch_grad<float, false>(b->block, grid, b->stream, A, b->pot, b->gd);
+  k_grad_final<<<A.ngrad + 1, 256, 0, b->stream>>>(A.out_g, A.out_loss, b->ncell, A.ngrad, grad, loss);
+  b->list_valid = true;
+  return launch_status();
 }
 
 // Average time of `reps` loss + gradient launches (HIP events on the batch stream), ms.

@@ -41,6 +41,7 @@ EXPORTS = [
     "mdk_batch_config", "mdk_batch_evaluate", "mdk_batch_synchronize", "mdk_batch_results",
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
+    "mdk_batch_loss_grad_noalloc",
     "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_set_integrator", "mdk_predic", "mdk_correc",
     "mdk_set_pressure", "mdk_scalev", "mdk_get_snhv", "mdk_set_snhv",
 ]
@@ -124,6 +125,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_batch_loss_grad": [vp, dp, dp],
         "mdk_batch_vjp": [vp, dp, dp, dp],
         "mdk_batch_time_loss_grad": [vp, C.c_int, dp],
+        "mdk_batch_loss_grad_noalloc": [vp, vp, vp],
     })
     for name, args in sig.items():
         f = getattr(L, name)
@@ -470,6 +472,11 @@ class BatchEngine:
         g = np.empty(self.ngrad)
         _bcheck(self.L.mdk_batch_loss_grad(self.h, C.byref(loss), _dp(g)))
         return loss.value, g
+
+    def loss_grad_noalloc(self, loss_ptr: int, grad_ptr: int):
+        """loss_grad into device memory (1 and ngrad doubles at the given device
+        addresses) with no allocation; asynchronous: synchronize() before reading."""
+        _bcheck(self.L.mdk_batch_loss_grad_noalloc(self.h, C.c_void_p(loss_ptr), C.c_void_p(grad_ptr)))
 
     def vjp(self, dE, dF):
         """sum_c dE_c dE_c/dtheta + sum_i dF_i . dF_i/dtheta at the last forward evaluation."""
