@@ -40,6 +40,7 @@ class System:
     mass: np.ndarray
     z: np.ndarray
     deltat_fs: float
+    info: dict | None = None  # how a generated cell was made (make_batch: nc, fcc, vacancy)
 
     @property
     def n(self) -> int:
@@ -121,8 +122,9 @@ def make_batch(ncell: int = 4096, seed: int = 7, nmin: int = 3, nmax: int = 10,
         e = rng.uniform(-max_strain, max_strain, 6)
         strain = np.array([[e[0], e[3], e[4]], [e[3], e[1], e[5]], [e[4], e[5], e[2]]])
         vac = float(rng.uniform(0.0, max_vacancy))
-        cells.append(make_system(nc=n, a=a_bcc * 2 ** (1 / 3) if fcc else a_bcc,
-                                 basis=FCC_BASIS if fcc else BCC_BASIS, sigma=sigma, temp=0.0,
-                                 seed=int(rng.integers(1 << 31)), strain=strain,
-                                 vacancy_frac=vac))
+        cell = make_system(nc=n, a=a_bcc * 2 ** (1 / 3) if fcc else a_bcc,
+                           basis=FCC_BASIS if fcc else BCC_BASIS, sigma=sigma, temp=0.0,
+                           seed=int(rng.integers(1 << 31)), strain=strain, vacancy_frac=vac)
+        cell.info = {"nc": n, "fcc": fcc, "vacancy": vac}
+        cells.append(cell)
     return cells
