@@ -76,7 +76,13 @@ struct TimedEv {
   hipEvent_t a, b;
 };
 
-#define MON_STRIDE 16
+// monitor sample: [0] max squared displacement since the list was built,
+// [1] max squared displacement of the next drift, then B0, b1 + b2 (the next
+// drift's box change) and Bref (B0 at the list build), 9 each
+#define MON_STRIDE 32
+#define MON_B0 2
+#define MON_DB 11
+#define MON_BREF 20
 #define MDK_NSPEC 128  // species (atomic numbers) covered by the mass/del2 table
 
 // list grid for cell-relative packing (gx == 0: absolute packing)
@@ -140,8 +146,8 @@ struct mdk_ctx {
   // per-block partials, value-major: force() (NPART_F values: force pass,
   // kinten, embedding energy); half kick
   double *part_f, *part_k;
-  double *stage_f, *stage_k;  // first-stage reductions of the partials
-  unsigned *fin_cnt;          // arrival counters of the finishing kernels (force, kick), zeroed
+  double *stage_f, *stage_k;  // first-stage reductions of force()'s and scalev's partials
+  unsigned *fin_cnt;          // arrival counters of k_finalize_force and k_fold, zeroed
   int nblk;
   // host-mapped monitor ring, MON_STRIDE doubles per step slot (4 slots):
   // [0] maxdisp2, [1] maxstep2, [2..10] B0 after the step, [11] full-list
@@ -1237,6 +1243,296 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_part(double *atoms, doubl
   density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
+enum { FIN_FORCE = 0, FIN_VV = 1, FIN_VV_THERM = 2, FIN_INIT = 3 };
+
+// update_therm_avs box part (src/dynamics_m.F90:672-700)
+__device__ void therm_box(BoxState &s, int ivol) {
+#pragma clang fp contract(off)
+  for (int k = 0; k < 9; k++) { s.tg[k] = 0.0; s.tgdot[k] = 0.0; }
+  for (int i = 1; i <= 3; i++)
+    for (int k = 1; k <= 3; k++)
+      for (int j = 1; j <= 3; j++) {
+        TT(s.tg, i, j) = TT(s.tg, i, j) + TT(s.b0, k, i) * TT(s.b0, k, j);
+        if (ivol != 1)
+          TT(s.tgdot, i, j) = TT(s.tgdot, i, j) + TT(s.b1, k, i) * TT(s.b0, k, j) + TT(s.b0, k, i) * TT(s.b1, k, j);
+      }
+  dev_invert(s.tg, s.tginv, &s.status);
+  dev_matmul(s.tginv, s.tgdot, s.tgid);
+  s.vol = dev_volume(s.b0);
+  const double *b = s.b0, v = s.vol;
+  TT(s.binv, 1, 1) = (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) / v;  // xx
+  TT(s.binv, 1, 2) = (TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 3, 3) * TT(b, 2, 1)) / v;  // xy
+  TT(s.binv, 1, 3) = (TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 3, 1) * TT(b, 2, 2)) / v;  // xz
+  TT(s.binv, 2, 1) = (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) / v;  // yx
+  TT(s.binv, 2, 2) = (TT(b, 1, 1) * TT(b, 3, 3) - TT(b, 1, 3) * TT(b, 3, 1)) / v;  // yy
+  TT(s.binv, 2, 3) = (TT(b, 3, 1) * TT(b, 1, 2) - TT(b, 1, 1) * TT(b, 3, 2)) / v;  // yz
+  TT(s.binv, 3, 1) = (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3)) / v;  // zx
+  TT(s.binv, 3, 2) = (TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3)) / v;  // zy
+  TT(s.binv, 3, 3) = (TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 1, 2) * TT(b, 2, 1)) / v;  // zz
+}
+
+// Stage-and-finish reduction for the finishing kernels, launched as p =
+// ceil(nb / 64) blocks of one wave (MDK_FIN_BLOCK).  Block b folds per-block
+// partials [64 b, 64 b + 64) lane-wise and across the wave in a fixed order;
+// lane 0 writes the K results to stage[k*p + b] and increments the arrival
+// counter with one agent-scope release/acquire, so each block makes a single
+// L2 write-back and invalidate.  The last block to arrive folds the p stage
+// columns the same way into out[] (every lane receives them), resets the
+// counter and returns true; the others return false.
+#define MDK_FIN_BLOCK 64
+template <int K, int KM>
+__device__ __forceinline__ void fold_columns(const double *part, int nb, int b0, int b1, double (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? 0.0 : -HUGE_VAL;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += MDK_FIN_BLOCK) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double x = part[(size_t)k * nb + b];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+  wave_reduce_all<K, KM>(v);
+}
+
+template <int K, int KM>
+__device__ bool stage_and_finish(const double *part, int nb, double *stage, unsigned *cnt, double (&out)[K]) {
+  const int p = gridDim.x, b0 = blockIdx.x * 64;
+  double v[K];
+  fold_columns<K, KM>(part, nb, b0, min(nb, b0 + 64), v);
+  int last = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) stage[(size_t)k * p + blockIdx.x] = v[k];
+    const unsigned prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = prev + 1 == (unsigned)p;
+  }
+  // one wave: lane 0's verdict to all lanes, which are past lane 0's acquire
+  if (!__builtin_amdgcn_readfirstlane(last)) return false;
+  fold_columns<K, KM>(stage, p, 0, p, out);
+  if (threadIdx.x == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// One-workgroup reduction of the per-block partials for the finishing
+// kernels (MDK_FOLD_BLOCK threads): thread t folds columns t, t + 1024, ...
+// in order, then the lanes of each wave and the waves in index order; thread
+// 0 receives the K totals.  No atomics and a single memory round trip.
+#define MDK_FOLD_BLOCK 1024
+template <int K, int KM>
+__device__ __forceinline__ void fold_all(const double *part, int nb, double (&out)[K]) {
+  __shared__ double sh[MDK_FOLD_BLOCK / 64][K];
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? 0.0 : -HUGE_VAL;
+  for (int b = (int)threadIdx.x; b < nb; b += MDK_FOLD_BLOCK) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const double x = part[(size_t)k * nb + b];
+      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
+    }
+  }
+  wave_reduce_all<K, KM>(v);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) sh[threadIdx.x >> 6][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double s = sh[0][k];
+      for (int q = 1; q < MDK_FOLD_BLOCK / 64; q++) s = (k < K - KM) ? s + sh[q][k] : fmax(s, sh[q][k]);
+      out[k] = s;
+    }
+  }
+}
+
+// Arguments of the finishing kernels of force() and of the half kick.
+struct FinForce {
+  double *stage;   // ceil(nb / 64) x NPART_F first-stage sums
+  unsigned *cnt;   // arrival counter, zero between launches
+  BoxState *bs;
+  int mode, ivol;
+  double press, bdel2, dt;
+};
+struct FinKick {
+  BoxState *bs;
+  BoxState *bout;  // the advanced copy of a fused drift: receives the same sums
+  double *trace_row, *mon;
+  double dt, bdel2, press;
+};
+
+// Finishing kernel of force(), ceil(nb / 64) blocks of one wave: the NPART_F
+// columns of per-block partials (force pass, kinten, embedding energy) are
+// reduced by stage_and_finish (spread over many CUs: seventeen columns are
+// too much data for one), then the last block completes force(): pe, tp, tk,
+// tc, fb (dynamics_m.F90:365-394), then per mode the velocity-Verlet box kick (quench_m.F90:595-604), the initial
+// tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part
+// (dynamics_m.F90:672-700).  Threads 0..8 each own one tensor element (i,j);
+// phases are separated by barriers; only changed fields are written back.
+__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *part, int nb, FinForce F) {
+#pragma clang fp contract(off)  // therm_box's rounding: the step API and the fused step agree bit for bit
+  // small matrices live in LDS: the nine tensor threads index them with their
+  // own (ii,jj), which private arrays would turn into scratch accesses.  The
+  // box fields are fetched before the partial sums so the two memory round
+  // trips overlap.
+  __shared__ double b0[9], b1[9], b2[9], tgid0[9], tg[9], tgd[9], tgi[9], sv[NPART_F], sbi[9];
+  BoxState *bs = F.bs;
+  const int mode = F.mode, ivol = F.ivol;
+  const double press = F.press, bdel2 = F.bdel2, dt = F.dt;
+  const int t = threadIdx.x;
+  const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
+  if (t < 9) {
+    b0[t] = bs->b0[t];
+    b1[t] = bs->b1[t];
+    b2[t] = bs->b2[t];
+    tgid0[t] = bs->tgid[t];
+  }
+  double v[NPART_F];
+  if (!stage_and_finish<NPART_F, 0>(part, nb, F.stage, F.cnt, v)) return;
+  if (t == 0)
+    for (int k = 0; k < NPART_F; k++) sv[k] = v[k];
+  __syncthreads();
+  if (t == 0) {
+    bs->pe_emb = v[PART_EMB];
+    bs->pe = v[PART_EMB] + v[0];
+    bs->vol = dev_volume(b0);
+    inv3(b0, sbi);
+  }
+  __syncthreads();
+  if (t < 9) {
+    // W = 1/2 sum r r^T fp (symmetric; sv[1..6] = xx yy zz xy xz yz) ->
+    // tp(a,c) = sum_k W(a,k) B0^-1(c,k)
+    double tpv = 0.0;
+    for (int k = 1; k <= 3; k++) {
+      const int w = ii == k ? ii : (ii + k == 3 ? 4 : (ii + k == 4 ? 5 : 6));
+      tpv += sv[w] * TT(sbi, jj, k);
+    }
+    const double tkv = sv[PART_KIN + t] / (dt * dt);
+    // tc = cofactor matrix of b0 (dynamics_m.F90:365-373)
+    const int i1 = ii % 3 + 1, i2 = (ii + 1) % 3 + 1, j1 = jj % 3 + 1, j2 = (jj + 1) % 3 + 1;
+    const double tcv = TT(b0, i1, j1) * TT(b0, i2, j2) - TT(b0, i1, j2) * TT(b0, i2, j1);  // cofactor C(ii,jj)
+    double tgv = 0.0, tgdv = 0.0;
+    for (int k = 1; k <= 3; k++) {
+      tgv += TT(b0, k, ii) * TT(b0, k, jj);
+      tgdv = tgdv + TT(b0, k, ii) * TT(b1, k, jj) + TT(b1, k, ii) * TT(b0, k, jj);
+    }
+    const double fbv = tkv + tpv - press * tcv;
+    bs->tp[t] = tpv;
+    bs->tk[t] = tkv;
+    bs->tc[t] = tcv;
+    bs->fb[t] = fbv;
+    tg[t] = tgv;
+    tgd[t] = tgdv;
+    if (mode == FIN_FORCE || mode == FIN_INIT) {
+      bs->tg[t] = tgv;
+      bs->tgdot[t] = tgdv;  // kinten's tgdot
+    }
+    if (mode == FIN_VV || mode == FIN_VV_THERM) {
+      bs->tgid_lag[t] = tgid0[t];
+      const bool upd = ivol < 1 || (ivol == 4 && t == 8);
+      double b2v = b2[t];
+      if (upd) b2v = bdel2 * fbv;
+      bs->b2[t] = b2v;
+      if (upd) b1[t] = b1[t] + b2v;
+      bs->b1[t] = b1[t];
+    } else if (mode == FIN_INIT) {
+      bs->b2[t] = bdel2 * fbv;
+    }
+  }
+  __syncthreads();
+  if (mode == FIN_VV_THERM || mode == FIN_INIT) {
+    // update_therm_avs box part (VV_THERM) or moldin's first tgid (INIT)
+    if (t < 9 && mode == FIN_VV_THERM) {
+      double tgdv = 0.0;
+      if (ivol != 1)
+        for (int k = 1; k <= 3; k++) tgdv = tgdv + TT(b1, k, ii) * TT(b0, k, jj) + TT(b0, k, ii) * TT(b1, k, jj);
+      tgd[t] = tgdv;
+      bs->tg[t] = tg[t];
+      bs->tgdot[t] = tgdv;
+    }
+    if (t < 9) {
+      // inverse of tg, element (ii,jj): cofactor(jj,ii)/det
+      const int a1 = jj % 3 + 1, a2 = (jj + 1) % 3 + 1, c1 = ii % 3 + 1, c2 = (ii + 1) % 3 + 1;
+      const double det = TT(tg, 1, 1) * (TT(tg, 2, 2) * TT(tg, 3, 3) - TT(tg, 2, 3) * TT(tg, 3, 2)) +
+                         TT(tg, 2, 1) * (TT(tg, 3, 2) * TT(tg, 1, 3) - TT(tg, 1, 2) * TT(tg, 3, 3)) +
+                         TT(tg, 3, 1) * (TT(tg, 1, 2) * TT(tg, 2, 3) - TT(tg, 2, 2) * TT(tg, 1, 3));
+      if (fabs(det) == 0.0) bs->status = MDK_ERR_SINGULAR_BOX;
+      tgi[t] = (TT(tg, a1, c1) * TT(tg, a2, c2) - TT(tg, a1, c2) * TT(tg, a2, c1)) / det;
+      bs->tginv[t] = tgi[t];
+    }
+    __syncthreads();
+    if (t < 9) {
+      double g = 0.0;
+      for (int k = 1; k <= 3; k++) g += TT(tgi, ii, k) * TT(tgd, k, jj);
+      bs->tgid[t] = g;
+      if (mode == FIN_VV_THERM) {
+        // cofactor(b0)/vol arranged as update_therm_avs's b0_XXdet ... (dynamics_m.F90:692-700)
+        const double vol = dev_volume(b0);
+        const int r1 = ii % 3 + 1, r2 = (ii + 1) % 3 + 1, q1 = jj % 3 + 1, q2 = (jj + 1) % 3 + 1;
+        bs->binv[t] = (TT(b0, r1, q1) * TT(b0, r2, q2) - TT(b0, r1, q2) * TT(b0, r2, q1)) / vol;
+      }
+    }
+  }
+}
+
+// Completion of the half kick's sums: s1,s2,s3, KE, box KE, TE, H, TH
+// (moldin.F90:763-790), the trace row and the host-visible skin monitor
+// (thread 0 of the finishing kernel's last block).
+__device__ void finish_kick(const double (&v)[NPART_KICK], const FinKick &F) {
+#pragma clang fp contract(off)
+  BoxState *bs = F.bs;
+  const double dt = F.dt, bdel2 = F.bdel2, press = F.press;
+  double *trace_row = F.trace_row, *mon = F.mon;
+  // only the fields this function changes are read and written
+  const double pe = bs->pe, vol = bs->vol;
+  double b0[9], b1[9], b2[9], br[9];
+  for (int k = 0; k < 9; k++) {  // every load issued before the first use
+    b0[k] = bs->b0[k];
+    b1[k] = bs->b1[k];
+    b2[k] = bs->b2[k];
+    br[k] = bs->bref[k];
+  }
+  const double ke = v[2] / (dt * dt) / 2.0;
+  double bk = 0.0;
+  for (int i = 1; i <= 3; i++)
+    for (int j = 1; j <= 3; j++) bk = bk + (TT(b1, i, j) * TT(b1, i, j)) / bdel2;
+  const double te = ke + pe, hh = te + press * vol, th = hh + bk;
+  for (BoxState *o : {bs, F.bout}) {  // the fused drift's copy was taken before these sums
+    if (!o) continue;
+    o->s1 = v[0];
+    o->s2 = v[1];
+    o->s3 = v[2];
+    o->maxdisp2 = v[3];
+    o->maxstep2 = v[4];
+    o->ke = ke;
+    o->boxke = bk;
+    o->te = te;
+    o->h = hh;
+    o->th = th;
+  }
+  if (trace_row) {
+    trace_row[0] = pe; trace_row[1] = ke; trace_row[2] = te;
+    trace_row[3] = hh; trace_row[4] = th; trace_row[5] = vol;
+  }
+  if (mon) {  // the host forms the box strains from B0, b1 + b2 and Bref
+    mon[0] = v[3];
+    mon[1] = v[4];
+    for (int k = 0; k < 9; k++) {
+      mon[MON_B0 + k] = b0[k];
+      mon[MON_DB + k] = b1[k] + b2[k];
+      mon[MON_BREF + k] = br[k];
+    }
+  }
+}
+
+// Finishing kernel of the half kick, one workgroup of MDK_FOLD_BLOCK threads.
+__global__ void __launch_bounds__(MDK_FOLD_BLOCK) k_finalize_kick(const double *part, int nb, FinKick F) {
+  double v[NPART_KICK];
+  fold_all<NPART_KICK, 2>(part, nb, v);
+  if (threadIdx.x == 0) finish_kick(v, F);
+}
+
 // Force pass (src/dynamics_m.F90:235-346); kinten's partial sums come from
 // the kernel that last set the velocities (kin_terms).
 // fp = -(V' + phi'(F'_i + F'_j))/r;  F_i = sum_j r_ij fp (Cartesian) and
@@ -1478,190 +1774,6 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force_f32(double *atoms, const fl
   block_store<NPART_FORCE, 0>(v, part);
 }
 
-enum { FIN_FORCE = 0, FIN_VV = 1, FIN_VV_THERM = 2, FIN_INIT = 3 };
-
-// update_therm_avs box part (src/dynamics_m.F90:672-700)
-__device__ void therm_box(BoxState &s, int ivol) {
-#pragma clang fp contract(off)
-  for (int k = 0; k < 9; k++) { s.tg[k] = 0.0; s.tgdot[k] = 0.0; }
-  for (int i = 1; i <= 3; i++)
-    for (int k = 1; k <= 3; k++)
-      for (int j = 1; j <= 3; j++) {
-        TT(s.tg, i, j) = TT(s.tg, i, j) + TT(s.b0, k, i) * TT(s.b0, k, j);
-        if (ivol != 1)
-          TT(s.tgdot, i, j) = TT(s.tgdot, i, j) + TT(s.b1, k, i) * TT(s.b0, k, j) + TT(s.b0, k, i) * TT(s.b1, k, j);
-      }
-  dev_invert(s.tg, s.tginv, &s.status);
-  dev_matmul(s.tginv, s.tgdot, s.tgid);
-  s.vol = dev_volume(s.b0);
-  const double *b = s.b0, v = s.vol;
-  TT(s.binv, 1, 1) = (TT(b, 2, 2) * TT(b, 3, 3) - TT(b, 2, 3) * TT(b, 3, 2)) / v;  // xx
-  TT(s.binv, 1, 2) = (TT(b, 2, 3) * TT(b, 3, 1) - TT(b, 3, 3) * TT(b, 2, 1)) / v;  // xy
-  TT(s.binv, 1, 3) = (TT(b, 2, 1) * TT(b, 3, 2) - TT(b, 3, 1) * TT(b, 2, 2)) / v;  // xz
-  TT(s.binv, 2, 1) = (TT(b, 3, 2) * TT(b, 1, 3) - TT(b, 1, 2) * TT(b, 3, 3)) / v;  // yx
-  TT(s.binv, 2, 2) = (TT(b, 1, 1) * TT(b, 3, 3) - TT(b, 1, 3) * TT(b, 3, 1)) / v;  // yy
-  TT(s.binv, 2, 3) = (TT(b, 3, 1) * TT(b, 1, 2) - TT(b, 1, 1) * TT(b, 3, 2)) / v;  // yz
-  TT(s.binv, 3, 1) = (TT(b, 1, 2) * TT(b, 2, 3) - TT(b, 2, 2) * TT(b, 1, 3)) / v;  // zx
-  TT(s.binv, 3, 2) = (TT(b, 1, 3) * TT(b, 2, 1) - TT(b, 1, 1) * TT(b, 2, 3)) / v;  // zy
-  TT(s.binv, 3, 3) = (TT(b, 1, 1) * TT(b, 2, 2) - TT(b, 1, 2) * TT(b, 2, 1)) / v;  // zz
-}
-
-// Finishing block of force: pe, tp, tk, tc, fb (dynamics_m.F90:365-394),
-// then per mode the velocity-Verlet box kick (quench_m.F90:595-604), the
-// initial tgid/b2 (moldin.F90:452-477) or the update_therm_avs box part
-// (dynamics_m.F90:672-700).  Threads 0..8 each own one tensor element (i,j);
-// phases are separated by barriers; only changed fields are written back.
-// Stage-and-finish reduction for the finishing kernels, launched as p =
-// ceil(nb / 64) blocks of one wave (MDK_FIN_BLOCK).  Block b folds per-block
-// partials [64 b, 64 b + 64) lane-wise and across the wave in a fixed order;
-// lane 0 writes the K results to stage[k*p + b] and increments the arrival
-// counter with one agent-scope release/acquire, so each block makes a single
-// L2 write-back and invalidate.  The last block to arrive folds the p stage
-// columns the same way into out[] (every lane receives them), resets the
-// counter and returns true; the others return false.
-#define MDK_FIN_BLOCK 64
-template <int K, int KM>
-__device__ __forceinline__ void fold_columns(const double *part, int nb, int b0, int b1, double (&v)[K]) {
-#pragma unroll
-  for (int k = 0; k < K; k++) v[k] = (k < K - KM) ? 0.0 : -HUGE_VAL;
-  for (int b = b0 + (int)threadIdx.x; b < b1; b += MDK_FIN_BLOCK) {
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const double x = part[(size_t)k * nb + b];
-      v[k] = (k < K - KM) ? v[k] + x : fmax(v[k], x);
-    }
-  }
-  wave_reduce_all<K, KM>(v);
-}
-
-template <int K, int KM>
-__device__ bool stage_and_finish(const double *part, int nb, double *stage, unsigned *cnt, double (&out)[K]) {
-  const int p = gridDim.x, b0 = blockIdx.x * 64;
-  double v[K];
-  fold_columns<K, KM>(part, nb, b0, min(nb, b0 + 64), v);
-  int last = 0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; k++) stage[(size_t)k * p + blockIdx.x] = v[k];
-    const unsigned prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = prev + 1 == (unsigned)p;
-  }
-  // one wave: lane 0's verdict to all lanes, which are past lane 0's acquire
-  if (!__builtin_amdgcn_readfirstlane(last)) return false;
-  fold_columns<K, KM>(stage, p, 0, p, out);
-  if (threadIdx.x == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
-// Finishing kernel of force().  The per-block partials are in one value-major
-// array (NPART_F values of nb blocks: force pass, kinten, embedding energy);
-// reduced by stage_and_finish, then the last block completes force().
-__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *pf, int nb, double *stage,
-                                                                  unsigned *cnt, BoxState *bs, int mode,
-                                                                  int ivol, double press, double bdel2,
-                                                                  double dt) {
-#pragma clang fp contract(off)  // therm_box's rounding: the step API and the fused step agree bit for bit
-  // small matrices live in LDS: the nine tensor threads index them with their
-  // own (ii,jj), which private arrays would turn into scratch accesses.  The
-  // box fields are fetched before the partial sums so the two memory round
-  // trips overlap.
-  __shared__ double b0[9], b1[9], b2[9], tgid0[9], tg[9], tgd[9], tgi[9], sv[NPART_F], sbi[9];
-  const int t = threadIdx.x;
-  const int ii = t % 3 + 1, jj = t / 3 + 1;  // element (ii,jj) of thread t < 9
-  if (t < 9) {
-    b0[t] = bs->b0[t];
-    b1[t] = bs->b1[t];
-    b2[t] = bs->b2[t];
-    tgid0[t] = bs->tgid[t];
-  }
-  double v[NPART_F];
-  if (!stage_and_finish<NPART_F, 0>(pf, nb, stage, cnt, v)) return;
-  if (t == 0)
-    for (int k = 0; k < NPART_F; k++) sv[k] = v[k];
-  __syncthreads();
-  if (t == 0) {
-    bs->pe_emb = v[PART_EMB];
-    bs->pe = v[PART_EMB] + v[0];
-    bs->vol = dev_volume(b0);
-    inv3(b0, sbi);
-  }
-  __syncthreads();
-  if (t < 9) {
-    // W = 1/2 sum r r^T fp (symmetric; sv[1..6] = xx yy zz xy xz yz) ->
-    // tp(a,c) = sum_k W(a,k) B0^-1(c,k)
-    double tpv = 0.0;
-    for (int k = 1; k <= 3; k++) {
-      const int w = ii == k ? ii : (ii + k == 3 ? 4 : (ii + k == 4 ? 5 : 6));
-      tpv += sv[w] * TT(sbi, jj, k);
-    }
-    const double tkv = sv[PART_KIN + t] / (dt * dt);
-    // tc = cofactor matrix of b0 (dynamics_m.F90:365-373)
-    const int i1 = ii % 3 + 1, i2 = (ii + 1) % 3 + 1, j1 = jj % 3 + 1, j2 = (jj + 1) % 3 + 1;
-    const double tcv = TT(b0, i1, j1) * TT(b0, i2, j2) - TT(b0, i1, j2) * TT(b0, i2, j1);  // cofactor C(ii,jj)
-    double tgv = 0.0, tgdv = 0.0;
-    for (int k = 1; k <= 3; k++) {
-      tgv += TT(b0, k, ii) * TT(b0, k, jj);
-      tgdv = tgdv + TT(b0, k, ii) * TT(b1, k, jj) + TT(b1, k, ii) * TT(b0, k, jj);
-    }
-    const double fbv = tkv + tpv - press * tcv;
-    bs->tp[t] = tpv;
-    bs->tk[t] = tkv;
-    bs->tc[t] = tcv;
-    bs->fb[t] = fbv;
-    tg[t] = tgv;
-    tgd[t] = tgdv;
-    if (mode == FIN_FORCE || mode == FIN_INIT) {
-      bs->tg[t] = tgv;
-      bs->tgdot[t] = tgdv;  // kinten's tgdot
-    }
-    if (mode == FIN_VV || mode == FIN_VV_THERM) {
-      bs->tgid_lag[t] = tgid0[t];
-      const bool upd = ivol < 1 || (ivol == 4 && t == 8);
-      double b2v = b2[t];
-      if (upd) b2v = bdel2 * fbv;
-      bs->b2[t] = b2v;
-      if (upd) b1[t] = b1[t] + b2v;
-      bs->b1[t] = b1[t];
-    } else if (mode == FIN_INIT) {
-      bs->b2[t] = bdel2 * fbv;
-    }
-  }
-  __syncthreads();
-  if (mode == FIN_VV_THERM || mode == FIN_INIT) {
-    // update_therm_avs box part (VV_THERM) or moldin's first tgid (INIT)
-    if (t < 9 && mode == FIN_VV_THERM) {
-      double tgdv = 0.0;
-      if (ivol != 1)
-        for (int k = 1; k <= 3; k++) tgdv = tgdv + TT(b1, k, ii) * TT(b0, k, jj) + TT(b0, k, ii) * TT(b1, k, jj);
-      tgd[t] = tgdv;
-      bs->tg[t] = tg[t];
-      bs->tgdot[t] = tgdv;
-    }
-    if (t < 9) {
-      // inverse of tg, element (ii,jj): cofactor(jj,ii)/det
-      const int a1 = jj % 3 + 1, a2 = (jj + 1) % 3 + 1, c1 = ii % 3 + 1, c2 = (ii + 1) % 3 + 1;
-      const double det = TT(tg, 1, 1) * (TT(tg, 2, 2) * TT(tg, 3, 3) - TT(tg, 2, 3) * TT(tg, 3, 2)) +
-                         TT(tg, 2, 1) * (TT(tg, 3, 2) * TT(tg, 1, 3) - TT(tg, 1, 2) * TT(tg, 3, 3)) +
-                         TT(tg, 3, 1) * (TT(tg, 1, 2) * TT(tg, 2, 3) - TT(tg, 2, 2) * TT(tg, 1, 3));
-      if (fabs(det) == 0.0) bs->status = MDK_ERR_SINGULAR_BOX;
-      tgi[t] = (TT(tg, a1, c1) * TT(tg, a2, c2) - TT(tg, a1, c2) * TT(tg, a2, c1)) / det;
-      bs->tginv[t] = tgi[t];
-    }
-    __syncthreads();
-    if (t < 9) {
-      double g = 0.0;
-      for (int k = 1; k <= 3; k++) g += TT(tgi, ii, k) * TT(tgd, k, jj);
-      bs->tgid[t] = g;
-      if (mode == FIN_VV_THERM) {
-        // cofactor(b0)/vol arranged as update_therm_avs's b0_XXdet ... (dynamics_m.F90:692-700)
-        const double vol = dev_volume(b0);
-        const int r1 = ii % 3 + 1, r2 = (ii + 1) % 3 + 1, q1 = jj % 3 + 1, q2 = (jj + 1) % 3 + 1;
-        bs->binv[t] = (TT(b0, r1, q1) * TT(b0, r2, q2) - TT(b0, r1, q2) * TT(b0, r2, q1)) / vol;
-      }
-    }
-  }
-}
-
 // scalev (src/thermostat_m.F90:317-395), Nose-Hoover velocity scaling, as
 // three passes with deterministic block partials: (1) sums of x1 over all
 // atoms; (2) moving atoms (Z != 0, mass >= 0.1) lose the mean, partial
@@ -1846,72 +1958,6 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
   if (DRIFT && blockIdx.x == 0) drift_box(bs, bout, ivol);
   block_store<NPART_KICK, 2>(v, part);
   if (DRIFT) block_store<9, 0>(kv, kin);
-}
-
-// Finishing kernel of the half kick: s1,s2,s3, KE, box KE, TE, H, TH
-// (moldin.F90:763-790), the trace row and the host-visible skin monitor
-// (thread 0 of the last block, after stage_and_finish).
-__global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_kick(const double *pk, int nb, double *stage,
-                                                                 unsigned *cnt, BoxState *bs, double dt,
-                                                                 double bdel2, double press,
-                                                                 double *trace_row, double *mon) {
-#pragma clang fp contract(off)
-  double v[NPART_KICK];
-  if (!stage_and_finish<NPART_KICK, 2>(pk, nb, stage, cnt, v)) return;
-  if (threadIdx.x != 0) return;
-  // only the fields this kernel changes are read and written
-  const double pe = bs->pe, vol = bs->vol;
-  double b1[9], b0[9];
-  for (int k = 0; k < 9; k++) {
-    b1[k] = bs->b1[k];
-    b0[k] = bs->b0[k];
-  }
-  const double ke = v[2] / (dt * dt) / 2.0;
-  double bk = 0.0;
-  for (int i = 1; i <= 3; i++)
-    for (int j = 1; j <= 3; j++) bk = bk + (TT(b1, i, j) * TT(b1, i, j)) / bdel2;
-  const double te = ke + pe, hh = te + press * vol, th = hh + bk;
-  bs->s1 = v[0];
-  bs->s2 = v[1];
-  bs->s3 = v[2];
-  bs->maxdisp2 = v[3];
-  bs->maxstep2 = v[4];
-  bs->ke = ke;
-  bs->boxke = bk;
-  bs->te = te;
-  bs->h = hh;
-  bs->th = th;
-  if (trace_row) {
-    trace_row[0] = pe; trace_row[1] = ke; trace_row[2] = te;
-    trace_row[3] = hh; trace_row[4] = th; trace_row[5] = vol;
-  }
-  if (mon) {
-    // box strain since the list was built, Frobenius norm of B0 Bref^-1 - I,
-    // and the strain of one step, that of (b1 + b2) Bref^-1
-    double bri[9], b2[9];
-    for (int k = 0; k < 9; k++) {
-      bri[k] = bs->bref[k];
-      b2[k] = bs->b2[k];
-    }
-    double inv[9];
-    inv3(bri, inv);
-    double e2 = 0.0, d2 = 0.0;
-    for (int i = 1; i <= 3; i++)
-      for (int j = 1; j <= 3; j++) {
-        double x = (i == j) ? -1.0 : 0.0, y = 0.0;
-        for (int k = 1; k <= 3; k++) {
-          x += TT(b0, i, k) * TT(inv, k, j);
-          y += (TT(b1, i, k) + TT(b2, i, k)) * TT(inv, k, j);
-        }
-        e2 += x * x;
-        d2 += y * y;
-      }
-    mon[0] = v[3];
-    mon[1] = v[4];
-    for (int k = 0; k < 9; k++) mon[2 + k] = b0[k];
-    mon[11] = sqrt(e2);
-    mon[12] = sqrt(d2);
-  }
 }
 
 // energy_calc (src/dynamics_m.F90:410-470): en_i = F(rho_i) + 1/2 sum_j V(r_ij)
@@ -2396,10 +2442,9 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   }
   tpush(c, te);
   RCK(check_launch("k_force"));
-  const int nb = block_count(c);
-  k_finalize_force<<<(nb + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_f, nb, c->stage_f, c->fin_cnt,
-                                                                c->box + c->cur, fin_mode, c->p.ivol,
-                                                                c->p.press, c->p.bdel2, c->p.deltat);
+  const FinForce F = {c->stage_f, c->fin_cnt, c->box + c->cur, fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
+                      c->p.deltat};
+  k_finalize_force<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_f, c->nblk, F);
   return check_launch("k_finalize_force");
 }
 
@@ -2442,7 +2487,8 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   TimedEv te;
   tbegin(c, TC_KICK, te);
   const int nxt = 1 - c->cur;
-  if (fuse_drift && mode == 0)
+  const bool fused = fuse_drift && mode == 0;
+  if (fused)
     k_kick<true><<<c->nblk, MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol, mode, c->sref, c->part_k,
         lean_path(c) ? nullptr : c->pos4, p4f_of(c), c->img, c->cell_of, pack_geo(c), c->box + nxt, spec_of(c),
@@ -2453,13 +2499,13 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
         nullptr, nullptr, pack_geo(c), nullptr, spec_of(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
-  if (mode != 2) {
-    k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
-        c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
-        trace_row_dev, mon);
+  if (mode != 2) {  // initial accelerations have no sums
+    const FinKick F = {c->box + c->cur, fused ? c->box + nxt : nullptr, trace_row_dev,
+                       mon, c->p.deltat, c->p.bdel2, c->p.press};
+    k_finalize_kick<<<1, MDK_FOLD_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, F);
     RCK(check_launch("k_finalize_kick"));
   }
-  if (fuse_drift && mode == 0) {
+  if (fused) {
     c->pos4_valid = !lean_path(c);
     c->p4f_valid = true;
     c->cur = nxt;
@@ -2497,37 +2543,67 @@ static const double *poll_monitor(mdk_ctx *c) {
   const int slot = (int)(probe & 3);
   hipEventSynchronize(c->mon_ev[slot]);
   const double *m = c->mon_host + MON_STRIDE * slot;
-  for (int k = 0; k < 9; k++) c->b0_lag[k] = m[2 + k];
+  for (int k = 0; k < 9; k++) c->b0_lag[k] = m[MON_B0 + k];
   c->b0_lag_step = probe;
   return m;
 }
 
+// Frobenius norm of A Bref^-1 - s I for the monitor's Bref (column-major 3x3)
+static double strain_norm(const double *a, const double *bref, double s) {
+  const double det = bref[0] * (bref[4] * bref[8] - bref[7] * bref[5]) -
+                     bref[3] * (bref[1] * bref[8] - bref[7] * bref[2]) +
+                     bref[6] * (bref[1] * bref[5] - bref[4] * bref[2]);
+  double inv[9];  // cofactors / determinant, column-major
+  inv[0] = (bref[4] * bref[8] - bref[7] * bref[5]) / det;
+  inv[3] = -(bref[3] * bref[8] - bref[6] * bref[5]) / det;
+  inv[6] = (bref[3] * bref[7] - bref[6] * bref[4]) / det;
+  inv[1] = -(bref[1] * bref[8] - bref[7] * bref[2]) / det;
+  inv[4] = (bref[0] * bref[8] - bref[6] * bref[2]) / det;
+  inv[7] = -(bref[0] * bref[7] - bref[6] * bref[1]) / det;
+  inv[2] = (bref[1] * bref[5] - bref[4] * bref[2]) / det;
+  inv[5] = -(bref[0] * bref[5] - bref[3] * bref[2]) / det;
+  inv[8] = (bref[0] * bref[4] - bref[3] * bref[1]) / det;
+  double e2 = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double x = (i == j) ? -s : 0.0;
+      for (int k = 0; k < 3; k++) x += a[i + 3 * k] * inv[k + 3 * j];
+      e2 += x * x;
+    }
+  return sqrt(e2);
+}
+// box strain since the list was built (B0 Bref^-1 - I) and that of the next
+// drift's box change ((b1 + b2) Bref^-1)
+static double strain_since_build(const double *m) { return strain_norm(m + MON_B0, m + MON_BREF, 1.0); }
+static double strain_of_step(const double *m) { return strain_norm(m + MON_DB, m + MON_BREF, 0.0); }
+
 // The list a step uses holds every pair within the potential's range while
 // each atom has moved less than d and the box has strained by less than e
-// (monitor sample: d^2 = m[0], e = m[11]) since the list was built, with
-// 2 d + e r_list < skin: no pair separation has then changed by the skin.
+// (monitor sample: d^2 = m[0], e = strain_since_build) since the list was
+// built, with 2 d + e r_list < skin: no pair separation has then changed by
+// the skin.
 static double skin_used(const mdk_ctx *c, const double *m) {
-  return sqrt(fmax(m[0], 0.0)) + 0.5 * m[11] * c->r_list;  // against skin / 2
+  return sqrt(fmax(m[0], 0.0)) + 0.5 * strain_since_build(m) * c->r_list;  // against skin / 2
 }
 static int skin_violation(const mdk_ctx *c, const double *m, long step) {
   return set_err(MDK_ERR_SKIN,
                  "step %ld: an atom moved %.4g A and the box strained %.3g since the list was built "
                  "(bound: skin/2 = %.4g A); pairs may have been missed",
-                 step, sqrt(fmax(m[0], 0.0)), m[11], 0.5 * c->skin);
+                 step, sqrt(fmax(m[0], 0.0)), strain_since_build(m), 0.5 * c->skin);
 }
 
 // Checks the completed sample of step s-2 against the skin bound (an error if
 // it was exceeded), then decides whether the next force evaluation could see
 // it exceeded: that sample's displacement and strain plus three times its
-// step's (m[1]: squared displacement of the next drift, m[12]: strain of one
-// box step).
+// step's (m[1]: squared displacement of the next drift, strain_of_step: that
+// of the next drift's box change).
 static int skin_check(mdk_ctx *c, bool *due) {
   const double *m = poll_monitor(c);
   *due = !c->list_valid;
   if (!m) return MDK_OK;
   if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, c->step_counter - 2);
   if (c->step_counter - c->last_rebuild_step < 3) return MDK_OK;  // the sample predates the list
-  const double stp = sqrt(fmax(m[1], 0.0)) + 0.5 * m[12] * c->r_list;
+  const double stp = sqrt(fmax(m[1], 0.0)) + 0.5 * strain_of_step(m) * c->r_list;
   *due = *due || skin_used(c, m) + 3.0 * stp >= 0.5 * c->skin;
   return MDK_OK;
 }
@@ -3312,9 +3388,9 @@ extern "C" int mdk_update_therm_avs(void *h, double *s1, double *s2, double *s3)
       nullptr, pack_geo(c), nullptr, spec_of(c), nullptr);
   tend(c, te);
   RCK(check_launch("k_kick"));
-  k_finalize_kick<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(
-      c->part_k, c->nblk, c->stage_k, c->fin_cnt + 1, c->box + c->cur, c->p.deltat, c->p.bdel2, c->p.press,
-      nullptr, nullptr);
+  const FinKick F = {c->box + c->cur, nullptr, nullptr, nullptr,
+                     c->p.deltat, c->p.bdel2, c->p.press};
+  k_finalize_kick<<<1, MDK_FOLD_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, F);
   RCK(check_launch("k_finalize_kick"));
   BoxState s;
   RCK(read_box(c, &s));
