@@ -105,6 +105,21 @@ def batch_legs(reps):
     loss, grad = be.loss_grad()
     ms_grad = be.time_loss_grad(reps)
     be.close()
+    na, nf = info["natom"], info["pairs"] / info["natom"]
+    # per-atom stream models: evaluation = positions 24 + forces 24 + F' 8 +
+    # the list written once and read by the density and force phases (12 n_f),
+    # plus energy and virial per cell; loss + gradient = positions, force
+    # targets, forces, force seeds written and read (120) + the list written
+    # once and read four times (20 n_f)
+    b_eval = na * (56.0 + 12.0 * nf) + 56.0 * len(cells)
+    b_grad = na * (120.0 + 20.0 * nf) + 56.0 * len(cells)
+
+    def roof(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"bytes_per_launch": nbytes, "ms_per_launch": ms, "achieved_GBs": gbs,
+                "frac": gbs / PEAK_HBM_GBS,
+                "bound": "VALU issue and latency (far from the HBM roof; per-type VALU counts "
+                         "are in profiles/)"}
     return {
         "workload": "4096 periodic Fe cells, bcc/fcc n^3 supercells n in 3..10, strain +-3%, "
                     "0-2% vacancies, sigma 0.1 A (seed 7)",
@@ -113,8 +128,9 @@ def batch_legs(reps):
         "tuned_block": block, "tuned_workgroups_per_cu": wg,
         "ms_per_evaluation": ms_eval, "atoms_per_s": info["natom"] / ms_eval * 1e3,
         "cells_per_s": len(cells) / ms_eval * 1e3,
+        "roofline": roof(b_eval, ms_eval),
         "adjoint": {"ms_per_loss_grad": ms_grad, "grads_per_s": 1e3 / ms_grad, "loss": loss,
-                    "n_parameters": len(grad)},
+                    "n_parameters": len(grad), "roofline": roof(b_grad, ms_grad)},
         "dtype": "f32 pair/spline arithmetic, f64 accumulation",
     }
 
@@ -152,30 +168,72 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(sd, steps, threads):
-    """The reference Fortran (OpenMP) on the host cores, else the C port."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_step_seconds(refrun, sd, threads, n1, n2, repeats, **kw):
+    """Per-step wall seconds of the reference's MD loop: (loop(n2) - loop(n1))
+    / (n2 - n1) for runs of n1 and n2 steps, median of `repeats` pairs."""
+    per = []
+    for _ in range(repeats):
+        t1 = refrun.time_md_loop(sd, n1, threads, timeout=900, **kw)
+        t2 = refrun.time_md_loop(sd, n2, threads, timeout=900, **kw)
+        per.append((t2 - t1) / (n2 - n1))
+    return float(np.median(per)), per
+
+
+def cpu_baseline(sd, threads, with_c2=True):
+    """The reference Fortran (its modules built with amdflang -O2 -fopenmp) on
+    the host cores, OMP_PROC_BIND=close and OMP_PLACES=cores.  Per-step wall
+    time by step differencing of the MD loop: runs of 1 and 11 steps, so the
+    difference holds ten steps and exactly one list build (nbrupdate 10), as
+    the reference's cadence; median of 5 pairs at `threads` threads and of 3
+    pairs at 1 thread.  The 3.46M-atom liquid sample: runs of 1 and 3 steps
+    (no list build in the difference), median of 3 pairs.  Falls back to the
+    C port when the reference driver is not built."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
+    host = {"model": _cpu_model(), "host_cpus": len(os.sched_getaffinity(0)), "nproc": os.cpu_count(),
+            "omp": "OMP_PROC_BIND=close OMP_PLACES=cores"}
     if os.path.exists(refrun.DRIVER_OMP):
-        secs = refrun.time_md_loop(sd, steps, threads, ivol=0, ntcm=10, press=0.0, boxmass=50.0,
-                                   timeout=900)
-        return {"value": sd.n * steps / secs, "unit": "atom-steps/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"{sd.n} atoms x {steps} MD steps of the same NPT workload (one "
-                          f"nbrupdate=10 cycle), reference modules built with amdflang -O2 "
-                          f"-fopenmp, wall clock of the MD loop"}
+        kw = dict(ivol=0, ntcm=10, press=0.0, boxmass=50.0)
+        sec, per = _ref_step_seconds(refrun, sd, threads, 1, 11, 5, **kw)
+        sec1, per1 = _ref_step_seconds(refrun, sd, 1, 1, 11, 3, **kw)
+        out = {"value": sd.n / sec, "unit": "atom-steps/s", "cores": threads, "kind": "reference",
+               "sample": f"{sd.n} atoms (the same NPT workload), per-step wall time of the MD loop from "
+                         f"runs of 1 and 11 steps (ten steps incl. one nbrupdate=10 list build), "
+                         f"median of 5 pairs",
+               "per_step_s": per, **host,
+               "single_thread": {"value": sd.n / sec1, "cores": 1, "per_step_s": per1,
+                                 "sample": "same, median of 3 pairs"}}
+        if with_c2:
+            liq = c2_system()
+            sec2, per2 = _ref_step_seconds(refrun, liq, threads, 1, 3, 3, ivol=1, ntcm=1000, press=0.0,
+                                           boxmass=50.0)
+            out["hot_liquid_3456000"] = {"value": liq.n / sec2, "cores": threads, "per_step_s": per2,
+                                         "sample": f"{liq.n} atoms, runs of 1 and 3 steps (no list build "
+                                                   f"in the difference), median of 3 pairs"}
+        return out
     import pyoracle as po
     pot = po.Potential()
     po.lib().orc_set_threads(threads)
     S = po.OracleSystem(sd, pot, ivol=0, ntcm=10)
     S.call("linkup")
     S.call("md_init")
+    steps = 10
     t0 = time.perf_counter()
     for _ in range(steps):
         S.call("md_step")
     secs = time.perf_counter() - t0
     return {"value": sd.n * steps / secs, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{sd.n} atoms x {steps} MD steps, C restatement with OpenMP"}
+            "sample": f"{sd.n} atoms x {steps} MD steps, C restatement with OpenMP", **host}
 
 
 def c0_leg(steps=1000):
@@ -189,24 +247,37 @@ def c0_leg(steps=1000):
         e = engine.Engine(sd, ivol=1, precision=prec, ntcm=10, rebuild=0)
         e.linkup()
         e.md_init()
+        e.synchronize()
+        t0 = time.perf_counter()
         tr = e.md_steps(steps, trace=True)
+        e.synchronize()
+        secs = time.perf_counter() - t0
         e.close()
         te = tr[:, 2] / sd.n
         t_ps = np.arange(steps) * 1e-3
         slope = float(np.polyfit(t_ps, te, 1)[0])
         out[tag] = {"te_first_eV_per_atom": float(te[0]), "te_last_eV_per_atom": float(te[-1]),
                     "max_abs_dev_eV_per_atom": float(np.max(np.abs(te - te[0]))),
-                    "drift_eV_per_atom_per_ps": slope}
+                    "drift_eV_per_atom_per_ps": slope,
+                    "ms_per_step": secs / steps * 1e3, "atom_steps_per_s": sd.n * steps / secs,
+                    "bound": "launch latency (2000 atoms: each kernel is a few workgroups)"}
     return out
+
+
+def c2_system():
+    """120^3 bcc cells (3,456,000 atoms) at 7.0 g/cm^3, 2500 K, dt = 0.5 fs."""
+    from moldy_amd import lattice
+    a = lattice.liquid_lattice_parameter(7.0)
+    return lattice.make_system(nc=120, a=a, sigma=0.05, temp=2500.0, deltat_fs=0.5, seed=2500)
 
 
 def c2_leg(steps=1000):
     """Hot liquid: 120^3 bcc cells (3,456,000 atoms) at 7.0 g/cm^3, 2500 K,
     NVE, dt = 0.5 fs, skin 0.3 A, rebuilds on the displacement criterion only;
     rebuild count and neighbour-build time reported apart from the step rate."""
-    from moldy_amd import engine, lattice
-    a = lattice.liquid_lattice_parameter(7.0)
-    sd = lattice.make_system(nc=120, a=a, sigma=0.05, temp=2500.0, deltat_fs=0.5, seed=2500)
+    from moldy_amd import engine
+    sd = c2_system()
+    a = sd.b0[0, 0] / 120
     out = {"workload": f"bcc Fe 120^3 cells = {sd.n} atoms, a={a:.5f} A (7.0 g/cm^3), 2500 K, NVE, "
                        f"dt=0.5 fs, skin 0.3 A, {steps} steps, rebuild when max displacement > skin/2"}
     for prec, tag in ((engine.PREC_FP32, "fp32"), (engine.PREC_FP64, "fp64")):
@@ -215,21 +286,30 @@ def c2_leg(steps=1000):
         e.md_init()
         e.md_steps(10, trace=False)
         e.synchronize()
-        e.timing(True, classes=("rebuild",), every=1)
+        e.timing(True, classes=("rebuild", "force"), every=1)
         r0 = e.stats()[0]
         t0 = time.perf_counter()
         tr = e.md_steps(steps, trace=True)
         e.synchronize()
         secs = time.perf_counter() - t0
-        reb = e.stats()[0] - r0
+        reb, full = e.stats()
+        reb -= r0
         kt = e.timing_read()
         e.timing(False)
         e.close()
         rb = kt.get("rebuild", (0.0, 0))
+        kf = kt.get("force", (0.0, 0))
+        us_f = 1e3 * kf[0] / max(kf[1], 1)
+        nb = sd.n * (56.0 + 4.0 * full / sd.n)  # the same per-atom stream model as the main roofline
         out[tag] = {"ms_per_step": secs / steps * 1e3, "atom_steps_per_s": sd.n * steps / secs,
                     "rebuilds": reb, "rebuild_us_avg": 1e3 * rb[0] / max(rb[1], 1),
                     "rebuild_ms_total": rb[0],
-                    "te_drift_eV_per_atom": float((tr[-1, 2] - tr[0, 2]) / sd.n)}
+                    "te_drift_eV_per_atom": float((tr[-1, 2] - tr[0, 2]) / sd.n),
+                    "roofline": {"kernel": "force pass", "us_per_launch": us_f, "bytes_per_launch": nb,
+                                 "achieved_GBs": nb / (us_f * 1e-6) / 1e9,
+                                 "frac": nb / (us_f * 1e-6) / 1e9 / PEAK_HBM_GBS,
+                                 "full_list_neighbours_per_atom": full / sd.n},
+                    "timing_note": "HIP events around every force launch and every rebuild in this leg"}
     return out
 
 
@@ -238,8 +318,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=10)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0))))
+    ap.add_argument("--no-cpu-c2", action="store_true", help="skip the 3.46M-atom CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -292,8 +373,9 @@ def main():
     achieved = bytes_per_launch / (kf["us_avg"] * 1e-6) / 1e9 if kf["us_avg"] else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": (achieved / PEAK_HBM_GBS) if achieved else None, "traffic": measured_traffic(),
-                "kernel": "k_force<float, false, POT_FE, true> (FP32 pair arithmetic, Fe-Fe knot counts, 16-byte partner records)", "us_per_launch": kf["us_avg"],
-                "bytes_per_launch": bytes_per_launch}
+                "kernel": "k_force_f32 (FP32 pair terms from the interval table, 16-byte FP32 partner "
+                          "records, FP64 sums across lanes)",
+                "us_per_launch": kf["us_avg"], "bytes_per_launch": bytes_per_launch}
 
     batch = None
     if not args.no_batch:
@@ -316,8 +398,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(sd, args.cpu_steps, args.cpu_threads)
-        log(f"cpu baseline {cpu['value']:.4e} atom-steps/s on {cpu['cores']} cores ({cpu['kind']})")
+        cpu = cpu_baseline(sd, args.cpu_threads, with_c2=not args.no_cpu_c2)
+        log(f"cpu baseline {cpu['value']:.4e} atom-steps/s on {cpu['cores']} cores ({cpu['kind']}, "
+            f"{cpu['model']})")
+        if "single_thread" in cpu:
+            log(f"cpu baseline, 1 thread: {cpu['single_thread']['value']:.4e} atom-steps/s")
 
     line = {
         "metric": "atom-steps/s, 0.5M-atom Fe EAM (FP32/FP64-acc vs FP64); batched grads/s",

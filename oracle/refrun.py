@@ -52,8 +52,8 @@ def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
         env = dict(os.environ)
         if threads is not None:
             env["OMP_NUM_THREADS"] = str(threads)
-            env.setdefault("OMP_PROC_BIND", "close")
-            env.setdefault("OMP_PLACES", "cores")
+            env["OMP_PROC_BIND"] = "close"
+            env["OMP_PLACES"] = "cores"
         res = subprocess.run([driver, inp, outp], cwd=td, capture_output=True, text=True, env=env,
                              timeout=timeout)
         if res.returncode != 0:
