@@ -152,7 +152,7 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_per_dispatch": 2 * 4 * (1 << 28)}))
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_bcc524288_fp32_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_f32_bcc524288_traffic.md")
 
 
 def measured_traffic():
