@@ -1110,7 +1110,7 @@ __device__ __forceinline__ double density_atom(double *atoms, double4 *pos4, flo
     daf[i] = dfi;
     pe = f;
   }
-  pos4[i].w = dfi;
+  if (pos4) pos4[i].w = dfi;
   if (p4f) p4f[i].w = (float)dfi;
   return pe;
 }
@@ -1676,12 +1676,15 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_f32(double *atoms, double
     const float4 pi = p4f[i];
     const int nn = numn[i];
     int na = 0;
+    int4 e4 = make_int4(0, 0, 0, 0);
+    if (nn > 0) e4 = *(const int4 *)(nbr + ((size_t)i << 2));
     for (int q = 0; 4 * q < nn; q++) {
-      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       float4 pj[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) pj[u] = p4f[e[u] & NBR_IDX_MASK];
+      // the next quad's indices load while this quad's partners arrive
+      if (4 * (q + 1) < nn) e4 = *(const int4 *)(nbr + (((size_t)(q + 1) * n + i) << 2));
       float mn = rc2;
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -1738,12 +1741,15 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_force_f32(double *atoms, const fl
     float fx = 0.f, fy = 0.f, fz = 0.f;
     // sums of V and of r r^T fp (the factor 1/2 of both is applied once below)
     float pe2 = 0.f, wxx = 0.f, wyy = 0.f, wzz = 0.f, wxy = 0.f, wxz = 0.f, wyz = 0.f;
+    int4 e4 = make_int4(0, 0, 0, 0);
+    if (nn > 0) e4 = *(const int4 *)(nbr + ((size_t)i << 2));
     for (int q = 0; 4 * q < nn; q++) {
-      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
       const unsigned e[4] = {(unsigned)e4.x, (unsigned)e4.y, (unsigned)e4.z, (unsigned)e4.w};
       float4 pj[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) pj[u] = p4f[e[u] & NBR_IDX_MASK];
+      // the next quad's indices load while this quad's partners arrive
+      if (4 * (q + 1) < nn) e4 = *(const int4 *)(nbr + (((size_t)(q + 1) * n + i) << 2));
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         float rx, ry, rz;
@@ -2317,10 +2323,12 @@ static void lean_density(mdk_ctx *c, int do_embed) {
     k_density_part<float, POT_FE, true><<<block_count(c), MDK_BLOCK, 0, c->stream>>>(
         c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
         c->nact);
-  else
-    k_density_f32<<<block_count(c), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn,
+  else {  // F' goes to the FP32 records only: the FP64 copy is not kept on this path
+    k_density_f32<<<block_count(c), MDK_BLOCK, 0, c->stream>>>(c->atoms, nullptr, c->p4f, c->n, c->nbr, c->numn,
                                                                c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
                                                                c->nact);
+    c->pos4_valid = false;
+  }
   c->part_due = false;
 }
 static void lean_force(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
