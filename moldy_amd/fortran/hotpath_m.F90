@@ -487,11 +487,206 @@ module quench_m
   use particles_m
   use parrinellorahman_m
   use dynamics_m
+  use system_m
   use mdk_device_m
   implicit none
   private
   public :: velocityverlet, predic, correc
+  public :: quench, init_quench_m, cleanup_quench_m
+  integer, save :: nvari = 0, nevals = 0
+  real(kind_wp), allocatable, save :: qx(:), qg(:)   ! variables and gradient
 contains
+  !> init_quench_m / cleanup_quench_m (src/quench_m.F90:83-91): the work
+  !> vectors of 3 nm + 9 variables
+  subroutine init_quench_m
+    type(simparameters) :: sp
+    integer :: istat
+    sp = get_params()
+    nvari = 3*sp%nm + 9
+    allocate(qx(nvari), qg(nvari), stat=istat)
+    if (istat /= 0) stop 'ALLOCATION ERROR (init_quench_m) - Possibly out of memory.'
+  end subroutine init_quench_m
+  subroutine cleanup_quench_m
+    if (allocated(qx)) deallocate(qx, qg)
+  end subroutine cleanup_quench_m
+
+  !> The objective of quench (derivs, src/quench_m.F90:214-322): H = pe +
+  !> press vol at the point x (fractional positions atom by atom, then B0 row
+  !> by row) and its gradient, dH/ds_i = -tg f_i with tg = B0^T B0, dH/dB0 =
+  !> -tp + press tc for the box elements that move (all for ivol < 1, zz for
+  !> ivol = 4), from one device force evaluation.
+  subroutine q_eval(x, hval, g)
+    real(kind_wp), intent(in) :: x(:)
+    real(kind_wp), intent(out) :: hval, g(:)
+    type(simparameters) :: sp
+    integer :: i, j, m
+    real(kind_wp) :: t(3,3)
+    sp = get_params()
+    m = 3*sp%nm
+    do i = 1, 3
+       do j = 1, 3
+          m = m + 1
+          b0(i,j) = x(m)
+       end do
+    end do
+    do i = 1, sp%nm
+       x0(i) = x(3*i-2)
+       y0(i) = x(3*i-1)
+       z0(i) = x(3*i)
+    end do
+    call force
+    nevals = nevals + 1
+    t = matmul(transpose(b0), b0)
+    do i = 1, sp%nm
+       g(3*i-2) = -(t(1,1)*fx(i) + t(1,2)*fy(i) + t(1,3)*fz(i))
+       g(3*i-1) = -(t(2,1)*fx(i) + t(2,2)*fy(i) + t(2,3)*fz(i))
+       g(3*i)   = -(t(3,1)*fx(i) + t(3,2)*fy(i) + t(3,3)*fz(i))
+    end do
+    m = 3*sp%nm
+    do i = 1, 3
+       do j = 1, 3
+          m = m + 1
+          g(m) = 0.0_kind_wp
+          if (sp%ivol < 1 .or. (sp%ivol == 4 .and. i*j == 9)) g(m) = -tp(i,j) + sp%press*tc(i,j)
+       end do
+    end do
+    hval = pe + sp%press*vol
+  end subroutine q_eval
+
+  !> Minimum of phi(a) = H(x + a d), a > 0: bracketing with doubling steps,
+  !> then safeguarded secant steps on phi'(a) = g(x + a d).d; accepts a point
+  !> with |phi'| <= 0.1 |phi'(0)| and H below H(x), else returns the lowest
+  !> point seen (a = 0 when none is lower).
+  subroutine q_line(x, hx, gx, d, step0, hb, xb, gb, ab)
+    real(kind_wp), intent(in) :: x(:), hx, gx(:), d(:), step0
+    real(kind_wp), intent(out) :: hb, xb(:), gb(:), ab
+    real(kind_wp), allocatable :: xa(:), ga(:)
+    real(kind_wp) :: d0, lo, flo, dlo, hi, fhi, dhi, a, fa, da, t, w
+    logical :: have_hi
+    integer :: it
+    allocate(xa(size(x)), ga(size(x)))
+    d0 = dot_product(gx, d)
+    lo = 0.0_kind_wp; flo = hx; dlo = d0
+    hi = 0.0_kind_wp; fhi = 0.0_kind_wp; dhi = 0.0_kind_wp; have_hi = .false.
+    a = step0
+    hb = hx; xb = x; gb = gx; ab = 0.0_kind_wp
+    do it = 1, 40
+       xa = x + a*d
+       call q_eval(xa, fa, ga)
+       da = dot_product(ga, d)
+       if (fa < hb) then
+          hb = fa; xb = xa; gb = ga; ab = a
+       end if
+       if (abs(da) <= 0.1_kind_wp*abs(d0) .and. fa <= hx) then
+          hb = fa; xb = xa; gb = ga; ab = a
+          exit
+       end if
+       if (da > 0.0_kind_wp .or. fa > flo) then
+          hi = a; fhi = fa; dhi = da; have_hi = .true.
+       else
+          lo = a; flo = fa; dlo = da
+       end if
+       if (.not. have_hi) then
+          if (da < 0.0_kind_wp) a = 2.0_kind_wp*a
+          cycle
+       end if
+       if (dhi > 0.0_kind_wp .and. dlo < 0.0_kind_wp) then
+          t = lo + (hi - lo)*(-dlo)/(dhi - dlo)
+       else
+          t = 0.5_kind_wp*(lo + hi)
+       end if
+       w = hi - lo
+       a = min(max(t, lo + 0.1_kind_wp*w), hi - 0.1_kind_wp*w)
+    end do
+    deallocate(xa, ga)
+  end subroutine q_line
+
+  !> quench (src/quench_m.F90:100-205): relaxes positions and, for ivol < 1
+  !> (zz only for ivol = 4), the box to the minimum of H = pe + press vol,
+  !> each point evaluated by the device force.  The minimiser is this
+  !> project's Polak-Ribiere conjugate gradient (PR+, restarted every
+  !> max(3 nm + 9, 50) iterations or on a non-descent direction) with the line
+  !> search above; as the reference's VA14CD call, the first step is sized for
+  !> an expected decrease dfn = 0.1 and the run ends once two successive
+  !> iterations lower H by less than acc = 5e-7 nm (src/quench_m.F90:134-136).
+  !> Afterwards velocities, higher derivatives and box rates are zero, as
+  !> there.
+  subroutine quench
+    type(simparameters) :: sp
+    real(kind_wp), allocatable :: d(:), xn(:), gn(:)
+    real(kind_wp) :: hx, hn, a, acc, dfn, gd, gdn, beta, step, dec
+    integer :: i, j, m, it, small
+    sp = get_params()
+    if (.not. allocated(qx)) call init_quench_m
+    write(*,'(A,I9,A)') ' QUENCH, FINDING MINIMUM ENERGY CONFIGURATION OF', sp%nm, ' ATOMS'
+    do i = 1, sp%nm
+       qx(3*i-2) = x0(i)
+       qx(3*i-1) = y0(i)
+       qx(3*i) = z0(i)
+    end do
+    m = 3*sp%nm
+    do i = 1, 3
+       do j = 1, 3
+          m = m + 1
+          qx(m) = b0(i,j)
+       end do
+    end do
+    allocate(d(nvari), xn(nvari), gn(nvari))
+    dfn = 0.1_kind_wp
+    acc = 5.0e-7_kind_wp*sp%nm
+    nevals = 0
+    call q_eval(qx, hx, qg)
+    d = -qg
+    step = 2.0_kind_wp*dfn/max(dot_product(qg, qg), 1.0e-300_kind_wp)
+    small = 0
+    do it = 1, 20000
+       gd = dot_product(qg, d)
+       if (gd >= 0.0_kind_wp) then
+          d = -qg
+          gd = -dot_product(qg, qg)
+       end if
+       call q_line(qx, hx, qg, d, step, hn, xn, gn, a)
+       if (a == 0.0_kind_wp) then
+          if (all(d == -qg)) exit
+          d = -qg
+          cycle
+       end if
+       dec = hx - hn
+       beta = max(0.0_kind_wp, dot_product(gn, gn - qg)/max(dot_product(qg, qg), 1.0e-300_kind_wp))
+       if (mod(it, max(3*sp%nm + 9, 50)) == 0) beta = 0.0_kind_wp
+       d = -gn + beta*d
+       gdn = dot_product(gn, d)
+       if (gdn < 0.0_kind_wp) then
+          step = a*gd/gdn
+       else
+          step = a
+       end if
+       qx = xn; hx = hn; qg = gn
+       if (dec < acc) then
+          small = small + 1
+       else
+          small = 0
+       end if
+       if (small >= 2) exit
+    end do
+    call q_eval(qx, hx, qg)   ! the host and device state at the minimum
+    write(*,'(A,ES18.10,A,F10.6,A,ES12.4,A,ES13.7)') 'FINAL U= ', pe/sp%nm, ' V=', vol/sp%nm, &
+         ' P=', sp%press*press_to_gpa, ' H=', hx/sp%nm
+    deallocate(d, xn, gn)
+    b1 = 0.0_kind_wp; b2 = 0.0_kind_wp; b3 = 0.0_kind_wp
+    call pr_get_metric_tensor
+    x1 = 0.0_kind_wp; y1 = 0.0_kind_wp; z1 = 0.0_kind_wp
+    x2 = 0.0_kind_wp; y2 = 0.0_kind_wp; z2 = 0.0_kind_wp
+    x3 = 0.0_kind_wp; y3 = 0.0_kind_wp; z3 = 0.0_kind_wp
+    ke = 0.0_kind_wp
+    te = ke + pe
+    h = te + sp%press*vol
+    th = h
+    sp = get_params()
+    sp%boxtem = 0.0_kind_wp
+    call set_params(sp)
+  end subroutine quench
+
   !> predic (src/quench_m.F90:636-671); a no-op until correc or
   !> velocityverlet has run, as the reference's (see mdk_predic)
   subroutine predic

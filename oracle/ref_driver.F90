@@ -216,13 +216,11 @@ program ref_driver
      write(20) (demb(rhogrid(k),26), k = 1, nrho)
   end if
 
-#ifndef MDK_DROPIN
   if (iand(flags, 32) /= 0) then
      call init_quench_m
      call quench
      call cleanup_quench_m
   end if
-#endif
   allocate(trace(6, max(nsteps,1)), xtrace(13, max(nsteps,1)))
   sp = get_params()
   sp%rqke = 1.5d0*bk*sp%temprq*(sp%nm-3)

@@ -63,3 +63,30 @@ def test_fortran_dropin_matches_reference(oracle, case):
             else:
                 knot = F["rgrid"] > (2.05 if kw.get("material") == "iron_carbon" else 1.9)
                 assert rel(d[k][knot], F[k][knot]) < 1e-13, k
+
+
+def test_fortran_dropin_quench_matches_reference():
+    """quench_m's init_quench_m / quench / cleanup_quench_m through the drop-in
+    (the driver's flag 32: a quench after the first force, as moldin.F90:566-572)
+    against the reference's VA14CD quench of the 249-atom vacancy cell.  Both
+    stop once iterations lower H by less than 5e-7 n eV, so the energies agree
+    within 5e-7 n eV, positions within 2e-4 fractional and the box within 1e-4
+    relative."""
+    import gen_golden
+    import refrun
+    F = load_golden("quench_vac_249")
+    kw = {k[4:]: (int(F[k]) if F[k].dtype.kind == "i" else float(F[k])) for k in F.files if k.startswith("run_")}
+    if not os.path.exists(DROPIN):
+        pytest.fail("drop-in driver not built (moldy_amd.build.build_fortran)")
+    print("[gpu-test] fortran drop-in quench", flush=True)
+    d = gen_golden.run_reference(golden_system(F), driver=DROPIN, pot_file=refrun.POT_FILE, **kw)
+    n = F["s"].shape[0]
+    print("[gpu-test] pe", d["pe_q"], "reference", float(F["pe_q"]), flush=True)
+    assert abs(d["pe_q"] - float(F["pe_q"])) <= 5e-7 * n
+    fin = np.stack([d["x0_f"], d["y0_f"], d["z0_f"]], axis=1)
+    ref = np.stack([F["x0_f"], F["y0_f"], F["z0_f"]], axis=1)
+    dd = fin - ref
+    dd -= np.round(dd)
+    assert np.max(np.abs(dd)) < 2e-4
+    np.testing.assert_allclose(np.diag(d["b0_f"]), np.diag(F["b0_f"]), rtol=1e-4)
+    assert np.all(d["x1_f"] == 0.0)  # quench leaves the atoms at rest
