@@ -188,21 +188,130 @@ contains
     demb = eval1(5_c_int, rho_)
   end function demb
 
-  !> embed: F(rho), F'(rho) and the embedding energy (src/potential_m.F90:148-169),
-  !> with the densities recomputed on the device from the current positions
+  !> embed (src/potential_m.F90:148-169): F(rho) and F'(rho) of the caller's
+  !> RHO array, atoms with rho <= 1e-40 keeping their AFRHO and DAFRHO, and
+  !> the embedding energy.  The embedding functions are evaluated by the
+  !> device for all atoms in one call each (the exact functions of the
+  !> density pass); the sum runs in atom order.
   subroutine embed(pexx)
     real(kind_wp) :: pexx
-    real(c_double) :: pe_c
-    call mdk_dev_push()
-    call mdk_check(mdk_embed(mdk_dev_handle(), pe_c), 'mdk_embed')
-    pexx = pe_c
-    call mdk_check(mdk_download(mdk_dev_handle(), MDK_RHO, rho), 'download rho')
-    call mdk_check(mdk_download(mdk_dev_handle(), MDK_AFRHO, afrho), 'download afrho')
-    call mdk_check(mdk_download(mdk_dev_handle(), MDK_DAFRHO, dafrho), 'download dafrho')
+    real(c_double), allocatable :: f(:), df(:)
+    integer :: i, n
+    n = size(rho)
+    allocate(f(n), df(n))
+    call mdk_check(mdk_eval_potential(mdk_dev_handle(), 4_c_int, rho, f, int(n, c_int)), 'mdk_eval_potential')
+    call mdk_check(mdk_eval_potential(mdk_dev_handle(), 5_c_int, rho, df, int(n, c_int)), 'mdk_eval_potential')
+    pexx = 0.0_kind_wp
+    do i = 1, n
+       if (rho(i) <= 1.0e-40_kind_wp) cycle
+       afrho(i) = f(i)
+       dafrho(i) = df(i)
+       pexx = pexx + afrho(i)
+    end do
+    deallocate(f, df)
   end subroutine embed
 
-  !> potin writes potential.dat for plotting; not part of the device path
+  !> The functions of one kind (0 V, 1 V', 2 phi, 3 phi', 4 F, 5 F') at many
+  !> points in one device call.
+  subroutine evalv(which, x, y)
+    integer(c_int), intent(in) :: which
+    real(kind_wp), intent(in) :: x(:)
+    real(kind_wp), intent(out) :: y(:)
+    call mdk_check(mdk_eval_potential(mdk_dev_handle(), which, x, y, int(size(x), c_int)), 'mdk_eval_potential')
+  end subroutine evalv
+
+  !> potin (src/potential_m.F90:429-642): potential.dat, for 1000 nearest-
+  !> neighbour distances over the potential's range, the pair and density
+  !> functions there and the cohesive energies of bcc, fcc and ideal hcp at
+  !> the same volume per atom (shell radii and counts, the literals' kinds and
+  !> the reference's choice of embedding term for hcp kept, so the file
+  !> agrees with the reference's).  The device holds the Fe-Fe functions
+  !> only: blocks are written for Fe-Fe species pairs.
   subroutine potin
+    integer, parameter :: npts = 1000
+    real(kind_wp), parameter :: rminplot = 1.5_kind_wp
+    ! shell weights of the pair sums (half the shell counts) and of the
+    ! densities: bcc, fcc and ideal hcp
+    real(kind_wp), parameter :: nbv(6) = (/4.0, 3.0, 6.0, 12.0, 4.0, 3.0/)
+    real(kind_wp), parameter :: nbp(6) = (/8.0, 6.0, 12.0, 24.0, 8.0, 6.0/)
+    real(kind_wp), parameter :: nfv(5) = (/6.0, 3.0, 12.0, 6.0, 12.0/)
+    real(kind_wp), parameter :: nfp(5) = (/12.0, 6.0, 24.0, 12.0, 24.0/)
+    real(kind_wp), parameter :: nhv(7) = (/6.0, 3.0, 1.0, 9.0, 6.0, 3.0, 9.0/)
+    real(kind_wp) :: rb(npts, 6), rf(npts, 5), rh(npts, 7)
+    real(kind_wp) :: vb(npts, 6), pb(npts, 6), vf(npts, 5), pf(npts, 5), vh(npts, 7), ph(npts, 7)
+    real(kind_wp) :: ccb(npts), ccf(npts), eb(npts), ef(npts), dph(npts), dvh(npts)
+    real(kind_wp) :: rmin, rmax, ecr, ecbccb, ecfcc, echcp, veff
+    type(simparameters) :: sp
+    integer :: i, j, ix, k, u
+    sp = get_params()
+    call get_available_potential_range(rmin, rmax)
+    rmin = max(rmin, rminplot)
+    open(newunit=u, file="potential.dat", status='unknown')
+    do i = 1, sp%nspec
+       do j = i, sp%nspec
+          if (ispec(i) /= 26 .or. ispec(j) /= 26) cycle
+          do ix = 1, npts
+             rb(ix, 1) = 2.0 + (ix*(rmax - rmin))/npts
+             rb(ix, 2) = rb(ix, 1)*(2.0/sqrt(3.0d0))
+             rb(ix, 3) = rb(ix, 2)*sqrt(2.0d0)
+             rb(ix, 4) = rb(ix, 1)*sqrt(11.0/3.0)
+             rb(ix, 5) = rb(ix, 1)*2.0
+             rb(ix, 6) = rb(ix, 1)*sqrt(16.0/3.0)
+             ! fcc at the same volume: second shell x2a, first x2a/sqrt(2)
+             rf(ix, 2) = rb(ix, 2)*2d0**(0.3333333333)
+             rf(ix, 1) = rf(ix, 2)/sqrt(2.0d0)
+             rf(ix, 3) = rf(ix, 1)*sqrt(3.0d0)
+             rf(ix, 4) = rf(ix, 1)*2d0
+             rf(ix, 5) = rf(ix, 1)*sqrt(5d0)
+             ! ideal hcp, the same x2a
+             rh(ix, 1) = rf(ix, 1)
+             rh(ix, 2) = rf(ix, 2)
+             rh(ix, 3) = rh(ix, 1)*sqrt(8.0d0/3.0d0)
+             rh(ix, 4) = rh(ix, 1)*sqrt(3.0d0)
+             rh(ix, 5) = rh(ix, 1)*sqrt(11.0d0/3.0d0)
+             rh(ix, 6) = rh(ix, 1)*2d0
+             rh(ix, 7) = rh(ix, 1)*sqrt(5d0)
+          end do
+          do k = 1, 6
+             call evalv(0_c_int, rb(:, k), vb(:, k))
+             call evalv(2_c_int, rb(:, k), pb(:, k))
+          end do
+          do k = 1, 5
+             call evalv(0_c_int, rf(:, k), vf(:, k))
+             call evalv(2_c_int, rf(:, k), pf(:, k))
+          end do
+          do k = 1, 7
+             call evalv(0_c_int, rh(:, k), vh(:, k))
+             call evalv(2_c_int, rh(:, k), ph(:, k))
+          end do
+          do ix = 1, npts
+             ccb(ix) = nbp(1)*pb(ix, 1) + nbp(2)*pb(ix, 2) + nbp(3)*pb(ix, 3) + nbp(4)*pb(ix, 4) &
+                       + nbp(5)*pb(ix, 5) + nbp(6)*pb(ix, 6)
+             ccf(ix) = nfp(1)*pf(ix, 1) + nfp(2)*pf(ix, 2) + nfp(3)*pf(ix, 3) + nfp(4)*pf(ix, 4) &
+                       + nfp(5)*pf(ix, 5)
+          end do
+          call evalv(4_c_int, ccb, eb)
+          call evalv(4_c_int, ccf, ef)
+          call evalv(3_c_int, rh(:, 1), dph)
+          call evalv(1_c_int, rh(:, 1), dvh)
+          do ix = 1, npts
+             ecr = nbv(1)*vb(ix, 1) + nbv(2)*vb(ix, 2) + nbv(3)*vb(ix, 3) + nbv(4)*vb(ix, 4) &
+                   + nbv(5)*vb(ix, 5) + nbv(6)*vb(ix, 6)
+             ecbccb = ecr + eb(ix)
+             ecr = nfv(1)*vf(ix, 1) + nfv(2)*vf(ix, 2) + nfv(3)*vf(ix, 3) + nfv(4)*vf(ix, 4) &
+                   + nfv(5)*vf(ix, 5)
+             ecfcc = ecr + ef(ix)
+             ecr = nhv(1)*vh(ix, 1) + nhv(2)*vh(ix, 2) + nhv(3)*vh(ix, 3) + nhv(4)*vh(ix, 4) &
+                   + nhv(5)*vh(ix, 5) + nhv(6)*vh(ix, 6) + nhv(7)*vh(ix, 7)
+             echcp = ecr + ef(ix)   ! the fcc embedding term, as the reference
+             if (ph(ix, 1) > 1d-10) then
+                veff = vh(ix, 1) + ph(ix, 1)/ef(ix)
+                write(u, '(9e15.7)') rh(ix, 1), vh(ix, 1), ph(ix, 1), veff, dph(ix), dvh(ix), ecbccb, ecfcc, echcp
+             end if
+          end do
+       end do
+    end do
+    close(u)
   end subroutine potin
 end module potential_m
 

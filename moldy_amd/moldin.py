@@ -265,6 +265,7 @@ def run(workdir: str = ".", material: str = "fetransmetal") -> dict:
                                                         eng.tensor("b0"), eng.tensor("tp"), eng.tensor("tk")))
                 if p.nchkpt > 0 and counters["lastchkpt"] % p.nchkpt == 0:
                     _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir)
+                    write_efs(eng, workdir)                           # moldin.F90:815
                     out.write("Checkpoint at step:%6d\n\n\n\n" % istep)
                 thermotimer = thermotimer + p.deltat                  # moldin.F90:828-835
                 if thermotimer >= fs_ref:
@@ -331,6 +332,30 @@ def _checkpoint(eng, p, counters, sums, z, mass, aidx, ispec, workdir):
                      arr["x0"], arr["x1"], arr["x2"], arr["x3"],
                      [eng.tensor(t) for t in ("b0", "b1", "b2", "b3")], z, mass, aidx,
                      np.array(ispec, dtype=np.int32), sums.as_record(), sums.numsum)
+
+
+def write_efs(eng, workdir: str) -> None:
+    """efs.out, as write_energy_forces_stress (src/dynamics_m.F90:588-619)
+    writes it at each in-run checkpoint: the word energy and pe, the
+    Cartesian force B0 f of every atom (reference order), then sigma(i,j) =
+    -sum_k b0(j,k) tp(i,k) / vol in column order; whitespace-separated, as
+    list-directed output is read."""
+    b0, tp = eng.tensor("b0"), eng.tensor("tp")
+    sigma = -(tp @ b0.T) / eng.scalar("vol")
+    with open(os.path.join(workdir, "efs.out"), "w") as fh:
+        fh.write(" energy %25.16E\n" % eng.scalar("pe"))
+        for row in eng.forces_cart():
+            fh.write(" %25.16E %25.16E %25.16E\n" % tuple(row))
+        fh.write(" " + " ".join("%25.16E" % v for v in sigma.ravel(order="F")) + "\n")
+
+
+def read_efs(path: str):
+    """(energy, forces (n, 3), sigma (3, 3)) of an efs.out."""
+    toks = open(path).read().split()
+    assert toks[0] == "energy"
+    vals = np.array([float(t.replace("D", "E").replace("d", "e")) for t in toks[1:]])
+    n = (len(vals) - 10) // 3
+    return vals[0], vals[1:1 + 3 * n].reshape(n, 3), vals[1 + 3 * n:].reshape(3, 3, order="F")
 
 
 def _iron_in(workdir: str):

@@ -21,6 +21,8 @@
 !         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
 !                 (moldin.F90:175), the 3 draws per atom of read_system
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 6 = potin (potential.dat) for the Fe-Fe pair, one species, after
+!                 init_linkcell_m (moldin.F90:266-282)
 !         bit 5 = quench after the first force (moldin.F90:566-572: init_quench_m,
 !                 quench, cleanup_quench_m) instead of MD; pe and vol follow
 !                 the final record
@@ -55,7 +57,7 @@ program ref_driver
 
   character(len=1024) :: inpath, outpath
   integer :: n, ivol, ntcm, nsteps, uselk, nnbrs_in, flags
-  integer :: ngrid, nrho, i, k, istep, ierror, c0, c1, crate
+  integer :: ngrid, nrho, i, k, istep, ierror, c0, c1, crate, nspec_save, ispec_save
   real(kind_wp) :: dt_fs, rpad_in, press_gpa, boxmass, b0in(3,3)
   real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke, nose_in, thermotimer
   real(kind_wp) :: fs_ref = 0.098227
@@ -160,6 +162,21 @@ program ref_driver
   if (sp%nlcz < 3) sp%nlcz = 3
   call set_params(sp)
   call init_linkcell_m
+
+  if (iand(flags, 64) /= 0) then
+     ! potin's potential.dat for the Fe-Fe pair alone (one species)
+     sp = get_params()
+     nspec_save = sp%nspec
+     ispec_save = ispec(1)
+     sp%nspec = 1
+     call set_params(sp)
+     ispec(1) = 26
+     call potin
+     ispec(1) = ispec_save
+     sp%nspec = nspec_save
+     call set_params(sp)
+  end if
+
   call init_dynamics_m
   call pr_get_metric_tensor
   do i = 1, n

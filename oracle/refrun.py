@@ -40,10 +40,17 @@ def write_input(path, sysdef, ivol, ntcm, nsteps, uselookup, nnbrs, flags, rpad,
         fh.write(np.asarray(rhogrid, dtype="<f8").tobytes())
 
 
-def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3, press=0.0,
-        boxmass=50.0, flags=0, rgrid=(), rhogrid=(), driver=DRIVER, pot_file=POT_FILE,
-        threads=None, timeout=None, temprq=300.0, nose=0.0):
+def run(sysdef, **kw):
     """Runs the driver; returns (stdout, output bytes or None)."""
+    out, data, _ = run_keep(sysdef, (), **kw)
+    return out, data
+
+
+def run_keep(sysdef, keep, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3, press=0.0,
+             boxmass=50.0, flags=0, rgrid=(), rhogrid=(), driver=DRIVER, pot_file=POT_FILE,
+             threads=None, timeout=None, temprq=300.0, nose=0.0):
+    """Runs the driver; returns (stdout, output bytes or None, {name: bytes}
+    of the files named in keep that the run wrote in its directory)."""
     with tempfile.TemporaryDirectory() as td:
         os.symlink(pot_file, os.path.join(td, "pot_026_026.in"))
         inp, outp = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
@@ -60,7 +67,9 @@ def run(sysdef, *, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rpad=0.3,
             raise RuntimeError(f"reference driver failed ({res.returncode}): "
                                f"{res.stdout[-2000:]} {res.stderr[-2000:]}")
         data = None if flags & 1 else open(outp, "rb").read()
-        return res.stdout, data
+        files = {nm: open(os.path.join(td, nm), "rb").read() for nm in keep
+                 if os.path.exists(os.path.join(td, nm))}
+        return res.stdout, data, files
 
 
 def time_md_loop(sysdef, nsteps, threads, **kw) -> float:

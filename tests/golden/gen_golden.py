@@ -329,6 +329,20 @@ def shear_inputs_fixture():
     print(f"shear_inputs -> {path} ({os.path.getsize(path) // 1024} KiB)")
 
 
+def potin_fixture():
+    """potential.dat of the reference's potin (driver flag 64: the Fe-Fe pair,
+    one species) as a (lines, 9) array: r, V, phi, V + phi/F, phi', V' at the
+    hcp first-neighbour distance and the bcc, fcc and hcp cohesive energies."""
+    import refrun
+    sd = lattice.make_system(nc=3, a=2.8665, sigma=0.0, temp=0.0, seed=0)
+    _, _, files = refrun.run_keep(sd, ("potential.dat",), driver=DRIVER, pot_file=REF_POT, flags=1 | 64,
+                                  nsteps=0)
+    rows = [[float(t) for t in ln.split()] for ln in files["potential.dat"].decode().splitlines() if ln.strip()]
+    path = os.path.join(OUT, "potin_fe.npz")
+    np.savez_compressed(path, table=np.array(rows))
+    print(f"potin_fe: {len(rows)} lines -> {path}")
+
+
 def main(names=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -338,6 +352,8 @@ def main(names=None):
         bcc_iron_fixture()
     if names and "shear_inputs" in names:
         shear_inputs_fixture()
+    if names and "potin_fe" in names:
+        potin_fixture()
     for name, (skw, rkw) in CASES.items():
         if names and name not in names:
             continue

@@ -90,3 +90,27 @@ def test_fortran_dropin_quench_matches_reference():
     assert np.max(np.abs(dd)) < 2e-4
     np.testing.assert_allclose(np.diag(d["b0_f"]), np.diag(F["b0_f"]), rtol=1e-4)
     assert np.all(d["x1_f"] == 0.0)  # quench leaves the atoms at rest
+
+
+def _potin_table(buf: bytes) -> np.ndarray:
+    return np.array([[float(t) for t in ln.split()] for ln in buf.decode().splitlines() if ln.strip()])
+
+
+def test_fortran_dropin_potin_matches_reference():
+    """potential_m's potin through the drop-in (the driver's flag 64: the Fe-Fe
+    pair, one species) against the reference's potential.dat
+    (tests/golden/potin_fe.npz, written by the reference's potin): the same
+    lines, every number equal to the printed precision (e15.7, seven
+    significant digits; a last-digit flip is 1e-6 relative)."""
+    import refrun
+    from moldy_amd import lattice
+    if not os.path.exists(DROPIN):
+        pytest.fail("drop-in driver not built (moldy_amd.build.build_fortran)")
+    print("[gpu-test] fortran drop-in potin", flush=True)
+    sd = lattice.make_system(nc=3, a=2.8665, sigma=0.0, temp=0.0, seed=0)
+    _, _, files = refrun.run_keep(sd, ("potential.dat",), driver=DROPIN, pot_file=refrun.POT_FILE,
+                                  flags=1 | 64, nsteps=0)
+    got = _potin_table(files["potential.dat"])
+    want = np.load(os.path.join(ROOT, "tests", "golden", "potin_fe.npz"))["table"]
+    assert got.shape == want.shape
+    assert np.all(np.abs(got - want) <= 1.01e-6 * np.abs(want))

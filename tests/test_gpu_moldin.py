@@ -116,12 +116,12 @@ THERMAL_SYSTEM = """2
 """
 
 
-def _thermal_params(nsteps, iquen):
+def _thermal_params(nsteps, iquen, nchkpt=1000):
     return (f'title1 = "thermal"\ntitle2 = "fe_bcc_101010"\nfile_system = system.in\n'
             f"file_textout = output66.txt\nfile_checkpointwrite = chkp_out.dat\n"
             f"boxmass = 5.\ndeltat = 1.0d0\nrpad = 1.0d0\ntemprq = 10.d0\npress = 0.0\nivol = 0\n"
             f"iquen = {iquen}\nnose = 0\ndsp = 0.0d0\nnm = 2000\nnspec = 1\nnsteps = {nsteps}\n"
-            f"nbrupdate = 2000\nrestart = 0\nnprint = 100\nnchkpt = 1000\nnnbrs = 150\niverlet = 1\n"
+            f"nbrupdate = 2000\nrestart = 0\nnprint = 100\nnchkpt = {nchkpt}\nnnbrs = 150\niverlet = 1\n"
             f"uselookup = .false.\n")
 
 
@@ -130,7 +130,7 @@ def test_gpu_main_program_box_quench_matches_oracle(tmp_path, oracle):
     nsteps = 300
     d = str(tmp_path)
     with open(os.path.join(d, "params.in"), "w") as fh:
-        fh.write(_thermal_params(nsteps, 2))
+        fh.write(_thermal_params(nsteps, 2, nchkpt=nsteps))   # an in-run checkpoint (and efs.out) at the end
     with open(os.path.join(d, "system.in"), "w") as fh:
         fh.write(THERMAL_SYSTEM)
     print("[gpu-test] main program, iquen = 2 on the thermal inputs", flush=True)
@@ -164,6 +164,13 @@ def test_gpu_main_program_box_quench_matches_oracle(tmp_path, oracle):
     for k, nm in enumerate(("x0", "y0", "z0")):
         assert np.max(np.abs(ck["x0"][:, k] - S.array(nm))) < 1e-11, nm
     np.testing.assert_allclose(got["en_atom"], S.energy_calc(), rtol=0, atol=1e-9)
+    # efs.out of the in-run checkpoint (moldin.F90:815): energy, Cartesian forces, stress
+    e, f, sig = moldin.read_efs(os.path.join(d, "efs.out"))
+    assert e == got["pe"]
+    fr = S.forces_cart()
+    assert np.max(np.abs(f - fr)) <= 1e-9 * np.max(np.abs(fr))
+    sw = -(S.tensor("tp") @ bw.T) / S.scalar("vol")
+    assert np.max(np.abs(sig - sw)) <= 1e-8 * np.max(np.abs(sw))
     # without the quench the same run ends measurably elsewhere
     U = oracle_run(False)
     assert abs(U.scalar("pe") - S.scalar("pe")) > 1e-6 * abs(S.scalar("pe"))

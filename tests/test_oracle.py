@@ -623,3 +623,26 @@ def test_ranmar_known_sequence(oracle):
     g.rand1(20000)
     got = g.rand1(6) * 4096.0 * 4096.0
     np.testing.assert_array_equal(got, [6533892.0, 14220222.0, 7275067.0, 6172232.0, 8354498.0, 10633180.0])
+
+
+def test_potin_fixture_against_oracle_functions(oracle):
+    """The reference's potential.dat (tests/golden/potin_fe.npz): its r column
+    is the hcp first-neighbour distance of potin's grid, and its V, phi, phi'
+    and V' columns are the oracle's FP64 functions there, to the printed
+    precision (seven significant digits)."""
+    import os
+    from conftest import ROOT
+    t = np.load(os.path.join(ROOT, "tests", "golden", "potin_fe.npz"))["table"]
+    pot = oracle.Potential()
+    r = t[:, 0]
+    for col, fn in ((1, "vee"), (2, "phi"), (4, "dphi"), (5, "dvee")):
+        want = pot.f(fn, r)
+        # r itself is printed to 7 digits: allow the slope times its rounding
+        slope = np.abs(pot.f("d" + fn, r)) if fn in ("vee", "phi") else 50.0 * np.abs(want) + 10.0
+        tol = 1.01e-6 * np.abs(t[:, col]) + slope * 1.01e-6 * r + 1e-12
+        assert np.all(np.abs(t[:, col] - want) <= tol), fn
+    # bcc first-shell distance of the same point: x2a / sqrt 2 = x1b (2 / sqrt 3) 2^(1/3) / sqrt 2
+    ix = np.arange(1, 1001)
+    x1 = 2.0 + ix * 4.5 / 1000
+    hcp = x1 * (2.0 / np.sqrt(3.0)) * 2.0 ** float(np.float32(0.3333333333)) / np.sqrt(2.0)
+    assert np.allclose(r, hcp[:len(r)], rtol=1.01e-6)
