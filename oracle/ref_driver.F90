@@ -21,6 +21,8 @@
 !         bit 1 = velocities as moldin makes them for a new run: rstart(97,33,21,82)
 !                 (moldin.F90:175), the 3 draws per atom of read_system
 !                 (io_m.F90:233-235), then setvel and scalev (moldin.F90:307-308)
+!         bit 7 = extra record after the final one: embed(pexx) on RHO scaled by
+!                 1.01 with RHO(1) = 0, then pexx, AFRHO, DAFRHO
 !         bit 6 = potin (potential.dat) for the Fe-Fe pair, one species, after
 !                 init_linkcell_m (moldin.F90:266-282)
 !         bit 5 = quench after the first force (moldin.F90:566-572: init_quench_m,
@@ -60,7 +62,7 @@ program ref_driver
   integer :: ngrid, nrho, i, k, istep, ierror, c0, c1, crate, nspec_save, ispec_save
   real(kind_wp) :: dt_fs, rpad_in, press_gpa, boxmass, b0in(3,3)
   real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke, nose_in, thermotimer
-  real(kind_wp) :: fs_ref = 0.098227
+  real(kind_wp) :: fs_ref = 0.098227, pexx
   real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:), xtrace(:,:)
   type(simparameters) :: sp
   logical :: timing
@@ -297,6 +299,14 @@ program ref_driver
         write(20) xtrace
         write(20) en_atom
         write(20) x1, y1, z1
+     end if
+     if (iand(flags, 128) /= 0) then
+        ! embed on a caller-changed RHO: scaled by 1.01, atom 1 set to 0 (below the
+        ! 1e-40 cut: its AFRHO and DAFRHO keep their values)
+        rho = 1.01d0*rho
+        rho(1) = 0.0d0
+        call embed(pexx)
+        write(20) pexx, afrho, dafrho
      end if
      close(20)
   end if

@@ -124,6 +124,11 @@ def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rp
         for k, nm in enumerate(["x3", "y3", "z3"]):
             d[nm + "_f"] = x3[k]
         d["b3_f"] = R.f8(9).reshape(3, 3, order="F")
+    if flags & 4:
+        R.f8(13 * max(nsteps, 1) + 4 * n)   # extra record (per-step trace, en_atom, x1)
+    if flags & 128:
+        d["embed_pe"] = float(R.f8(1)[0])
+        d["embed_afrho"], d["embed_dafrho"] = R.f8(n), R.f8(n)
     return d
 
 
@@ -165,6 +170,9 @@ CASES = {
                       dict(ivol=1, ntcm=10, nsteps=15, uselookup=0, flags=16, nose=50.0, temprq=600.0)),
     # quench (flag 32, the reference's VA14CD conjugate gradients): a 5^3 bcc
     # cell without its first atom, displaced by 0.1 A, box free at 0 GPa
+    # embed on a caller-changed RHO after 3 steps (driver flag 128)
+    "embed_rho_250": (dict(nc=5, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=82),
+                      dict(ivol=1, ntcm=10, nsteps=3, uselookup=0, flags=128)),
     "quench_vac_249": (dict(nc=5, a=2.8665, sigma=0.1, temp=0.0, deltat_fs=1.0, seed=81, vacancy=0),
                        dict(ivol=0, ntcm=10, nsteps=0, uselookup=0, flags=32, press=0.0, boxmass=50.0)),
     # iron_carbon hot dense state: pairs inside the exp-cubic join (r < 2.05 A)
@@ -378,6 +386,10 @@ def main(names=None):
             fx[k] = d[k]
         if "pe_q" in d:
             fx["pe_q"], fx["vol_q"] = np.float64(d["pe_q"]), np.float64(d["vol_q"])
+        if "embed_pe" in d:
+            fx["embed_pe"] = np.float64(d["embed_pe"])
+            fx["embed_afrho"], fx["embed_dafrho"] = d["embed_afrho"], d["embed_dafrho"]
+            fx["rho_f"], fx["dafrho_f"] = d["rho_f"], d["dafrho_f"]
         for k in ("nlc", "rnear", "deltat", "bdel2", "press", "vol0", "pe0", "trace", "numn0"):
             fx[k] = np.asarray(d[k])
         fx["numn0"] = d["numn0"].astype(np.int16)

@@ -28,7 +28,7 @@ def rel(a, b):
 
 @pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
                                   "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024",
-                                  "gear_npt_1024", "gear_z_1024", "nose_nvt_1024"])
+                                  "gear_npt_1024", "gear_z_1024", "nose_nvt_1024", "embed_rho_250"])
 def test_fortran_dropin_matches_reference(oracle, case):
     import gen_golden
     import refrun
@@ -55,6 +55,10 @@ def test_fortran_dropin_matches_reference(oracle, case):
     for k in ("x1_f", "fx_f"):
         assert rel(d[k], F[k]) < 1e-9, k
     assert rel(d["b0_f"], F["b0_f"]) < 1e-14
+    if "embed_pe" in F:   # embed on the caller's (changed) RHO
+        assert abs(d["embed_pe"] - float(F["embed_pe"])) < 1e-12 * abs(float(F["embed_pe"]))
+        for k in ("embed_afrho", "embed_dafrho"):
+            assert rel(d[k], F[k]) < 1e-12, k
     if "rgrid" in F:
         exact_cmp = bool(int(F["run_uselookup"]))
         for k in ("vee", "dvee", "phi", "dphi"):

@@ -646,3 +646,17 @@ def test_potin_fixture_against_oracle_functions(oracle):
     x1 = 2.0 + ix * 4.5 / 1000
     hcp = x1 * (2.0 / np.sqrt(3.0)) * 2.0 ** float(np.float32(0.3333333333)) / np.sqrt(2.0)
     assert np.allclose(r, hcp[:len(r)], rtol=1.01e-6)
+
+
+def test_embed_fixture_against_oracle_functions(oracle):
+    """The reference's embed on a changed RHO (tests/golden/embed_rho_250.npz:
+    RHO of the last force scaled by 1.01, atom 1 set to 0): F and F' of
+    1.01 rho for every other atom, atom 1 keeping the F' of the last force, and
+    pe the sum of the new F (1e-12 relative)."""
+    F = load_golden("embed_rho_250")
+    pot = oracle.Potential()
+    rho = 1.01 * F["rho_f"][1:]
+    np.testing.assert_allclose(F["embed_afrho"][1:], pot.f("emb", rho), rtol=1e-12)
+    np.testing.assert_allclose(F["embed_dafrho"][1:], pot.f("demb", rho), rtol=1e-12)
+    assert F["embed_dafrho"][0] == F["dafrho_f"][0]
+    assert abs(F["embed_pe"] - F["embed_afrho"][1:].sum()) < 1e-12 * abs(F["embed_pe"])
