@@ -379,3 +379,30 @@ def test_gpu_fp32_interval_table(oracle):
     assert not eng.eval_potential("dphi32", np.array([1.7, 1.8, rstar - 1e-4])).any()
     assert eng.eval_potential("dphi32", np.array([rstar + 1e-3]))[0] != 0.0
     eng.close()
+
+
+def test_gpu_fp32_close_knots_fall_back_to_knot_sums(oracle):
+    """A 6+2 knot set whose first two pair knots lie 0.003 A apart, closer than
+    the finest bin of the FP32 interval table: the table is refused (its
+    functions are not offered), the FP32 force pass sums the knots directly,
+    and its forces stay within 1e-5 eV/A of the oracle's (FP64).  Its energy
+    error is larger than the table's (6.3e-6 eV/atom measured: the FP32 knot
+    sums cancel terms up to ten times V), bound here at 1e-5 eV/atom."""
+    from moldy_amd import potential
+    c = potential.coeffs_from_theta(potential.theta_vector())
+    c["rv"][1] = c["rv"][0] - 0.003
+    sd = lattice.make_system(nc=8, a=2.8665, sigma=0.08, temp=300.0, deltat_fs=1.0, seed=5)
+    say("fp32 form, close knots")
+    eng = engine.Engine(sd, ivol=1, precision=engine.PREC_FP32, coeffs=c)
+    with pytest.raises(engine.MdkError):
+        eng.eval_potential("vee32", np.array([2.5]))
+    eng.linkup()
+    eng.md_init()
+    S = oracle.OracleSystem(sd, oracle.Potential(c), ivol=1)
+    S.call("linkup")
+    S.call("md_init")
+    err = float(np.max(np.abs(eng.forces_cart() - S.forces_cart())))
+    say(f"close knots: fp32 max force error {err:.3e} eV/A")
+    assert err <= 1e-5
+    assert abs(eng.scalar("pe") - S.scalar("pe")) / S.n < 1e-5
+    eng.close()
