@@ -62,6 +62,7 @@ struct BatchArgs {
   const double *s;       // 3 per atom: fractional, wrapped into [0,1), sorted by sub-cell
   const int *perm;       // sorted atom -> caller's atom index
   double4 *pos4;         // Cartesian position, F'
+  float4 *p4f;           // FP32 form: position from its sub-cell's centre (FP32), F'
   int *nbr;              // per cell region: nbr[a0*maxnb + k*nc + il]
   int *numn;
   double *rho;
@@ -71,6 +72,7 @@ struct BatchArgs {
   int *counter;
   int *flags;            // [0] max neighbour count seen when over capacity
   double rc2;
+  float rc2f;            // FP32 form's list radius squared, 1e-4 A past the outermost knot
   // adjoint
   const double *e_in;    // per cell: dL/dE_c (vector-Jacobian product) or target E_c (loss)
   const double *f_in;    // 3 per caller atom: dL/dF_i (product) or target F_i (loss)
@@ -147,42 +149,89 @@ __device__ __forceinline__ bool next_cell(const BatchArgs &A, int *sh_cell, Cell
   return true;
 }
 
-// image shift table of the cell (codes 27..31 never occur) and Cartesian
-// positions; the caller places a barrier after it
-template <int BLOCK>
-__device__ void phase_positions(const BatchArgs &A, const CellView &V, double (*S)[3]) {
+// Per-workgroup shift tables.  FP64 form: S[code] = B t, the image shift of
+// the pair (positions are absolute).  FP32 form: positions are taken from the
+// centre of the atom's sub-cell (of the cell's centre along a dimension with
+// one sub-cell), so |R| stays below a sub-cell width and FP32 keeps it to
+// 1e-6 A; the pair's shift is B delta with delta = q / n per dimension (q the
+// sub-cell offset of j from i, negated) or q (the rounding image where there
+// is one sub-cell), held as FP32 hi and lo parts: r = ((R_i - R_j) + Sf) + Sl.
+struct Shifts {
+  double S[32][3];
+  float4 Sf[32], Sl[32];
+};
+
+// shift tables and packed positions of the cell; the caller places a barrier
+// after it (codes 27..31 never occur)
+template <typename T, int BLOCK>
+__device__ void phase_positions(const BatchArgs &A, const CellView &V, Shifts &SH) {
   const int t = threadIdx.x;
   for (int k = t; k < 32; k += BLOCK) {
     if (k >= 27) {
-      S[k][0] = 1.0e6; S[k][1] = 0.0; S[k][2] = 0.0;
-    } else {
+      SH.S[k][0] = 1.0e6; SH.S[k][1] = 0.0; SH.S[k][2] = 0.0;
+      SH.Sf[k] = make_float4(1.0e6f, 0.f, 0.f, 0.f);
+      SH.Sl[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else if constexpr (sizeof(T) == 8) {
       const double sx = (double)(k % 3 - 1), sy = (double)((k / 3) % 3 - 1), sz = (double)(k / 9 - 1);
-      S[k][0] = V.b[0] * sx + V.b[3] * sy + V.b[6] * sz;
-      S[k][1] = V.b[1] * sx + V.b[4] * sy + V.b[7] * sz;
-      S[k][2] = V.b[2] * sx + V.b[5] * sy + V.b[8] * sz;
+      SH.S[k][0] = V.b[0] * sx + V.b[3] * sy + V.b[6] * sz;
+      SH.S[k][1] = V.b[1] * sx + V.b[4] * sy + V.b[7] * sz;
+      SH.S[k][2] = V.b[2] * sx + V.b[5] * sy + V.b[8] * sz;
+    } else {
+      const int qx = k % 3 - 1, qy = (k / 3) % 3 - 1, qz = k / 9 - 1;
+      const double sx = V.nx >= 3 ? (double)qx / V.nx : (double)qx;
+      const double sy = V.ny >= 3 ? (double)qy / V.ny : (double)qy;
+      const double sz = V.nz >= 3 ? (double)qz / V.nz : (double)qz;
+      const double dx = V.b[0] * sx + V.b[3] * sy + V.b[6] * sz, dy = V.b[1] * sx + V.b[4] * sy + V.b[7] * sz,
+                   dz = V.b[2] * sx + V.b[5] * sy + V.b[8] * sz;
+      const float hx = (float)dx, hy = (float)dy, hz = (float)dz;
+      SH.Sf[k] = make_float4(hx, hy, hz, 0.f);
+      SH.Sl[k] = make_float4((float)(dx - (double)hx), (float)(dy - (double)hy), (float)(dz - (double)hz), 0.f);
     }
   }
   for (int il = t; il < V.nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
-    const double sx = A.s[3 * a], sy = A.s[3 * a + 1], sz = A.s[3 * a + 2];
-    A.pos4[a] = make_double4(V.b[0] * sx + V.b[3] * sy + V.b[6] * sz,
-                             V.b[1] * sx + V.b[4] * sy + V.b[7] * sz,
-                             V.b[2] * sx + V.b[5] * sy + V.b[8] * sz, 0.0);
+    double sx = A.s[3 * a], sy = A.s[3 * a + 1], sz = A.s[3 * a + 2];
+    if constexpr (sizeof(T) == 8) {
+      A.pos4[a] = make_double4(V.b[0] * sx + V.b[3] * sy + V.b[6] * sz,
+                               V.b[1] * sx + V.b[4] * sy + V.b[7] * sz,
+                               V.b[2] * sx + V.b[5] * sy + V.b[8] * sz, 0.0);
+    } else {
+      const int sc = A.subc[a], nx = V.nx, ny = V.ny, nz = V.nz;
+      const int cx = sc % nx, cy = (sc / nx) % ny, cz = sc / (nx * ny);
+      sx -= nx >= 3 ? (cx + 0.5) / nx : 0.5;
+      sy -= ny >= 3 ? (cy + 0.5) / ny : 0.5;
+      sz -= nz >= 3 ? (cz + 0.5) / nz : 0.5;
+      A.p4f[a] = make_float4((float)(V.b[0] * sx + V.b[3] * sy + V.b[6] * sz),
+                             (float)(V.b[1] * sx + V.b[4] * sy + V.b[7] * sz),
+                             (float)(V.b[2] * sx + V.b[5] * sy + V.b[8] * sz), 0.f);
+    }
   }
+}
+
+__device__ __forceinline__ float3 sep32(const float4 &pi, const float4 &pj, const Shifts &SH, int code) {
+  const float4 d = SH.Sf[code], l = SH.Sl[code];
+  return make_float3(((pi.x - pj.x) + d.x) + l.x, ((pi.y - pj.y) + d.y) + l.y, ((pi.z - pj.z) + d.z) + l.z);
 }
 
 // full neighbour list within the cell and its periodic images: the 3^3
 // sub-cell stencil in dimensions with >= 3 sub-cells (image from the wrap),
-// rounding of the fractional separation in dimensions with one sub-cell
-template <int BLOCK>
-__device__ void phase_list(const BatchArgs &A, const CellView &V, const double (*S)[3]) {
+// rounding of the fractional separation in dimensions with one sub-cell.
+// FP64 form: the code is the image shift, the test r < rc in FP64.  FP32
+// form: the code is the packed-shift offset q, the test on the FP32
+// separation against (rc + 1e-4)^2 (pairs past the outermost knot add exact
+// zeros).
+template <typename T, int BLOCK>
+__device__ void phase_list(const BatchArgs &A, const CellView &V, const Shifts &SH) {
   const int nx = V.nx, ny = V.ny, nz = V.nz, nc = V.nc;
   const int rx_ = nx >= 3 ? 1 : 0, ry_ = ny >= 3 ? 1 : 0, rz_ = nz >= 3 ? 1 : 0;
   for (int il = threadIdx.x; il < nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
     const int sc = A.subc[a];
     const int cx = sc % nx, cy = (sc / nx) % ny, cz = sc / (nx * ny);
-    const double4 pi = A.pos4[a];
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (sizeof(T) == 8) pi = A.pos4[a];
+    else pf = A.p4f[a];
     const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
     int cnt = 0;
     for (int dz = -rz_; dz <= rz_; dz++) {
@@ -198,15 +247,26 @@ __device__ void phase_list(const BatchArgs &A, const CellView &V, const double (
           for (int jl = V.sst[cj]; jl < V.sst[cj + 1]; jl++) {
             if (jl == il) continue;
             const size_t aj = (size_t)V.a0 + jl;
-            int tx = wx, ty = wy, tz = wz;
-            if (!rx_) tx = -(int)round(sxi - A.s[3 * aj]);
-            if (!ry_) ty = -(int)round(syi - A.s[3 * aj + 1]);
-            if (!rz_) tz = -(int)round(szi - A.s[3 * aj + 2]);
-            const int code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
-            const double4 pj = A.pos4[aj];
-            const double ex = pi.x - pj.x + S[code][0], ey = pi.y - pj.y + S[code][1],
-                         ez = pi.z - pj.z + S[code][2];
-            if (ex * ex + ey * ey + ez * ez < A.rc2) {
+            int tx = rx_ ? wx : -(int)round(sxi - A.s[3 * aj]);
+            int ty = ry_ ? wy : -(int)round(syi - A.s[3 * aj + 1]);
+            int tz = rz_ ? wz : -(int)round(szi - A.s[3 * aj + 2]);
+            bool in;
+            int code;
+            if constexpr (sizeof(T) == 8) {
+              code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+              const double4 pj = A.pos4[aj];
+              const double ex = pi.x - pj.x + SH.S[code][0], ey = pi.y - pj.y + SH.S[code][1],
+                           ez = pi.z - pj.z + SH.S[code][2];
+              in = ex * ex + ey * ey + ez * ez < A.rc2;
+            } else {
+              if (rx_) tx = -dx;
+              if (ry_) ty = -dy;
+              if (rz_) tz = -dz;
+              code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+              const float3 e = sep32(pf, A.p4f[aj], SH, code);
+              in = e.x * e.x + e.y * e.y + e.z * e.z < A.rc2f;
+            }
+            if (in) {
               if (cnt < A.maxnb) V.nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
               cnt++;
             }
@@ -219,36 +279,49 @@ __device__ void phase_list(const BatchArgs &A, const CellView &V, const double (
   }
 }
 
-// separation of the k-th neighbour of local atom il (arithmetic type T)
+// separation of the k-th neighbour of local atom il, and the partner's F'
+// (FP64 form: from pos4; FP32 form: from the FP32 records)
 template <typename T>
-__device__ __forceinline__ double4 nb_sep(const BatchArgs &A, const CellView &V, const double (*S)[3],
-                                          const double4 &pi, int il, int k, T &dx, T &dy, T &dz,
-                                          int &jl) {
+__device__ __forceinline__ T nb_sep(const BatchArgs &A, const CellView &V, const Shifts &SH, const double4 &pi,
+                                    const float4 &pf, int il, int k, T &dx, T &dy, T &dz, int &jl) {
   const unsigned e = (unsigned)V.nbr[(size_t)k * V.nc + il];
   jl = (int)(e & NB_IDX_MASK);
-  const double4 pj = A.pos4[V.a0 + jl];
   const int code = (int)(e >> NB_CODE_SHIFT);
-  dx = (T)(pi.x - pj.x + S[code][0]);
-  dy = (T)(pi.y - pj.y + S[code][1]);
-  dz = (T)(pi.z - pj.z + S[code][2]);
-  return pj;
+  if constexpr (sizeof(T) == 8) {
+    const double4 pj = A.pos4[V.a0 + jl];
+    dx = (T)(pi.x - pj.x + SH.S[code][0]);
+    dy = (T)(pi.y - pj.y + SH.S[code][1]);
+    dz = (T)(pi.z - pj.z + SH.S[code][2]);
+    return (T)pj.w;
+  } else {
+    const float4 pj = A.p4f[V.a0 + jl];
+    const float3 r = sep32(pf, pj, SH, code);
+    dx = r.x;
+    dy = r.y;
+    dz = r.z;
+    return pj.w;
+  }
 }
 
 // densities and embedding (src/dynamics_m.F90:497-579, src/potential_m.F90:148-169):
-// rho (FP64 sum), F' into pos4.w; returns this thread's embedding energy
+// rho (FP64 sum), F' into pos4.w (and the FP32 record's w); returns this
+// thread's embedding energy
 template <typename T, int BLOCK>
-__device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellView &V,
-                                const double (*S)[3], bool store_rho) {
+__device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellView &V, const Shifts &SH,
+                                bool store_rho) {
   double e_emb = 0.0;
   for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
-    const double4 pi = A.pos4[a];
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (sizeof(T) == 8) pi = A.pos4[a];
+    else pf = A.p4f[a];
     const int nn = A.numn[a];
     double acc = 0.0;
     for (int k = 0; k < nn; k++) {
       T dx, dy, dz, r, ri;
       int jl;
-      nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
+      nb_sep<T>(A, V, SH, pi, pf, il, k, dx, dy, dz, jl);
       r_inv(dx * dx + dy * dy + dz * dz, r, ri);
       acc += (double)mdk_phi<T>(P, r);
     }
@@ -259,39 +332,68 @@ __device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellV
       e_emb += f;
     }
     A.pos4[a].w = df;
+    if constexpr (sizeof(T) == 4) A.p4f[a].w = (float)df;
   }
   return e_emb;
 }
 
 // forces and virial (src/dynamics_m.F90:235-346).  With LOSS, also the force
 // seed dL/dF_i = 2 f_weight (F_i - F_t,i) into gvec and this thread's share of
-// sum |F_i - F_t,i|^2 in *fsq.
+// sum |F_i - F_t,i|^2 in *fsq.  FP32 form: each atom's force, pair energy
+// and virial terms are summed in FP32 over its partners, then added to the
+// thread's FP64 sums; FP64 form: every term goes to the FP64 sums.
 template <typename T, int BLOCK, bool LOSS>
-__device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView &V,
-                            const double (*S)[3], double &e_pair, double *w, double *fsq) {
+__device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView &V, const Shifts &SH,
+                            double &e_pair, double *w, double *fsq) {
   for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
-    const double4 pi = A.pos4[a];
-    const T dfi = (T)pi.w;
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (sizeof(T) == 8) pi = A.pos4[a];
+    else pf = A.p4f[a];
+    const T dfi = sizeof(T) == 8 ? (T)pi.w : (T)pf.w;
     const int nn = A.numn[a];
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int k = 0; k < nn; k++) {
-      T dx, dy, dz, r, rr;
-      int jl;
-      const double4 pj = nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
-      r_inv(dx * dx + dy * dy + dz * dz, r, rr);
-      T vv, dv;
-      mdk_vee<T>(P, r, vv, dv);
-      const T dph = mdk_dphi<T>(P, r);
-      const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * (T)pj.w) * rr);
-      const double fpd = (double)fp, ddx = (double)dx, ddy = (double)dy, ddz = (double)dz;
-      e_pair += 0.5 * (double)vv;
-      fx += ddx * fpd;
-      fy += ddy * fpd;
-      fz += ddz * fpd;
-      const double h = 0.5 * fpd;
-      w[0] += ddx * ddx * h; w[1] += ddy * ddy * h; w[2] += ddz * ddz * h;
-      w[3] += ddx * ddy * h; w[4] += ddx * ddz * h; w[5] += ddy * ddz * h;
+    if constexpr (sizeof(T) == 8) {
+      for (int k = 0; k < nn; k++) {
+        T dx, dy, dz, r, rr;
+        int jl;
+        const T dfj = nb_sep<T>(A, V, SH, pi, pf, il, k, dx, dy, dz, jl);
+        r_inv(dx * dx + dy * dy + dz * dz, r, rr);
+        T vv, dv;
+        mdk_vee<T>(P, r, vv, dv);
+        const T dph = mdk_dphi<T>(P, r);
+        const T fp = -dv * rr + (-T(1) * (dph * dfi + dph * dfj) * rr);
+        const double fpd = (double)fp, ddx = (double)dx, ddy = (double)dy, ddz = (double)dz;
+        e_pair += 0.5 * (double)vv;
+        fx += ddx * fpd;
+        fy += ddy * fpd;
+        fz += ddz * fpd;
+        const double h = 0.5 * fpd;
+        w[0] += ddx * ddx * h; w[1] += ddy * ddy * h; w[2] += ddz * ddz * h;
+        w[3] += ddx * ddy * h; w[4] += ddx * ddz * h; w[5] += ddy * ddz * h;
+      }
+    } else {
+      float sx = 0.f, sy = 0.f, sz = 0.f, se = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f;
+      for (int k = 0; k < nn; k++) {
+        float dx, dy, dz, r, rr;
+        int jl;
+        const float dfj = nb_sep<float>(A, V, SH, pi, pf, il, k, dx, dy, dz, jl);
+        r_inv(dx * dx + dy * dy + dz * dz, r, rr);
+        float vv, dv;
+        mdk_vee<float>(P, r, vv, dv);
+        const float dph = mdk_dphi<float>(P, r);
+        const float fp = -dv * rr + (-1.f * (dph * dfi + dph * dfj) * rr);
+        const float gx = dx * fp, gy = dy * fp, gz = dz * fp;
+        se += vv;
+        sx += gx; sy += gy; sz += gz;
+        s0 += dx * gx; s1 += dy * gy; s2 += dz * gz;
+        s3 += dx * gy; s4 += dx * gz; s5 += dy * gz;
+      }
+      e_pair += 0.5 * (double)se;
+      fx = sx; fy = sy; fz = sz;
+      w[0] += 0.5 * (double)s0; w[1] += 0.5 * (double)s1; w[2] += 0.5 * (double)s2;
+      w[3] += 0.5 * (double)s3; w[4] += 0.5 * (double)s4; w[5] += 0.5 * (double)s5;
     }
     const size_t u = (size_t)A.perm[a];
     A.out_f[3 * u] = fx;
@@ -316,16 +418,19 @@ __device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView 
 //   lambda_i = F''(rho_i) Q_i,  Q_i = sum_j phi'(r_ij) u_ij,  F'' = -2 F'^3,
 // over the full list (every pair from both ends).  The lambda term is
 // accumulated as lambda_i d(rho_i)/dtheta after the neighbour loop, so one
-// pass over the list suffices.  F' comes from pos4.w, recomputed by the
+// pass over the list suffices.  F' comes from pos4.w (FP32 form: the
+// partners' from the FP32 records), recomputed by the
 // caller's density phase; G from gvec.
 template <typename T, int BLOCK>
 __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev &D,
-                              const CellView &V, const double (*S)[3], double ec, double *ga,
+                              const CellView &V, const Shifts &SH, double ec, double *ga,
                               double *gr, double *gA, double *gR) {
   PotView<T> Q(P);
   for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
-    const double4 pi = A.pos4[a];
+    const double4 pi = A.pos4[a];   // its w: F'_i in FP64 (both forms)
+    float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (sizeof(T) == 4) pf = A.p4f[a];
     const double gix = A.gvec[3 * a], giy = A.gvec[3 * a + 1], giz = A.gvec[3 * a + 2];
     const T fpi = (T)pi.w;
     const int nn = A.numn[a];
@@ -335,7 +440,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
     for (int k = 0; k < nn; k++) {
       T dx, dy, dz, r, ri;
       int jl;
-      const double4 pj = nb_sep<T>(A, V, S, pi, il, k, dx, dy, dz, jl);
+      const T fpj = nb_sep<T>(A, V, SH, pi, pf, il, k, dx, dy, dz, jl);
       r_inv(dx * dx + dy * dy + dz * dz, r, ri);
       const size_t aj = (size_t)V.a0 + jl;
       const T u = ((T)(gix - A.gvec[3 * aj]) * dx + (T)(giy - A.gvec[3 * aj + 1]) * dy +
@@ -343,7 +448,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
       const T dph = mdk_dphi<T>(P, r);
       q += (double)(dph * u);
       const T cv = T(0.5) * (T)ec, cvp = T(-0.5) * u;
-      const T cp = (T)ec * fpi, cpp = T(-0.5) * (fpi + (T)pj.w) * u;
+      const T cp = (T)ec * fpi, cpp = T(-0.5) * (fpi + fpj) * u;
       // pair parameters
       if (r > Q.bc2()) {
 #pragma unroll
@@ -393,7 +498,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
 #pragma unroll
         for (int m = 0; m < MDK_MAXP; m++) {
           if (m < P.np) {
-            const T d = Q.rp(m) - r;
+            const T d = Q.dp(m, r);
             if (d > T(0)) {
               const T d2 = d * d, pa = d2 * d, pr = T(3) * Q.ap(m) * d2;
               gA[m] += (double)(cp * pa - T(3) * cpp * d2);
@@ -443,14 +548,14 @@ __device__ double store_cell(const BatchArgs &A, const CellView &V, double e, co
 // Forward: energies, forces and virials of every cell.
 template <typename T, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_batch_eval(BatchArgs A, PotDev P) {
-  __shared__ double S[32][3];
+  __shared__ Shifts S;
   __shared__ double red[BLOCK / 64][7];
   __shared__ int sh_cell;
   CellView V;
   while (next_cell(A, &sh_cell, V)) {
-    phase_positions<BLOCK>(A, V, S);
+    phase_positions<T, BLOCK>(A, V, S);
     __syncthreads();
-    phase_list<BLOCK>(A, V, S);
+    phase_list<T, BLOCK>(A, V, S);
     __syncthreads();
     const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
     __syncthreads();
@@ -498,13 +603,13 @@ __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &
 // list of the last forward launch.
 template <typename T, int BLOCK, bool VJP>
 __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
-  __shared__ double S[32][3];
+  __shared__ Shifts S;
   __shared__ double red[BLOCK / 64][7];
   __shared__ double red1[BLOCK / 64];
   __shared__ int sh_cell;
   CellView V;
   while (next_cell(A, &sh_cell, V)) {
-    phase_positions<BLOCK>(A, V, S);
+    phase_positions<T, BLOCK>(A, V, S);
     double ec;
     if (VJP) {
       for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
@@ -517,7 +622,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, Gra
       __syncthreads();
     } else {
       __syncthreads();
-      phase_list<BLOCK>(A, V, S);
+      phase_list<T, BLOCK>(A, V, S);
       __syncthreads();
       const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
       __syncthreads();
@@ -606,6 +711,7 @@ struct mdk_batch {
   int *order, *cell_off, *nsub, *sub_off, *sub_start, *subc, *perm, *numn, *nbr, *counter, *flags;
   double *box, *s, *rho, *out_e, *out_w, *out_f;
   double4 *pos4;
+  float4 *p4f;
   // adjoint buffers: targets, product seeds, force seeds (sorted order),
   // per-cell gradient and loss partials, reduced results
   double *e_tgt, *f_tgt, *e_seed, *f_seed, *gvec, *out_g, *out_loss, *res;
@@ -649,9 +755,10 @@ BatchArgs args_of(mdk_batch *b) {
   BatchArgs A;
   A.ncell = b->ncell; A.maxnb = b->maxnb; A.order = b->order; A.cell_off = b->cell_off;
   A.box = b->box; A.nsub = b->nsub; A.sub_off = b->sub_off; A.sub_start = b->sub_start;
-  A.subc = b->subc; A.s = b->s; A.perm = b->perm; A.pos4 = b->pos4; A.nbr = b->nbr;
+  A.subc = b->subc; A.s = b->s; A.perm = b->perm; A.pos4 = b->pos4; A.p4f = b->p4f; A.nbr = b->nbr;
   A.numn = b->numn; A.rho = b->rho; A.out_e = b->out_e; A.out_w = b->out_w; A.out_f = b->out_f;
   A.counter = b->counter; A.flags = b->flags; A.rc2 = b->rc * b->rc;
+  A.rc2f = (float)((b->rc + 1e-4) * (b->rc + 1e-4));
   A.ngrad = 2 * b->knots.nv + 2 * b->knots.np;
   A.e_in = b->e_tgt; A.f_in = b->f_tgt; A.gvec = b->gvec; A.out_g = b->out_g;
   A.out_loss = b->out_loss; A.f_weight = 0.1 / (double)b->natom;
@@ -746,7 +853,7 @@ extern "C" int mdk_batch_destroy(void *h) {
   drop_graph(b);
   void *p[] = {b->order, b->cell_off, b->nsub, b->sub_off, b->sub_start, b->subc, b->perm,
                b->numn, b->nbr, b->counter, b->flags, b->box, b->s, b->rho, b->out_e, b->out_w,
-               b->out_f, b->pos4, b->e_tgt, b->f_tgt, b->e_seed, b->f_seed, b->gvec, b->out_g,
+               b->out_f, b->pos4, b->p4f, b->e_tgt, b->f_tgt, b->e_seed, b->f_seed, b->gvec, b->out_g,
                b->out_loss, b->res};
   for (void *q : p)
     if (q) hipFree(q);
@@ -869,6 +976,7 @@ extern "C" int mdk_batch_create(void **handle, int ncell, const int *natoms, con
   BALLOC(b->perm, sizeof(int) * natom);
   BALLOC(b->s, sizeof(double) * 3 * (size_t)natom);
   BALLOC(b->pos4, sizeof(double4) * natom);
+  BALLOC(b->p4f, sizeof(float4) * natom);
   BALLOC(b->nbr, sizeof(int) * (size_t)natom * b->maxnb);
   BALLOC(b->numn, sizeof(int) * natom);
   BALLOC(b->rho, sizeof(double) * natom);

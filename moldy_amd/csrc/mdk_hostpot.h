@@ -134,6 +134,7 @@ static double fill_potdev(PotDev &P, const Knots &k) {
   }
   for (int i = 0; i < k.np; i++) {
     P.ap[i] = k.ap[i]; P.rp[i] = k.rp[i]; P.apf[i] = (float)k.ap[i]; P.rpf[i] = (float)k.rp[i];
+    P.rplf[i] = (float)(k.rp[i] - (double)P.rpf[i]);
     rint = std::max(rint, k.rp[i]);
   }
   P.bc1 = k.bc1; P.bc2 = k.bc2;
@@ -183,9 +184,11 @@ inline bool build_fe_table(const Knots &k, float *tab, float &binh, float &binc)
   float *iv = tab + 4 * MDK_FE_NB;
   for (int m = 0; m < MDK_FE_NI; m++) {
     // expansion point: the interval's midpoint (its upper end for the two
-    // unbounded ones), which halves |x| and the rounding of the Horner steps
+    // unbounded ones), which halves |x| and the rounding of the Horner steps;
+    // rounded to FP32 first, so the device's x = u - r and these expansions
+    // share one u (a rounded u would shift every pair of the interval alike)
     const int mm = std::min(m, nb);
-    const double u = mm == 0 ? B[0] : (mm < nb ? 0.5 * (B[mm - 1] + B[mm]) : B[mm - 1]);
+    const double u = (double)(float)(mm == 0 ? B[0] : (mm < nb ? 0.5 * (B[mm - 1] + B[mm]) : B[mm - 1]));
     double c[4] = {0, 0, 0, 0}, q[3] = {0, 0, 0};
     if (m > 0) {
       const double top = B[mm - 1];

@@ -40,6 +40,10 @@ struct PotDev {
   const double *tab_v, *tab_dv, *tab_p, *tab_dp, *tab_e, *tab_de;
   // float copies of the knot data for the reduced-precision form
   float avf[MDK_MAXV], rvf[MDK_MAXV], apf[MDK_MAXP], rpf[MDK_MAXP];
+  // rp - rpf: the density knots' FP32 rounding (up to 3e-8 A) biases every
+  // pair's phi the same way (+8e-7 eV/atom in bcc Fe), so the FP32 form
+  // carries each density radius as rpf + rplf
+  float rplf[MDK_MAXP];
   float bc1f, bc2f, phi_cut_rf, phi_cut_vf, r_intf;
   // pre-derived join constants: log(V) and V'/V at both ends
   double jx1, jv1, jx2, jv2;
@@ -77,6 +81,7 @@ template <> struct PotView<double> {
   __device__ double rv(int k) const { return P.rv[k]; }
   __device__ double ap(int k) const { return P.ap[k]; }
   __device__ double rp(int k) const { return P.rp[k]; }
+  __device__ double dp(int k, double r) const { return P.rp[k] - r; }
   __device__ double bc1() const { return P.bc1; }
   __device__ double bc2() const { return P.bc2; }
   __device__ double pcr() const { return P.phi_cut_r; }
@@ -95,6 +100,8 @@ template <> struct PotView<float> {
   __device__ float rv(int k) const { return P.rvf[k]; }
   __device__ float ap(int k) const { return P.apf[k]; }
   __device__ float rp(int k) const { return P.rpf[k]; }
+  // R_k - r with the radius's low part added after the (exact near R_k) difference
+  __device__ float dp(int k, float r) const { return (P.rpf[k] - r) + P.rplf[k]; }
   __device__ float bc1() const { return P.bc1f; }
   __device__ float bc2() const { return P.bc2f; }
   __device__ float pcr() const { return P.phi_cut_rf; }
@@ -209,7 +216,7 @@ __device__ __forceinline__ T mdk_phi_n(const PotDev &P, T r) {
   T s = T(0);
 #pragma unroll
   for (int k = 0; k < NP; k++) {
-    const T d = mdk_pos<T>(Q.rp(k) - r);
+    const T d = mdk_pos<T>(Q.dp(k, r));
     s = knot_acc(s, Q.ap(k), d * d * d);
   }
   return r < Q.pcr() ? Q.pcv() : s;
@@ -222,7 +229,7 @@ __device__ __forceinline__ T mdk_dphi_n(const PotDev &P, T r) {
   T s = T(0);
 #pragma unroll
   for (int k = 0; k < NP; k++) {
-    const T d = mdk_pos<T>(Q.rp(k) - r);
+    const T d = mdk_pos<T>(Q.dp(k, r));
     s = knot_acc(s, T(-3) * Q.ap(k), d * d);
   }
   return r < Q.pcr() ? T(0) : s;
