@@ -161,6 +161,13 @@ int mdk_kinten(void *h);
  * prints), in reference atom order.  Needs the list and densities of a
  * previous force evaluation (MDK_ERR_STATE otherwise). */
 int mdk_energy_calc(void *h, double *en_atom);
+/* Per-atom diagonal stress of force's atomstress option
+ * (src/dynamics_m.F90:279-283, arrays of src/particles_m.F90:66): sx[i] =
+ * sum of rx_ij fp_ij over atom i's own reference half-list partners (NLIST of
+ * the last update_neighbourlist), at the current positions and the F' of the
+ * last force evaluation, in FP64 and reference atom order.  MDK_ERR_STATE
+ * before the first half list. */
+int mdk_atomic_stress(void *h, double *sx, double *sy, double *sz);
 /* moldin.F90:446-477: first force, tgid and initial accelerations x2, b2 */
 int mdk_md_init(void *h);
 /* scalev (src/thermostat_m.F90:317-395): Nose-Hoover scaling of the atom

@@ -138,6 +138,13 @@ struct mdk_ctx {
   int *lblocks, nlblocks, lblocks_cap, lblock_g[3];
   long rebuilds;
   bool list_valid;
+  bool half_valid;         // numn_ref/nlist_ref hold a reference half list
+  // reference link-cell chains (src/linkcell_m.F90:122-274) on the host in
+  // reference atom order, 1-based: L0 per cell, LINK, IC; chains_exact is
+  // false after relinks inside mdk_md_steps, which update only the device ic
+  int *h_l0, *h_link, *h_ic, h_ncell;
+  bool chains_exact;
+  int *ref_key, *rank_orig, *rank_slot;  // device: cell keys and chain ranks
   int last_rebuild_step;
   // box state: 2 copies
   BoxState *box;
@@ -531,12 +538,15 @@ __global__ void k_minus1(const int *in, int *out, int n) {
   if (i < n) out[i] = in[i] - 1;
 }
 
-// Reference half list (src/neighbourlist_m.F90:143-255) as an unordered set:
-// atom i lists j when j lies in i's own link cell after it (ascending original
-// index) or in one of the 13 forward cells, and r < rnear with the reference's
-// arithmetic.  Output in reference layout NLIST(nnbrs, nm), 1-based.
+// Reference half list (src/neighbourlist_m.F90:143-255): atom i lists j when
+// j lies in i's own link cell after it in the cell's chain from L0 (rank =
+// position in that chain) or in one of the 13 forward cells, and r < rnear
+// with the reference's arithmetic; cells in the reference's stencil order and
+// members in chain order (bin_atoms sorts them by rank), so NLIST(:, i) is the
+// reference's, entry for entry.  Output in reference layout NLIST(nnbrs, nm),
+// 1-based.
 __global__ void k_half_list(const double *x0, const double *y0, const double *z0, const int *ic,
-                            const int *orig, const int *cstart, const int *members, int n,
+                            const int *orig, const int *rank, const int *cstart, const int *members, int n,
                             int nlcx, int nlcy, int nlcz, const BoxState *bs, double rn2,
                             int pbc, int nnbrs, int *numn_ref, int *nlist_ref, int *flags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,7 +555,7 @@ __global__ void k_half_list(const double *x0, const double *y0, const double *z0
   const int c = ic[i] - 1;
   const int cx = c % nlcx, cy = (c / nlcx) % nlcy, cz = c / (nlcx * nlcy);
   const double xi = x0[i], yi = y0[i], zi = z0[i];
-  const int oi = orig[i];
+  const int oi = orig[i], ri = rank[i];
   int cnt = 0;
   for (int dz = -1; dz <= 1; dz++) {
     int jz = (cz + dz + nlcz) % nlcz;
@@ -560,7 +570,7 @@ __global__ void k_half_list(const double *x0, const double *y0, const double *z0
         int cj = jx + nlcx * (jy + nlcy * jz);
         for (int q = cstart[cj]; q < cstart[cj + 1]; q++) {
           int j = members[q];
-          if (own && orig[j] <= oi) continue;
+          if (own && rank[j] <= ri) continue;
           double ddx = xi - x0[j], ddy = yi - y0[j], ddz = zi - z0[j];
           min_image(pbc, ddx, ddy, ddz);
           double r2 = sep2_exact(b, ddx, ddy, ddz);
@@ -1994,6 +2004,45 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_energy_calc(const double *atoms, 
   en[i] = atoms[(size_t)F_AFRHO * n + i] + s;
 }
 
+// Per-atom diagonal stress of the reference's force loop
+// (src/dynamics_m.F90:279-283): stress_x(i) = sum of rx_ij fp_ij over atom i's
+// own half-list partners j only (the partner gets nothing from that pair),
+// with fp = -V'/r - (phi' F'_i + phi' F'_j)/r as the force loop forms it.
+// One thread per atom in reference order; positions at the atom's current slot.
+template <int POTM>
+__global__ void __launch_bounds__(MDK_BLOCK) k_atomic_stress(const double *atoms, const int *slot_of, int n,
+                                                             const int *numn_ref, const int *nlist_ref,
+                                                             int nnbrs, const BoxState *bs, int pbc, PotDev P,
+                                                             double *sx, double *sy, double *sz) {
+#pragma clang fp contract(off)
+  const int oi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (oi >= n) return;
+  const double *b = bs->b0;
+  const int i = slot_of[oi];
+  const double *x0 = atoms, *y0 = atoms + n, *z0 = atoms + 2 * (size_t)n;
+  const double *df = atoms + (size_t)F_DAFRHO * n;
+  const double xi = x0[i], yi = y0[i], zi = z0[i], dfi = df[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const int cnt = min(numn_ref[oi], nnbrs);
+  for (int q = 0; q < cnt; q++) {
+    const int j = slot_of[nlist_ref[(size_t)oi * nnbrs + q] - 1];
+    double dx = xi - x0[j], dy = yi - y0[j], dz = zi - z0[j];
+    min_image(pbc, dx, dy, dz);
+    const double r = sqrt(sep2_exact(b, dx, dy, dz));
+    const double rr = 1.0 / r;
+    double v, dv;
+    mdk_pair_vdv<double, POTM>(P, r, v, dv);
+    const double dph = mdk_pair_dphi<double, POTM>(P, r);
+    const double fp = -dv * rr + (-1.0 * (dph * dfi + dph * df[j]) * rr);
+    ax += (b[0] * dx + b[3] * dy + b[6] * dz) * fp;
+    ay += (b[1] * dx + b[4] * dy + b[7] * dz) * fp;
+    az += (b[2] * dx + b[5] * dy + b[8] * dz) * fp;
+  }
+  sx[oi] = ax;
+  sy[oi] = ay;
+  sz[oi] = az;
+}
+
 // potential evaluation for mdk_eval_potential
 __global__ void k_eval_pot(PotDev P, int which, const double *x, double *y, int cnt, const float *fetab) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2742,6 +2791,9 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->numn, sizeof(int) * n);
   ALLOC(c->nact, sizeof(int) * n);
   ALLOC(c->cell_of, sizeof(int) * n);
+  ALLOC(c->ref_key, sizeof(int) * n);
+  ALLOC(c->rank_orig, sizeof(int) * n);
+  ALLOC(c->rank_slot, sizeof(int) * n);
   ALLOC(c->members, sizeof(int) * n);
   ALLOC(c->dflags, sizeof(int) * 4);
   ALLOC(c->box, sizeof(BoxState) * 2);
@@ -2802,7 +2854,8 @@ extern "C" int mdk_destroy(void *h) {
   void *ptrs[] = {c->pos4, c->p4f, c->img, c->atoms, c->atoms_alt, c->zat, c->zat_alt, c->orig, c->orig_alt, c->slot_of,
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
-                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red, c->fetab};
+                  c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red, c->fetab,
+                  c->ref_key, c->rank_orig, c->rank_slot};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
@@ -2811,6 +2864,9 @@ extern "C" int mdk_destroy(void *h) {
   free(c->h_mass);
   free(c->h_del2);
   free(c->h_z);
+  free(c->h_l0);
+  free(c->h_link);
+  free(c->h_ic);
   for (int k = 0; k < 4; k++)
     if (c->mon_ev[k]) hipEventDestroy(c->mon_ev[k]);
   for (auto &e : c->evs) {
@@ -2982,6 +3038,33 @@ extern "C" int mdk_energy_calc(void *h, double *en_atom) {
   return MDK_OK;
 }
 
+extern "C" int mdk_atomic_stress(void *h, double *sx, double *sy, double *sz) {
+  mdk_ctx *c = H(h);
+  if (!c || !sx || !sy || !sz) return set_err(MDK_ERR_BAD_ARG, "bad atomic_stress arguments");
+  if (!c->pot_set || !c->half_valid)
+    return set_err(MDK_ERR_STATE, "atomic_stress needs the reference half list and densities of a force evaluation");
+  const int n = c->n;
+  double *o = fld(c->atoms_alt, 0, n);  // three scratch columns
+  const dim3 g(nblocks(n)), b(MDK_BLOCK);
+  const BoxState *bs = c->box + c->cur;
+#define MDK_STRESS_ARGS c->atoms, c->slot_of, n, c->numn_ref, c->nlist_ref, c->p.nnbrs, bs, c->pbc, c->pot, o, o + n, o + 2 * (size_t)n
+  if (c->pot.use_lookup)
+    k_atomic_stress<POT_TABLE><<<g, b, 0, c->stream>>>(MDK_STRESS_ARGS);
+  else if (c->material == MDK_MAT_IRON_CARBON)
+    k_atomic_stress<POT_FEC><<<g, b, 0, c->stream>>>(MDK_STRESS_ARGS);
+  else if (c->pot.nv == MDK_FE_NV && c->pot.np == MDK_FE_NP)
+    k_atomic_stress<POT_FE><<<g, b, 0, c->stream>>>(MDK_STRESS_ARGS);
+  else
+    k_atomic_stress<POT_KNOTS><<<g, b, 0, c->stream>>>(MDK_STRESS_ARGS);
+#undef MDK_STRESS_ARGS
+  RCK(check_launch("k_atomic_stress"));
+  HIPCK(hipMemcpyAsync(sx, o, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(sy, o + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(sz, o + 2 * (size_t)n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
 static const int field_map[MDK_NFIELDS] = {F_X0,  F_Y0,  F_Z0,   F_X1,     F_Y1,      F_Z1,   F_X2,
                                            F_Y2,  F_Z2,  F_FX,   F_FY,     F_FZ,      F_RHO,  F_AFRHO,
                                            F_DAFRHO, F_MASS, F_DEL2, F_X3, F_Y3, F_Z3};
@@ -3147,60 +3230,153 @@ extern "C" int mdk_get_ntc(void *h, int *ntc) {
   return MDK_OK;
 }
 
-extern "C" int mdk_linkup(void *h) {
-  mdk_ctx *c = H(h);
-  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+// Reference cells of every atom in reference order (host), from the device ic.
+static int ic_to_host(mdk_ctx *c, int *ico) {
   const int n = c->n;
-  k_ref_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, n,
-                                                      c->p.nlcx, c->p.nlcy, c->p.nlcz, c->ic);
-  RCK(check_launch("k_ref_cells"));
-  c->linked = true;
+  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->ref_key, c->ic, c->slot_of, n);
+  RCK(check_launch("k_gather_i"));
+  HIPCK(hipMemcpyAsync(ico, c->ref_key, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
   return MDK_OK;
 }
 
-extern "C" int mdk_relink(void *h) { return mdk_linkup(h); }
+static int chains_reserve(mdk_ctx *c) {
+  const int m = c->p.nlcx * c->p.nlcy * c->p.nlcz;
+  if (c->h_l0 && c->h_ncell == m) return MDK_OK;
+  free(c->h_l0);
+  c->h_l0 = (int *)calloc((size_t)m + 1, sizeof(int));
+  if (!c->h_link) c->h_link = (int *)calloc((size_t)c->n + 1, sizeof(int));
+  if (!c->h_ic) c->h_ic = (int *)calloc((size_t)c->n + 1, sizeof(int));
+  if (!c->h_l0 || !c->h_link || !c->h_ic) return set_err(MDK_ERR_STATE, "out of host memory for the link-cell chains");
+  c->h_ncell = m;
+  return MDK_OK;
+}
+
+// linkup's chains (src/linkcell_m.F90:140-182): each cell's atoms in
+// ascending order, the last linked back to the first
+static void chains_linkup(mdk_ctx *c, const int *ico) {
+  const int n = c->n, m = c->h_ncell;
+  int *l0 = c->h_l0, *link = c->h_link;
+  std::vector<int> last(m + 1, 0);
+  for (int k = 0; k <= m; k++) l0[k] = 0;
+  for (int i = 1; i <= n; i++) {
+    const int ip = ico[i - 1];
+    c->h_ic[i] = ip;
+    if (last[ip]) link[last[ip]] = i; else l0[ip] = i;
+    last[ip] = i;
+  }
+  for (int ip = 1; ip <= m; ip++)
+    if (last[ip]) link[last[ip]] = l0[ip];
+}
+
+// relink's chain surgery (src/linkcell_m.F90:224-271), atoms in reference
+// order: an atom whose cell changed leaves its old chain (the atom after it
+// becomes that cell's starter) and enters its new cell right after the
+// starter, becoming the starter itself.
+static int chains_relink(mdk_ctx *c, const int *ico) {
+  int *l0 = c->h_l0, *link = c->h_link, *icr = c->h_ic;
+  for (int i = 1; i <= c->n; i++) {
+    const int ip = ico[i - 1];
+    if (ip == icr[i]) continue;
+    const int me = icr[i];
+    int j = l0[me], hops = 0;
+    while (j > 0 && link[j] != i && hops++ < c->n) j = link[j];
+    if (j <= 0 || link[j] != i) return set_err(MDK_ERR_STATE, "link-cell chains inconsistent at atom %d", i);
+    link[j] = link[i];
+    l0[me] = link[j];
+    if (i == j) l0[me] = 0;
+    j = l0[ip] == 0 ? i : l0[ip];
+    link[i] = link[j];
+    link[j] = i;
+    icr[i] = ip;
+    l0[ip] = i;
+  }
+  return MDK_OK;
+}
+
+// each atom's position in its cell's chain from L0, uploaded in reference order
+static int chains_rank(mdk_ctx *c) {
+  std::vector<int> rank(c->n, 0);
+  for (int ip = 1; ip <= c->h_ncell; ip++) {
+    const int s0 = c->h_l0[ip];
+    if (!s0) continue;
+    int i = s0, r = 0;
+    do {
+      rank[i - 1] = r++;
+      i = c->h_link[i];
+    } while (i != s0 && r <= c->n);
+  }
+  HIPCK(hipMemcpyAsync(c->rank_orig, rank.data(), sizeof(int) * c->n, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return MDK_OK;
+}
+
+static int launch_ref_cells(mdk_ctx *c) {
+  const int n = c->n;
+  k_ref_cells<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, n,
+                                                      c->p.nlcx, c->p.nlcy, c->p.nlcz, c->ic);
+  return check_launch("k_ref_cells");
+}
+
+extern "C" int mdk_linkup(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  RCK(launch_ref_cells(c));
+  RCK(chains_reserve(c));
+  std::vector<int> ico(c->n);
+  RCK(ic_to_host(c, ico.data()));
+  chains_linkup(c, ico.data());
+  RCK(chains_rank(c));
+  c->linked = true;
+  c->chains_exact = true;
+  return MDK_OK;
+}
+
+extern "C" int mdk_relink(void *h) {
+  mdk_ctx *c = H(h);
+  if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
+  if (!c->linked || !c->chains_exact) return mdk_linkup(h);
+  RCK(launch_ref_cells(c));
+  std::vector<int> ico(c->n);
+  RCK(ic_to_host(c, ico.data()));
+  RCK(chains_relink(c, ico.data()));
+  return chains_rank(c);
+}
+
+// The relink at the wrap of ntc inside mdk_md_steps: the device ic only (no
+// host round trip in the loop).  The chain order it would give is not
+// tracked; the next chain query or half list starts from linkup's chains.
+static int relink_in_loop(mdk_ctx *c) {
+  RCK(launch_ref_cells(c));
+  c->chains_exact = false;
+  return MDK_OK;
+}
 
 extern "C" int mdk_get_linkcells(void *h, int *l0, int *link, int *ic) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
-  const int n = c->n, m = c->p.nlcx * c->p.nlcy * c->p.nlcz;
-  std::vector<int> ico(n);
-  int *tmp = c->cell_of;
-  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(tmp, c->ic, c->slot_of, n);
-  RCK(check_launch("k_gather_i"));
-  HIPCK(hipMemcpyAsync(ico.data(), tmp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
-  if (ic) memcpy(ic, ico.data(), sizeof(int) * n);
-  // chains in ascending atom order, closed (last -> first): linkup's layout
-  std::vector<int> first(m + 1, 0), last(m + 1, 0);
-  std::vector<int> lk(n + 1, 0);
-  for (int i = 1; i <= n; i++) {
-    int cc = ico[i - 1];
-    if (cc < 1 || cc > m) return set_err(MDK_ERR_STATE, "link cells not set (call linkup)");
-    if (last[cc]) lk[last[cc]] = i; else first[cc] = i;
-    last[cc] = i;
-  }
-  for (int cc = 1; cc <= m; cc++)
-    if (last[cc]) lk[last[cc]] = first[cc];
-  if (l0)
-    for (int cc = 1; cc <= m; cc++) l0[cc - 1] = first[cc];
-  if (link)
-    for (int i = 1; i <= n; i++) link[i - 1] = lk[i];
+  if (!c->linked) return set_err(MDK_ERR_STATE, "link cells not set (call linkup)");
+  if (!c->chains_exact) RCK(mdk_linkup(h));
+  if (ic) memcpy(ic, c->h_ic + 1, sizeof(int) * c->n);
+  if (l0) memcpy(l0, c->h_l0 + 1, sizeof(int) * c->h_ncell);
+  if (link) memcpy(link, c->h_link + 1, sizeof(int) * c->n);
   return MDK_OK;
 }
 
 extern "C" int mdk_update_neighbourlist(void *h) {
   mdk_ctx *c = H(h);
   if (!c) return set_err(MDK_ERR_BAD_ARG, "null handle");
-  if (!c->linked) RCK(mdk_linkup(h));
+  if (!c->linked || !c->chains_exact) RCK(mdk_linkup(h));
   const int n = c->n, m = c->p.nlcx * c->p.nlcy * c->p.nlcz;
-  // bin by reference cell (0-based key), in-cell order by original index
-  int *key = c->cell_of;
-  k_minus1<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->ic, key, n);
-  RCK(bin_atoms(c, key, m, c->orig));
+  // bin by reference cell (0-based key), in-cell order by chain rank; the
+  // device list's own cell keys (cell_of) are left alone
+  k_minus1<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->ic, c->ref_key, n);
+  k_gather_i<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->rank_slot, c->rank_orig, c->orig, n);
+  RCK(check_launch("k_gather_i"));
+  RCK(bin_atoms(c, c->ref_key, m, c->rank_slot));
   HIPCK(hipMemsetAsync(c->dflags, 0, sizeof(int) * 4, c->stream));
   k_half_list<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(
-      c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, c->ic, c->orig, c->cell_start, c->members, n,
+      c->atoms, c->atoms + n, c->atoms + 2 * (size_t)n, c->ic, c->orig, c->rank_slot, c->cell_start, c->members, n,
       c->p.nlcx, c->p.nlcy, c->p.nlcz, c->box + c->cur, c->rnear2_exact, c->pbc, c->p.nnbrs,
       c->numn_ref, c->nlist_ref, c->dflags);
   RCK(check_launch("k_half_list"));
@@ -3208,6 +3384,7 @@ extern "C" int mdk_update_neighbourlist(void *h) {
   HIPCK(hipMemcpyAsync(fl, c->dflags, sizeof(fl), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   if (fl[1]) return set_err(MDK_ERR_NBR_OVERFLOW, "NEIGHBOURLIST: EXCESS NEIGHBOURS! (nnbrs=%d)", c->p.nnbrs);
+  c->half_valid = true;
   return MDK_OK;
 }
 
@@ -3420,7 +3597,7 @@ extern "C" int mdk_md_steps(void *h, int nsteps, double *trace) {
   for (int s = 0; s < nsteps; s++) {
     RCK(md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true, s + 1 < nsteps));
     // relink when the reference's ntc counter wraps (moldin.F90:791)
-    if (c->ntc == 0) RCK(mdk_relink(h));
+    if (c->ntc == 0) RCK(relink_in_loop(c));
   }
   if (trace && nsteps > 0)
     HIPCK(hipMemcpyAsync(trace, c->trace_dev, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));

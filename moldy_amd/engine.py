@@ -42,7 +42,7 @@ EXPORTS = [
     "mdk_batch_tune", "mdk_batch_time", "mdk_batch_info",
     "mdk_timing_every", "mdk_batch_set_targets", "mdk_batch_loss_grad", "mdk_batch_vjp", "mdk_batch_time_loss_grad",
     "mdk_batch_loss_grad_noalloc",
-    "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_set_integrator", "mdk_predic", "mdk_correc",
+    "mdk_set_iron_carbon", "mdk_energy_calc", "mdk_atomic_stress", "mdk_set_integrator", "mdk_predic", "mdk_correc",
     "mdk_set_pressure", "mdk_scalev", "mdk_get_snhv", "mdk_set_snhv",
 ]
 
@@ -84,6 +84,7 @@ def load_library(path: str = LIB_PATH):
         "mdk_set_lookup": [vp, C.c_int],
         "mdk_set_iron_carbon": [vp, dp],
         "mdk_energy_calc": [vp, dp],
+        "mdk_atomic_stress": [vp, dp, dp, dp],
         "mdk_set_integrator": [vp, C.c_int], "mdk_predic": [vp], "mdk_correc": [vp],
         "mdk_set_pressure": [vp, C.c_double],
         "mdk_scalev": [vp, C.c_double, C.c_double], "mdk_get_snhv": [vp, dp], "mdk_set_snhv": [vp, C.c_double],
@@ -335,6 +336,13 @@ class Engine:
         out = np.empty(self.n)
         _check(self.L.mdk_energy_calc(self.h, _dp(out)))
         return out
+
+    def atomic_stress(self) -> np.ndarray:
+        """(n, 3) per-atom diagonal stress of force's atomstress option
+        (reference atom order)."""
+        out = np.empty((3, self.n))
+        _check(self.L.mdk_atomic_stress(self.h, _dp(out[0]), _dp(out[1]), _dp(out[2])))
+        return out.T.copy()
 
     def velocityverlet(self):
         self.call("velocityverlet")

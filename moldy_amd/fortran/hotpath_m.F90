@@ -5,9 +5,13 @@
 !                    with -DIRON_CARBON the iron_carbon Fe-Fe functions, src/iron_carbon.F90)
 !   linkcell_m       src/linkcell_m.F90:52-54
 !   neighbourlist_m  src/neighbourlist_m.F90:52-58
-!   thermostat_m     src/thermostat_m.F90:94-395   (init, kinten, scalev, snhv)
-!   dynamics_m       src/dynamics_m.F90:69-83      (force, rhoset, energy_calc, update_therm_avs)
-!   quench_m         src/quench_m.F90:59-63        (velocityverlet, predic, correc)
+!   thermostat_m     src/thermostat_m.F90:52-395   (init, temp, setvel, kinten, scalev, snhv)
+!   dynamics_m       src/dynamics_m.F90:69-83      (force with atomstress, rhoset, energy_calc,
+!                    update_therm_avs, efs.out, write_stress, write_atomic_stress, energy_pair_emb)
+!   quench_m         src/quench_m.F90:59-63        (velocityverlet, predic, correc, quench)
+! They build against this package's state modules (state_m.F90) or, unchanged,
+! against the reference's own constants_m, params_m, random_m, particles_m,
+! system_m, matrix_m and parrinellorahman_m.
 !
 module potential_m
   use iso_c_binding
@@ -420,11 +424,92 @@ module thermostat_m
   use params_m
   use particles_m
   use parrinellorahman_m
+  use system_m, only : ke
+  use random_m, only : rand1
   use mdk_device_m
   implicit none
   private
   public :: init_thermostat_m, cleanup_thermostat_m, kinten, scalev, set_snhv, get_snhv
+  public :: set_temp, get_temp, setvel
+  real(kind_wp), save :: temp = 0.0d0   ! the run's temperature (src/thermostat_m.F90:58,106-115)
 contains
+  !> setvel (src/thermostat_m.F90:191-306): initial velocities on the host
+  !> (once per run).  Movable atoms draw x1, y1, z1 = rand1() - 1/2, the
+  !> mean momentum per movable atom is removed, and all velocities are scaled
+  !> so the kinetic energy is rqke nm/nmove; for ivol 1, 2, 3 and 5 the box
+  !> velocities b1 are drawn after the atoms, rows centred by sum/nm and
+  !> scaled to the box temperature temprq.  boxtem is set as the reference
+  !> leaves it: from the sums of squares before and after the scaling.
+  subroutine setvel
+    type(simparameters) :: sp
+    integer :: i, j, nmove
+    real(kind_wp) :: px, py, pz, acc, vx, vy, vz, f, rsum
+    sp = get_params()
+    nmove = 0
+    do i = 1, sp%nm
+       if (atomic_number(i) == 0 .or. atomic_mass(i) < 0.1) cycle
+       x1(i) = rand1() - 0.5
+       y1(i) = rand1() - 0.5
+       z1(i) = rand1() - 0.5
+       nmove = nmove + 1
+    end do
+    write(*,*) sp%nm, "atoms, of which  ", nmove, "  are moveable"
+    px = 0.0d0; py = 0.0d0; pz = 0.0d0
+    do i = 1, sp%nm
+       px = px + x1(i)*atomic_mass(i)
+       py = py + y1(i)*atomic_mass(i)
+       pz = pz + z1(i)*atomic_mass(i)
+    end do
+    px = px/nmove; py = py/nmove; pz = pz/nmove
+    acc = 0.0d0
+    do i = 1, sp%nm
+       if (atomic_number(i) == 0 .or. atomic_mass(i) < 0.1) cycle
+       x1(i) = x1(i) - px/atomic_mass(i)
+       y1(i) = y1(i) - py/atomic_mass(i)
+       z1(i) = z1(i) - pz/atomic_mass(i)
+       vx = b0(1,1)*x1(i) + b0(1,2)*y1(i) + b0(1,3)*z1(i)
+       vy = b0(2,1)*x1(i) + b0(2,2)*y1(i) + b0(2,3)*z1(i)
+       vz = b0(3,1)*x1(i) + b0(3,2)*y1(i) + b0(3,3)*z1(i)
+       acc = acc + atomic_mass(i)*(vx*vx + vy*vy + vz*vz)
+    end do
+    ke = acc/sp%deltat**2/2.d0
+    ! the atom-count ratio and its root are single precision in the reference
+    f = sqrt(sp%rqke/ke)*sqrt(real(nmove)/real(sp%nm))
+    x1 = x1*f; y1 = y1*f; z1 = z1*f
+    call set_temp(sp%rqke/(1.5d0*bk))
+    ke = sp%rqke
+    if (sp%ivol < 1 .or. sp%ivol == 4) return
+    acc = 0.0d0
+    do i = 1, nmat
+       rsum = 0.0d0
+       do j = 1, nmat
+          b1(i,j) = rand1() - 0.5d0
+          rsum = rsum + b1(i,j)
+       end do
+       rsum = rsum/sp%nm
+       do j = 1, nmat
+          b1(i,j) = b1(i,j) - rsum
+          acc = acc + b1(i,j)**2
+       end do
+    end do
+    sp%boxtem = (acc/sp%bdel2)/(4.5d0*bk)
+    f = sqrt(sp%temprq/sp%boxtem)
+    do i = 1, nmat
+       do j = 1, nmat
+          b1(i,j) = b1(i,j)*f
+          acc = acc + b1(i,j)**2
+       end do
+    end do
+    sp%boxtem = (acc/sp%bdel2)/(4.5d0*bk)
+    call set_params(sp)
+  end subroutine setvel
+  subroutine set_temp(t)
+    real(kind_wp), intent(in) :: t
+    temp = t
+  end subroutine set_temp
+  pure real(kind_wp) function get_temp()
+    get_temp = temp
+  end function get_temp
   !> scalev (src/thermostat_m.F90:317-395): Nose-Hoover velocity scaling on
   !> the device; snhv lives with the device handle
   subroutine scalev
@@ -499,6 +584,7 @@ module dynamics_m
   private
   public :: init_dynamics_m, cleanup_dynamics_m, force, rhoset, update_therm_avs, energy_calc
   public :: update_neighbourlist, write_energy_forces_stress, dynamics_pull_forces
+  public :: write_stress, write_atomic_stress, energy_pair_emb, dynamics_after_force
   real(kind_wp), allocatable, public :: fx(:), fy(:), fz(:)
   real(kind_wp), allocatable, public :: x1_dt2(:), y1_dt2(:), z1_dt2(:)
 contains
@@ -531,8 +617,22 @@ contains
     call mdk_check(mdk_force(mdk_dev_handle()), 'mdk_force')
     call mdk_dev_pull(.false.)
     call dynamics_pull_forces()
-    if (list_due) call update_neighbourlist
+    call dynamics_after_force(list_due)
   end subroutine force
+
+  !> What the reference's force leaves besides forces, after a device force
+  !> evaluation (also the one inside velocityverlet): NLIST/NUMN rebuilt when
+  !> its ntc was 1 (list_due), and with atomstress the per-atom diagonal
+  !> stress over that half list (src/dynamics_m.F90:186-198, 279-283).  The
+  !> positions are those of the evaluation.
+  subroutine dynamics_after_force(list_due)
+    logical, intent(in) :: list_due
+    type(simparameters) :: sp
+    sp = get_params()
+    if (list_due) call update_neighbourlist
+    if (sp%atomstress) call mdk_check(mdk_atomic_stress(mdk_dev_handle(), stressx, stressy, stressz), &
+                                      'mdk_atomic_stress')
+  end subroutine dynamics_after_force
 
   !> rhoset (src/dynamics_m.F90:497-579).  The density function is the
   !> registered potential's phi; phi_local is accepted for interface parity.
@@ -586,6 +686,53 @@ contains
     write(u,*) sigma
     close(u)
   end subroutine write_energy_forces_stress
+
+  !> write_stress (src/dynamics_m.F90:629-647): the stress tensor
+  !> -B0 (tp + tk)^T / vol in GPa, one row per line
+  subroutine write_stress(unit)
+    integer :: unit
+    integer :: i, j, k
+    real(kind_wp) :: sigma(3,3)
+    do i = 1, 3
+       do j = 1, 3
+          sigma(i,j) = -1.d0*sum((/(b0(j,k)*(tp(i,k)+tk(i,k)), k = 1, 3)/))/vol
+       end do
+    end do
+    do i = 1, 3
+       write(unit, '("STRESS I=",I2, 3e14.6," GPa")') i, (sigma(i,j)*press_to_gpa, j = 1, 3)
+    end do
+  end subroutine write_stress
+
+  !> write_atomic_stress (src/dynamics_m.F90:765-792): atomicStressNN.out
+  !> (NN the counter, zero-padded below 10), or lastAtomicStress.out for -1;
+  !> one line per atom: stressx, stressy, stressz, x0, y0, z0
+  subroutine write_atomic_stress(file_counter)
+    integer :: file_counter
+    integer :: u, i
+    character(len=40) :: fname
+    type(simparameters) :: sp
+    sp = get_params()
+    if (file_counter == -1) then
+       fname = 'lastAtomicStress.out'
+    else if (file_counter < 10) then
+       write(fname, '("atomicStress0",i0,".out")') file_counter
+    else
+       write(fname, '("atomicStress",i0,".out")') file_counter
+    end if
+    open(newunit=u, file=trim(fname), status='unknown')
+    do i = 1, sp%nm
+       write(u,*) stressx(i), stressy(i), stressz(i), x0(i), y0(i), z0(i)
+    end do
+    close(u)
+  end subroutine write_atomic_stress
+
+  !> energy_pair_emb (src/dynamics_m.F90:797-806): total potential energy,
+  !> the sum of energy_calc's per-atom energies
+  function energy_pair_emb()
+    real(kind_wp) :: energy_pair_emb
+    call energy_calc
+    energy_pair_emb = sum(en_atom)
+  end function energy_pair_emb
 end module dynamics_m
 
 module quench_m
@@ -814,10 +961,15 @@ contains
   !> velocityverlet(dt) (src/quench_m.F90:562-624), force included
   subroutine velocityverlet(dt)
     real(kind_wp) :: dt
+    type(simparameters) :: sp
+    logical :: list_due
+    sp = get_params()
+    list_due = (sp%ntc + 1 == 1)   ! the force inside rebuilds NLIST/NUMN
     call mdk_dev_push()
     call mdk_check(mdk_velocityverlet(mdk_dev_handle()), 'mdk_velocityverlet')
     call mdk_dev_pull(.true.)
     call dynamics_pull_forces()
     call mdk_check(mdk_get_tensor(mdk_dev_handle(), MDK_TGID, tgid), 'get tgid')
+    call dynamics_after_force(list_due)
   end subroutine velocityverlet
 end module quench_m

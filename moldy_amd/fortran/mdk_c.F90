@@ -80,6 +80,11 @@ module mdk_c
        type(c_ptr), value :: h
        real(c_double) :: en_atom(*)
      end function mdk_energy_calc
+     integer(c_int) function mdk_atomic_stress(h, sx, sy, sz) bind(C, name="mdk_atomic_stress")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double) :: sx(*), sy(*), sz(*)
+     end function mdk_atomic_stress
      integer(c_int) function mdk_set_lookup(h, nlookup) bind(C, name="mdk_set_lookup")
        import :: c_int, c_ptr
        type(c_ptr), value :: h

@@ -19,6 +19,13 @@ module mdk_device_m
   public :: mdk_dev_set_potential, mdk_dev_set_lookup, mdk_dev_push, mdk_dev_pull
   public :: mdk_dev_set_iron_carbon
   public :: mdk_dev_push_positions
+  ! device options with no counterpart in the reference's simparameters, so
+  ! the drop-in also builds against the reference's own params_m: set them
+  ! before the first init_linkcell_m.  precision 0 FP64 / 1 FP32 form;
+  ! rebuild 0 the reference's ntcm cadence plus the skin monitor, 1 skin
+  ! monitor only; rskin the list skin (<= 0: the reference's rpad)
+  integer, public, save :: mdk_precision = 0, mdk_rebuild = 0
+  real(c_double), public, save :: mdk_rskin = 0.0d0
 
   type(c_ptr), save :: handle = c_null_ptr
   logical, save :: pot_pending = .false.
@@ -48,9 +55,9 @@ contains
     sp = get_params()
     p%nm = sp%nm; p%ivol = sp%ivol; p%nnbrs = sp%nnbrs; p%ntcm = sp%ntcm
     p%nlcx = sp%nlcx; p%nlcy = sp%nlcy; p%nlcz = sp%nlcz
-    p%precision = sp%precision; p%rebuild = sp%rebuild; p%max_full = 0
+    p%precision = mdk_precision; p%rebuild = mdk_rebuild; p%max_full = 0
     p%deltat = sp%deltat; p%rnear = sp%rnear; p%press = sp%press; p%bdel2 = sp%bdel2
-    p%rskin = sp%rskin
+    p%rskin = mdk_rskin
     if (p%rskin <= 0.0d0) p%rskin = sp%rpad
     call mdk_check(mdk_create(handle, p), 'mdk_create')
     if (pot_pending) call apply_potential()

@@ -2,7 +2,7 @@
 ! state_m.F90 -- the reference's base and state modules, restated for the GPU
 ! drop-in: same module names and the public names the hot path's callers use
 ! (src/constants_m.F90, src/params_m.F90:44-184, src/matrix_m.F90:60-119,
-! src/system_m.F90:45-49, src/particles_m.F90:52-94,
+! src/system_m.F90:45-49, src/random_m.F90:54-143, src/particles_m.F90:52-94,
 ! src/parrinellorahman_m.F90:91-261).  Host arrays keep the reference's layout;
 ! the device copies live behind the C ABI (mdk_device_m).
 !
@@ -40,9 +40,6 @@ module params_m
      integer :: nsteps = 0, ntcm = 10, ntc = 0
      logical :: uselookup = .false., atomstress = .false.
      real(kind_wp) :: boxtem = 0._kind_wp, bdel2 = 0._kind_wp, bmass = 0._kind_wp
-     ! device options (no counterpart in the reference)
-     integer :: precision = 0, rebuild = 0
-     real(kind_wp) :: rskin = 0._kind_wp
   end type simparameters
 
   type(simparameters), save :: simparam_global
@@ -106,6 +103,61 @@ module system_m
   real(kind_wp) :: pe = 0, ke = 0, te = 0, h = 0, th = 0
 end module system_m
 
+!> random_m (src/random_m.F90:54-143): the Marsaglia-Zaman-Tsang RANMAR
+!> generator (Marsaglia, Zaman and Tsang, Stat. Prob. Lett. 9 (1990) 35-39):
+!> a lagged-Fibonacci difference sequence over a 97-entry table seeded by
+!> rstart, combined with an arithmetic sequence of step 7654321/2^24.
+module random_m
+  use constants_m, only : kind_wp
+  implicit none
+  private
+  public :: rand1, rstart
+  real(kind_wp), save :: tab(97) = 0.0_kind_wp, cc = 0.0_kind_wp, cstep = 0.0_kind_wp, cmod = 0.0_kind_wp
+  integer, save :: ip = 97, jp = 33
+contains
+  !> next uniform deviate in [0, 1)
+  function rand1()
+    real(kind_wp) :: rand1, x
+    x = tab(ip) - tab(jp)
+    if (x < 0.0_kind_wp) x = x + 1.0_kind_wp
+    tab(ip) = x
+    ip = merge(97, ip - 1, ip == 1)
+    jp = merge(97, jp - 1, jp == 1)
+    cc = cc - cstep
+    if (cc < 0.0_kind_wp) cc = cc + cmod
+    x = x - cc
+    if (x < 0.0_kind_wp) x = x + 1.0_kind_wp
+    rand1 = x
+  end function rand1
+
+  !> seeds in 1..168, not all 1 (moldin uses 97, 33, 21, 82): each table
+  !> entry takes 24 bits from a 3-lag multiplicative sequence mod 179 and a
+  !> linear congruential sequence mod 169
+  subroutine rstart(iw, ix, iy, iz)
+    integer, intent(in) :: iw, ix, iy, iz
+    integer :: a, b, c, d, m, e, bit
+    real(kind_wp) :: acc, w
+    a = iw; b = ix; c = iy; d = iz
+    do e = 1, 97
+       acc = 0.0_kind_wp
+       w = 0.5_kind_wp
+       do bit = 1, 24
+          m = mod(mod(a*b, 179)*c, 179)
+          a = b; b = c; c = m
+          d = mod(53*d + 1, 169)
+          if (mod(d*m, 64) >= 32) acc = acc + w
+          w = 0.5_kind_wp*w
+       end do
+       tab(e) = acc
+    end do
+    cc = 362436.0_kind_wp/16777216.0_kind_wp
+    cstep = 7654321.0_kind_wp/16777216.0_kind_wp
+    cmod = 16777213.0_kind_wp/16777216.0_kind_wp
+    ip = 97
+    jp = 33
+  end subroutine rstart
+end module random_m
+
 module particles_m
   use constants_m, only : kind_wp
   use params_m, only : simparameters, get_params
@@ -118,6 +170,7 @@ module particles_m
   real(kind_wp), allocatable, public :: x0(:), y0(:), z0(:), x1(:), y1(:), z1(:)
   real(kind_wp), allocatable, public :: x2(:), y2(:), z2(:), x3(:), y3(:), z3(:)
   real(kind_wp), allocatable, public :: afrho(:), dafrho(:), rho(:), en_atom(:)
+  real(kind_wp), allocatable, public :: stressx(:), stressy(:), stressz(:)   ! atomstress (:66)
 contains
   subroutine init_particles_m()
     type(simparameters) :: sp
@@ -126,7 +179,8 @@ contains
     n = sp%nm
     allocate(atomic_index(n), atomic_number(n), atomic_mass(n), del2(n), ispec(112))
     allocate(x0(n), y0(n), z0(n), x1(n), y1(n), z1(n), x2(n), y2(n), z2(n), x3(n), y3(n), z3(n))
-    allocate(afrho(n), dafrho(n), rho(n), en_atom(n))
+    allocate(afrho(n), dafrho(n), rho(n), en_atom(n), stressx(n), stressy(n), stressz(n))
+    stressx = 0; stressy = 0; stressz = 0
     ispec = 0
     x0 = 0; y0 = 0; z0 = 0; x1 = 0; y1 = 0; z1 = 0; x2 = 0; y2 = 0; z2 = 0; x3 = 0; y3 = 0; z3 = 0
     afrho = 0; dafrho = 0; rho = 0; en_atom = 0; del2 = 0
@@ -134,6 +188,7 @@ contains
   subroutine cleanup_particles_m()
     deallocate(atomic_index, atomic_number, atomic_mass, del2, ispec)
     deallocate(x0, y0, z0, x1, y1, z1, x2, y2, z2, x3, y3, z3, afrho, dafrho, rho, en_atom)
+    deallocate(stressx, stressy, stressz)
   end subroutine cleanup_particles_m
 end module particles_m
 

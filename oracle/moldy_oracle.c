@@ -1401,6 +1401,14 @@ int orc_get_ic(const orc_sys *s, int *ic) {
   for (int i = 0; i < s->n; i++) ic[i] = s->ic[i + 1];
   return 0;
 }
+/* L0 (ncell starters) and LINK (n closed chains), 1-based, as linkup and
+ * relink leave them */
+int orc_get_chains(const orc_sys *s, int *l0, int *link) {
+  const int ncell = s->nlcx * s->nlcy * s->nlcz;
+  for (int c = 0; c < ncell; c++) l0[c] = s->l0[c];
+  for (int i = 0; i < s->n; i++) link[i] = s->link[i + 1];
+  return 0;
+}
 
 long orc_bruteforce_pairs(int n, const double *b0, const double *x, const double *y,
                           const double *z, int ivol, double rcut, int *pi, int *pj,

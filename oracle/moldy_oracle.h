@@ -165,6 +165,7 @@ long orc_get_pairs(const orc_sys *s, int *pi, int *pj, long cap);
 int  orc_get_numn(const orc_sys *s, int *numn);
 /* link cell of each atom (1-based cell index like ic) */
 int  orc_get_ic(const orc_sys *s, int *ic);
+int  orc_get_chains(const orc_sys *s, int *l0, int *link);
 
 /* Brute-force O(N^2) minimum-image pair enumeration, r < rcut, i<j, 0-based.
  * Independent of link cells: used to pin the cell algorithm. */

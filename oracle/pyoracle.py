@@ -86,6 +86,7 @@ def lib():
         L.orc_get_pairs.restype = C.c_long
         L.orc_get_numn.argtypes = [C.c_void_p, ip]
         L.orc_get_ic.argtypes = [C.c_void_p, ip]
+        L.orc_get_chains.argtypes = [C.c_void_p, ip, ip]
         L.orc_bruteforce_pairs.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_double,
                                            ip, ip, C.c_long]
         L.orc_bruteforce_pairs.restype = C.c_long
@@ -177,6 +178,7 @@ class OracleSystem:
                  rcut: float = 0.0, nnbrs: int = 150, ntcm: int = 10,
                  press_gpa: float = 0.0, boxmass: float = 50.0, iverlet: int = 1):
         self.pot = pot
+        self.ivol = int(ivol)
         n = sysdef.n
         b0 = np.asfortranarray(sysdef.b0).ravel(order="F").copy()
         s = np.ascontiguousarray(sysdef.s.T)
@@ -245,6 +247,14 @@ class OracleSystem:
         lib().orc_get_ic(self.h, _ip(out))
         return out
 
+    def chains(self):
+        """(L0, LINK) of the link cells, 1-based, as linkup/relink leave them."""
+        m = self.int("nlcx") * self.int("nlcy") * self.int("nlcz")
+        l0 = np.empty(m, dtype=np.int32)
+        link = np.empty(self.n, dtype=np.int32)
+        lib().orc_get_chains(self.h, _ip(l0), _ip(link))
+        return l0, link
+
     def scalev(self, nose: float, rqke: float) -> None:
         lib().orc_scalev(self.h, float(nose), float(rqke))
 
@@ -262,6 +272,28 @@ class OracleSystem:
         """en_atom of energy_calc (needs afrho of a previous force())."""
         out = np.empty(self.n)
         lib().orc_energy_calc(self.h, _dp(out))
+        return out
+
+    def atomic_stress(self) -> np.ndarray:
+        """(n, 3) per-atom diagonal stress of the force loop's atomstress
+        option (src/dynamics_m.F90:279-283): stress_x(i) = sum of rx_ij fp_ij
+        over atom i's own half-list partners j, rx = B0 (s_i - s_j) after
+        pr_get_realsep_from_dx's minimum image, fp = -V'/r - phi'(F'_i + F'_j)/r.
+        Needs the list and F' of a previous force()."""
+        p = self.pairs()
+        i, j = p[:, 0], p[:, 1]
+        s = np.stack([self.array("x0"), self.array("y0"), self.array("z0")], axis=1)
+        d = s[i] - s[j]
+        per = {0: (0, 1, 2), 1: (0, 1, 2), 4: (0, 1, 2), 2: (0, 1), 5: (2,)}.get(self.ivol, ())
+        for k in per:
+            x = d[:, k]
+            d[:, k] = x - np.where(x >= 0, np.floor(x + 0.5), np.ceil(x - 0.5))   # nint
+        rvec = d @ self.tensor("b0").T
+        r = np.sqrt(np.sum(rvec * rvec, axis=1))
+        df = self.array("dafrho")
+        fp = -self.pot.f("dvee", r) / r - self.pot.f("dphi", r) * (df[i] + df[j]) / r
+        out = np.zeros((self.n, 3))
+        np.add.at(out, i, rvec * fp[:, None])
         return out
 
     def forces_cart(self) -> np.ndarray:

@@ -36,8 +36,10 @@
 !                 and b3 follow the final record
 !         bit 2 = extra record: per step pe, ke, vol, box kinetic energy
 !                 (moldin.F90:768-774), b0(3,3); after the run en_atom (energy_calc)
-! Compiled against the drop-in modules (-DMDK_DROPIN), bit 1 is not available:
-! random_m, setvel and scalev are outside the replaced hot path.
+!         bit 8 = atomstress on (the force loop's per-atom diagonal stress,
+!                 dynamics_m.F90:279-283); extra record after all others: the
+!                 stress after the first force as (3, n), then stressx,
+!                 stressy, stressz after the last
 !
 program ref_driver
   use constants_m
@@ -52,9 +54,7 @@ program ref_driver
   use thermostat_m
   use dynamics_m
   use quench_m
-#ifndef MDK_DROPIN
   use random_m
-#endif
   implicit none
 
   character(len=1024) :: inpath, outpath
@@ -63,7 +63,7 @@ program ref_driver
   real(kind_wp) :: dt_fs, rpad_in, press_gpa, boxmass, b0in(3,3)
   real(kind_wp) :: rmaxpot, s1, s2, s3, a1, a2, a3, temprq_in, rdum, bke, nose_in, thermotimer
   real(kind_wp) :: fs_ref = 0.098227, pexx
-  real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:), xtrace(:,:)
+  real(kind_wp), allocatable :: rgrid(:), rhogrid(:), trace(:,:), xtrace(:,:), stress0(:,:)
   type(simparameters) :: sp
   logical :: timing
 
@@ -100,6 +100,7 @@ program ref_driver
   sp%bmass = boxmass
   sp%bmass = sp%nm*sp%bmass
   sp%bdel2 = sp%deltat*sp%deltat/(2.d0*sp%bmass)
+  sp%atomstress = iand(flags, 256) /= 0
   call set_params(sp)
 
   call init_particles_m
@@ -189,7 +190,6 @@ program ref_driver
      end if
   end do
 
-#ifndef MDK_DROPIN
   if (iand(flags, 2) /= 0) then
      call rstart(97, 33, 21, 82)
      do i = 1, 3*n
@@ -198,10 +198,13 @@ program ref_driver
      call setvel
      call scalev
   end if
-#endif
 
   call linkup
   call force
+  if (iand(flags, 256) /= 0) then
+     allocate(stress0(3, n))
+     stress0(1,:) = stressx; stress0(2,:) = stressy; stress0(3,:) = stressz
+  end if
   call matrix_invert(tg, tginv)
   call matrix_multiply(nmat, tginv, tgdot, tgid)
   do i = 1, n
@@ -308,6 +311,7 @@ program ref_driver
         call embed(pexx)
         write(20) pexx, afrho, dafrho
      end if
+     if (iand(flags, 256) /= 0) write(20) stress0, stressx, stressy, stressz
      close(20)
   end if
 end program ref_driver

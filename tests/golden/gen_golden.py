@@ -129,6 +129,9 @@ def run_reference(sysdef, ivol=1, ntcm=10, nsteps=10, uselookup=0, nnbrs=150, rp
     if flags & 128:
         d["embed_pe"] = float(R.f8(1)[0])
         d["embed_afrho"], d["embed_dafrho"] = R.f8(n), R.f8(n)
+    if flags & 256:
+        d["stress0"] = R.f8(3 * n).reshape(n, 3)
+        d["stress_f"] = R.f8(3 * n).reshape(3, n).T.copy()
     return d
 
 
@@ -142,6 +145,12 @@ CASES = {
                            strain=np.array([[0.01, 0.004, -0.003], [0.004, -0.006, 0.002],
                                             [-0.003, 0.002, 0.003]])),
                       dict(ivol=0, ntcm=10, nsteps=12, uselookup=0, press=0.0, boxmass=50.0)),
+    # the force loop's per-atom diagonal stress (flag 256, atomstress) on the
+    # sheared NPT box, the half list rebuilt every second force (ntcm=2)
+    "atomstress_tric_1024": (dict(nc=8, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=97,
+                                  strain=np.array([[0.01, 0.004, -0.003], [0.004, -0.006, 0.002],
+                                                   [-0.003, 0.002, 0.003]])),
+                             dict(ivol=0, ntcm=2, nsteps=5, uselookup=0, press=0.0, boxmass=50.0, flags=256)),
     # lookup tables (set_up before potential_set_lookup)
     "lookup_2000": (dict(nc=10, a=2.8665, sigma=0.05, temp=300.0, deltat_fs=1.0, seed=77),
                     dict(ivol=1, ntcm=10, nsteps=3, uselookup=1)),
@@ -386,6 +395,8 @@ def main(names=None):
             fx[k] = d[k]
         if "pe_q" in d:
             fx["pe_q"], fx["vol_q"] = np.float64(d["pe_q"]), np.float64(d["vol_q"])
+        if "stress0" in d:
+            fx["stress0"], fx["stress_f"] = d["stress0"], d["stress_f"]
         if "embed_pe" in d:
             fx["embed_pe"] = np.float64(d["embed_pe"])
             fx["embed_afrho"], fx["embed_dafrho"] = d["embed_afrho"], d["embed_dafrho"]

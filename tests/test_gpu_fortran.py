@@ -28,7 +28,8 @@ def rel(a, b):
 
 @pytest.mark.parametrize("case", ["c0_nve_2000", "npt_tric_1024", "lookup_2000", "hot_1024",
                                   "fec_npt_1024", "fec_lookup_2000", "fec_hot_1024",
-                                  "gear_npt_1024", "gear_z_1024", "nose_nvt_1024", "embed_rho_250"])
+                                  "gear_npt_1024", "gear_z_1024", "nose_nvt_1024", "embed_rho_250",
+                                  "setvel_slab_ivol2_1024", "atomstress_tric_1024"])
 def test_fortran_dropin_matches_reference(oracle, case):
     import gen_golden
     import refrun
@@ -55,6 +56,9 @@ def test_fortran_dropin_matches_reference(oracle, case):
     for k in ("x1_f", "fx_f"):
         assert rel(d[k], F[k]) < 1e-9, k
     assert rel(d["b0_f"], F["b0_f"]) < 1e-14
+    if "stress0" in F:    # atomstress: per-atom diagonal stress after the first and last force
+        assert rel(d["stress0"], F["stress0"]) < 1e-10
+        assert rel(d["stress_f"], F["stress_f"]) < 1e-10
     if "embed_pe" in F:   # embed on the caller's (changed) RHO
         assert abs(d["embed_pe"] - float(F["embed_pe"])) < 1e-12 * abs(float(F["embed_pe"]))
         for k in ("embed_afrho", "embed_dafrho"):

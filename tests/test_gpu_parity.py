@@ -108,6 +108,61 @@ def test_gpu_reference_half_list_bit_exact(oracle, case):
     eng.close()
 
 
+def test_gpu_relink_chains_and_half_list_match_reference(oracle):
+    """relink keeps the reference's chain order (src/linkcell_m.F90:224-271:
+    an atom that changes cell becomes the new cell's starter, the atom after
+    it the old cell's), and the half list follows it: which atom of an
+    own-cell pair lists the other, and the order inside NLIST(:, i).  After
+    two rounds of random displacements and relinks, L0, LINK and IC equal the
+    oracle's, and so does every NLIST(:, i), entry for entry."""
+    F = load_golden("hot_1024")
+    say("relink chains")
+    eng = make_engine(F)
+    S = oracle.OracleSystem(golden_system(F), oracle.Potential(), **run_kwargs(F))
+    eng.linkup()
+    S.call("linkup")
+    rng = np.random.default_rng(5)
+    s = golden_system(F).s.copy()
+    moved = 0
+    for _ in range(2):
+        s = s + rng.normal(0.0, 0.03, s.shape)
+        for k, nm in enumerate(("x0", "y0", "z0")):
+            eng.upload(nm, s[:, k])
+            S.set_array(nm, s[:, k])
+        ic_before = S.ic()
+        eng.relink()
+        S.call("relink")
+        moved += int(np.sum(S.ic() != ic_before))
+        l0, link, ic = eng.linkcells()
+        l0o, linko = S.chains()
+        np.testing.assert_array_equal(ic, S.ic())
+        np.testing.assert_array_equal(l0, l0o)
+        np.testing.assert_array_equal(link, linko)
+    assert moved > 100
+    eng.update_neighbourlist()
+    assert S.call("update_neighbourlist") == 0
+    np.testing.assert_array_equal(eng.pairs(), S.pairs())
+    eng.close()
+
+
+def test_gpu_atomic_stress_matches_reference():
+    """mdk_atomic_stress after the first force and its half list: the
+    reference's per-atom diagonal stress (atomstress,
+    src/dynamics_m.F90:279-283) on the sheared NPT box to 1e-10 of the largest
+    component; before any half list the entry point refuses."""
+    from moldy_amd.engine import MdkError
+    F = load_golden("atomstress_tric_1024")
+    say("atomic stress")
+    eng = make_engine(F)
+    eng.linkup()
+    with pytest.raises(MdkError):
+        eng.atomic_stress()
+    eng.md_init()
+    eng.update_neighbourlist()
+    assert rel(eng.atomic_stress(), F["stress0"]) < 1e-10
+    eng.close()
+
+
 @pytest.mark.parametrize("case", CASES + ["lookup_2000"])
 def test_gpu_first_force_matches_reference(case):
     F = load_golden(case)

@@ -660,3 +660,28 @@ def test_embed_fixture_against_oracle_functions(oracle):
     np.testing.assert_allclose(F["embed_dafrho"][1:], pot.f("demb", rho), rtol=1e-12)
     assert F["embed_dafrho"][0] == F["dafrho_f"][0]
     assert abs(F["embed_pe"] - F["embed_afrho"][1:].sum()) < 1e-12 * abs(F["embed_pe"])
+
+
+def test_atomic_stress_against_reference(oracle):
+    """The force loop's per-atom diagonal stress (atomstress,
+    src/dynamics_m.F90:279-283), written by the reference driver after its
+    first force and after 5 NPT steps with the half list rebuilt every second
+    force: the oracle's restatement over its half list (bit-exact to the
+    reference's) agrees to 1e-12 of the largest component (the sums over an
+    atom's partners may run in another order).  Leaving out the embedding
+    term (F' = 0) misses by far more than that bound."""
+    F = load_golden("atomstress_tric_1024")
+    pot, S = _oracle_for(oracle, F)
+    S.call("linkup")
+    assert S.call("md_init") == 0
+    scale = np.max(np.abs(F["stress0"]))
+    s0 = S.atomic_stress()
+    assert np.max(np.abs(s0 - F["stress0"])) < 1e-12 * scale
+    df = S.array("dafrho")
+    S.set_array("dafrho", np.zeros_like(df))
+    assert np.max(np.abs(S.atomic_stress() - F["stress0"])) > 1e-3 * scale
+    S.set_array("dafrho", df)
+    for _ in range(int(F["run_nsteps"])):
+        assert S.call("md_step") == 0
+    sf = S.atomic_stress()
+    assert np.max(np.abs(sf - F["stress_f"])) < 1e-12 * np.max(np.abs(F["stress_f"]))
