@@ -219,11 +219,18 @@ __device__ __forceinline__ float3 sep32(const float4 &pi, const float4 &pj, cons
 // FP64 form: the code is the image shift, the test r < rc in FP64.  FP32
 // form: the code is the packed-shift offset q, the test on the FP32
 // separation against (rc + 1e-4)^2 (pairs past the outermost knot add exact
-// zeros).
+// zeros).  The density pass rides along: each listed partner's phi is added
+// to the atom's FP64 density in list order, from the separation the test
+// used (the one the force pass recomputes), and the atom's embedding
+// (src/dynamics_m.F90:497-579, src/potential_m.F90:148-169) follows its
+// walk: rho, F' into pos4.w (and the FP32 record's w).  Returns this thread's
+// embedding energy.
 template <typename T, int BLOCK>
-__device__ void phase_list(const BatchArgs &A, const CellView &V, const Shifts &SH) {
+__device__ double phase_list(const BatchArgs &A, const PotDev &P, const CellView &V, const Shifts &SH,
+                             bool store_rho) {
   const int nx = V.nx, ny = V.ny, nz = V.nz, nc = V.nc;
   const int rx_ = nx >= 3 ? 1 : 0, ry_ = ny >= 3 ? 1 : 0, rz_ = nz >= 3 ? 1 : 0;
+  double e_emb = 0.0;
   for (int il = threadIdx.x; il < nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
     const int sc = A.subc[a];
@@ -234,6 +241,7 @@ __device__ void phase_list(const BatchArgs &A, const CellView &V, const Shifts &
     else pf = A.p4f[a];
     const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
     int cnt = 0;
+    double acc = 0.0;
     for (int dz = -rz_; dz <= rz_; dz++) {
       int kz = cz + dz, wz = 0;
       if (kz < 0) { kz += nz; wz = 1; } else if (kz >= nz) { kz -= nz; wz = -1; }
@@ -252,23 +260,29 @@ __device__ void phase_list(const BatchArgs &A, const CellView &V, const Shifts &
             int tz = rz_ ? wz : -(int)round(szi - A.s[3 * aj + 2]);
             bool in;
             int code;
+            T r2;
             if constexpr (sizeof(T) == 8) {
               code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
               const double4 pj = A.pos4[aj];
               const double ex = pi.x - pj.x + SH.S[code][0], ey = pi.y - pj.y + SH.S[code][1],
                            ez = pi.z - pj.z + SH.S[code][2];
-              in = ex * ex + ey * ey + ez * ez < A.rc2;
+              r2 = ex * ex + ey * ey + ez * ez;
+              in = r2 < A.rc2;
             } else {
               if (rx_) tx = -dx;
               if (ry_) ty = -dy;
               if (rz_) tz = -dz;
               code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
               const float3 e = sep32(pf, A.p4f[aj], SH, code);
-              in = e.x * e.x + e.y * e.y + e.z * e.z < A.rc2f;
+              r2 = e.x * e.x + e.y * e.y + e.z * e.z;
+              in = r2 < A.rc2f;
             }
             if (in) {
               if (cnt < A.maxnb) V.nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
               cnt++;
+              T r, ri;
+              r_inv(r2, r, ri);
+              acc += (double)mdk_phi<T>(P, r);
             }
           }
         }
@@ -276,7 +290,16 @@ __device__ void phase_list(const BatchArgs &A, const CellView &V, const Shifts &
     }
     A.numn[a] = min(cnt, A.maxnb);
     if (cnt > A.maxnb) atomicMax(&A.flags[0], cnt);
+    if (store_rho) A.rho[a] = acc;
+    double f = 0.0, df = 0.0;
+    if (acc > 1e-40) {
+      mdk_embed_f(P, acc, f, df);
+      e_emb += f;
+    }
+    A.pos4[a].w = df;
+    if constexpr (sizeof(T) == 4) A.p4f[a].w = (float)df;
   }
+  return e_emb;
 }
 
 // separation of the k-th neighbour of local atom il, and the partner's F'
@@ -303,9 +326,9 @@ __device__ __forceinline__ T nb_sep(const BatchArgs &A, const CellView &V, const
   }
 }
 
-// densities and embedding (src/dynamics_m.F90:497-579, src/potential_m.F90:148-169):
-// rho (FP64 sum), F' into pos4.w (and the FP32 record's w); returns this
-// thread's embedding energy
+// densities and embedding over an existing list (the VJP's backward pass;
+// the forward passes fold them into phase_list): rho (FP64 sum), F' into
+// pos4.w (and the FP32 record's w); returns this thread's embedding energy
 template <typename T, int BLOCK>
 __device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellView &V, const Shifts &SH,
                                 bool store_rho) {
@@ -555,9 +578,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(BatchArgs A, PotDev P) {
   while (next_cell(A, &sh_cell, V)) {
     phase_positions<T, BLOCK>(A, V, S);
     __syncthreads();
-    phase_list<T, BLOCK>(A, V, S);
-    __syncthreads();
-    const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
+    const double e_emb = phase_list<T, BLOCK>(A, P, V, S, true);
     __syncthreads();
     double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
     phase_force<T, BLOCK, false>(A, P, V, S, e_pair, w, nullptr);
@@ -598,9 +619,9 @@ __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &
 // Loss and its gradient in one pass per cell:
 //   L_c = (E_c - E_t,c)^2 / N_c^2 + f_weight sum_i |F_i - F_t,i|^2,
 // forward phases, seeds e_c = 2 (E_c - E_t,c) / N_c^2 and G_i, then the
-// densities are recomputed and the adjoint phase runs.
-// VJP=true instead takes the seeds from e_in / f_in and reuses the neighbour
-// list of the last forward launch.
+// adjoint phase on the forward's list and F'.
+// VJP=true instead takes the seeds from e_in / f_in, reuses the neighbour
+// list of the last forward launch and recomputes the densities on it.
 template <typename T, int BLOCK, bool VJP>
 __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
   __shared__ Shifts S;
@@ -622,9 +643,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, Gra
       __syncthreads();
     } else {
       __syncthreads();
-      phase_list<T, BLOCK>(A, V, S);
-      __syncthreads();
-      const double e_emb = phase_density<T, BLOCK>(A, P, V, S, true);
+      const double e_emb = phase_list<T, BLOCK>(A, P, V, S, true);
       __syncthreads();
       double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0}, fsq = 0.0;
       phase_force<T, BLOCK, true>(A, P, V, S, e_pair, w, &fsq);
@@ -635,8 +654,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, Gra
       ec = 2.0 * de * inv_n2;
       if (threadIdx.x == 0) A.out_loss[V.c] = de * de * inv_n2 + A.f_weight * fs;
     }
-    // backward: densities recomputed from the positions (F' into pos4.w)
-    phase_density<T, BLOCK>(A, P, V, S, false);
+    // backward: the VJP recomputes the densities on the last forward's list
+    // (F' into pos4.w); the loss path still holds them from its forward
+    // phases.  The barrier also publishes the force seeds G for the partners.
+    if (VJP) phase_density<T, BLOCK>(A, P, V, S, false);
     __syncthreads();
     double ga[MDK_MAXV], gr[MDK_MAXV], gA[MDK_MAXP], gR[MDK_MAXP];
 #pragma unroll
