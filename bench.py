@@ -107,12 +107,12 @@ def batch_legs(reps):
     be.close()
     na, nf = info["natom"], info["pairs"] / info["natom"]
     # per-atom stream models: evaluation = positions 24 + forces 24 + F' 8 +
-    # the list written once and read by the density and force phases (12 n_f),
-    # plus energy and virial per cell; loss + gradient = positions, force
-    # targets, forces, force seeds written and read (120) + the list written
-    # once and read four times (20 n_f)
-    b_eval = na * (56.0 + 12.0 * nf) + 56.0 * len(cells)
-    b_grad = na * (120.0 + 20.0 * nf) + 56.0 * len(cells)
+    # the list written once (with the densities) and read by the force phase
+    # (8 n_f), plus energy and virial per cell; loss + gradient = positions,
+    # force targets, forces, force seeds written and read (120) + the list
+    # written once and read by the force and adjoint phases (12 n_f)
+    b_eval = na * (56.0 + 8.0 * nf) + 56.0 * len(cells)
+    b_grad = na * (120.0 + 12.0 * nf) + 56.0 * len(cells)
 
     def roof(nbytes, ms):
         gbs = nbytes / (ms * 1e-3) / 1e9

@@ -239,50 +239,87 @@ __device__ double phase_list(const BatchArgs &A, const PotDev &P, const CellView
     float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (sizeof(T) == 8) pi = A.pos4[a];
     else pf = A.p4f[a];
-    const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
     int cnt = 0;
     double acc = 0.0;
-    for (int dz = -rz_; dz <= rz_; dz++) {
-      int kz = cz + dz, wz = 0;
-      if (kz < 0) { kz += nz; wz = 1; } else if (kz >= nz) { kz -= nz; wz = -1; }
-      for (int dy = -ry_; dy <= ry_; dy++) {
-        int ky = cy + dy, wy = 0;
-        if (ky < 0) { ky += ny; wy = 1; } else if (ky >= ny) { ky -= ny; wy = -1; }
-        for (int dx = -rx_; dx <= rx_; dx++) {
-          int kx = cx + dx, wx = 0;
-          if (kx < 0) { kx += nx; wx = 1; } else if (kx >= nx) { kx -= nx; wx = -1; }
-          const int cj = kx + nx * (ky + ny * kz);
-          for (int jl = V.sst[cj]; jl < V.sst[cj + 1]; jl++) {
-            if (jl == il) continue;
-            const size_t aj = (size_t)V.a0 + jl;
-            int tx = rx_ ? wx : -(int)round(sxi - A.s[3 * aj]);
-            int ty = ry_ ? wy : -(int)round(syi - A.s[3 * aj + 1]);
-            int tz = rz_ ? wz : -(int)round(szi - A.s[3 * aj + 2]);
-            bool in;
-            int code;
-            T r2;
-            if constexpr (sizeof(T) == 8) {
-              code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
-              const double4 pj = A.pos4[aj];
-              const double ex = pi.x - pj.x + SH.S[code][0], ey = pi.y - pj.y + SH.S[code][1],
-                           ez = pi.z - pj.z + SH.S[code][2];
-              r2 = ex * ex + ey * ey + ez * ez;
-              in = r2 < A.rc2;
-            } else {
-              if (rx_) tx = -dx;
-              if (ry_) ty = -dy;
-              if (rz_) tz = -dz;
-              code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
-              const float3 e = sep32(pf, A.p4f[aj], SH, code);
-              r2 = e.x * e.x + e.y * e.y + e.z * e.z;
-              in = r2 < A.rc2f;
+    if (sizeof(T) == 4 && rx_ && ry_ && rz_) {
+      // FP32 form, stencil in every dimension (uniform branch): the shift of
+      // a sub-cell's candidates is one code, loaded once per sub-cell
+      const float4 *pc = A.p4f + V.a0;
+      int *nb = V.nbr + il;  // entry k at nb[k * nc]
+      for (int dz = -1; dz <= 1; dz++) {
+        int kz = cz + dz;
+        kz = kz < 0 ? kz + nz : (kz >= nz ? kz - nz : kz);
+        for (int dy = -1; dy <= 1; dy++) {
+          int ky = cy + dy;
+          ky = ky < 0 ? ky + ny : (ky >= ny ? ky - ny : ky);
+          for (int dx = -1; dx <= 1; dx++) {
+            int kx = cx + dx;
+            kx = kx < 0 ? kx + nx : (kx >= nx ? kx - nx : kx);
+            const int cj = kx + nx * (ky + ny * kz);
+            const int code = (1 - dx) + 3 * (1 - dy) + 9 * (1 - dz);
+            const float4 d = SH.Sf[code], l = SH.Sl[code];
+            const int tag = code << NB_CODE_SHIFT;
+            const int j1 = V.sst[cj + 1];
+            for (int jl = V.sst[cj]; jl < j1; jl++) {
+              const float4 pj = pc[jl];
+              const float ex = ((pf.x - pj.x) + d.x) + l.x, ey = ((pf.y - pj.y) + d.y) + l.y,
+                          ez = ((pf.z - pj.z) + d.z) + l.z;
+              const float r2 = ex * ex + ey * ey + ez * ez;
+              if (r2 < A.rc2f && jl != il) {
+                if (cnt < A.maxnb) nb[cnt * nc] = jl | tag;
+                cnt++;
+                float r, ri;
+                r_inv(r2, r, ri);
+                acc += (double)mdk_phi<float>(P, r);
+              }
             }
-            if (in) {
-              if (cnt < A.maxnb) V.nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
-              cnt++;
-              T r, ri;
-              r_inv(r2, r, ri);
-              acc += (double)mdk_phi<T>(P, r);
+          }
+        }
+      }
+    } else {
+      const double sxi = A.s[3 * a], syi = A.s[3 * a + 1], szi = A.s[3 * a + 2];
+      for (int dz = -rz_; dz <= rz_; dz++) {
+        int kz = cz + dz, wz = 0;
+        if (kz < 0) { kz += nz; wz = 1; } else if (kz >= nz) { kz -= nz; wz = -1; }
+        for (int dy = -ry_; dy <= ry_; dy++) {
+          int ky = cy + dy, wy = 0;
+          if (ky < 0) { ky += ny; wy = 1; } else if (ky >= ny) { ky -= ny; wy = -1; }
+          for (int dx = -rx_; dx <= rx_; dx++) {
+            int kx = cx + dx, wx = 0;
+            if (kx < 0) { kx += nx; wx = 1; } else if (kx >= nx) { kx -= nx; wx = -1; }
+            const int cj = kx + nx * (ky + ny * kz);
+            for (int jl = V.sst[cj]; jl < V.sst[cj + 1]; jl++) {
+              if (jl == il) continue;
+              const size_t aj = (size_t)V.a0 + jl;
+              int tx = rx_ ? wx : -(int)round(sxi - A.s[3 * aj]);
+              int ty = ry_ ? wy : -(int)round(syi - A.s[3 * aj + 1]);
+              int tz = rz_ ? wz : -(int)round(szi - A.s[3 * aj + 2]);
+              bool in;
+              int code;
+              T r2;
+              if constexpr (sizeof(T) == 8) {
+                code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+                const double4 pj = A.pos4[aj];
+                const double ex = pi.x - pj.x + SH.S[code][0], ey = pi.y - pj.y + SH.S[code][1],
+                             ez = pi.z - pj.z + SH.S[code][2];
+                r2 = ex * ex + ey * ey + ez * ez;
+                in = r2 < A.rc2;
+              } else {
+                if (rx_) tx = -dx;
+                if (ry_) ty = -dy;
+                if (rz_) tz = -dz;
+                code = (tx + 1) + 3 * (ty + 1) + 9 * (tz + 1);
+                const float3 e = sep32(pf, A.p4f[aj], SH, code);
+                r2 = e.x * e.x + e.y * e.y + e.z * e.z;
+                in = r2 < A.rc2f;
+              }
+              if (in) {
+                if (cnt < A.maxnb) V.nbr[(size_t)cnt * nc + il] = jl | (code << NB_CODE_SHIFT);
+                cnt++;
+                T r, ri;
+                r_inv(r2, r, ri);
+                acc += (double)mdk_phi<T>(P, r);
+              }
             }
           }
         }
@@ -568,9 +605,12 @@ __device__ double store_cell(const BatchArgs &A, const CellView &V, double e, co
   return etot;
 }
 
-// Forward: energies, forces and virials of every cell.
+// Forward: energies, forces and virials of every cell.  The kernel is bound
+// by latency: six waves per SIMD (80 VGPRs, some spilled to scratch) run the
+// 4096-cell batch in 2.1 ms against 3.7 ms at the three that 131 VGPRs allow.
 template <typename T, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_batch_eval(BatchArgs A, PotDev P) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_batch_eval(BatchArgs A, PotDev P) {
   __shared__ Shifts S;
   __shared__ double red[BLOCK / 64][7];
   __shared__ int sh_cell;
@@ -622,8 +662,11 @@ __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &
 // adjoint phase on the forward's list and F'.
 // VJP=true instead takes the seeds from e_in / f_in, reuses the neighbour
 // list of the last forward launch and recomputes the densities on it.
+// Four waves per SIMD (128 VGPRs and scratch spills) against the two that
+// its 225 VGPRs allow: 3.7 against 5.4 ms on the 4096-cell batch.
 template <typename T, int BLOCK, bool VJP>
-__global__ void __launch_bounds__(BLOCK) k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
   __shared__ Shifts S;
   __shared__ double red[BLOCK / 64][7];
   __shared__ double red1[BLOCK / 64];
