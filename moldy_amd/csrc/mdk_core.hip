@@ -139,6 +139,7 @@ struct mdk_ctx {
   long rebuilds;
   bool list_valid;
   bool half_valid;         // numn_ref/nlist_ref hold a reference half list
+  bool x3_live;            // X3..Z3 may be non-zero (uploaded, or the predictor-corrector ran)
   // reference link-cell chains (src/linkcell_m.F90:122-274) on the host in
   // reference atom order, 1-based: L0 per cell, LINK, IC; chains_exact is
   // false after relinks inside mdk_md_steps, which update only the device ic
@@ -395,10 +396,15 @@ __global__ void k_iota(int *a, int n) {
   if (i < n) a[i] = i;
 }
 // new slot k takes old slot members[k]
-__global__ void k_permute(double *dst, const double *src, const int *members, int n, int nf) {
+// the per-atom fields a re-sort moves (the others are rewritten before any read)
+struct FieldSet {
+  int f[F_N];
+  int cnt;
+};
+__global__ void k_permute(double *dst, const double *src, const int *members, int n, FieldSet fs) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)n * nf) return;
-  int f = (int)(t / n), k = (int)(t % n);
+  if (t >= (size_t)n * fs.cnt) return;
+  const int f = fs.f[t / n], k = (int)(t % n);
   dst[(size_t)f * n + k] = src[(size_t)f * n + members[k]];
 }
 __global__ void k_permute_int(int *zdst, const int *zsrc, int *odst, const int *osrc, int *slot_of,
@@ -2238,7 +2244,24 @@ static int plan_list_blocks(mdk_ctx *c, const int g[3]) {
 
 // defer: one attempt whose flags the caller checks after queueing the next
 // kernels (confirm_list); otherwise every attempt is checked here.
-static int rebuild_full(mdk_ctx *c, bool defer = false) {
+// Fields a re-sort must carry.  Inside force (force_follows) the density and
+// force passes rewrite RHO and FX..FZ of every atom before anything reads
+// them; X3..Z3 are zero in both buffers until the predictor-corrector or an
+// upload sets them (both buffers are cleared at create, and the scratch uses
+// of the alternate buffer stay in its X0..Z0 columns).
+static FieldSet resort_fields(const mdk_ctx *c, bool force_follows) {
+  FieldSet fs;
+  fs.cnt = 0;
+  for (int f = 0; f < F_N; f++) {
+    const bool out = f == F_FX || f == F_FY || f == F_FZ || f == F_RHO;
+    const bool third = f == F_X3 || f == F_Y3 || f == F_Z3;
+    if ((out && force_follows) || (third && !c->x3_live)) continue;
+    fs.f[fs.cnt++] = f;
+  }
+  return fs;
+}
+
+static int rebuild_full(mdk_ctx *c, bool defer = false, bool force_follows = false) {
   const int n = c->n;
   c->list_valid = false;
   TimedEv te;
@@ -2271,7 +2294,8 @@ static int rebuild_full(mdk_ctx *c, bool defer = false) {
                                                          n, g[0], g[1], g[2], c->cell_of);
     RCK(bin_atoms(c, c->cell_of, m, nullptr));
     // physical re-sort of every per-atom array into cell order
-    k_permute<<<nblocks((long)n * F_N), MDK_BLOCK, 0, c->stream>>>(c->atoms_alt, c->atoms, c->members, n, F_N);
+    const FieldSet fs = resort_fields(c, force_follows);
+    k_permute<<<nblocks((long)n * fs.cnt), MDK_BLOCK, 0, c->stream>>>(c->atoms_alt, c->atoms, c->members, n, fs);
     k_permute_int<<<nblocks(n), MDK_BLOCK, 0, c->stream>>>(c->zat_alt, c->zat, c->orig_alt, c->orig,
                                                           c->slot_of, c->members, n);
     std::swap(c->atoms, c->atoms_alt);
@@ -2519,7 +2543,7 @@ static int confirm_list(mdk_ctx *c) {
   HIPCK(hipStreamSynchronize(c->stream));
   c->rebuilds--;
   c->b0_lag_step = -1;
-  RCK(rebuild_full(c, false));
+  RCK(rebuild_full(c, false, true));
   return launch_density(c, 1);
 }
 
@@ -2530,7 +2554,7 @@ static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
   bool ref_due = (c->ntc == 1);
   if (c->ntc == c->p.ntcm) c->ntc = 0;
   bool due = !c->list_valid || skin_due || (ref_due && c->p.rebuild == 0);
-  if (due) RCK(rebuild_full(c, true));
+  if (due) RCK(rebuild_full(c, true, true));
   RCK(launch_density(c, 1));
   RCK(confirm_list(c));
   return launch_force(c, fin_mode);
@@ -2669,6 +2693,7 @@ static int skin_check(mdk_ctx *c, bool *due) {
 // step's drift, except when relink follows this step (it must see the
 // positions of this step).
 static int launch_predic(mdk_ctx *c) {
+  c->x3_live = true;
   TimedEv te;
   tbegin(c, TC_DRIFT, te);
   const int nxt = 1 - c->cur;
@@ -2683,6 +2708,7 @@ static int launch_predic(mdk_ctx *c) {
 }
 
 static int launch_correc(mdk_ctx *c) {
+  c->x3_live = true;
   TimedEv te;
   tbegin(c, TC_KICK, te);
   k_correc<<<c->nblk, MDK_BLOCK, 0, c->stream>>>(c->atoms, c->zat, c->n, c->box + c->cur, c->p.ivol,
@@ -2824,6 +2850,9 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
     goto fail;
   }
   hipMemsetAsync(c->atoms, 0, sizeof(double) * (size_t)F_N * n, c->stream);
+  // both buffers start at zero: a re-sort leaves X3..Z3 unmoved while they
+  // are zero in both (x3_live false; scratch uses of atoms_alt stay in X0..Z0)
+  hipMemsetAsync(c->atoms_alt, 0, sizeof(double) * (size_t)F_N * n, c->stream);
   hipMemsetAsync(c->zat, 0, sizeof(int) * n, c->stream);
   hipMemsetAsync(c->box, 0, sizeof(BoxState) * 2, c->stream);
   hipMemsetAsync(c->ic, 0, sizeof(int) * n, c->stream);
@@ -3114,6 +3143,7 @@ extern "C" int mdk_upload(void *h, int field, const double *data) {
     c->b0_lag_step = -1;
   }
   if (field <= MDK_Z0 || field == MDK_DAFRHO) c->pos4_valid = c->p4f_valid = false;
+  if (field == MDK_X3 || field == MDK_Y3 || field == MDK_Z3) c->x3_live = true;
   if (field == MDK_MASS) {
     // the per-pair test m_i + m_j > 0.1 (REAL(4) literal) can only fail with a mass <= 0.05
     bool any = false;
