@@ -747,8 +747,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_full_list(
 // positions relative to the centre of the block's first cell, in FP32 (the
 // 1e-6 A rounding is immaterial against the skin) -- instead of gathering them
 // from global memory.  Each lane then walks the 27 staged cells around its own
-// atom's cell (about 155 candidates instead of all 368 staged ones); a
-// partner's code is the offset of its cell, uniform over the walk of one cell.
+// atom's cell (about 155 candidates instead of all 368 staged ones), row by
+// row; a partner's code is the offset of its cell.
 // Fully periodic boxes with >= 4 cells per dimension (no staged cell occurs
 // twice); a block whose neighbourhood exceeds LIST_CAP atoms uses the
 // global-gather loop.  blocks[] lists the Morton key / 8 of every block that
@@ -814,22 +814,26 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
                 pz = (float)(pd.z + (b[2] * ex + b[5] * ey + b[8] * ez));
     const int h0 = ax + 4 * ay + 16 * az;  // staged cell at offset (-1,-1,-1) from the atom's
     int m = 0;
+    // the three cells of a row along x are consecutive staged cells, so one
+    // loop walks their candidates (about 17, instead of three loops of about
+    // 6 whose lengths differ from lane to lane); a partner's cell offset, and
+    // so its code, follows from where it lies in the row
     for (int dz = 0; dz < 3; dz++)
-      for (int dy = 0; dy < 3; dy++)
-        for (int dx = 0; dx < 3; dx++) {
-          const int h = h0 + dx + 4 * dy + 16 * dz;
-          const int lo = hoff[h], hi = lo + hcnt[h];
-          const int tag = (dx + 3 * dy + 9 * dz) << NBR_CODE_SHIFT;  // the cell offset's code
-          for (int c = lo; c < hi; c++) {
-            const float4 q = cpos[c];
-            const float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
-            const int j = __float_as_int(q.w);
-            if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
-              if (m < maxn) nbr[nbr_at(m, i, n)] = j | tag;
-              m++;
-            }
+      for (int dy = 0; dy < 3; dy++) {
+        const int h = h0 + 4 * dy + 16 * dz;  // the row's cell at dx = 0
+        const int lo = hoff[h], m1 = hoff[h + 1], m2 = hoff[h + 2], hi = m2 + hcnt[h + 2];
+        const int tagb = 3 * dy + 9 * dz;
+        for (int c = lo; c < hi; c++) {
+          const float4 q = cpos[c];
+          const float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
+          const int j = __float_as_int(q.w);
+          if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
+            const int code = tagb + (c >= m1 ? 1 : 0) + (c >= m2 ? 1 : 0);
+            if (m < maxn) nbr[nbr_at(m, i, n)] = j | (code << NBR_CODE_SHIFT);
+            m++;
           }
         }
+      }
     numn[i] = min(m, maxn);
     if (m > maxn) atomicMax(&flags[0], m);
     for (int k = m; k < ((m + 3) & ~3) && k < maxn; k++)  // pad the last quad with null entries
