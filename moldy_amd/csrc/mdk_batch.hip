@@ -73,6 +73,8 @@ struct BatchArgs {
   int *flags;            // [0] max neighbour count seen when over capacity
   double rc2;
   float rc2f;            // FP32 form's list radius squared, 1e-4 A past the outermost knot
+  const float *fetab;    // FP32 form: V, V', phi' interval table of the knots (fe_ok)
+  int fe_ok;
   // adjoint
   const double *e_in;    // per cell: dL/dE_c (vector-Jacobian product) or target E_c (loss)
   const double *f_in;    // 3 per caller atom: dL/dF_i (product) or target F_i (loss)
@@ -397,6 +399,16 @@ __device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellV
   return e_emb;
 }
 
+// the FP32 interval table (bins, then interval rows) into LDS
+__device__ __forceinline__ void load_fe_table(const BatchArgs &A, float4 *fbin, float *fiv) {
+  if (!A.fe_ok) return;
+  const float4 *src = (const float4 *)A.fetab;
+  for (int t = threadIdx.x; t < MDK_FE_NB + 2 * MDK_FE_NI; t += blockDim.x) {
+    if (t < MDK_FE_NB) fbin[t] = src[t];
+    else ((float4 *)fiv)[t - MDK_FE_NB] = src[t];
+  }
+}
+
 // forces and virial (src/dynamics_m.F90:235-346).  With LOSS, also the force
 // seed dL/dF_i = 2 f_weight (F_i - F_t,i) into gvec and this thread's share of
 // sum |F_i - F_t,i|^2 in *fsq.  FP32 form: each atom's force, pair energy
@@ -404,7 +416,7 @@ __device__ double phase_density(const BatchArgs &A, const PotDev &P, const CellV
 // thread's FP64 sums; FP64 form: every term goes to the FP64 sums.
 template <typename T, int BLOCK, bool LOSS>
 __device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView &V, const Shifts &SH,
-                            double &e_pair, double *w, double *fsq) {
+                            const float4 *fbin, const float *fiv, double &e_pair, double *w, double *fsq) {
   for (int il = threadIdx.x; il < V.nc; il += BLOCK) {
     const size_t a = (size_t)V.a0 + il;
     double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -440,9 +452,13 @@ __device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView 
         int jl;
         const float dfj = nb_sep<float>(A, V, SH, pi, pf, il, k, dx, dy, dz, jl);
         r_inv(dx * dx + dy * dy + dz * dz, r, rr);
-        float vv, dv;
-        mdk_vee<float>(P, r, vv, dv);
-        const float dph = mdk_dphi<float>(P, r);
+        float vv, dv, dph;
+        if (A.fe_ok) {  // uniform: the knots admit the interval table
+          mdk_fe_terms(P, fbin, fiv, r, vv, dv, dph);
+        } else {
+          mdk_vee<float>(P, r, vv, dv);
+          dph = mdk_dphi<float>(P, r);
+        }
         const float fp = -dv * rr + (-1.f * (dph * dfi + dph * dfj) * rr);
         const float gx = dx * fp, gy = dy * fp, gz = dz * fp;
         se += vv;
@@ -614,6 +630,9 @@ k_batch_eval(BatchArgs A, PotDev P) {
   __shared__ Shifts S;
   __shared__ double red[BLOCK / 64][7];
   __shared__ int sh_cell;
+  __shared__ float4 fbin[MDK_FE_NB];
+  __shared__ __align__(16) float fiv[8 * MDK_FE_NI];
+  load_fe_table(A, fbin, fiv);  // ordered before its use by next_cell's barriers
   CellView V;
   while (next_cell(A, &sh_cell, V)) {
     phase_positions<T, BLOCK>(A, V, S);
@@ -621,7 +640,7 @@ k_batch_eval(BatchArgs A, PotDev P) {
     const double e_emb = phase_list<T, BLOCK>(A, P, V, S, true);
     __syncthreads();
     double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0};
-    phase_force<T, BLOCK, false>(A, P, V, S, e_pair, w, nullptr);
+    phase_force<T, BLOCK, false>(A, P, V, S, fbin, fiv, e_pair, w, nullptr);
     store_cell<BLOCK>(A, V, e_emb + e_pair, w, red);
   }
 }
@@ -671,6 +690,9 @@ k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
   __shared__ double red[BLOCK / 64][7];
   __shared__ double red1[BLOCK / 64];
   __shared__ int sh_cell;
+  __shared__ float4 fbin[MDK_FE_NB];
+  __shared__ __align__(16) float fiv[8 * MDK_FE_NI];
+  if (!VJP) load_fe_table(A, fbin, fiv);  // ordered before its use by next_cell's barriers
   CellView V;
   while (next_cell(A, &sh_cell, V)) {
     phase_positions<T, BLOCK>(A, V, S);
@@ -689,7 +711,7 @@ k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
       const double e_emb = phase_list<T, BLOCK>(A, P, V, S, true);
       __syncthreads();
       double e_pair = 0.0, w[6] = {0, 0, 0, 0, 0, 0}, fsq = 0.0;
-      phase_force<T, BLOCK, true>(A, P, V, S, e_pair, w, &fsq);
+      phase_force<T, BLOCK, true>(A, P, V, S, fbin, fiv, e_pair, w, &fsq);
       const double fs = block_sum<BLOCK>(fsq, red1);
       const double ecell = store_cell<BLOCK>(A, V, e_emb + e_pair, w, red);
       const double inv_n2 = 1.0 / ((double)V.nc * (double)V.nc);
@@ -776,6 +798,8 @@ struct mdk_batch {
   double *box, *s, *rho, *out_e, *out_w, *out_f;
   double4 *pos4;
   float4 *p4f;
+  float *fetab;  // FP32 interval table of the current knots (when they admit one)
+  bool fe_ok;
   // adjoint buffers: targets, product seeds, force seeds (sorted order),
   // per-cell gradient and loss partials, reduced results
   double *e_tgt, *f_tgt, *e_seed, *f_seed, *gvec, *out_g, *out_loss, *res;
@@ -823,6 +847,8 @@ BatchArgs args_of(mdk_batch *b) {
   A.numn = b->numn; A.rho = b->rho; A.out_e = b->out_e; A.out_w = b->out_w; A.out_f = b->out_f;
   A.counter = b->counter; A.flags = b->flags; A.rc2 = b->rc * b->rc;
   A.rc2f = (float)((b->rc + 1e-4) * (b->rc + 1e-4));
+  A.fetab = b->fetab;
+  A.fe_ok = (b->fe_ok && b->precision == MDK_PREC_FP32) ? 1 : 0;
   A.ngrad = 2 * b->knots.nv + 2 * b->knots.np;
   A.e_in = b->e_tgt; A.f_in = b->f_tgt; A.gvec = b->gvec; A.out_g = b->out_g;
   A.out_loss = b->out_loss; A.f_weight = 0.1 / (double)b->natom;
@@ -917,7 +943,7 @@ extern "C" int mdk_batch_destroy(void *h) {
   drop_graph(b);
   void *p[] = {b->order, b->cell_off, b->nsub, b->sub_off, b->sub_start, b->subc, b->perm,
                b->numn, b->nbr, b->counter, b->flags, b->box, b->s, b->rho, b->out_e, b->out_w,
-               b->out_f, b->pos4, b->p4f, b->e_tgt, b->f_tgt, b->e_seed, b->f_seed, b->gvec, b->out_g,
+               b->out_f, b->pos4, b->p4f, b->fetab, b->e_tgt, b->f_tgt, b->e_seed, b->f_seed, b->gvec, b->out_g,
                b->out_loss, b->res};
   for (void *q : p)
     if (q) hipFree(q);
@@ -1041,6 +1067,7 @@ extern "C" int mdk_batch_create(void **handle, int ncell, const int *natoms, con
   BALLOC(b->s, sizeof(double) * 3 * (size_t)natom);
   BALLOC(b->pos4, sizeof(double4) * natom);
   BALLOC(b->p4f, sizeof(float4) * natom);
+  BALLOC(b->fetab, sizeof(float) * MDK_FE_TAB);
   BALLOC(b->nbr, sizeof(int) * (size_t)natom * b->maxnb);
   BALLOC(b->numn, sizeof(int) * natom);
   BALLOC(b->rho, sizeof(double) * natom);
@@ -1091,6 +1118,16 @@ extern "C" int mdk_batch_set_potential(void *h, int nv, const double *av, const 
   hostpot::set_up(k);
   const double rint = hostpot::fill_potdev(b->pot, k);
   hostpot::grad_consts(k, b->gd);
+  // the FP32 form's interval table of V, V', phi' (knot sums where the knots
+  // do not admit one); launches in flight still read the previous table
+  float tab[MDK_FE_TAB];
+  b->fe_ok = nv == MDK_FE_NV && np == MDK_FE_NP && hostpot::build_fe_table(k, tab, b->pot.fe_binh, b->pot.fe_binc);
+  if (b->fe_ok) {
+    if (hipStreamSynchronize(b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->fetab, tab, sizeof(tab), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return berr(MDK_ERR_HIP, "interval table upload failed");
+  }
   if (rint > 4.5) return berr(MDK_ERR_BAD_ARG, "knots beyond 4.5 A are not supported by the batch grid");
   if (2.0 * rint >= b->min_width)
     return berr(MDK_ERR_BAD_ARG, "a cell is narrower (%.3f A) than twice the interaction range", b->min_width);
