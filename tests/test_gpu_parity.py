@@ -145,6 +145,28 @@ def test_gpu_relink_chains_and_half_list_match_reference(oracle):
     eng.close()
 
 
+def test_gpu_resort_keeps_what_is_read_again():
+    """A list rebuild outside force (rhoset on moved positions) re-sorts every
+    field that can still be read: the forces of the last force evaluation come
+    back unchanged in reference order, and so do third derivatives that were
+    never set (zero) after MD steps with velocity Verlet."""
+    F = load_golden("npt_tric_1024")
+    say("re-sort outside force")
+    eng = make_engine(F)
+    eng.linkup()
+    eng.md_init()
+    eng.md_steps(12)  # several rebuilds inside force
+    for nm in ("x3", "y3", "z3"):
+        assert not np.any(eng.array(nm)), nm
+    f = np.stack([eng.array(nm) for nm in ("fx", "fy", "fz")], axis=1)
+    x = eng.array("x0")
+    eng.upload("x0", x + 1e-3)  # moved positions: the next pass rebuilds the list
+    eng.call("rhoset")
+    np.testing.assert_array_equal(np.stack([eng.array(nm) for nm in ("fx", "fy", "fz")], axis=1), f)
+    np.testing.assert_array_equal(eng.array("x0"), x + 1e-3)
+    eng.close()
+
+
 def test_gpu_atomic_stress_matches_reference():
     """mdk_atomic_stress after the first force and its half list: the
     reference's per-atom diagonal stress (atomstress,
