@@ -1724,6 +1724,69 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_density_f32(double *atoms, double
   density_finish(atoms, pos4, p4f, n, i, acc, P, do_embed, part);
 }
 
+// The lean path's density pass on a new list: k_density_f32's arithmetic (so
+// the densities are those of every later pass on this list), and each atom's
+// list reordered in place as k_density_part does -- partners within rc first,
+// in list order, then the others (DENS_OBUF of them held in LDS, more stay in
+// place) -- so that the force pass stops after the in-range part.
+__global__ void __launch_bounds__(MDK_BLOCK) k_density_part_f32(double *atoms, float4 *p4f, int n, int *nbr,
+                                                                const int *__restrict__ numn, const BoxState *bs,
+                                                                PackGeo pg, PotDev P, int do_embed, double *part,
+                                                                int *nact) {
+  __shared__ double S[32][3];
+  __shared__ float4 Sf[32], Sl[32];
+  __shared__ int obuf[DENS_OBUF][MDK_BLOCK];
+  shift_table(bs->b0, pg, S);
+  __syncthreads();
+  shift_table_f32(S, Sf, Sl);
+  __syncthreads();
+  const int t = threadIdx.x, i = xcd_tile() * blockDim.x + t;
+  const float rc = (float)P.r_int + 1e-4f, rc2 = rc * rc;
+  double acc = 0.0;
+  if (i < n) {
+    const float4 pi = p4f[i];
+    const int nn = numn[i];
+    int m = 0, no = 0, last = 0;  // entries written back, entries held, end of the last in-range one
+    int4 qd = make_int4(0, 0, 0, 0);
+    for (int q = 0; 4 * q < nn; q++) {
+      const int4 e4 = *(const int4 *)(nbr + (((size_t)q * n + i) << 2));
+      const int e[4] = {e4.x, e4.y, e4.z, e4.w};
+      float4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = p4f[(unsigned)e[u] & NBR_IDX_MASK];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        float rx, ry, rz;
+        const float r2 = pair_r2_f32(pi, pj[u], Sf, Sl, (unsigned)e[u], rx, ry, rz);
+        if (r2 < rc2) {
+          float r, ri;
+          r_and_inv(r2, r, ri);
+          acc += (double)mdk_pair_phi<float, POT_FE>(P, r);
+        }
+        if (4 * q + u < nn) {
+          const bool in = r2 < rc2;
+          if (in || no == DENS_OBUF) {
+            quad_put(nbr, i, n, e[u], m, qd);
+            if (in) last = m;
+          } else {
+            obuf[no++][t] = e[u];
+          }
+        }
+      }
+    }
+    for (int u = 0; u < no; u++) quad_put(nbr, i, n, obuf[u][t], m, qd);
+    if (m & 3) {  // the last quad's null entries
+      const int nul = i | (NBR_NULL_CODE << NBR_CODE_SHIFT), sl = m & 3;
+      qd.y = sl <= 1 ? nul : qd.y;
+      qd.z = sl <= 2 ? nul : qd.z;
+      qd.w = nul;
+      *(int4 *)(nbr + nbr_at(m & ~3, i, n)) = qd;
+    }
+    nact[i] = (last + 3) & ~3;
+  }
+  density_finish(atoms, nullptr, p4f, n, i, acc, P, do_embed, part);
+}
+
 // Force pass (src/dynamics_m.F90:235-346): V, V' and phi' from the FP32 interval table, fp = -(V' + phi'(F'_i +
 // F'_j))/r, F_i = sum r fp.  Each atom's force and the lane's pair energy and
 // virial W = 1/2 sum r r^T fp are FP32 sums over the atom's few tens of terms;
@@ -2396,16 +2459,16 @@ static double *kin_part(mdk_ctx *c) { return c->part_f + (size_t)PART_KIN * c->n
 
 static void lean_density(mdk_ctx *c, int do_embed) {
   double *pe = c->part_f + (size_t)PART_EMB * block_count(c);
+  // F' goes to the FP32 records only: the FP64 copy is not kept on this path
   if (c->part_due)
-    k_density_part<float, POT_FE, true><<<block_count(c), MDK_BLOCK, 0, c->stream>>>(
-        c->atoms, c->pos4, c->p4f, c->n, c->nbr, c->numn, c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
-        c->nact);
-  else {  // F' goes to the FP32 records only: the FP64 copy is not kept on this path
+    k_density_part_f32<<<block_count(c), MDK_BLOCK, 0, c->stream>>>(c->atoms, c->p4f, c->n, c->nbr, c->numn,
+                                                                    c->box + c->cur, pack_geo(c), c->pot, do_embed,
+                                                                    pe, c->nact);
+  else
     k_density_f32<<<block_count(c), MDK_BLOCK, 0, c->stream>>>(c->atoms, nullptr, c->p4f, c->n, c->nbr, c->numn,
                                                                c->box + c->cur, pack_geo(c), c->pot, do_embed, pe,
                                                                c->nact);
-    c->pos4_valid = false;
-  }
+  c->pos4_valid = false;
   c->part_due = false;
 }
 static void lean_force(mdk_ctx *c, hipEvent_t ea, hipEvent_t eb) {
@@ -2493,7 +2556,7 @@ static int ensure_pos4(mdk_ctx *c) {
 }
 
 static int launch_density(mdk_ctx *c, int do_embed) {
-  if (!(lean_path(c) && !c->part_due && c->p4f_valid)) RCK(ensure_pos4(c));
+  if (!(lean_path(c) && c->p4f_valid)) RCK(ensure_pos4(c));  // the lean passes read only the FP32 records
   TimedEv te;
   tbegin(c, TC_DENSITY, te);
   if (lean_path(c)) {
