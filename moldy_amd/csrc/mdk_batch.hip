@@ -497,7 +497,7 @@ __device__ void phase_force(const BatchArgs &A, const PotDev &P, const CellView 
 // pass over the list suffices.  F' comes from pos4.w (FP32 form: the
 // partners' from the FP32 records), recomputed by the
 // caller's density phase; G from gvec.
-template <typename T, int BLOCK>
+template <typename T, int BLOCK, int NV, int NP>
 __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev &D,
                               const CellView &V, const Shifts &SH, double ec, double *ga,
                               double *gr, double *gA, double *gR) {
@@ -510,9 +510,9 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
     const double gix = A.gvec[3 * a], giy = A.gvec[3 * a + 1], giz = A.gvec[3 * a + 2];
     const T fpi = (T)pi.w;
     const int nn = A.numn[a];
-    double q = 0.0, drA[MDK_MAXP], drR[MDK_MAXP];
+    double q = 0.0, drA[NP], drR[NP];
 #pragma unroll
-    for (int m = 0; m < MDK_MAXP; m++) { drA[m] = 0.0; drR[m] = 0.0; }
+    for (int m = 0; m < NP; m++) { drA[m] = 0.0; drR[m] = 0.0; }
     for (int k = 0; k < nn; k++) {
       T dx, dy, dz, r, ri;
       int jl;
@@ -528,7 +528,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
       // pair parameters
       if (r > Q.bc2()) {
 #pragma unroll
-        for (int m = 0; m < MDK_MAXV; m++) {
+        for (int m = 0; m < NV; m++) {
           if (m < P.nv) {
             const T d = Q.rv(m) - r;
             if (d > T(0)) {
@@ -550,7 +550,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
         const double h01 = -2 * t3 + 3 * t2, h11 = (t3 - t2) * h;
         const double d01 = (-6 * t2 + 6 * t) / h, d11 = 3 * t2 - 2 * t;
 #pragma unroll
-        for (int m = 0; m < MDK_MAXV; m++) {
+        for (int m = 0; m < NV; m++) {
           if (m < P.nv) {
             const double sa = h01 * D.xa[m] + h11 * D.ya[m], sr = h01 * D.xr[m] + h11 * D.yr[m];
             const double pa = d01 * D.xa[m] + d11 * D.ya[m], pr = d01 * D.xr[m] + d11 * D.yr[m];
@@ -562,7 +562,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
       // density parameters
       if (r < Q.pcr()) {
 #pragma unroll
-        for (int m = 0; m < MDK_MAXP; m++) {
+        for (int m = 0; m < NP; m++) {
           if (m < P.np) {
             gA[m] += (double)cp * D.cpa[m];
             gR[m] += (double)cp * D.cpr[m];
@@ -572,7 +572,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
         }
       } else {
 #pragma unroll
-        for (int m = 0; m < MDK_MAXP; m++) {
+        for (int m = 0; m < NP; m++) {
           if (m < P.np) {
             const T d = Q.dp(m, r);
             if (d > T(0)) {
@@ -588,7 +588,7 @@ __device__ void phase_adjoint(const BatchArgs &A, const PotDev &P, const GradDev
     }
     const double lam = -2.0 * pi.w * pi.w * pi.w * q;  // F''(rho_i) Q_i
 #pragma unroll
-    for (int m = 0; m < MDK_MAXP; m++) {
+    for (int m = 0; m < NP; m++) {
       gA[m] -= lam * drA[m];
       gR[m] -= lam * drR[m];
     }
@@ -647,12 +647,12 @@ k_batch_eval(BatchArgs A, PotDev P) {
 
 // per-cell gradient partials (fixed order) into out_g[c*NGRAD_SLOTS + slot],
 // slots packed as a_0..a_{nv-1}, r_0.., A_0..A_{np-1}, R_0..
-template <int BLOCK>
+template <int BLOCK, int NV, int NP>
 __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &V, const double *ga,
                            const double *gr, const double *gA, const double *gR, double *red) {
   double *out = A.out_g + (size_t)V.c * NGRAD_SLOTS;
 #pragma unroll
-  for (int m = 0; m < MDK_MAXV; m++) {
+  for (int m = 0; m < NV; m++) {
     if (m < P.nv) {
       const double sa = block_sum<BLOCK>(ga[m], red);
       const double sr = block_sum<BLOCK>(gr[m], red);
@@ -663,7 +663,7 @@ __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &
     }
   }
 #pragma unroll
-  for (int m = 0; m < MDK_MAXP; m++) {
+  for (int m = 0; m < NP; m++) {
     if (m < P.np) {
       const double sa = block_sum<BLOCK>(gA[m], red);
       const double sr = block_sum<BLOCK>(gR[m], red);
@@ -683,7 +683,9 @@ __device__ void store_grad(const BatchArgs &A, const PotDev &P, const CellView &
 // list of the last forward launch and recomputes the densities on it.
 // Four waves per SIMD (128 VGPRs and scratch spills) against the two that
 // its 225 VGPRs allow: 3.7 against 5.4 ms on the 4096-cell batch.
-template <typename T, int BLOCK, bool VJP>
+// NV, NP: pair and density knot counts the per-thread gradient sums hold
+// (6 and 2 for the Fe potential; the maxima for any other knot set).
+template <typename T, int BLOCK, bool VJP, int NV, int NP>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
   __shared__ Shifts S;
@@ -724,13 +726,13 @@ k_batch_grad(BatchArgs A, PotDev P, GradDev D) {
     // phases.  The barrier also publishes the force seeds G for the partners.
     if (VJP) phase_density<T, BLOCK>(A, P, V, S, false);
     __syncthreads();
-    double ga[MDK_MAXV], gr[MDK_MAXV], gA[MDK_MAXP], gR[MDK_MAXP];
+    double ga[NV], gr[NV], gA[NP], gR[NP];
 #pragma unroll
-    for (int m = 0; m < MDK_MAXV; m++) { ga[m] = 0.0; gr[m] = 0.0; }
+    for (int m = 0; m < NV; m++) { ga[m] = 0.0; gr[m] = 0.0; }
 #pragma unroll
-    for (int m = 0; m < MDK_MAXP; m++) { gA[m] = 0.0; gR[m] = 0.0; }
-    phase_adjoint<T, BLOCK>(A, P, D, V, S, ec, ga, gr, gA, gR);
-    store_grad<BLOCK>(A, P, V, ga, gr, gA, gR, red1);
+    for (int m = 0; m < NP; m++) { gA[m] = 0.0; gR[m] = 0.0; }
+    phase_adjoint<T, BLOCK, NV, NP>(A, P, D, V, S, ec, ga, gr, gA, gR);
+    store_grad<BLOCK, NV, NP>(A, P, V, ga, gr, gA, gR, red1);
   }
 }
 
@@ -831,11 +833,20 @@ void launch_eval(int block, int grid, hipStream_t st, const BatchArgs &A, const 
 template <typename T, bool VJP>
 void launch_grad(int block, int grid, hipStream_t st, const BatchArgs &A, const PotDev &P,
                  const GradDev &D) {
+  if (P.nv == MDK_FE_NV && P.np == MDK_FE_NP) {
+    switch (block) {
+      case 64: k_batch_grad<T, 64, VJP, MDK_FE_NV, MDK_FE_NP><<<grid, 64, 0, st>>>(A, P, D); break;
+      case 128: k_batch_grad<T, 128, VJP, MDK_FE_NV, MDK_FE_NP><<<grid, 128, 0, st>>>(A, P, D); break;
+      case 512: k_batch_grad<T, 512, VJP, MDK_FE_NV, MDK_FE_NP><<<grid, 512, 0, st>>>(A, P, D); break;
+      default: k_batch_grad<T, 256, VJP, MDK_FE_NV, MDK_FE_NP><<<grid, 256, 0, st>>>(A, P, D); break;
+    }
+    return;
+  }
   switch (block) {
-    case 64: k_batch_grad<T, 64, VJP><<<grid, 64, 0, st>>>(A, P, D); break;
-    case 128: k_batch_grad<T, 128, VJP><<<grid, 128, 0, st>>>(A, P, D); break;
-    case 512: k_batch_grad<T, 512, VJP><<<grid, 512, 0, st>>>(A, P, D); break;
-    default: k_batch_grad<T, 256, VJP><<<grid, 256, 0, st>>>(A, P, D); break;
+    case 64: k_batch_grad<T, 64, VJP, MDK_MAXV, MDK_MAXP><<<grid, 64, 0, st>>>(A, P, D); break;
+    case 128: k_batch_grad<T, 128, VJP, MDK_MAXV, MDK_MAXP><<<grid, 128, 0, st>>>(A, P, D); break;
+    case 512: k_batch_grad<T, 512, VJP, MDK_MAXV, MDK_MAXP><<<grid, 512, 0, st>>>(A, P, D); break;
+    default: k_batch_grad<T, 256, VJP, MDK_MAXV, MDK_MAXP><<<grid, 256, 0, st>>>(A, P, D); break;
   }
 }
 
