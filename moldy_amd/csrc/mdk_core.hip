@@ -91,6 +91,15 @@ struct PackGeo {
   double ix, iy, iz;  // 1/g
 };
 
+// Arguments of the half kick's finishing (k_finalize_kick, or deferred into
+// the next force's finishing kernel).
+struct FinKick {
+  BoxState *bs;
+  BoxState *bout;  // the advanced copy of a fused drift: receives the same sums
+  double *trace_row, *mon;
+  double dt, bdel2, press;
+};
+
 struct mdk_ctx {
   mdk_params p;
   int n;
@@ -140,6 +149,11 @@ struct mdk_ctx {
   bool list_valid;
   bool half_valid;         // numn_ref/nlist_ref hold a reference half list
   bool x3_live;            // X3..Z3 may be non-zero (uploaded, or the predictor-corrector ran)
+  // a half kick whose sums the next force's finishing kernel completes
+  bool kick_pending;
+  FinKick kick_pend;
+  int kick_slot;
+  double *kstage;          // first-stage sums of the deferred kick
   // reference link-cell chains (src/linkcell_m.F90:122-274) on the host in
   // reference atom order, 1-based: L0 per cell, LINK, IC; chains_exact is
   // false after relinks inside mdk_md_steps, which update only the device ic
@@ -1333,6 +1347,33 @@ __device__ bool stage_and_finish(const double *part, int nb, double *stage, unsi
   return true;
 }
 
+// stage_and_finish over two partial arrays of nb blocks in one pass (part2 may
+// be null): the deferred half-kick sums ride on force()'s arrival counter.
+template <int K, int KM, int K2, int KM2>
+__device__ bool stage_and_finish2(const double *part, const double *part2, int nb, double *stage,
+                                  double *stage2, unsigned *cnt, double (&out)[K], double (&out2)[K2]) {
+  const int p = gridDim.x, b0 = blockIdx.x * 64;
+  double v[K], v2[K2];
+  fold_columns<K, KM>(part, nb, b0, min(nb, b0 + 64), v);
+  if (part2) fold_columns<K2, KM2>(part2, nb, b0, min(nb, b0 + 64), v2);
+  int last = 0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) stage[(size_t)k * p + blockIdx.x] = v[k];
+    if (part2) {
+#pragma unroll
+      for (int k = 0; k < K2; k++) stage2[(size_t)k * p + blockIdx.x] = v2[k];
+    }
+    const unsigned prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = prev + 1 == (unsigned)p;
+  }
+  if (!__builtin_amdgcn_readfirstlane(last)) return false;
+  fold_columns<K, KM>(stage, p, 0, p, out);
+  if (part2) fold_columns<K2, KM2>(stage2, p, 0, p, out2);
+  if (threadIdx.x == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
 // One-workgroup reduction of the per-block partials for the finishing
 // kernels (MDK_FOLD_BLOCK threads): thread t folds columns t, t + 1024, ...
 // in order, then the lanes of each wave and the waves in index order; thread
@@ -1374,13 +1415,13 @@ struct FinForce {
   BoxState *bs;
   int mode, ivol;
   double press, bdel2, dt;
+  // the previous half kick's sums, when its finishing was deferred to this
+  // kernel (the MD loop): its partials, their first-stage sums, its finish
+  const double *kpart;
+  double *kstage;
+  FinKick kick;
 };
-struct FinKick {
-  BoxState *bs;
-  BoxState *bout;  // the advanced copy of a fused drift: receives the same sums
-  double *trace_row, *mon;
-  double dt, bdel2, press;
-};
+__device__ void finish_kick(const double (&v)[NPART_KICK], const FinKick &F);
 
 // Finishing kernel of force(), ceil(nb / 64) blocks of one wave: the NPART_F
 // columns of per-block partials (force pass, kinten, embedding energy) are
@@ -1408,8 +1449,11 @@ __global__ void __launch_bounds__(MDK_FIN_BLOCK) k_finalize_force(const double *
     b2[t] = bs->b2[t];
     tgid0[t] = bs->tgid[t];
   }
-  double v[NPART_F];
-  if (!stage_and_finish<NPART_F, 0>(part, nb, F.stage, F.cnt, v)) return;
+  double v[NPART_F], kv[NPART_KICK];
+  if (!stage_and_finish2<NPART_F, 0, NPART_KICK, 2>(part, F.kpart, nb, F.stage, F.kstage, F.cnt, v, kv)) return;
+  // the previous step's kick sums (its box copy is untouched until this
+  // step's kick): only fields that this kernel does not write
+  if (F.kpart && t == 0) finish_kick(kv, F.kick);
   if (t == 0)
     for (int k = 0; k < NPART_F; k++) sv[k] = v[k];
   __syncthreads();
@@ -2590,10 +2634,30 @@ static int launch_force(mdk_ctx *c, int fin_mode) {
   }
   tpush(c, te);
   RCK(check_launch("k_force"));
-  const FinForce F = {c->stage_f, c->fin_cnt, c->box + c->cur, fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
-                      c->p.deltat};
+  FinForce F = {c->stage_f, c->fin_cnt, c->box + c->cur, fin_mode, c->p.ivol, c->p.press, c->p.bdel2,
+                c->p.deltat, nullptr, c->kstage, FinKick{}};
+  const bool pend = c->kick_pending;  // the previous step's kick sums finish here
+  if (pend) {
+    F.kpart = c->part_k;
+    F.kick = c->kick_pend;
+  }
   k_finalize_force<<<(c->nblk + 63) / 64, MDK_FIN_BLOCK, 0, c->stream>>>(c->part_f, c->nblk, F);
-  return check_launch("k_finalize_force");
+  RCK(check_launch("k_finalize_force"));
+  if (pend) {
+    c->kick_pending = false;
+    HIPCK(hipEventRecord(c->mon_ev[c->kick_slot], c->stream));  // that step's monitor sample is written
+  }
+  return MDK_OK;
+}
+
+// Finishes a deferred half kick on its own (the last step of an MD call).
+static int flush_kick(mdk_ctx *c) {
+  if (!c->kick_pending) return MDK_OK;
+  c->kick_pending = false;
+  k_finalize_kick<<<1, MDK_FOLD_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, c->kick_pend);
+  RCK(check_launch("k_finalize_kick"));
+  HIPCK(hipEventRecord(c->mon_ev[c->kick_slot], c->stream));
+  return MDK_OK;
 }
 
 // Checks the flags of a deferred build (the density pass on it is queued and
@@ -2631,7 +2695,10 @@ static int force_body(mdk_ctx *c, int fin_mode, bool skin_due) {
 // fuse_drift (mode 0) the kernel also performs the next step's drift; the
 // finishing block then completes this step's sums on the current box copy and
 // the next md_step skips its drift.
-static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon, bool fuse_drift = false) {
+// defer (mode 0): the sums' finishing is left to the next force's finishing
+// kernel (or flush_kick); the caller records the monitor event then.
+static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon, bool fuse_drift = false,
+                       bool defer = false) {
   TimedEv te;
   tbegin(c, TC_KICK, te);
   const int nxt = 1 - c->cur;
@@ -2650,8 +2717,13 @@ static int launch_kick(mdk_ctx *c, int mode, double *trace_row_dev, double *mon,
   if (mode != 2) {  // initial accelerations have no sums
     const FinKick F = {c->box + c->cur, fused ? c->box + nxt : nullptr, trace_row_dev,
                        mon, c->p.deltat, c->p.bdel2, c->p.press};
-    k_finalize_kick<<<1, MDK_FOLD_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, F);
-    RCK(check_launch("k_finalize_kick"));
+    if (defer && mode == 0) {
+      c->kick_pend = F;
+      c->kick_pending = true;
+    } else {
+      k_finalize_kick<<<1, MDK_FOLD_BLOCK, 0, c->stream>>>(c->part_k, c->nblk, F);
+      RCK(check_launch("k_finalize_kick"));
+    }
   }
   if (fused) {
     c->pos4_valid = !lean_path(c);
@@ -2682,12 +2754,14 @@ static int launch_drift(mdk_ctx *c) {
   return check_launch("k_drift");
 }
 
-// Reads the monitor sample of step s-2 (its kernels have completed or the
+// Reads the monitor sample of step s-3 (its kernels have completed or the
 // host waits for them): the box for the next grid plan and the skin bound.
+// In the MD loop a step's sample is finished by the next step's force
+// finishing kernel, so the host stays about two steps ahead of the GPU.
 static const double *poll_monitor(mdk_ctx *c) {
   const long s = c->step_counter;
-  if (s < 2) return nullptr;
-  const long probe = s - 2;
+  if (s < 3) return nullptr;
+  const long probe = s - 3;
   const int slot = (int)(probe & 3);
   hipEventSynchronize(c->mon_ev[slot]);
   const double *m = c->mon_host + MON_STRIDE * slot;
@@ -2740,19 +2814,20 @@ static int skin_violation(const mdk_ctx *c, const double *m, long step) {
                  step, sqrt(fmax(m[0], 0.0)), strain_since_build(m), 0.5 * c->skin);
 }
 
-// Checks the completed sample of step s-2 against the skin bound (an error if
+// Checks the completed sample of step s-3 against the skin bound (an error if
 // it was exceeded), then decides whether the next force evaluation could see
-// it exceeded: that sample's displacement and strain plus three times its
+// it exceeded: that sample's displacement and strain plus four times its
 // step's (m[1]: squared displacement of the next drift, strain_of_step: that
-// of the next drift's box change).
+// of the next drift's box change) -- three drifts since the sample and one
+// to spare.
 static int skin_check(mdk_ctx *c, bool *due) {
   const double *m = poll_monitor(c);
   *due = !c->list_valid;
   if (!m) return MDK_OK;
-  if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, c->step_counter - 2);
-  if (c->step_counter - c->last_rebuild_step < 3) return MDK_OK;  // the sample predates the list
+  if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, c->step_counter - 3);
+  if (c->step_counter - c->last_rebuild_step < 4) return MDK_OK;  // the sample predates the list
   const double stp = sqrt(fmax(m[1], 0.0)) + 0.5 * strain_of_step(m) * c->r_list;
-  *due = *due || skin_used(c, m) + 3.0 * stp >= 0.5 * c->skin;
+  *due = *due || skin_used(c, m) + 4.0 * stp >= 0.5 * c->skin;
   return MDK_OK;
 }
 
@@ -2803,6 +2878,8 @@ static int md_step_gear(mdk_ctx *c, double *trace_row_dev) {
   return MDK_OK;
 }
 
+// The half kick's sums are finished by the next step's force finishing kernel
+// (one launch fewer per step); mdk_md_steps finishes the last one itself.
 static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool allow_fuse = false) {
   if (c->iverlet == 0) return md_step_gear(c, trace_row_dev);
   bool skin_due;
@@ -2811,8 +2888,8 @@ static int md_step(mdk_ctx *c, double *trace_row_dev, bool fused_therm, bool all
   RCK(launch_drift(c));
   RCK(force_body(c, fused_therm ? FIN_VV_THERM : FIN_VV, skin_due));
   int slot = (int)(c->step_counter & 3);
-  RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + MON_STRIDE * slot, allow_fuse && c->ntc != 0));
-  HIPCK(hipEventRecord(c->mon_ev[slot], c->stream));
+  RCK(launch_kick(c, 0, trace_row_dev, c->mon_dev + MON_STRIDE * slot, allow_fuse && c->ntc != 0, true));
+  c->kick_slot = slot;
   c->step_counter++;
   return MDK_OK;
 }
@@ -2897,6 +2974,7 @@ extern "C" int mdk_create(void **handle, const mdk_params *p) {
   ALLOC(c->part_k, sizeof(double) * (size_t)c->nblk * NPART_KICK);
   ALLOC(c->stage_f, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_F);
   ALLOC(c->stage_k, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
+  ALLOC(c->kstage, sizeof(double) * (size_t)(c->nblk / 64 + 1) * NPART_KICK);
   ALLOC(c->fin_cnt, sizeof(unsigned) * 2);
   ALLOC(c->red, sizeof(double) * 4);
   ALLOC(c->spec_tab, sizeof(double) * 2 * MDK_NSPEC);
@@ -2951,7 +3029,7 @@ extern "C" int mdk_destroy(void *h) {
                   c->sref, c->ic, c->numn_ref, c->nlist_ref, c->nbr, c->numn, c->nact, c->cell_of,
                   c->members, c->dflags, c->box, c->part_f, c->part_k, c->stage_f, c->stage_k, c->fin_cnt, c->cell_count,
                   c->cell_start, c->cell_fill, c->tabs, c->trace_dev, c->lblocks, c->spec_tab, c->red, c->fetab,
-                  c->ref_key, c->rank_orig, c->rank_slot};
+                  c->ref_key, c->rank_orig, c->rank_slot, c->kstage};
   for (void *q : ptrs)
     if (q) hipFree(q);
   if (c->mon_host) hipHostFree(c->mon_host);
@@ -3691,16 +3769,20 @@ extern "C" int mdk_md_steps(void *h, int nsteps, double *trace) {
     HIPCK(hipMalloc(&c->trace_dev, sizeof(double) * 6 * (size_t)nsteps));
     c->trace_cap = nsteps;
   }
-  for (int s = 0; s < nsteps; s++) {
-    RCK(md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true, s + 1 < nsteps));
+  int rc = MDK_OK;
+  for (int s = 0; s < nsteps && rc == MDK_OK; s++) {
+    rc = md_step(c, trace ? c->trace_dev + 6 * (size_t)s : nullptr, true, s + 1 < nsteps);
     // relink when the reference's ntc counter wraps (moldin.F90:791)
-    if (c->ntc == 0) RCK(relink_in_loop(c));
+    if (rc == MDK_OK && c->ntc == 0) rc = relink_in_loop(c);
   }
+  const int rf = flush_kick(c);  // the last step's sums (also after a failed step)
+  if (rc != MDK_OK) return rc;
+  RCK(rf);
   if (trace && nsteps > 0)
     HIPCK(hipMemcpyAsync(trace, c->trace_dev, sizeof(double) * 6 * (size_t)nsteps, hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
-  // the samples of the last two steps, not yet polled
-  for (long st = std::max(c->step_counter - 2, c->step_counter - nsteps); st < c->step_counter; st++) {
+  // the samples of the last three steps, not yet polled
+  for (long st = std::max(c->step_counter - 3, c->step_counter - nsteps); st < c->step_counter; st++) {
     const double *m = c->mon_host + MON_STRIDE * (int)(st & 3);
     if (skin_used(c, m) >= 0.5 * c->skin) return skin_violation(c, m, st);
   }
