@@ -21,9 +21,11 @@ module mdk_device_m
   public :: mdk_dev_push_positions
   ! device options with no counterpart in the reference's simparameters, so
   ! the drop-in also builds against the reference's own params_m: set them
-  ! before the first init_linkcell_m.  precision 0 FP64 / 1 FP32 form;
-  ! rebuild 0 the reference's ntcm cadence plus the skin monitor, 1 skin
-  ! monitor only; rskin the list skin (<= 0: the reference's rpad)
+  ! before the first init_linkcell_m.  precision 0 FP64 / 1 FP32 form (an
+  ! unchanged caller selects the FP32 form with MDK_PRECISION=fp32 in the
+  ! environment); rebuild 0 the reference's ntcm cadence plus the skin
+  ! monitor, 1 skin monitor only; rskin the list skin (<= 0: the reference's
+  ! rpad)
   integer, public, save :: mdk_precision = 0, mdk_rebuild = 0
   real(c_double), public, save :: mdk_rskin = 0.0d0
 
@@ -51,11 +53,16 @@ contains
   subroutine mdk_dev_create()
     type(simparameters) :: sp
     type(mdk_params_t) :: p
+    character(len=16) :: envv
+    integer :: envlen, envst
     if (c_associated(handle)) return
     sp = get_params()
     p%nm = sp%nm; p%ivol = sp%ivol; p%nnbrs = sp%nnbrs; p%ntcm = sp%ntcm
     p%nlcx = sp%nlcx; p%nlcy = sp%nlcy; p%nlcz = sp%nlcz
     p%precision = mdk_precision; p%rebuild = mdk_rebuild; p%max_full = 0
+    call get_environment_variable('MDK_PRECISION', envv, envlen, envst)
+    if (envst == 0 .and. (trim(envv) == 'fp32' .or. trim(envv) == 'FP32' .or. trim(envv) == '1')) &
+       p%precision = 1
     p%deltat = sp%deltat; p%rnear = sp%rnear; p%press = sp%press; p%bdel2 = sp%bdel2
     p%rskin = mdk_rskin
     if (p%rskin <= 0.0d0) p%rskin = sp%rpad

@@ -73,6 +73,33 @@ def test_fortran_dropin_matches_reference(oracle, case):
                 assert rel(d[k][knot], F[k][knot]) < 1e-13, k
 
 
+def test_fortran_dropin_fp32_form_from_the_environment(oracle, monkeypatch):
+    """An unchanged caller selects the FP32 form with MDK_PRECISION=fp32: the
+    golden driver over the drop-in then reproduces the reference's first
+    force within the FP32 form's bounds (1e-5 eV/A, 1e-6 eV/atom) and its
+    12-step NVE trajectory's energies within 1e-6 eV/atom, while the pair set
+    stays the reference's."""
+    import gen_golden
+    import refrun
+    F = load_golden("c0_nve_2000")
+    kw = {k[4:]: (str(F[k]) if F[k].dtype.kind == "U" else int(F[k]) if F[k].dtype.kind == "i" else float(F[k]))
+          for k in F.files if k.startswith("run_")}
+    print("[gpu-test] fortran drop-in, FP32 form", flush=True)
+    monkeypatch.setenv("MDK_PRECISION", "fp32")
+    d = gen_golden.run_reference(golden_system(F), driver=DROPIN, pot_file=refrun.POT_FILE, **kw)
+    assert oracle.pairset_digest(d["pairs0"]) == str(F["pairs0_sha256"])
+    b0 = F["b0"]
+    f = np.stack([d["fx0"], d["fy0"], d["fz0"]], axis=1) @ b0.T
+    fr = np.stack([F["fx0"], F["fy0"], F["fz0"]], axis=1) @ b0.T
+    err = float(np.max(np.abs(f - fr)))
+    n = len(F["fx0"])
+    de = abs(d["pe0"] - float(F["pe0"])) / n
+    print(f"[gpu-test] fp32 drop-in: force error {err:.3e} eV/A, energy {de:.3e} eV/atom", flush=True)
+    assert 0.0 < err < 1e-5   # the FP32 form ran (FP64 agrees to 1e-10 relative)
+    assert de < 1e-6
+    assert np.max(np.abs(d["trace"][:, 0] - F["trace"][:, 0])) / n < 1e-6
+
+
 def test_fortran_dropin_quench_matches_reference():
     """quench_m's init_quench_m / quench / cleanup_quench_m through the drop-in
     (the driver's flag 32: a quench after the first force, as moldin.F90:566-572)
