@@ -629,17 +629,21 @@ __device__ __forceinline__ int morton3(int x, int y, int z) {
 }
 
 // Packed position of atom i from fractional s - img (see the entry format).
-__device__ __forceinline__ double3 packed_pos(const double *b, PackGeo pg, const int *cell, int i, double fx,
-                                              double fy, double fz) {
+// k: the atom's cell key (read only when pg.gx is set)
+__device__ __forceinline__ double3 packed_pos_key(const double *b, PackGeo pg, unsigned k, double fx, double fy,
+                                                  double fz) {
 #pragma clang fp contract(off)  // the same packed positions from every kernel that writes them
   if (pg.gx) {
-    const unsigned k = (unsigned)cell[i];
     fx -= ((double)compact3(k) + 0.5) * pg.ix;
     fy -= ((double)compact3(k >> 1) + 0.5) * pg.iy;
     fz -= ((double)compact3(k >> 2) + 0.5) * pg.iz;
   }
   return make_double3(b[0] * fx + b[3] * fy + b[6] * fz, b[1] * fx + b[4] * fy + b[7] * fz,
                       b[2] * fx + b[5] * fy + b[8] * fz);
+}
+__device__ __forceinline__ double3 packed_pos(const double *b, PackGeo pg, const int *cell, int i, double fx,
+                                              double fy, double fz) {
+  return packed_pos_key(b, pg, pg.gx ? (unsigned)cell[i] : 0u, fx, fy, fz);
 }
 __device__ __forceinline__ void store_packed(double4 *pos4, float4 *p4f, int i, double3 r, double w) {
   if (pos4) pos4[i] = make_double4(r.x, r.y, r.z, w);
@@ -2006,12 +2010,33 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
     double *x1p = atoms + (size_t)F_X1 * n, *y1p = atoms + (size_t)F_Y1 * n, *z1p = atoms + (size_t)F_Z1 * n;
     double *x2p = atoms + (size_t)F_X2 * n, *y2p = atoms + (size_t)F_Y2 * n, *z2p = atoms + (size_t)F_Z2 * n;
     const double *fxp = atoms + (size_t)F_FX * n, *fyp = atoms + (size_t)F_FY * n, *fzp = atoms + (size_t)F_FZ * n;
+    // Every per-atom load is issued here, before the first store: the fields
+    // share one allocation, so a load written after a store to another field
+    // could not be moved ahead of it, and each would cost a round trip of its own.
     const int z = zat[i];
-    // mass and del2 from the species table when they are functions of the species
-    const double m = spec ? spec[2 * z] : atoms[(size_t)F_MASS * n + i];
-    const double d2 = spec ? spec[2 * z + 1] : atoms[(size_t)F_DEL2 * n + i];
     double x1 = x1p[i], y1 = y1p[i], z1 = z1p[i];
     double x2 = x2p[i], y2 = y2p[i], z2 = z2p[i];
+    const double fx = fxp[i], fy = fyp[i], fz = fzp[i];
+    const double x0 = x0p[i], y0 = y0p[i], z0 = z0p[i];
+    int im0 = 0, im1 = 0, im2 = 0;
+    unsigned ck = 0u;
+    if (DRIFT) {
+      im0 = img[3 * i];
+      im1 = img[3 * i + 1];
+      im2 = img[3 * i + 2];
+      if (pg.gx) ck = (unsigned)cell[i];
+    }
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+    if (sref) {
+      r0 = sref[3 * i];
+      r1 = sref[3 * i + 1];
+      r2 = sref[3 * i + 2];
+    }
+    // mass and del2 from the species table when they are functions of the
+    // species: one 16-byte load of the pair (the table is hipMalloc-aligned)
+    const double2 md = spec ? *reinterpret_cast<const double2 *>(spec + 2 * z)
+                            : make_double2(atoms[(size_t)F_MASS * n + i], atoms[(size_t)F_DEL2 * n + i]);
+    const double m = md.x, d2 = md.y;
     if (mode == 0) {
       const double *g = bs->tgid_lag;
       const double xd = x1 + x2, yd = y1 + y2, zd = z1 + z2;  // x1_dt2
@@ -2023,9 +2048,9 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
           sz = TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1;
         }
         if (ivol == 4) sz = TT(g, 3, 3) * z1;
-        x2 = d2 * fxp[i] - 0.5 * sx;
-        y2 = d2 * fyp[i] - 0.5 * sy;
-        z2 = d2 * fzp[i] - 0.5 * sz;
+        x2 = d2 * fx - 0.5 * sx;
+        y2 = d2 * fy - 0.5 * sy;
+        z2 = d2 * fz - 0.5 * sz;
         x2p[i] = x2; y2p[i] = y2; z2p[i] = z2;
       }
       x1 = xd + x2; y1 = yd + y2; z1 = zd + z2;
@@ -2036,9 +2061,9 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
         double a1 = TT(g, 1, 1) * x1 + TT(g, 1, 2) * y1 + TT(g, 1, 3) * z1;
         double a2 = TT(g, 2, 1) * x1 + TT(g, 2, 2) * y1 + TT(g, 2, 3) * z1;
         double a3 = TT(g, 3, 1) * x1 + TT(g, 3, 2) * y1 + TT(g, 3, 3) * z1;
-        x2 = d2 * fxp[i] - 0.5 * a1;
-        y2 = d2 * fyp[i] - 0.5 * a2;
-        z2 = d2 * fzp[i] - 0.5 * a3;
+        x2 = d2 * fx - 0.5 * a1;
+        y2 = d2 * fy - 0.5 * a2;
+        z2 = d2 * fz - 0.5 * a3;
         x2p[i] = x2; y2p[i] = y2; z2p[i] = z2;
       }
     }
@@ -2060,10 +2085,8 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
       double rz = TT(b, 3, 1) * x1 + TT(b, 3, 2) * y1 + TT(b, 3, 3) * z1;
       v[2] = m * (rx * rx + ry * ry + rz * rz);
     }
-    const double x0 = x0p[i], y0 = y0p[i], z0 = z0p[i];
     if (sref) {
-      double ex = x0 - (double)sref[3 * i], ey = y0 - (double)sref[3 * i + 1],
-             ez = z0 - (double)sref[3 * i + 2];
+      double ex = x0 - (double)r0, ey = y0 - (double)r1, ez = z0 - (double)r2;
       double qx = b[0] * ex + b[3] * ey + b[6] * ez, qy = b[1] * ex + b[4] * ey + b[7] * ez,
              qz = b[2] * ex + b[5] * ey + b[8] * ez;
       v[3] = qx * qx + qy * qy + qz * qz;
@@ -2083,7 +2106,7 @@ __global__ void __launch_bounds__(MDK_BLOCK) k_kick(double *__restrict__ atoms, 
         const bool upd = ivol < 1 || (ivol == 4 && k == 8);
         bn[k] = upd ? bs->b0[k] + (bs->b1[k] + bs->b2[k]) : bs->b0[k];
       }
-      const double3 r = packed_pos(bn, pg, cell, i, xn - img[3 * i], yn - img[3 * i + 1], zn - img[3 * i + 2]);
+      const double3 r = packed_pos_key(bn, pg, ck, xn - im0, yn - im1, zn - im2);
       store_packed(pos4, p4f, i, r, 0.0);  // F' is stored by the density pass, which always runs next
       if (z != 0) kin_terms(bn, x1, y1, z1, m, kv);  // kinten of the next force evaluation
     }
