@@ -841,8 +841,7 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
         const int h = h0 + 4 * dy + 16 * dz;  // the row's cell at dx = 0
         const int lo = hoff[h], m1 = hoff[h + 1], m2 = hoff[h + 2], hi = m2 + hcnt[h + 2];
         const int tagb = 3 * dy + 9 * dz;
-        for (int c = lo; c < hi; c++) {
-          const float4 q = cpos[c];
+        auto test = [&](const float4 q, int c) {
           const float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
           const int j = __float_as_int(q.w);
           if (rx * rx + ry * ry + rz * rz < rl2f && j != i) {
@@ -850,7 +849,18 @@ __global__ void __launch_bounds__(64) k_full_list_blk(
             if (m < maxn) nbr[nbr_at(m, i, n)] = j | (code << NBR_CODE_SHIFT);
             m++;
           }
+        };
+        // four candidates per trip: their LDS reads are in flight together;
+        // the tests run in candidate order, so the list is the same
+        int c = lo;
+        for (; c + 4 <= hi; c += 4) {
+          const float4 q0 = cpos[c], q1 = cpos[c + 1], q2 = cpos[c + 2], q3 = cpos[c + 3];
+          test(q0, c);
+          test(q1, c + 1);
+          test(q2, c + 2);
+          test(q3, c + 3);
         }
+        for (; c < hi; c++) test(cpos[c], c);
       }
     numn[i] = min(m, maxn);
     if (m > maxn) atomicMax(&flags[0], m);
