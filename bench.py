@@ -252,6 +252,7 @@ def c0_leg(steps=1000):
         tr = e.md_steps(steps, trace=True)
         e.synchronize()
         secs = time.perf_counter() - t0
+        full = e.stats()[1]
         e.close()
         te = tr[:, 2] / sd.n
         t_ps = np.arange(steps) * 1e-3
@@ -260,8 +261,19 @@ def c0_leg(steps=1000):
                     "max_abs_dev_eV_per_atom": float(np.max(np.abs(te - te[0]))),
                     "drift_eV_per_atom_per_ps": slope,
                     "ms_per_step": secs / steps * 1e3, "atom_steps_per_s": sd.n * steps / secs,
+                    "roofline": step_roofline(sd.n, full / sd.n, secs / steps),
                     "bound": "launch latency (2000 atoms: each kernel is a few workgroups)"}
     return out
+
+
+def step_roofline(n, nf, sec_per_step):
+    """Whole MD step against the HBM roof: 304 + 8 n_f algorithmic bytes per
+    atom-step (positions, velocities, accelerations and forces in FP64, the
+    packed records, the density pass's and the force pass's list reads)."""
+    b = n * (304.0 + 8.0 * nf)
+    gbs = b / sec_per_step / 1e9
+    return {"model": "whole MD step, 304 + 8 n_f bytes per atom-step", "bytes_per_step": b,
+            "achieved_GBs": gbs, "frac": gbs / PEAK_HBM_GBS}
 
 
 def c2_system():
@@ -305,6 +317,7 @@ def c2_leg(steps=1000):
                     "rebuilds": reb, "rebuild_us_avg": 1e3 * rb[0] / max(rb[1], 1),
                     "rebuild_ms_total": rb[0],
                     "te_drift_eV_per_atom": float((tr[-1, 2] - tr[0, 2]) / sd.n),
+                    "step_roofline": step_roofline(sd.n, full / sd.n, secs / steps),
                     "roofline": {"kernel": "force pass", "us_per_launch": us_f, "bytes_per_launch": nb,
                                  "achieved_GBs": nb / (us_f * 1e-6) / 1e9,
                                  "frac": nb / (us_f * 1e-6) / 1e9 / PEAK_HBM_GBS,
@@ -432,6 +445,7 @@ def main():
                         "kernel class; in the timed region only every 10th force-kernel launch records "
                         "its own dispatch events",
         "roofline": roofline,
+        "step_roofline": step_roofline(sd.n, nf, r32["ms_per_step"] * 1e-3),
         "nve_drift_2000": c0,
         "hot_liquid_3456000": c2,
         "batch": batch,
