@@ -262,8 +262,7 @@ __device__ double phase_list(const BatchArgs &A, const PotDev &P, const CellView
             const float4 d = SH.Sf[code], l = SH.Sl[code];
             const int tag = code << NB_CODE_SHIFT;
             const int j1 = V.sst[cj + 1];
-            for (int jl = V.sst[cj]; jl < j1; jl++) {
-              const float4 pj = pc[jl];
+            auto test = [&](const float4 pj, int jl) {
               const float ex = ((pf.x - pj.x) + d.x) + l.x, ey = ((pf.y - pj.y) + d.y) + l.y,
                           ez = ((pf.z - pj.z) + d.z) + l.z;
               const float r2 = ex * ex + ey * ey + ez * ez;
@@ -274,7 +273,18 @@ __device__ double phase_list(const BatchArgs &A, const PotDev &P, const CellView
                 r_inv(r2, r, ri);
                 acc += (double)mdk_phi<float>(P, r);
               }
+            };
+            // four candidates per trip: their loads are in flight together;
+            // the tests (and the density sum) run in candidate order
+            int jl = V.sst[cj];
+            for (; jl + 4 <= j1; jl += 4) {
+              const float4 q0 = pc[jl], q1 = pc[jl + 1], q2 = pc[jl + 2], q3 = pc[jl + 3];
+              test(q0, jl);
+              test(q1, jl + 1);
+              test(q2, jl + 2);
+              test(q3, jl + 3);
             }
+            for (; jl < j1; jl++) test(pc[jl], jl);
           }
         }
       }
